@@ -1,0 +1,267 @@
+"""Python mirror of AM-CFD's per-time-step solver surface over ``libamcfd.so``.
+
+The reference drives its solve through C++ member functions
+(src/main.cpp:56-162):
+
+    Mesh::getDomainInfo / readMeshFile        -> Deck(path)
+    BoundCondManager::updateLaser             -> Solver.update_laser()
+    CFDSolverManager::iterativeLoopSolve      -> Solver.iterative_loop_solve()
+    CFDSolverManager::updateSolutions         -> Solver.update_solutions()
+    SolutionVariableManager::mapSolutionVars_ -> Solver.field(name)
+
+Everything below is a thin ctypes layer over the C ABI in include/amcfd.h;
+all arithmetic runs in the HIP kernels of libamcfd.so.  There is no CPU
+fallback: constructing a Solver without the built library raises.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libamcfd.so")
+CLI_PATH = os.path.join(_HERE, "amcfd")
+
+MAX_ZONES = 32
+OK, I_MAXIT = 0, 1
+MODE_FP64 = 0
+
+FIELD_NAMES = ("enthalpy", "temp", "fracl", "uvel", "vvel", "wvel", "pressure", "pp", "rho",
+               "diff", "vis", "hnot", "fraclnot", "unot", "vnot", "wnot", "sourceinput", "dux",
+               "dvy", "dwz", "csfrac", "solfrac", "mpPhase", "mpLiqPhase")
+
+# header symbols of include/amcfd.h (checked by the CPU tests)
+EXPORTS = ("amcfd_deck_load", "amcfd_deck_params", "amcfd_deck_free", "amcfd_create",
+           "amcfd_destroy", "amcfd_begin_step", "amcfd_solve_step", "amcfd_end_step",
+           "amcfd_step", "amcfd_get_field", "amcfd_set_field", "amcfd_get_geometry",
+           "amcfd_get_scalars", "amcfd_dims", "amcfd_last_timing", "amcfd_set_profiling",
+           "amcfd_kernel_stats", "amcfd_last_error", "amcfd_version")
+
+
+class Params(ctypes.Structure):
+    """ctypes mirror of ``amcfd_params``."""
+    _Z = ctypes.c_double * MAX_ZONES
+    _fields_ = [
+        ("nzx", ctypes.c_int), ("nzy", ctypes.c_int), ("nzz", ctypes.c_int),
+        ("xzone", _Z), ("ncvx", _Z), ("powrx", _Z),
+        ("yzone", _Z), ("ncvy", _Z), ("powry", _Z),
+        ("zzone", _Z), ("ncvz", _Z), ("powrz", _Z),
+    ] + [(n, ctypes.c_double) for n in (
+        "finaltime", "maxit", "nonlintol", "delt", "outtime",
+        "grav", "sigm", "tempamb", "pressamb", "gasconst")] + [
+        (n, ctypes.c_int) for n in (
+            "navierstokes", "discretization", "steady", "isevap", "powderbed", "udtoolpath",
+            "gaussheatsource", "volheatsource", "volheatsourcetable")] + [
+        (n, ctypes.c_double) for n in (
+            "lasertime", "xstart", "ystart", "scanvel",
+            "dens", "denl", "viscos", "tsolid", "tliquid", "hsmelt", "hlfriz", "acpa", "acpb", "acpl",
+            "thconsa", "thconsb", "thconsc", "thconla", "thconlb",
+            "urfu", "urfv", "urfw", "urfp", "urfh",
+            "htci", "htcj", "htck1", "htckn", "emiss",
+            "tempwest", "tempeast", "tempnorth", "tempbottom", "temppreheat",
+            "beta", "dgdtp", "darcyKo", "a_oxygen",
+            "alaspow", "alaseta", "alasrb", "alasfact",
+            "evaplatent", "evaptemp", "evapfact", "molarmass",
+            "avolpow", "apowseta", "heatthick", "heatrb", "avolfact")] + [
+        ("numlayers", ctypes.c_int), ("ncvzlayer", ctypes.c_int)] + [
+        (n, ctypes.c_double) for n in (
+            "layerheight", "powderden", "powdercpa", "powdercpb", "powderthcona", "powderthconb")] + [
+        ("n_toolpath", ctypes.c_int), ("toolpath", ctypes.POINTER(ctypes.c_double)),
+        ("n_dt_table", ctypes.c_int), ("dt_table", ctypes.POINTER(ctypes.c_double)),
+        ("n_heat_table", ctypes.c_int), ("heat_table", ctypes.POINTER(ctypes.c_double)),
+        ("ni", ctypes.c_int), ("nj", ctypes.c_int), ("nk", ctypes.c_int), ("nk_active0", ctypes.c_int),
+        ("hlcal", ctypes.c_double), ("hlatnt", ctypes.c_double)]
+
+
+class Scalars(ctypes.Structure):
+    """ctypes mirror of ``amcfd_scalars``."""
+    _fields_ = [("timet", ctypes.c_double), ("niter", ctypes.c_int), ("itertot", ctypes.c_int)] + [
+        (n, ctypes.c_double) for n in (
+            "resorh", "resorm", "resoru", "resorv", "resorw", "tpeak", "alen", "depth", "width",
+            "ratio", "heatinlaser", "heatout", "accul", "heatvol", "ahtoploss", "beamposx",
+            "beamposy", "scanvel")] + [
+        (n, ctypes.c_int) for n in (
+            "powerindicator", "trackindx", "istat", "iend", "jstat", "jend", "kstat", "istart",
+            "jstart", "nk", "step")]
+
+    def as_dict(self):
+        return {n: getattr(self, n) for n, _ in self._fields_}
+
+
+class AmcfdError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("amcfd error %d: %s" % (code, msg))
+        self.code = code
+
+
+_lib = None
+
+
+def load_library(path=LIB_PATH):
+    """Load libamcfd.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise RuntimeError("libamcfd.so is not built (%s); run __graft_entry__.build()" % path)
+    lib = ctypes.CDLL(path)
+    vp, ip, dp, cp = ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_double), ctypes.c_char_p
+    sig = {
+        "amcfd_deck_load": (ip, [cp, ctypes.POINTER(vp)]),
+        "amcfd_deck_params": (ctypes.POINTER(Params), [vp]),
+        "amcfd_deck_free": (None, [vp]),
+        "amcfd_create": (ip, [ctypes.POINTER(Params), ip, ctypes.POINTER(vp)]),
+        "amcfd_destroy": (None, [vp]),
+        "amcfd_begin_step": (ip, [vp]),
+        "amcfd_solve_step": (ip, [vp]),
+        "amcfd_end_step": (ip, [vp]),
+        "amcfd_step": (ip, [vp, ctypes.POINTER(ip)]),
+        "amcfd_get_field": (ip, [vp, cp, dp, ctypes.c_size_t]),
+        "amcfd_set_field": (ip, [vp, cp, dp, ctypes.c_size_t]),
+        "amcfd_get_geometry": (ip, [vp, cp, dp, ctypes.c_size_t]),
+        "amcfd_get_scalars": (ip, [vp, ctypes.POINTER(Scalars)]),
+        "amcfd_dims": (ip, [vp, ctypes.POINTER(ip), ctypes.POINTER(ip), ctypes.POINTER(ip)]),
+        "amcfd_last_timing": (ip, [vp, dp, dp]),
+        "amcfd_set_profiling": (ip, [vp, ip]),
+        "amcfd_kernel_stats": (ip, [vp, cp, ctypes.POINTER(ctypes.c_long), dp]),
+        "amcfd_last_error": (cp, []),
+        "amcfd_version": (cp, []),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def _check(code):
+    if code < 0:
+        raise AmcfdError(code, load_library().amcfd_last_error().decode())
+    return code
+
+
+class Deck:
+    """Parsed keyword deck (reference grammar, Mesh.cpp:39-550)."""
+
+    def __init__(self, path):
+        lib = load_library()
+        self._h = ctypes.c_void_p()
+        _check(lib.amcfd_deck_load(os.fsencode(path), ctypes.byref(self._h)))
+        self.params = lib.amcfd_deck_params(self._h).contents
+
+    def close(self):
+        if self._h:
+            load_library().amcfd_deck_free(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Solver:
+    """Device-resident solver with the reference's per-step call surface."""
+
+    def __init__(self, deck_or_params, mode=MODE_FP64):
+        lib = load_library()
+        if isinstance(deck_or_params, Deck):
+            self._deck = deck_or_params
+            params = deck_or_params.params
+        elif isinstance(deck_or_params, (str, os.PathLike)):
+            self._deck = Deck(deck_or_params)
+            params = self._deck.params
+        else:
+            self._deck = None
+            params = deck_or_params
+        self.params = params
+        self._h = ctypes.c_void_p()
+        _check(lib.amcfd_create(ctypes.byref(params), mode, ctypes.byref(self._h)))
+        ni, nj, nk = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        lib.amcfd_dims(self._h, ctypes.byref(ni), ctypes.byref(nj), ctypes.byref(nk))
+        self.shape = (nk.value, nj.value, ni.value)   # numpy (k, j, i) order
+
+    # -- reference call surface -------------------------------------------------
+    def update_laser(self):
+        """dt table + timet += dt + BoundCondManager::updateLaser."""
+        return _check(load_library().amcfd_begin_step(self._h))
+
+    def iterative_loop_solve(self):
+        """CFDSolverManager::iterativeLoopSolve; returns True if it converged."""
+        return _check(load_library().amcfd_solve_step(self._h)) == OK
+
+    def update_solutions(self):
+        """CFDSolverManager::updateSolutions."""
+        return _check(load_library().amcfd_end_step(self._h))
+
+    def step(self):
+        """One full time step; returns True once timet >= finaltime."""
+        fin = ctypes.c_int(0)
+        _check(load_library().amcfd_step(self._h, ctypes.byref(fin)))
+        return bool(fin.value)
+
+    def run(self, max_steps=None):
+        n = 0
+        while max_steps is None or n < max_steps:
+            n += 1
+            if self.step():
+                break
+        return n
+
+    # -- data access ------------------------------------------------------------
+    def field(self, name):
+        out = np.empty(self.shape, dtype=np.float64)
+        _check(load_library().amcfd_get_field(
+            self._h, name.encode(), out.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), out.size))
+        return out
+
+    def set_field(self, name, values):
+        arr = np.ascontiguousarray(values, dtype=np.float64).reshape(self.shape)
+        _check(load_library().amcfd_set_field(
+            self._h, name.encode(), arr.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), arr.size))
+
+    def geometry(self, name):
+        nk, nj, ni = self.shape
+        sizes = {"x": ni, "y": nj, "z": nk, "xu": ni, "yv": nj, "zw": nk, "dxpwinv": ni,
+                 "dypsinv": nj, "dzpbinv": nk, "fracx": ni, "fracy": nj, "fracz": nk,
+                 "areaij": nj * ni, "areajk": nk * nj, "areaik": nk * ni, "volume": nk * nj * ni}
+        out = np.empty(sizes[name], dtype=np.float64)
+        _check(load_library().amcfd_get_geometry(
+            self._h, name.encode(), out.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), out.size))
+        return out
+
+    def scalars(self):
+        s = Scalars()
+        _check(load_library().amcfd_get_scalars(self._h, ctypes.byref(s)))
+        return s.as_dict()
+
+    def last_timing(self):
+        t, d = ctypes.c_double(), ctypes.c_double()
+        load_library().amcfd_last_timing(self._h, ctypes.byref(t), ctypes.byref(d))
+        return t.value, d.value
+
+    def set_profiling(self, on=True):
+        _check(load_library().amcfd_set_profiling(self._h, int(bool(on))))
+
+    def kernel_stats(self, kernel):
+        n, ms = ctypes.c_long(), ctypes.c_double()
+        _check(load_library().amcfd_kernel_stats(self._h, kernel.encode(), ctypes.byref(n),
+                                                 ctypes.byref(ms)))
+        return n.value, ms.value
+
+    def close(self):
+        if self._h:
+            load_library().amcfd_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def version():
+    return load_library().amcfd_version().decode()

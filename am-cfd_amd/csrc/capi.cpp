@@ -1,0 +1,165 @@
+// extern "C" boundary of the library (declared in include/amcfd.h).
+
+#include "host.h"
+
+#include <cstring>
+#include <exception>
+#include <new>
+
+namespace amcfd {
+
+static thread_local std::string g_last_error;
+
+int set_error(int code, const std::string &msg) {
+    g_last_error = msg;
+    return code;
+}
+
+}  // namespace amcfd
+
+using amcfd::Deck;
+using amcfd::Solver;
+
+struct amcfd_deck {
+    Deck d;
+};
+struct amcfd_solver {
+    Solver s;
+};
+
+#define GUARD(expr)                                                              \
+    try {                                                                        \
+        return (expr);                                                           \
+    } catch (const std::bad_alloc &) {                                           \
+        return amcfd::set_error(AMCFD_E_CAPACITY, "host allocation failed");     \
+    } catch (const std::exception &e) {                                          \
+        return amcfd::set_error(AMCFD_E_PARSE, e.what());                        \
+    }
+
+extern "C" {
+
+int amcfd_deck_load(const char *deck_path, amcfd_deck **out) {
+    if (!deck_path || !out) return amcfd::set_error(AMCFD_E_ARG, "null argument");
+    *out = nullptr;
+    try {
+        amcfd_deck *d = new amcfd_deck();
+        int st = d->d.load(deck_path);
+        if (st) {
+            delete d;
+            return st;
+        }
+        *out = d;
+        return AMCFD_OK;
+    } catch (const std::bad_alloc &) {
+        return amcfd::set_error(AMCFD_E_CAPACITY, "host allocation failed");
+    } catch (const std::exception &e) {
+        return amcfd::set_error(AMCFD_E_PARSE, std::string("deck parse error: ") + e.what());
+    }
+}
+
+const amcfd_params *amcfd_deck_params(const amcfd_deck *deck) { return deck ? &deck->d.p : nullptr; }
+
+void amcfd_deck_free(amcfd_deck *deck) { delete deck; }
+
+int amcfd_create(const amcfd_params *params, int mode, amcfd_solver **out) {
+    if (!params || !out) return amcfd::set_error(AMCFD_E_ARG, "null argument");
+    *out = nullptr;
+    try {
+        amcfd_solver *s = new amcfd_solver();
+        int st = s->s.create(*params, mode);
+        if (st) {
+            delete s;
+            return st;
+        }
+        *out = s;
+        return AMCFD_OK;
+    } catch (const std::bad_alloc &) {
+        return amcfd::set_error(AMCFD_E_CAPACITY, "host allocation failed");
+    }
+}
+
+void amcfd_destroy(amcfd_solver *s) { delete s; }
+
+int amcfd_begin_step(amcfd_solver *s) {
+    if (!s) return amcfd::set_error(AMCFD_E_ARG, "null solver");
+    GUARD(s->s.begin_step());
+}
+
+int amcfd_solve_step(amcfd_solver *s) {
+    if (!s) return amcfd::set_error(AMCFD_E_ARG, "null solver");
+    GUARD(s->s.solve_step());
+}
+
+int amcfd_end_step(amcfd_solver *s) {
+    if (!s) return amcfd::set_error(AMCFD_E_ARG, "null solver");
+    GUARD(s->s.end_step());
+}
+
+int amcfd_step(amcfd_solver *s, int *finished) {
+    if (!s) return amcfd::set_error(AMCFD_E_ARG, "null solver");
+    if (s->s.finished()) {
+        if (finished) *finished = 1;
+        return AMCFD_OK;
+    }
+    int st = amcfd_begin_step(s);
+    if (st < 0) return st;
+    int rs = amcfd_solve_step(s);
+    if (rs < 0) return rs;
+    st = amcfd_end_step(s);
+    if (st < 0) return st;
+    if (finished) *finished = s->s.finished();
+    return rs;
+}
+
+int amcfd_get_field(amcfd_solver *s, const char *name, double *host, size_t n) {
+    if (!s || !name || !host) return amcfd::set_error(AMCFD_E_ARG, "null argument");
+    GUARD(s->s.get_field(name, host, n));
+}
+
+int amcfd_set_field(amcfd_solver *s, const char *name, const double *host, size_t n) {
+    if (!s || !name || !host) return amcfd::set_error(AMCFD_E_ARG, "null argument");
+    GUARD(s->s.set_field(name, host, n));
+}
+
+int amcfd_get_geometry(amcfd_solver *s, const char *name, double *host, size_t n) {
+    if (!s || !name || !host) return amcfd::set_error(AMCFD_E_ARG, "null argument");
+    GUARD(s->s.get_geometry(name, host, n));
+}
+
+int amcfd_get_scalars(amcfd_solver *s, amcfd_scalars *out) {
+    if (!s || !out) return amcfd::set_error(AMCFD_E_ARG, "null argument");
+    GUARD(s->s.get_scalars(out));
+}
+
+int amcfd_dims(amcfd_solver *s, int *ni, int *nj, int *nk) {
+    if (!s || !ni || !nj || !nk) return amcfd::set_error(AMCFD_E_ARG, "null argument");
+    s->s.dims(ni, nj, nk);
+    return AMCFD_OK;
+}
+
+int amcfd_last_timing(amcfd_solver *s, double *total_ms, double *tdma_ms) {
+    if (!s) return amcfd::set_error(AMCFD_E_ARG, "null solver");
+    if (total_ms) *total_ms = s->s.last_total_ms;
+    if (tdma_ms) *tdma_ms = s->s.last_tdma_ms;
+    return AMCFD_OK;
+}
+
+int amcfd_set_profiling(amcfd_solver *s, int on) {
+    if (!s) return amcfd::set_error(AMCFD_E_ARG, "null solver");
+    s->s.set_profiling(on);
+    return AMCFD_OK;
+}
+
+int amcfd_kernel_stats(amcfd_solver *s, const char *kernel, long *launches, double *total_ms) {
+    if (!s || !kernel) return amcfd::set_error(AMCFD_E_ARG, "null argument");
+    auto it = s->s.timers.find(kernel);
+    if (launches) *launches = it == s->s.timers.end() ? 0 : it->second.launches;
+    if (total_ms) *total_ms = it == s->s.timers.end() ? 0.0 : it->second.total_ms;
+    return AMCFD_OK;
+}
+
+const char *amcfd_last_error(void) { return amcfd::g_last_error.c_str(); }
+
+const char *amcfd_version(void) { return "amcfd-mi355x 0.1 (gfx950, FP64 reference-order mode)"; }
+
+}  // extern "C"

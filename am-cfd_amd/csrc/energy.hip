@@ -1,0 +1,829 @@
+// Energy-equation kernels of one SIMPLE iteration (FP64 plain mode).
+//
+// Data layout: every field is the reference's flattened (k*nj + j)*ni + i
+// array, i fastest, so each kernel reads x-contiguous rows; blocks are
+// 64 (i) x 4 (j) threads marching over a few k planes.  Reductions write
+// per-block partials that a single-workgroup kernel folds in a fixed order,
+// so every run is bitwise reproducible.
+
+#include "fv.h"
+
+namespace amcfd {
+
+namespace {
+
+constexpr int TX = 64, TY = 4, KB = 4;
+
+__device__ __forceinline__ size_t cix(const Dims &d, int i, int j, int k) {
+    return (size_t)k * d.sk + (size_t)j * d.sj + (size_t)i;
+}
+
+// ------------------------------------------------------------------------
+// Gaussian surface flux map + total absorbed power (one workgroup),
+// BoundCondManager.cpp:128-159
+// ------------------------------------------------------------------------
+__global__ __launch_bounds__(1024) void k_laser(Dims d, const double *__restrict__ x,
+                                                const double *__restrict__ y,
+                                                const double *__restrict__ areaij, double *heatin,
+                                                double beamx, double beamy, double peakhin, double alasfact,
+                                                double rb2, double varlas, int pi, DevScalars *sc) {
+    __shared__ double scratch[16];
+    const int tid = threadIdx.x, nt = blockDim.x;
+    const int n = d.ni * d.nj;
+    double acc = 0.0;
+    for (int c = tid; c < n; c += nt) {
+        int j = c / d.ni, i = c - j * d.ni;
+        double yd = beamy - y[j];
+        double xd = beamx - x[i];
+        double d2 = xd * xd + yd * yd;
+        double q = peakhin * exp(-alasfact * d2 / rb2) * (double)pi;
+        if (alasfact < 1.0e-6 && d2 < rb2) q = varlas / (M_PI * rb2) * (double)pi;
+        heatin[c] = q;
+        acc += areaij[c] * q;
+    }
+    double v[1] = {acc};
+    block_sum<1>(v, scratch);
+    if (tid == 0) sc->heatinlaser = v[0];
+}
+
+// ------------------------------------------------------------------------
+// enthalpy ghost values: top (laser - 2*radiation - evaporation), east/west
+// and north/south convection, BoundCondManager.cpp:565-651
+// ------------------------------------------------------------------------
+struct BcConst {
+    double sigm, emiss, htci, htcj, tempamb, pressamb, gasconst, evaplatent, evaptemp, evapfact,
+        molarmass;
+    int isevap;
+};
+
+__global__ __launch_bounds__(256) void k_bc_enthalpy(Dims d, BcConst k, const double *__restrict__ dxpwinv,
+                                                     const double *__restrict__ dypsinv,
+                                                     const double *__restrict__ dzpbinv,
+                                                     const double *__restrict__ areaij,
+                                                     const double *__restrict__ heatin,
+                                                     const double *__restrict__ temp,
+                                                     const double *__restrict__ diff, double *h,
+                                                     double *partial) {
+    __shared__ double scratch[8];
+    const int ntop = d.njm2 * d.nim2, new_ = d.nkm2 * d.njm2, nns = d.nkm2 * d.nim2;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    double loss = 0.0;
+    if (t < ntop) {
+        int j = 1 + t / d.nim2, i = 1 + t % d.nim2;
+        size_t top = cix(d, i, j, d.nkm1), below = top - d.sk;
+        double T = temp[top];
+        double hevap = 0.0;
+        if (k.isevap) {
+            double psat = k.pressamb * exp(-k.evaplatent * k.molarmass / k.gasconst * (1.0 / T - 1.0 / k.evaptemp));
+            double mdot = k.evapfact * psat * sqrt(k.molarmass / (2.0 * M_PI * k.gasconst * T));
+            hevap = mdot * k.evaplatent;
+        }
+        double ta = k.tempamb;
+        double hrad = k.emiss * k.sigm * (T * T * T * T - ta * ta * ta * ta);
+        double c1 = diff[below] * dzpbinv[d.nkm1];
+        h[top] = h[below] + (heatin[j * d.ni + i] - hrad - hrad - hevap) / c1;
+        loss = (hrad + hrad + hevap) * areaij[j * d.ni + i];
+    } else if (t < ntop + new_) {
+        int r = t - ntop;
+        int kk = 1 + r / d.njm2, j = 1 + r % d.njm2;
+        size_t w = cix(d, 0, j, kk), e = cix(d, d.nim1, j, kk);
+        double q = k.htci * (temp[w] - k.tempamb);
+        double c1 = diff[w + 1] * dxpwinv[1];
+        h[w] = h[w + 1] - q / c1;
+        q = k.htci * (temp[e] - k.tempamb);
+        c1 = diff[e - 1] * dxpwinv[d.nim2];
+        h[e] = h[e - 1] - q / c1;
+    } else if (t < ntop + new_ + nns) {
+        int r = t - ntop - new_;
+        int kk = 1 + r / d.nim2, i = 1 + r % d.nim2;
+        size_t s = cix(d, i, 0, kk), n = cix(d, i, d.njm1, kk);
+        double q = k.htcj * (temp[s] - k.tempamb);
+        double c1 = diff[s + d.sj] * dypsinv[1];
+        h[s] = h[s + d.sj] - q / c1;
+        q = k.htcj * (temp[n] - k.tempamb);
+        c1 = diff[n - d.sj] * dypsinv[d.njm2];
+        h[n] = h[n - d.sj] - q / c1;
+    }
+    double v[1] = {loss};
+    const int lane = threadIdx.x;
+    v[0] = wave_sum(v[0]);
+    if ((lane & 63) == 0) scratch[lane >> 6] = v[0];
+    __syncthreads();
+    if (lane == 0) {
+        double s = 0.0;
+        for (int w = 0; w < (int)(blockDim.x >> 6); ++w) s += scratch[w];
+        partial[blockIdx.x] = s;
+    }
+}
+
+// ------------------------------------------------------------------------
+// enthalpy coefficients + sources + boundary folding + relaxation, fused
+// with the residual and the block-correction column sums.
+//   discretizeEquations(5)  CFDSolverManager.cpp:1157-1263
+//   calculateSource(5)      CFDSolverManager.cpp:1677-1843
+//   calculateResiduals(5)   CFDSolverManager.cpp:2073-2101
+//   enhanceConvergenceSpeed CFDSolverManager.cpp:2305-2332 (column sums)
+// Each face coefficient is computed by the cell that owns it (gather), with
+// the same operand order as the reference's scatter loop.
+// ------------------------------------------------------------------------
+struct AsmConst {
+    double delt, hlatnt, urfh;
+    double avolpow, apowseta, heatthick, heatrb, avolfact, beamx, beamy;
+    int pi, scheme;
+};
+
+__global__ __launch_bounds__(TX *TY) void k_energy_assemble(
+    Dims d, AsmConst q, DevGeom g, const double *__restrict__ uvel, const double *__restrict__ vvel,
+    const double *__restrict__ wvel, const double *__restrict__ rho, const double *__restrict__ diff,
+    const double *__restrict__ hnot, const double *__restrict__ fracl, const double *__restrict__ fraclnot,
+    const double *__restrict__ h, double *an_o, double *as_o, double *ae_o, double *aw_o, double *at_o,
+    double *ab_o, double *ap_o, double *su_o, double *src_o, double *part_res, double *part_blk) {
+    __shared__ double red[4][TY][TX];
+    __shared__ double scratch[2 * 4];
+    const int tx = threadIdx.x, ty = threadIdx.y;
+    const int i = 1 + blockIdx.x * TX + tx;
+    const int j = 1 + blockIdx.y * TY + ty;
+    const bool col_ok = (i <= d.nim2) && (j <= d.njm2);
+    double sumd = 0.0, sumh = 0.0, bl = 0.0, blp = 0.0, blm = 0.0, blc = 0.0;
+    const int sch = q.scheme;
+    const double ztop = g.z[d.nkm1];
+    if (col_ok) {
+        const double fxi = g.fracx[i], fxw = g.fracx[i - 1], fyj = g.fracy[j], fys = g.fracy[j - 1];
+        const double dxc = g.xu[i + 1] - g.xu[i], dyc = g.yv[j + 1] - g.yv[j];
+        const double aij = g.areaij[j * d.ni + i];
+        const double xb = q.beamx - g.x[i], yb = q.beamy - g.y[j];
+        for (int kk = 0; kk < KB; ++kk) {
+            const int k = 1 + blockIdx.z * KB + kk;
+            if (k > d.nkm2) break;
+            const size_t c = cix(d, i, j, k);
+            const double aik = g.areaik[k * d.ni + i], ajk = g.areajk[k * d.nj + j];
+            const double dP = diff[c], rP = rho[c];
+            const double vol = dxc * dyc * (g.zw[k + 1] - g.zw[k]);
+            double an, as, ae, aw, at, ab;
+            // north face
+            if (j < d.njm2) {
+                double dN = diff[c + d.sj];
+                double fn = rho[c + d.sj] * vvel[c + d.sj] * aik;
+                double dif = dP * dN / ((1.0 - fyj) * dN + fyj * dP);
+                an = coef_minus(sch, dif * aik * g.dypsinv[j + 1], fn);
+            } else {
+                double fn = rho[c + d.sj] * vvel[c + d.sj] * aik;
+                an = coef_minus(sch, dP * aik * g.dypsinv[d.njm1], fn);
+            }
+            // south face
+            {
+                double fs = rP * vvel[c] * aik;
+                if (j > 1) {
+                    double dS = diff[c - d.sj];
+                    double dif = dS * dP / ((1.0 - fys) * dP + fys * dS);
+                    as = coef_plus(sch, dif * aik * g.dypsinv[j], fs);
+                } else {
+                    as = coef_plus(sch, dP * aik * g.dypsinv[1], fs);
+                }
+            }
+            // east face
+            if (i < d.nim2) {
+                double dE = diff[c + 1];
+                double fe = rho[c + 1] * uvel[c + 1] * ajk;
+                double dif = dP * dE / ((1.0 - fxi) * dE + fxi * dP);
+                ae = coef_minus(sch, dif * ajk * g.dxpwinv[i + 1], fe);
+            } else {
+                double fe = rho[c + 1] * uvel[c + 1] * ajk;
+                ae = coef_minus(sch, dP * ajk * g.dxpwinv[d.nim1], fe);
+            }
+            // west face
+            {
+                double fw = rP * uvel[c] * ajk;
+                if (i > 1) {
+                    double dW = diff[c - 1];
+                    double dif = dW * dP / ((1.0 - fxw) * dP + fxw * dW);
+                    aw = coef_plus(sch, dif * ajk * g.dxpwinv[i], fw);
+                } else {
+                    aw = coef_plus(sch, dP * ajk * g.dxpwinv[1], fw);
+                }
+            }
+            // top face
+            if (k < d.nkm2) {
+                double dT = diff[c + d.sk];
+                double ft = rho[c + d.sk] * wvel[c + d.sk] * aij;
+                double fz = g.fracz[k];
+                double dif = dP * dT / ((1.0 - fz) * dT + fz * dP);
+                at = coef_minus(sch, dif * aij * g.dzpbinv[k + 1], ft);
+            } else {
+                double ft = rho[c + d.sk] * wvel[c + d.sk] * aij;
+                at = coef_minus(sch, dP * aij * g.dzpbinv[d.nkm1], ft);
+            }
+            // bottom face
+            {
+                double fb = rP * wvel[c] * aij;
+                if (k > 1) {
+                    double dB = diff[c - d.sk];
+                    double fz = g.fracz[k - 1];
+                    double dif = dB * dP / ((1.0 - fz) * dP + fz * dB);
+                    ab = coef_plus(sch, dif * aij * g.dzpbinv[k], fb);
+                } else {
+                    ab = coef_plus(sch, dP * aij * g.dzpbinv[1], fb);
+                }
+            }
+            const double apnot = rP / q.delt * vol;
+            double sp = 0.0;
+            double su = apnot * hnot[c];
+            // volumetric heat source
+            double src = 0.0;
+            if (ztop - g.z[k] <= q.heatthick) {
+                src = q.avolpow * q.avolfact / M_PI / (q.heatrb * q.heatrb) / q.heatthick * q.apowseta *
+                      exp(-q.avolfact / (q.heatrb * q.heatrb) * ((xb * xb) + (yb * yb))) * (double)q.pi;
+                su += vol * src;
+            }
+            // latent heat: transient and advection terms
+            const double flP = fracl[c];
+            double v1 = q.hlatnt * rP / q.delt;
+            double volht = vol * v1;
+            su -= volht * (flP - fraclnot[c]);
+            {
+                double uw = uvel[c], ue = uvel[c + 1], vs = vvel[c], vn = vvel[c + d.sj];
+                double wb = wvel[c], wt = wvel[c + d.sk];
+                double flew = ajk * (dmax(uw, 0.0) * fracl[c - 1] - dmax(-uw, 0.0) * flP +
+                                     dmax(-ue, 0.0) * fracl[c + 1] - dmax(ue, 0.0) * flP);
+                double flns = aik * (dmax(vs, 0.0) * fracl[c - d.sj] - dmax(-vs, 0.0) * flP +
+                                     dmax(-vn, 0.0) * fracl[c + d.sj] - dmax(vn, 0.0) * flP);
+                double fltb = aij * (dmax(wb, 0.0) * fracl[c - d.sk] - dmax(-wb, 0.0) * flP +
+                                     dmax(-wt, 0.0) * fracl[c + d.sk] - dmax(wt, 0.0) * flP);
+                su += rP * q.hlatnt * (flew + flns + fltb);
+            }
+            const double hP = h[c];
+            const double hN = h[c + d.sj], hS = h[c - d.sj], hE = h[c + 1], hW = h[c - 1];
+            const double hT = h[c + d.sk], hB = h[c - d.sk];
+            // fixed boundary values folded into the source (bottom, top, south, north, west, east)
+            if (k == 1) { su += ab * hB; sp -= ab; ab = 0.0; }
+            if (k == d.nkm2) { su += at * hT; sp -= at; at = 0.0; }
+            if (j == 1) { su += as * hS; sp -= as; as = 0.0; }
+            if (j == d.njm2) { su += an * hN; sp -= an; an = 0.0; }
+            if (i == 1) { su += aw * hW; sp -= aw; aw = 0.0; }
+            if (i == d.nim2) { su += ae * hE; sp -= ae; ae = 0.0; }
+            double ap = an + as + ae + aw + at + ab + apnot - sp;
+            ap /= q.urfh;
+            su += (1.0 - q.urfh) * ap * hP;
+            an_o[c] = an; as_o[c] = as; ae_o[c] = ae; aw_o[c] = aw; at_o[c] = at; ab_o[c] = ab;
+            ap_o[c] = ap; su_o[c] = su; src_o[c] = src;
+            // residual (calculateResiduals(5))
+            double r = (an * hN + as * hS + ae * hE + aw * hW + at * hT + ab * hB + su) / ap - hP;
+            sumd += fabs(r);
+            sumh += fabs(hP);
+            // block-correction column sums
+            bl += ap - an - as - at - ab;
+            blp += ae;
+            blm += aw;
+            blc += ae * hE + aw * hW + an * hN + as * hS + at * hT + ab * hB + su - ap * hP;
+        }
+    }
+    red[0][ty][tx] = bl;
+    red[1][ty][tx] = blp;
+    red[2][ty][tx] = blm;
+    red[3][ty][tx] = blc;
+    double v[2] = {sumd, sumh};
+    block_sum<2>(v, scratch);   // contains __syncthreads
+    const int row = blockIdx.y + gridDim.y * blockIdx.z;
+    if (ty == 0 && i <= d.nim2) {
+#pragma unroll
+        for (int qq = 0; qq < 4; ++qq) {
+            double s = red[qq][0][tx] + red[qq][1][tx] + red[qq][2][tx] + red[qq][3][tx];
+            part_blk[((size_t)row * 4 + qq) * d.ni + i] = s;
+        }
+    }
+    if (tx == 0 && ty == 0) {
+        const int b = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
+        part_res[2 * b] = v[0];
+        part_res[2 * b + 1] = v[1];
+    }
+}
+
+// ------------------------------------------------------------------------
+// phase-wise diffusivity / density / viscosity, CFDSolverManager.cpp:2137-2265
+// ------------------------------------------------------------------------
+struct PropConst {
+    double tliquid, tsolid, viscos, dens, denl, acpa, acpb, acpl, ka, kb, kc, kla, klb;
+    double pden, pcpa, pcpb, pka, pkb;
+    int powder;
+};
+
+__global__ __launch_bounds__(256) void k_props(size_t n, PropConst c, const double *__restrict__ temp,
+                                               const double *__restrict__ fracl,
+                                               const double *__restrict__ csfrac, double *diff, double *rho,
+                                               double *vis) {
+    size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    double T = temp[t];
+    double diffl = (c.kla * T + c.klb) / c.acpl;
+    double dv = diffl, rv = c.denl, vv = c.viscos;
+    if (!(T >= c.tliquid)) {
+        double diffs, rhos;
+        if (!c.powder) {
+            diffs = (c.ka * (T * T) + c.kb * T + c.kc) / (c.acpa * T + c.acpb);
+            rhos = c.dens;
+        } else {
+            double cs = csfrac[t];
+            double cps = c.acpa * T + c.acpb;
+            double conds = c.ka * (T * T) + c.kb * T + c.kc;
+            double diffp = (c.pka * T + c.pkb) / (c.pcpa * T + c.pcpb);
+            diffs = (1.0 - cs) * diffp + cs * conds / cps;
+            rhos = (1.0 - cs) * c.pden + cs * c.dens;
+        }
+        dv = diffs;
+        rv = rhos;
+        vv = 1e10;
+        if (!(T <= c.tsolid)) {
+            double fl = fracl[t];
+            dv = fl * diffl + (1.0 - fl) * diffs;
+            rv = fl * c.denl + (1.0 - fl) * rhos;
+            vv = c.viscos;
+        }
+    }
+    diff[t] = dv;
+    rho[t] = rv;
+    vis[t] = vv;
+}
+
+// ------------------------------------------------------------------------
+// block correction: fold column sums, solve the 1-D system along x and
+// (second kernel) add the correction, CFDSolverManager.cpp:2270-2362.
+// Also folds the energy residual, CFDSolverManager.cpp:2099.
+// ------------------------------------------------------------------------
+constexpr int BC_THREADS = 1024, BC_MAXN = 4096;
+
+__global__ __launch_bounds__(BC_THREADS) void k_block_solve(Dims d, int nrow, int nres,
+                                                            const double *__restrict__ part_blk,
+                                                            const double *__restrict__ part_res,
+                                                            double *delphi, DevScalars *sc) {
+    extern __shared__ double lds[];
+    const int n = d.nim2;   // unknowns i = 1..nim2
+    double *a = lds, *b = lds + n, *c = lds + 2 * n, *r = lds + 3 * n;
+    const int tid = threadIdx.x;
+    for (int ii = tid; ii < n; ii += blockDim.x) {
+        const int i = ii + 1;
+        double bl = 0, blp = 0, blm = 0, blc = 0;
+        for (int row = 0; row < nrow; ++row) {
+            const double *p = part_blk + (size_t)row * 4 * d.ni;
+            bl += p[i];
+            blp += p[d.ni + i];
+            blm += p[2 * d.ni + i];
+            blc += p[3 * d.ni + i];
+        }
+        // bl*dphi[i] - blm*dphi[i-1] - blp*dphi[i+1] = blc; the reference
+        // recurrence drops blm at i=1 and blp at i=nim2
+        a[ii] = (ii == 0) ? 0.0 : -blm;
+        b[ii] = bl;
+        c[ii] = (ii == n - 1) ? 0.0 : -blp;
+        r[ii] = blc;
+    }
+    __syncthreads();
+    pcr_solve<BC_MAXN / BC_THREADS>(n, a, b, c, r);
+    for (int ii = tid; ii < n; ii += blockDim.x) delphi[ii + 1] = r[ii];
+    if (tid == 0) {
+        double sd = 0.0, sh = 0.0;
+        for (int bidx = 0; bidx < nres; ++bidx) {
+            sd += part_res[2 * bidx];
+            sh += part_res[2 * bidx + 1];
+        }
+        sc->resorh = sd / (sh + AMCFD_SMALL);
+    }
+}
+
+__global__ __launch_bounds__(256) void k_add_delphi(Dims d, const double *__restrict__ delphi, double *phi) {
+    const int i = 1 + blockIdx.x * 64 + threadIdx.x;
+    const int j = 1 + blockIdx.y * 4 + threadIdx.y;
+    if (i > d.nim2 || j > d.njm2) return;
+    const double dp = delphi[i];
+    for (int k = 1 + blockIdx.z * KB; k < 1 + (blockIdx.z + 1) * KB && k <= d.nkm2; ++k)
+        phi[cix(d, i, j, k)] += dp;
+}
+
+// ------------------------------------------------------------------------
+// enthalpy -> temperature / liquid fraction on every active node
+// (CFDSolverManager.cpp:2531-2571), fused with the peak temperature and the
+// heat-balance terms of BoundCondManager.cpp:746-844.
+// ------------------------------------------------------------------------
+struct H2tConst {
+    double hlcal, hsmelt, acpa, acpb, acpl, tliquid, tsolid, hlatnt, delt;
+    int steady;
+};
+
+__global__ __launch_bounds__(TX *TY) void k_h2t(Dims d, H2tConst q, DevGeom g, const double *__restrict__ h,
+                                                 const double *__restrict__ hnot,
+                                                 const double *__restrict__ fraclnot,
+                                                 const double *__restrict__ rho,
+                                                 const double *__restrict__ diff,
+                                                 const double *__restrict__ src, double *temp, double *fracl,
+                                                 double *partial) {
+    __shared__ double scratch[10 * 4];
+    const int tx = threadIdx.x, ty = threadIdx.y;
+    const int i = blockIdx.x * TX + tx;
+    const int j = blockIdx.y * TY + ty;
+    double tmax = -INFINITY;
+    double acc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};   // accul heatvol W E B T S N
+    const double dT = q.tliquid - q.tsolid;
+    if (i < d.ni && j < d.nj) {
+        for (int kk = 0; kk < KB; ++kk) {
+            const int k = blockIdx.z * KB + kk;
+            if (k >= d.nk) break;
+            const size_t c = cix(d, i, j, k);
+            const double hv = h[c];
+            double T, fl;
+            if (hv >= q.hlcal) {
+                fl = 1.0;
+                T = (hv - q.hlcal) / q.acpl + q.tliquid;
+            } else if (hv <= q.hsmelt) {
+                fl = 0.0;
+                T = (sqrt(q.acpb * q.acpb + 2 * q.acpa * hv) - q.acpb) / q.acpa;
+            } else {
+                fl = (hv - q.hsmelt) / (q.hlcal - q.hsmelt);
+                T = dT * fl + q.tsolid;
+            }
+            temp[c] = T;
+            fracl[c] = fl;
+            tmax = dmax(tmax, T);
+            const bool in = i >= 1 && i <= d.nim2 && j >= 1 && j <= d.njm2 && k >= 1 && k <= d.nkm2;
+            if (!in) continue;
+            if (!q.steady) {
+                double vol = (g.xu[i + 1] - g.xu[i]) * (g.yv[j + 1] - g.yv[j]) * (g.zw[k + 1] - g.zw[k]);
+                double dh = hv - hnot[c] + (fl - fraclnot[c]) * q.hlatnt;
+                acc[0] += vol * rho[c] * dh / q.delt;
+                acc[1] += src[c] * vol;
+            }
+            if (i == 1) acc[2] += g.areajk[k * d.nj + j] * (diff[c] * (h[c - 1] - hv) * g.dxpwinv[1]);
+            if (i == d.nim2) acc[3] += g.areajk[k * d.nj + j] * (diff[c] * (h[c + 1] - hv) * g.dxpwinv[d.nim1]);
+            if (k == 1) acc[4] += g.areaij[j * d.ni + i] * (diff[c] * (h[c - d.sk] - hv) * g.dzpbinv[1]);
+            if (k == d.nkm2) acc[5] += g.areaij[j * d.ni + i] * (diff[c] * (h[c + d.sk] - hv) * g.dzpbinv[d.nkm1]);
+            if (j == 1) acc[6] += g.areaik[k * d.ni + i] * (diff[c] * (h[c - d.sj] - hv) * g.dypsinv[1]);
+            if (j == d.njm2) acc[7] += g.areaik[k * d.ni + i] * (diff[c] * (h[c + d.sj] - hv) * g.dypsinv[d.njm1]);
+        }
+    }
+    // block reduction: max of T, sums of the rest
+    const int lane = tx + ty * TX, nw = (TX * TY) / 64;
+    tmax = wave_max(tmax);
+#pragma unroll
+    for (int qq = 0; qq < 8; ++qq) acc[qq] = wave_sum(acc[qq]);
+    if ((lane & 63) == 0) {
+        scratch[lane >> 6] = tmax;
+#pragma unroll
+        for (int qq = 0; qq < 8; ++qq) scratch[(qq + 1) * nw + (lane >> 6)] = acc[qq];
+    }
+    __syncthreads();
+    if (lane == 0) {
+        const int b = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
+        double m = scratch[0];
+        for (int w = 1; w < nw; ++w) m = dmax(m, scratch[w]);
+        partial[(size_t)b * 10] = m;
+        for (int qq = 0; qq < 8; ++qq) {
+            double s = 0.0;
+            for (int w = 0; w < nw; ++w) s += scratch[(qq + 1) * nw + w];
+            partial[(size_t)b * 10 + 1 + qq] = s;
+        }
+    }
+}
+
+// ------------------------------------------------------------------------
+// peak temperature, pool length / depth / width and the momentum box,
+// BoundCondManager.cpp:280-447 (one workgroup; every scan of the reference
+// is a first-crossing or extremum search, done here as parallel min/max).
+// ------------------------------------------------------------------------
+__device__ int block_min_int(int v, int *s) {
+    __syncthreads();
+    if (threadIdx.x == 0) s[0] = 0x7fffffff;
+    __syncthreads();
+    atomicMin(&s[0], v);
+    __syncthreads();
+    return s[0];
+}
+__device__ int block_max_int(int v, int *s) {
+    __syncthreads();
+    if (threadIdx.x == 0) s[0] = -0x7fffffff;
+    __syncthreads();
+    atomicMax(&s[0], v);
+    __syncthreads();
+    return s[0];
+}
+
+__global__ __launch_bounds__(1024) void k_melt_pool(Dims d, int nblk, const double *__restrict__ partial,
+                                                    const double *__restrict__ temp,
+                                                    const double *__restrict__ x, const double *__restrict__ y,
+                                                    const double *__restrict__ z, double ts, int istart,
+                                                    int jstart, DevScalars *sc) {
+    __shared__ int si[1];
+    __shared__ double sd[32];
+    __shared__ double tp_s;
+    const int tid = threadIdx.x, nt = blockDim.x;
+    // peak temperature (first-argument-wins max over block partials)
+    double m = -INFINITY;
+    for (int b = tid; b < nblk; b += nt) m = dmax(m, partial[(size_t)b * 10]);
+    m = wave_max(m);
+    if ((tid & 63) == 0) sd[tid >> 6] = m;
+    __syncthreads();
+    if (tid == 0) {
+        double mm = sd[0];
+        for (int w = 1; w < nt / 64; ++w) mm = dmax(mm, sd[w]);
+        tp_s = mm;
+        sc->tpeak = mm;
+    }
+    __syncthreads();
+    const double tpeak = tp_s;
+    if (!(tpeak > ts)) {
+        if (tid == 0) { sc->alen = 0.0; sc->depth = 0.0; sc->width = 0.0; }
+        return;
+    }
+    // The width scan can step one row past the last interior row (as the
+    // reference does); such reads see the neighbouring memory of the field
+    // when it exists and 0 otherwise, never outside the allocation.
+    const long long total = (long long)d.nkf * (long long)d.sk;
+    auto T = [&](int i, int j, int k) {
+        long long idx = (long long)k * (long long)d.sk + (long long)j * (long long)d.sj + i;
+        return (idx >= 0 && idx < total) ? temp[idx] : 0.0;
+    };
+    auto Y = [&](int j) { return (j >= 0 && j < d.nj) ? y[j] : 0.0; };
+    const int js = jstart, top = d.nkm1;
+    // length
+    int cand = d.nim2;
+    for (int i = istart + tid; i <= d.nim2; i += nt)
+        if (T(i, js, top) <= ts) cand = min(cand, i);
+    const int imax = block_min_int(cand, si);
+    cand = 1;
+    for (int i = 1 + tid; i <= istart; i += nt)
+        if (T(i, js, top) < ts) cand = max(cand, i);
+    const int imin = block_max_int(cand, si);
+    // depth: lowest k of the run that starts at nkm2 and stays >= ts
+    cand = d.nkm2;
+    for (int i = 1 + tid; i <= d.nim2; i += nt) {
+        for (int k = d.nkm2; k > 0; --k) {
+            if (T(i, js, k) < ts) break;
+            cand = min(cand, k);
+        }
+    }
+    const int kmin = block_min_int(cand, si) - 1;
+    // width: highest / lowest j of the runs along the top plane
+    cand = js;
+    for (int i = 1 + tid; i <= d.nim2; i += nt)
+        for (int j = js; j < d.njm1; ++j) {
+            if (T(i, j, top) < ts) break;
+            cand = max(cand, j);
+        }
+    const int jmax = block_max_int(cand, si) + 1;
+    cand = js;
+    for (int i = 1 + tid; i <= d.nim2; i += nt)
+        for (int j = js; j > 0; --j) {
+            if (T(i, j, top) < ts) break;
+            cand = min(cand, j);
+        }
+    const int jmin = block_min_int(cand, si) - 1;
+    // interpolated extents (max / min over i of per-row values)
+    double dep = -INFINITY, ymx = -INFINITY, ymn = INFINITY;
+    if (kmin != 0) {
+        for (int i = 1 + tid; i <= d.nim2; i += nt) {
+            if (T(i, js, kmin + 1) < ts) continue;
+            double dtdz = (z[kmin] - z[kmin - 1]) / (T(i, js, kmin) - T(i, js, kmin - 1));
+            dep = dmax(z[top] - z[kmin] + (T(i, js, kmin) - ts) * dtdz, dep);
+        }
+    }
+    if (jmax != js)
+        for (int i = 1 + tid; i <= d.nim2; i += nt) {
+            if (T(i, jmax - 1, top) < ts) continue;
+            double dtdy = (Y(jmax) - Y(jmax + 1)) / (T(i, jmax, top) - T(i, jmax + 1, top));
+            ymx = dmax(Y(jmax) + (ts - T(i, jmax, top)) * dtdy, ymx);
+        }
+    if (jmin != js)
+        for (int i = 1 + tid; i <= d.nim2; i += nt) {
+            if (T(i, jmin + 1, top) < ts) continue;
+            double dtdy = (Y(jmin) - Y(jmin - 1)) / (T(i, jmin, top) - T(i, jmin - 1, top));
+            ymn = dmin(Y(jmin) + (ts - T(i, jmin, top)) * dtdy, ymn);
+        }
+    dep = wave_max(dep);
+    ymx = wave_max(ymx);
+    ymn = -wave_max(-ymn);
+    __syncthreads();
+    if ((tid & 63) == 0) {
+        sd[tid >> 6] = dep;
+        sd[16 + (tid >> 6)] = ymx;
+    }
+    __syncthreads();
+    double depth_all = -INFINITY, ymax_all = -INFINITY;
+    if (tid == 0) {
+        for (int w = 0; w < nt / 64; ++w) {
+            depth_all = dmax(sd[w], depth_all);
+            ymax_all = dmax(sd[16 + w], ymax_all);
+        }
+    }
+    __syncthreads();
+    if ((tid & 63) == 0) sd[tid >> 6] = ymn;
+    __syncthreads();
+    if (tid == 0) {
+        double ymin_all = INFINITY;
+        for (int w = 0; w < nt / 64; ++w) ymin_all = dmin(sd[w], ymin_all);
+        // length from the interpolated crossing on each side
+        double dx = (x[imax] - x[imax + 1]) / (T(imax, js, top) - T(imax + 1, js, top));
+        double xxmax = x[imax] + (ts - T(imax, js, top)) * dx;
+        dx = (x[imin] - x[imin - 1]) / (T(imin, js, top) - T(imin - 1, js, top));
+        double xxmin = x[imin] + (ts - T(imin, js, top)) * dx;
+        sc->alen = xxmax - xxmin;
+        double depth = 0.0;
+        if (kmin == 0) depth = z[top] - z[0];
+        else depth = dmax(depth_all, 0.0);
+        sc->depth = depth;
+        double yymax = y[0];
+        if (jmax != js) yymax = dmax(ymax_all, yymax);
+        double yymin = y[js];
+        if (jmin != js) yymin = dmin(ymin_all, yymin);
+        sc->width = yymax - yymin;
+        sc->imin = imin;
+        sc->imax = imax;
+        sc->jmin = jmin;
+        sc->jmax = jmax;
+        sc->kmin = kmin;
+        sc->istat = max(imin - 3, 1);
+        sc->iend = min(imax + 3, d.nim2);
+        sc->jstat = max(jmin - 3, 1);
+        sc->jend = min(jmax + 2, d.njm2);
+        sc->kstat = max(kmin - 2, 2);
+    }
+}
+
+// ------------------------------------------------------------------------
+// end-of-iteration scalars: residual folds, heat balance, energy ratio
+// (CFDSolverManager.cpp:1989-2071, BoundCondManager.cpp:846-857)
+// ------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_finalize(int nblk_t, int nblk_b, int nblk_f, const double *__restrict__ part_t,
+                                                  const double *__restrict__ part_b,
+                                                  const double *__restrict__ part_f, int flow_done, int steady,
+                                                  double dens, double width, DevScalars *sc) {
+    __shared__ double s_acc[9];
+    const int tid = threadIdx.x;
+    if (tid < 8) {
+        double s = 0.0;
+        for (int b = 0; b < nblk_t; ++b) s += part_t[(size_t)b * 10 + 1 + tid];
+        s_acc[tid] = s;
+    } else if (tid == 8) {
+        double s = 0.0;
+        for (int b = 0; b < nblk_b; ++b) s += part_b[b];
+        s_acc[8] = s;
+    }
+    __syncthreads();
+    if (tid != 0) return;
+    sc->accul = steady ? 0.0 : s_acc[0];
+    sc->heatvol = steady ? 0.0 : s_acc[1];
+    sc->fluxwest = s_acc[2];
+    sc->fluxeast = s_acc[3];
+    sc->fluxbottom = s_acc[4];
+    sc->fluxtop = s_acc[5];
+    sc->fluxsouth = s_acc[6];
+    sc->fluxnorth = s_acc[7];
+    sc->ahtoploss = s_acc[8];
+    double heatout = (sc->fluxnorth + sc->fluxbottom + sc->fluxwest + sc->fluxeast + sc->fluxsouth) - sc->ahtoploss;
+    sc->heatout = heatout;
+    if (steady) sc->ratio = -heatout / sc->heatinlaser;
+    else sc->ratio = (sc->heatvol + sc->heatinlaser) / (sc->accul - heatout);
+    if (flow_done) {
+        // per variable: [sumd, umax (u only) / denom (p only), resorm sum (p only)]
+        double sum[4] = {0, 0, 0, 0}, aux[4] = {0, 0, 0, 0}, aux2 = 0.0;
+        for (int v = 0; v < 4; ++v)
+            for (int b = 0; b < nblk_f; ++b) {
+                const double *p = part_f + ((size_t)v * nblk_f + b) * 3;
+                sum[v] += p[0];
+                if (v == 0) aux[0] = dmax(aux[0], p[1]);
+                if (v == 3) { aux[3] += p[1]; aux2 += p[2]; }
+            }
+        double refmom = 0.25 * M_PI * (width * width) * dens * (aux[0] * aux[0]);
+        sc->refmom = refmom;
+        sc->resoru = sum[0] / refmom;
+        sc->resorv = sum[1] / refmom;
+        sc->resorw = sum[2] / refmom;
+        double denom = aux[3] * dens;
+        sc->resorm = aux2 / (denom + AMCFD_SMALL);
+    }
+}
+
+// ------------------------------------------------------------------------
+// end-of-step push-back, CFDSolverManager.cpp:2625-2687
+// ------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_end_step(size_t n, double tsolid, double tliquid, int powder, double track,
+                                                  double trackid, const double *__restrict__ temp,
+                                                  const double *__restrict__ h, const double *__restrict__ fracl,
+                                                  double *u, double *v, double *w, double *unot, double *vnot,
+                                                  double *wnot, double *hnot, double *fraclnot, double *csfrac,
+                                                  double *solfrac, double *mpphase, double *mpliq) {
+    size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    double T = temp[t];
+    double uu = u[t], vv = v[t], ww = w[t];
+    if (T <= tsolid) {
+        uu = 0.0; vv = 0.0; ww = 0.0;
+        u[t] = 0.0; v[t] = 0.0; w[t] = 0.0;
+    }
+    unot[t] = uu;
+    vnot[t] = vv;
+    wnot[t] = ww;
+    hnot[t] = h[t];
+    double fl = fracl[t];
+    fraclnot[t] = fl;
+    if (powder) {
+        csfrac[t] = dmax(csfrac[t], fl);
+        solfrac[t] = dmax(solfrac[t], fl * track);
+        if (T > tsolid) mpphase[t] = trackid;
+        if (T > tliquid) mpliq[t] = trackid;
+    }
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------------
+void launch_laser(hipStream_t st, const Dims &d, const DevGeom &g, DevFields f, Partials pr, double beamx,
+                  double beamy, double peakhin, double alasfact, double rb2, double varlas, int powerindicator,
+                  DevScalars *sc) {
+    (void)pr;
+    hipLaunchKernelGGL(k_laser, dim3(1), dim3(1024), 0, st, d, g.x, g.y, g.areaij, f.heatin, beamx, beamy,
+                       peakhin, alasfact, rb2, varlas, powerindicator, sc);
+}
+
+void launch_bc_enthalpy(hipStream_t st, const Dims &d, const DevGeom &g, DevFields f, Partials pr,
+                        const amcfd_params &p) {
+    BcConst k{p.sigm, p.emiss, p.htci, p.htcj, p.tempamb, p.pressamb, p.gasconst, p.evaplatent,
+              p.evaptemp, p.evapfact, p.molarmass, p.isevap};
+    int n = d.njm2 * d.nim2 + d.nkm2 * d.njm2 + d.nkm2 * d.nim2;
+    int nb = (n + 255) / 256;
+    hipLaunchKernelGGL(k_bc_enthalpy, dim3(nb), dim3(256), 0, st, d, k, g.dxpwinv, g.dypsinv, g.dzpbinv,
+                       g.areaij, f.heatin, f.temp, f.diff, f.enthalpy, pr.bc);
+}
+
+static dim3 energy_grid(const Dims &d) {
+    return dim3((d.nim2 + TX - 1) / TX, (d.njm2 + TY - 1) / TY, (d.nkm2 + KB - 1) / KB);
+}
+
+void launch_energy_assemble(hipStream_t st, const Dims &d, const DevGeom &g, DevFields f, Partials pr,
+                            const amcfd_params &p, const SourceParams &s) {
+    AsmConst q{p.delt, p.hlatnt, p.urfh, s.avolpow, s.apowseta, s.heatthick, s.heatrb, s.avolfact,
+               s.beamposx, s.beamposy, s.powerindicator, p.discretization};
+    hipLaunchKernelGGL(k_energy_assemble, energy_grid(d), dim3(TX, TY), 0, st, d, q, g, f.uvel, f.vvel, f.wvel,
+                       f.rho, f.diff, f.hnot, f.fracl, f.fraclnot, f.enthalpy, f.an, f.as, f.ae, f.aw, f.at,
+                       f.ab, f.ap, f.su, f.sourceinput, pr.energy, pr.blk);
+}
+
+void launch_props(hipStream_t st, const Dims &d, DevFields f, const amcfd_params &p) {
+    PropConst c{p.tliquid, p.tsolid, p.viscos, p.dens, p.denl, p.acpa, p.acpb, p.acpl, p.thconsa,
+                p.thconsb, p.thconsc, p.thconla, p.thconlb, p.powderden, p.powdercpa, p.powdercpb,
+                p.powderthcona, p.powderthconb, p.powderbed};
+    size_t n = (size_t)d.ni * d.nj * d.nk;
+    hipLaunchKernelGGL(k_props, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, n, c, f.temp, f.fracl,
+                       f.csfrac, f.diff, f.rho, f.vis);
+}
+
+void launch_block_correction(hipStream_t st, const Dims &d, DevFields f, Partials pr, DevScalars *sc) {
+    size_t lds = sizeof(double) * 4 * (size_t)d.nim2;
+    const dim3 e = energy_grid(d);   // partial counts of the active grid
+    hipLaunchKernelGGL(k_block_solve, dim3(1), dim3(BC_THREADS), lds, st, d, (int)(e.y * e.z),
+                       (int)(e.x * e.y * e.z), pr.blk, pr.energy, pr.delphi, sc);
+    hipLaunchKernelGGL(k_add_delphi, energy_grid(d), dim3(64, 4), 0, st, d, pr.delphi, f.enthalpy);
+}
+
+static dim3 node_grid(const Dims &d) {
+    return dim3((d.ni + TX - 1) / TX, (d.nj + TY - 1) / TY, (d.nk + KB - 1) / KB);
+}
+
+void launch_h2t(hipStream_t st, const Dims &d, const DevGeom &g, DevFields f, Partials pr, const amcfd_params &p) {
+    H2tConst q{p.hlcal, p.hsmelt, p.acpa, p.acpb, p.acpl, p.tliquid, p.tsolid, p.hlatnt, p.delt, p.steady};
+    hipLaunchKernelGGL(k_h2t, node_grid(d), dim3(TX, TY), 0, st, d, q, g, f.enthalpy, f.hnot, f.fraclnot, f.rho,
+                       f.diff, f.sourceinput, f.temp, f.fracl, pr.h2t);
+}
+
+void launch_melt_pool(hipStream_t st, const Dims &d, const DevGeom &g, DevFields f, Partials pr,
+                      const amcfd_params &p, int istart, int jstart, DevScalars *sc) {
+    dim3 gr = node_grid(d);
+    int nblk = gr.x * gr.y * gr.z;
+    hipLaunchKernelGGL(k_melt_pool, dim3(1), dim3(1024), 0, st, d, nblk, pr.h2t, f.temp, g.x, g.y, g.z, p.tsolid,
+                       istart, jstart, sc);
+}
+
+void launch_finalize(hipStream_t st, const Dims &d, Partials pr, DevScalars *sc, const amcfd_params &p,
+                     int flow_done, int steady, double width) {
+    dim3 gr = node_grid(d);
+    int nblk_t = gr.x * gr.y * gr.z;
+    const int nblk_b = (d.njm2 * d.nim2 + d.nkm2 * d.njm2 + d.nkm2 * d.nim2 + 255) / 256;
+    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, st, nblk_t, nblk_b, pr.nblk_f, pr.h2t, pr.bc, pr.flow,
+                       flow_done, steady, p.dens, width, sc);
+}
+
+void launch_end_step(hipStream_t st, const Dims &d, DevFields f, const amcfd_params &p, double track,
+                     double trackid) {
+    size_t n = (size_t)d.ni * d.nj * d.nk;
+    hipLaunchKernelGGL(k_end_step, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, n, p.tsolid, p.tliquid,
+                       p.powderbed, track, trackid, f.temp, f.enthalpy, f.fracl, f.uvel, f.vvel, f.wvel, f.unot,
+                       f.vnot, f.wnot, f.hnot, f.fraclnot, f.csfrac, f.solfrac, f.mpphase, f.mpliqphase);
+}
+
+int energy_partial_sizes(const Dims &d, int *nblk_e, int *nrow_e, int *nblk_t) {
+    dim3 e = energy_grid(d), t = node_grid(d);
+    *nblk_e = e.x * e.y * e.z;
+    *nrow_e = e.y * e.z;
+    *nblk_t = t.x * t.y * t.z;
+    return 0;
+}
+
+}  // namespace amcfd

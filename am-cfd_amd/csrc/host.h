@@ -1,0 +1,100 @@
+// Host-side solver object: owns the device state and drives one time step
+// (laser update, SIMPLE iterations, push-back) through the HIP kernels.
+#pragma once
+
+#include "internal.h"
+
+#include <map>
+#include <memory>
+
+namespace amcfd {
+
+struct HostGrid {
+    int ni = 0, nj = 0, nk = 0;
+    std::vector<double> x, y, z, xu, yv, zw, dxpwinv, dypsinv, dzpbinv, fracx, fracy, fracz;
+    std::vector<double> areaij, areauij, areavij, areaik, areawik, areauik, areajk, areavjk, areawjk;
+    void build(const amcfd_params &p);
+    double volume(int i, int j, int k) const;
+};
+
+struct InitialState {
+    std::vector<double> enthalpy, hnot, temp, rho, vis, diff, csfrac;
+};
+
+void initial_fields(const amcfd_params &p, const HostGrid &g, InitialState &s);
+
+int energy_partial_sizes(const Dims &d, int *nblk_e, int *nrow_e, int *nblk_t);
+int flow_partial_blocks(const Box &b, const Dims &d);
+int sweep_max_line();
+
+// accumulated device time of one kernel family (HIP events on the solver stream)
+struct KernelTimer {
+    long launches = 0;
+    double total_ms = 0.0;
+};
+
+class Solver {
+  public:
+    ~Solver();
+    int create(const amcfd_params &p, int mode);
+    int begin_step();
+    int solve_step();
+    int end_step();
+    int finished() const { return !(timet_ < p_.finaltime); }
+    int get_field(const char *name, double *host, size_t n);
+    int set_field(const char *name, const double *host, size_t n);
+    int get_geometry(const char *name, double *host, size_t n);
+    int get_scalars(amcfd_scalars *out);
+    void dims(int *ni, int *nj, int *nk) const { *ni = d_.ni; *nj = d_.nj; *nk = d_.nkf; }
+    double last_total_ms = -1.0, last_tdma_ms = -1.0;
+    void set_profiling(int on) { profile_ = on; }
+    std::map<std::string, KernelTimer> timers;
+
+  private:
+    amcfd_params p_{};
+    std::vector<double> tool_, dtt_, heatt_;
+    HostGrid hg_;
+    Dims d_{};
+    DevGeom g_{};
+    DevFields f_{};
+    Partials pr_{};
+    DevScalars *sc_ = nullptr;       // device
+    DevScalars *sc_host_ = nullptr;  // pinned mirror
+    hipStream_t st_ = nullptr;
+    std::vector<void *> allocs_;
+    SweepPlan plan_dom_, plan_box_;
+    unsigned long long *counter_ = nullptr;
+    unsigned long long counter_base_ = 0;
+    int *versions_ = nullptr;
+    int epoch_ = 4;
+    int profile_ = 0;
+    std::vector<hipEvent_t> ev_pool_;
+    std::vector<std::pair<std::string, std::pair<int, int>>> ev_pending_;
+    size_t ev_next_ = 0;
+    // host mirror of the reference's laser / driver state
+    double timet_ = 0.0, scanvel_ = 0.0;
+    double beamx_ = 0.0, beamy_ = 0.0, beamz_ = 0.0, oldx_ = 0.0, oldy_ = 0.0, oldz_ = 0.0;
+    int powerindicator_ = 1, laser_init_ = 1, trackindx_ = 1, istart_ = 0, jstart_ = 0;
+    int heat_it_ = 0, dt_state_ = 0, step_ = 0, niter_ = 0, itertot_ = 0;
+    double alen_ = 0.0;
+    bool steadylength_ = false;
+    Box box_{};
+
+    void set_active_nk(int nk);
+    int alloc(double **ptr, size_t n);
+    int upload(double *dst, const std::vector<double> &src);
+    void laser_position();
+    int launch_laser_map();
+    SourceParams source_params();
+    int sweep(double *phi, SweepPlan &plan, int ibeg, int iend, int jbeg, int jend, int kbeg, int kend, int guard,
+              const char *timer);
+    int ensure_plan(SweepPlan &plan, int kbeg, int kend, int jbeg, int jend);
+    int read_scalars();
+    int flow_iteration();
+    void tic(const char *name, int *slot);
+    void toc(const char *name, int slot);
+    void collect_timers();
+    double *field_ptr(const char *name);
+};
+
+}  // namespace amcfd

@@ -1,0 +1,140 @@
+// Internal declarations of the MI355X SIMPLE-solve library.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/amcfd.h"
+
+namespace amcfd {
+
+int set_error(int code, const std::string &msg);
+
+// ---------------------------------------------------------------------------
+// parsed input deck (Mesh + DomainManager inputs)
+// ---------------------------------------------------------------------------
+struct Flags {
+    int navierstokes = 1, steady = 0, isevap = 0, powderbed = 0, udtoolpath = 0;
+    int gaussheatsource = 0, volheatsource = 0, volheatsourcetable = 0;
+};
+
+struct Deck {
+    amcfd_params p{};
+    Flags flags;
+    std::string keyword_id, mesh_file, tool_file, heat_file;
+    std::vector<std::vector<double>> tool_rows, dt_rows, heat_rows;
+    std::vector<double> flat_tool, flat_dt, flat_heat;
+    int load(const std::string &path);
+    int read_mesh();
+    void finalize_tables();
+};
+
+// ---------------------------------------------------------------------------
+// device-side scalar block: everything the host reads once per iteration
+// ---------------------------------------------------------------------------
+struct DevScalars {
+    double resorh, resorm, resoru, resorv, resorw, refmom;
+    double tpeak, alen, depth, width;
+    double ratio, heatout, accul, heatvol, ahtoploss, heatinlaser;
+    double fluxwest, fluxeast, fluxbottom, fluxtop, fluxsouth, fluxnorth;
+    int istat, iend, jstat, jend, kstat, imin, imax, jmin, jmax, kmin;
+    int error;      // nonzero: a bounded wait in the line-sweep kernel expired
+    int pad;
+};
+
+// geometry on the device (1-D and 2-D arrays of the reference grid)
+struct DevGeom {
+    double *x, *y, *z, *xu, *yv, *zw, *dxpwinv, *dypsinv, *dzpbinv, *fracx, *fracy, *fracz;
+    double *areaij, *areauij, *areavij;   // [nj*ni]
+    double *areaik, *areawik, *areauik;   // [nk*ni]
+    double *areajk, *areavjk, *areawjk;   // [nk*nj]
+};
+
+// solution + coefficient fields (ni*nj*nkf each)
+struct DevFields {
+    double *enthalpy, *temp, *fracl, *uvel, *vvel, *wvel, *pressure, *pp, *rho, *diff, *vis;
+    double *hnot, *fraclnot, *unot, *vnot, *wnot, *sourceinput, *dux, *dvy, *dwz;
+    double *csfrac, *solfrac, *mpphase, *mpliqphase;
+    double *ap, *an, *as, *ae, *aw, *at, *ab, *su, *sp, *apnot;
+    double *heatin;   // [nj*ni]
+};
+
+// dimensions and constants every kernel needs (passed by value)
+struct Dims {
+    int ni, nj, nkf;              // allocation
+    int nk, nim1, njm1, nkm1, nim2, njm2, nkm2;
+    size_t sj, sk;                // strides in j and k
+};
+
+struct Box {
+    int istat, iend, jstat, jend, kstat;
+    int istatp1() const { return istat + 1; }
+    int iendm1() const { return iend - 1; }
+};
+
+// per-iteration energy-source inputs (heat-source table entry, laser)
+struct SourceParams {
+    double avolpow, apowseta, heatthick, heatrb, avolfact;
+    double beamposx, beamposy;
+    int powerindicator;
+};
+
+// partial-sum buffers (deterministic two-pass reductions)
+struct Partials {
+    double *energy;     // [nblk_e][2]       resid sumd, sumh
+    double *blk;        // [nrow_e][4][ni]   block-correction column sums
+    double *h2t;        // [nblk_t][10]      tpeak + fluxes + accul/heatvol
+    double *flow;       // [4][nblk_f][3]    per variable: sumd / umax / denom
+    double *bc;         // [nblk_b]          ahtoploss
+    double *laser;      // [nblk_l]          heatinlaser
+    double *delphi;     // [ni]
+    int nblk_e, nrow_e, nblk_t, nblk_f, nblk_b, nblk_l;
+};
+
+// line-sweep schedule (x-lines in Gauss-Seidel program order)
+struct SweepPlan {
+    int *tasks = nullptr;          // device: packed (pass, k, j) in wavefront order
+    int ntasks = 0, width = 0, cap = 0;
+    int kbeg = 0, kend = -1, jbeg = 0, jend = -1;   // cached shape
+};
+
+// ---------------------------------------------------------------------------
+// kernel launchers (defined in the .hip files)
+// ---------------------------------------------------------------------------
+void launch_laser(hipStream_t st, const Dims &d, const DevGeom &g, DevFields f, Partials pr,
+                  double beamx, double beamy, double peakhin, double alasfact, double rb2, double varlas,
+                  int powerindicator, DevScalars *sc);
+void launch_bc_enthalpy(hipStream_t st, const Dims &d, const DevGeom &g, DevFields f, Partials pr,
+                        const amcfd_params &p);
+void launch_energy_assemble(hipStream_t st, const Dims &d, const DevGeom &g, DevFields f, Partials pr,
+                            const amcfd_params &p, const SourceParams &sp);
+void launch_props(hipStream_t st, const Dims &d, DevFields f, const amcfd_params &p);
+void launch_block_correction(hipStream_t st, const Dims &d, DevFields f, Partials pr, DevScalars *sc);
+void launch_h2t(hipStream_t st, const Dims &d, const DevGeom &g, DevFields f, Partials pr,
+                const amcfd_params &p);
+void launch_melt_pool(hipStream_t st, const Dims &d, const DevGeom &g, DevFields f, Partials pr,
+                      const amcfd_params &p, int istart, int jstart, DevScalars *sc);
+void launch_clean_velocities(hipStream_t st, const Dims &d, DevFields f, const Box &b, double tsolid);
+void launch_bc_flow(hipStream_t st, const Dims &d, const DevGeom &g, DevFields f, const Box &b,
+                    const amcfd_params &p, int ivar);
+void launch_momentum(hipStream_t st, const Dims &d, const DevGeom &g, DevFields f, Partials pr,
+                     const Box &b, const amcfd_params &p, double scanvel, int ivar);
+void launch_pressure(hipStream_t st, const Dims &d, const DevGeom &g, DevFields f, Partials pr,
+                     const Box &b, const amcfd_params &p);
+void launch_correct(hipStream_t st, const Dims &d, DevFields f, const Box &b, const amcfd_params &p);
+void launch_finalize(hipStream_t st, const Dims &d, Partials pr, DevScalars *sc, const amcfd_params &p,
+                     int flow_done, int steady, double width);
+void launch_end_step(hipStream_t st, const Dims &d, DevFields f, const amcfd_params &p, double track,
+                     double trackid);
+void launch_line_sweeps(hipStream_t st, const Dims &d, DevFields f, double *phi, const SweepPlan &plan,
+                        int ibeg, int iend, int guard, unsigned long long *counter,
+                        unsigned long long counter_base, int *versions, int epoch, DevScalars *sc,
+                        int nblocks);
+
+// wavefront task list for the Gauss-Seidel line sweeps (host side)
+void build_sweep_tasks(int kbeg, int kend, int jbeg, int jend, std::vector<int> &tasks, int &width);
+
+}  // namespace amcfd

@@ -1,0 +1,575 @@
+// Host driver of the device-resident SIMPLE solve.
+//
+// The time-step and iteration structure is the reference's:
+//   step:      dt table, timet += dt, updateLaser       src/main.cpp:125-141
+//   solve:     CFDSolverManager::iterativeLoopSolve     CFDSolverManager.cpp:595-722
+//   push-back: CFDSolverManager::updateSolutions        CFDSolverManager.cpp:2625-2687
+// All field arithmetic runs in HIP kernels on one stream; the host keeps the
+// scalar control state (laser position, heat-source table row, convergence
+// test) and reads back one small scalar block per SIMPLE iteration.
+
+#include "host.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+namespace amcfd {
+
+#define HIPCALL(x)                                                                     \
+    do {                                                                               \
+        hipError_t e_ = (x);                                                           \
+        if (e_ != hipSuccess) return set_error(AMCFD_E_HIP, std::string(#x) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+Solver::~Solver() {
+    if (st_) hipStreamSynchronize(st_);
+    for (void *p : allocs_) hipFree(p);
+    if (sc_host_) hipHostFree(sc_host_);
+    for (auto &e : ev_pool_) hipEventDestroy(e);
+    if (st_) hipStreamDestroy(st_);
+}
+
+int Solver::alloc(double **ptr, size_t n) {
+    void *p = nullptr;
+    HIPCALL(hipMalloc(&p, std::max<size_t>(n, 1) * sizeof(double)));
+    HIPCALL(hipMemsetAsync(p, 0, std::max<size_t>(n, 1) * sizeof(double), st_));
+    allocs_.push_back(p);
+    *ptr = (double *)p;
+    return AMCFD_OK;
+}
+
+int Solver::upload(double *dst, const std::vector<double> &src) {
+    HIPCALL(hipMemcpyAsync(dst, src.data(), src.size() * sizeof(double), hipMemcpyHostToDevice, st_));
+    return AMCFD_OK;
+}
+
+void Solver::set_active_nk(int nk) {
+    d_.nk = nk;
+    d_.nkm1 = nk - 1;
+    d_.nkm2 = nk - 2;
+}
+
+int Solver::create(const amcfd_params &p, int mode) {
+    if (mode != AMCFD_MODE_FP64) return set_error(AMCFD_E_ARG, "unknown solver mode");
+    p_ = p;
+    if (p.n_toolpath > 0) tool_.assign(p.toolpath, p.toolpath + 6 * p.n_toolpath);
+    if (p.n_dt_table > 0) dtt_.assign(p.dt_table, p.dt_table + 3 * p.n_dt_table);
+    if (p.n_heat_table > 0) heatt_.assign(p.heat_table, p.heat_table + 6 * p.n_heat_table);
+    p_.toolpath = tool_.empty() ? nullptr : tool_.data();
+    p_.dt_table = dtt_.empty() ? nullptr : dtt_.data();
+    p_.heat_table = heatt_.empty() ? nullptr : heatt_.data();
+    if (p.ni < 4 || p.nj < 4 || p.nk < 4) return set_error(AMCFD_E_ARG, "grid too small");
+    if (p.ni - 2 > sweep_max_line())
+        return set_error(AMCFD_E_CAPACITY, "x-lines longer than " + std::to_string(sweep_max_line()) + " cells");
+    if (p.nj >= 16384 || p.nk >= 16384) return set_error(AMCFD_E_CAPACITY, "more than 16383 nodes in y or z");
+    if (p.maxit < 1) return set_error(AMCFD_E_ARG, "maxit must be >= 1");
+    if (p.acpa == 0.0) return set_error(AMCFD_E_ARG, "acpa must be nonzero (solid H->T divides by it)");
+
+    int dev = 0;
+    HIPCALL(hipGetDevice(&dev));
+    HIPCALL(hipStreamCreateWithFlags(&st_, hipStreamNonBlocking));
+
+    d_.ni = p.ni;
+    d_.nj = p.nj;
+    d_.nkf = p.nk;
+    d_.nim1 = p.ni - 1;
+    d_.nim2 = p.ni - 2;
+    d_.njm1 = p.nj - 1;
+    d_.njm2 = p.nj - 2;
+    d_.sj = (size_t)p.ni;
+    d_.sk = (size_t)p.ni * p.nj;
+    set_active_nk(p.nk);
+
+    hg_.build(p_);
+    const size_t nn = (size_t)d_.ni * d_.nj * d_.nkf;
+    const int n1 = d_.ni + d_.nj + d_.nkf + 8;
+    int st;
+    // geometry
+    double **g1[] = {&g_.x, &g_.y, &g_.z, &g_.xu, &g_.yv, &g_.zw, &g_.dxpwinv, &g_.dypsinv, &g_.dzpbinv,
+                     &g_.fracx, &g_.fracy, &g_.fracz};
+    const std::vector<double> *h1[] = {&hg_.x, &hg_.y, &hg_.z, &hg_.xu, &hg_.yv, &hg_.zw, &hg_.dxpwinv,
+                                       &hg_.dypsinv, &hg_.dzpbinv, &hg_.fracx, &hg_.fracy, &hg_.fracz};
+    for (int a = 0; a < 12; ++a) {
+        if ((st = alloc(g1[a], n1))) return st;
+        if ((st = upload(*g1[a], *h1[a]))) return st;
+    }
+    double **g2[] = {&g_.areaij, &g_.areauij, &g_.areavij, &g_.areaik, &g_.areawik, &g_.areauik,
+                     &g_.areajk, &g_.areavjk, &g_.areawjk};
+    const std::vector<double> *h2[] = {&hg_.areaij, &hg_.areauij, &hg_.areavij, &hg_.areaik, &hg_.areawik,
+                                       &hg_.areauik, &hg_.areajk, &hg_.areavjk, &hg_.areawjk};
+    for (int a = 0; a < 9; ++a) {
+        if ((st = alloc(g2[a], h2[a]->size()))) return st;
+        if ((st = upload(*g2[a], *h2[a]))) return st;
+    }
+    // fields
+    double **fl[] = {&f_.enthalpy, &f_.temp, &f_.fracl, &f_.uvel, &f_.vvel, &f_.wvel, &f_.pressure, &f_.pp,
+                     &f_.rho, &f_.diff, &f_.vis, &f_.hnot, &f_.fraclnot, &f_.unot, &f_.vnot, &f_.wnot,
+                     &f_.sourceinput, &f_.dux, &f_.dvy, &f_.dwz, &f_.csfrac, &f_.solfrac, &f_.mpphase,
+                     &f_.mpliqphase, &f_.ap, &f_.an, &f_.as, &f_.ae, &f_.aw, &f_.at, &f_.ab, &f_.su, &f_.sp,
+                     &f_.apnot};
+    for (auto *q : fl)
+        if ((st = alloc(q, nn))) return st;
+    if ((st = alloc(&f_.heatin, (size_t)d_.ni * d_.nj))) return st;
+    InitialState is;
+    initial_fields(p_, hg_, is);
+    if ((st = upload(f_.enthalpy, is.enthalpy)) || (st = upload(f_.hnot, is.hnot)) ||
+        (st = upload(f_.temp, is.temp)) || (st = upload(f_.rho, is.rho)) || (st = upload(f_.vis, is.vis)) ||
+        (st = upload(f_.diff, is.diff)) || (st = upload(f_.csfrac, is.csfrac)))
+        return st;
+    // partial-sum buffers sized for the whole interior
+    energy_partial_sizes(d_, &pr_.nblk_e, &pr_.nrow_e, &pr_.nblk_t);
+    Box whole{1, d_.nim2, 1, d_.njm2, 2};
+    int nf = flow_partial_blocks(whole, d_) + 64;
+    if ((st = alloc(&pr_.energy, (size_t)2 * pr_.nblk_e))) return st;
+    if ((st = alloc(&pr_.blk, (size_t)pr_.nrow_e * 4 * d_.ni))) return st;
+    if ((st = alloc(&pr_.h2t, (size_t)10 * pr_.nblk_t))) return st;
+    if ((st = alloc(&pr_.flow, (size_t)4 * nf * 3))) return st;
+    pr_.nblk_b = (d_.njm2 * d_.nim2 + d_.nkm2 * d_.njm2 + d_.nkm2 * d_.nim2 + 255) / 256;
+    if ((st = alloc(&pr_.bc, (size_t)pr_.nblk_b))) return st;
+    if ((st = alloc(&pr_.delphi, (size_t)d_.ni + 2))) return st;
+    pr_.laser = nullptr;
+    void *v = nullptr;
+    HIPCALL(hipMalloc(&v, sizeof(DevScalars)));
+    allocs_.push_back(v);
+    sc_ = (DevScalars *)v;
+    HIPCALL(hipMemsetAsync(sc_, 0, sizeof(DevScalars), st_));
+    HIPCALL(hipHostMalloc((void **)&sc_host_, sizeof(DevScalars), hipHostMallocDefault));
+    std::memset(sc_host_, 0, sizeof(DevScalars));
+    HIPCALL(hipMalloc(&v, sizeof(unsigned long long)));
+    allocs_.push_back(v);
+    counter_ = (unsigned long long *)v;
+    HIPCALL(hipMemsetAsync(counter_, 0, sizeof(unsigned long long), st_));
+    HIPCALL(hipMalloc(&v, sizeof(int) * (size_t)d_.nkf * d_.nj));
+    allocs_.push_back(v);
+    versions_ = (int *)v;
+    HIPCALL(hipMemsetAsync(versions_, 0, sizeof(int) * (size_t)d_.nkf * d_.nj, st_));
+
+    scanvel_ = p_.scanvel;
+    set_active_nk(p_.nk_active0);
+    if (p_.steady) {   // main.cpp:105-106
+        beamx_ = p_.xstart;
+        int s2 = launch_laser_map();
+        if (s2) return s2;
+    }
+    HIPCALL(hipStreamSynchronize(st_));
+    return AMCFD_OK;
+}
+
+// ---------------------------------------------------------------------------
+// laser: BoundCondManager.cpp:68-275
+// ---------------------------------------------------------------------------
+void Solver::laser_position() {
+    if (!p_.udtoolpath) {
+        oldx_ = beamx_;
+        if (timet_ <= p_.lasertime) {
+            beamx_ = p_.xstart + timet_ * scanvel_;
+            if (laser_init_) {
+                oldx_ = beamx_;
+                laser_init_ = 0;
+            }
+        } else {
+            beamx_ = p_.xstart + p_.lasertime * scanvel_;
+            powerindicator_ = 0;
+        }
+        beamy_ = p_.ystart;
+        return;
+    }
+    const double small = 1.0e-8;
+    double tx = beamx_, ty = beamy_, tz = beamz_, dty = 0.0;
+    oldx_ = beamx_;
+    oldy_ = beamy_;
+    oldz_ = beamz_;
+    int oldpi = powerindicator_;
+    for (int r = 1; r < p_.n_toolpath; ++r) {
+        const double *b = &tool_[6 * r], *a = &tool_[6 * (r - 1)];
+        if (timet_ <= b[0] + small) {
+            double ratio = (timet_ - a[0]) / (b[0] - a[0]);
+            double dtx = b[1] - a[1];
+            dty = b[2] - a[2];
+            double dtz = b[3] - a[3];
+            tx = ratio * dtx + a[1];
+            ty = ratio * dty + a[2];
+            tz = ratio * dtz + a[3];
+            powerindicator_ = (int)b[4];
+            if (laser_init_) {
+                oldx_ = a[1];
+                oldy_ = a[2];
+                oldz_ = a[3];
+                oldpi = (int)a[4];
+                laser_init_ = 0;
+            }
+            break;
+        }
+    }
+    beamx_ = tx;
+    beamy_ = ty;
+    beamz_ = tz;
+    double dx = beamx_ - oldx_, dy = beamy_ - oldy_, dz = beamz_ - oldz_;
+    scanvel_ = std::sqrt(dx * dx + dy * dy + dz * dz) / p_.delt;
+    if (oldpi - powerindicator_ > 0 && dty > small) trackindx_++;
+    if (powerindicator_ - oldpi > 0 && p_.powderbed) {
+        trackindx_++;
+        set_active_nk(d_.nk + p_.ncvzlayer);
+    }
+}
+
+static int nearest(const std::vector<double> &c, int nlast, double pos) {
+    int out = 0;
+    for (int i = 1; i < nlast; ++i) {
+        if (pos <= c[i]) break;
+        out = i;
+    }
+    if (std::fabs(pos - c[out + 1]) < std::fabs(pos - c[out])) out++;
+    return out;
+}
+
+int Solver::launch_laser_map() {
+    istart_ = nearest(hg_.x, d_.nim1, beamx_);
+    jstart_ = nearest(hg_.y, d_.njm1, beamy_);
+    const double rb2 = p_.alasrb * p_.alasrb;
+    const double varlas = p_.alaspow * p_.alaseta;
+    const double peakhin = p_.alasfact * varlas / (M_PI * rb2);
+    launch_laser(st_, d_, g_, f_, pr_, beamx_, beamy_, peakhin, p_.alasfact, rb2, varlas, powerindicator_, sc_);
+    HIPCALL(hipGetLastError());
+    return AMCFD_OK;
+}
+
+int Solver::begin_step() {
+    if (!dtt_.empty()) {
+        if (timet_ > dtt_[3 * dt_state_] && dt_state_ < p_.n_dt_table - 1) dt_state_ += 1;
+        p_.delt = dtt_[3 * dt_state_ + 1];
+        p_.outtime = dtt_[3 * dt_state_ + 2];
+    }
+    timet_ += p_.delt;
+    ++step_;
+    if (p_.steady) return AMCFD_OK;
+    laser_position();
+    return launch_laser_map();
+}
+
+// volumetric-source parameters, CFDSolverManager.cpp:1686-1718 (the table
+// row advances at most once per call, i.e. once per SIMPLE iteration)
+SourceParams Solver::source_params() {
+    SourceParams s{0, 0, 0, 0, 0, beamx_, beamy_, powerindicator_};
+    if (p_.volheatsource && !p_.volheatsourcetable) {
+        s.avolpow = p_.avolpow;
+        s.apowseta = p_.apowseta;
+        s.heatthick = p_.heatthick;
+        s.heatrb = p_.heatrb;
+        s.avolfact = p_.avolfact;
+    } else if (!heatt_.empty()) {
+        if (timet_ > heatt_[6 * heat_it_] && heat_it_ < p_.n_heat_table - 1) heat_it_ += 1;
+        const double *r = &heatt_[6 * heat_it_];
+        s.avolpow = r[1];
+        s.apowseta = r[2];
+        s.heatthick = r[3];
+        s.heatrb = r[4];
+        s.avolfact = r[5];
+    }
+    return s;
+}
+
+// ---------------------------------------------------------------------------
+// profiling helpers (events on the solver stream)
+// ---------------------------------------------------------------------------
+void Solver::tic(const char *name, int *slot) {
+    (void)name;
+    *slot = -1;
+    if (!profile_) return;
+    if (ev_next_ + 2 > ev_pool_.size()) {
+        for (int a = 0; a < 64; ++a) {
+            hipEvent_t e;
+            if (hipEventCreate(&e) != hipSuccess) return;
+            ev_pool_.push_back(e);
+        }
+    }
+    *slot = (int)ev_next_;
+    ev_next_ += 2;
+    hipEventRecord(ev_pool_[*slot], st_);
+}
+
+void Solver::toc(const char *name, int slot) {
+    if (slot < 0) return;
+    hipEventRecord(ev_pool_[slot + 1], st_);
+    ev_pending_.push_back({name, {slot, slot + 1}});
+}
+
+void Solver::collect_timers() {
+    for (auto &pe : ev_pending_) {
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, ev_pool_[pe.second.first], ev_pool_[pe.second.second]) == hipSuccess) {
+            KernelTimer &t = timers[pe.first];
+            t.launches += 1;
+            t.total_ms += ms;
+        }
+    }
+    ev_pending_.clear();
+    ev_next_ = 0;
+}
+
+// ---------------------------------------------------------------------------
+// line sweeps
+// ---------------------------------------------------------------------------
+int Solver::ensure_plan(SweepPlan &plan, int kbeg, int kend, int jbeg, int jend) {
+    if (plan.kbeg == kbeg && plan.kend == kend && plan.jbeg == jbeg && plan.jend == jend && plan.tasks) return 0;
+    std::vector<int> tasks;
+    int width = 0;
+    build_sweep_tasks(kbeg, kend, jbeg, jend, tasks, width);
+    if ((int)tasks.size() > plan.cap) {
+        if (plan.tasks) {
+            HIPCALL(hipStreamSynchronize(st_));
+            HIPCALL(hipFree(plan.tasks));
+        }
+        void *v = nullptr;
+        HIPCALL(hipMalloc(&v, sizeof(int) * std::max<size_t>(tasks.size(), 1)));
+        plan.tasks = (int *)v;
+        plan.cap = (int)tasks.size();
+    }
+    if (!tasks.empty()) {
+        // the copy is ordered on the stream after every earlier sweep
+        HIPCALL(hipMemcpyAsync(plan.tasks, tasks.data(), tasks.size() * sizeof(int), hipMemcpyHostToDevice, st_));
+        HIPCALL(hipStreamSynchronize(st_));
+    }
+    plan.ntasks = (int)tasks.size();
+    plan.width = width;
+    plan.kbeg = kbeg;
+    plan.kend = kend;
+    plan.jbeg = jbeg;
+    plan.jend = jend;
+    return 0;
+}
+
+int Solver::sweep(double *phi, SweepPlan &plan, int ibeg, int iend, int jbeg, int jend, int kbeg, int kend,
+                  int guard, const char *timer) {
+    int st = ensure_plan(plan, kbeg, kend, jbeg, jend);
+    if (st) return st;
+    if (plan.ntasks == 0 || iend < ibeg) return 0;
+    int nblocks = std::min(plan.ntasks, std::max(2 * plan.width, 1));
+    nblocks = std::min(nblocks, 2048);
+    int slot;
+    tic(timer, &slot);
+    launch_line_sweeps(st_, d_, f_, phi, plan, ibeg, iend, guard, counter_, counter_base_, versions_, epoch_, sc_,
+                       nblocks);
+    HIPCALL(hipGetLastError());
+    toc(timer, slot);
+    counter_base_ += (unsigned long long)plan.ntasks + (unsigned long long)nblocks;
+    epoch_ += 4;
+    return 0;
+}
+
+int Solver::read_scalars() {
+    HIPCALL(hipMemcpyAsync(sc_host_, sc_, sizeof(DevScalars), hipMemcpyDeviceToHost, st_));
+    HIPCALL(hipStreamSynchronize(st_));
+    if (sc_host_->error) return set_error(AMCFD_E_HIP, "line-sweep dependency wait expired");
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// momentum + pressure correction on the melt-pool box, CFDSolverManager.cpp:660-675
+// ---------------------------------------------------------------------------
+int Solver::flow_iteration() {
+    box_ = Box{sc_host_->istat, sc_host_->iend, sc_host_->jstat, sc_host_->jend, sc_host_->kstat};
+    pr_.nblk_f = flow_partial_blocks(box_, d_);
+    launch_clean_velocities(st_, d_, f_, box_, p_.tsolid);
+    double *phis[5] = {nullptr, f_.uvel, f_.vvel, f_.wvel, f_.pp};
+    static const char *names[5] = {"", "sweep_u", "sweep_v", "sweep_w", "sweep_p"};
+    for (int ivar = 1; ivar <= 4; ++ivar) {
+        launch_bc_flow(st_, d_, g_, f_, box_, p_, ivar);
+        if (ivar < 4) launch_momentum(st_, d_, g_, f_, pr_, box_, p_, scanvel_, ivar);
+        else launch_pressure(st_, d_, g_, f_, pr_, box_, p_);
+        int st = sweep(phis[ivar], plan_box_, box_.istatp1(), box_.iendm1(), box_.jstat, box_.jend, box_.kstat,
+                       d_.nkm2, 1, names[ivar]);
+        if (st) return st;
+        if (ivar == 4) launch_correct(st_, d_, f_, box_, p_);
+    }
+    HIPCALL(hipGetLastError());
+    return 0;
+}
+
+int Solver::solve_step() {
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    hipEventCreate(&e0);
+    hipEventCreate(&e1);
+    hipEventRecord(e0, st_);
+    double tdma_before = timers.count("sweep_h") ? timers["sweep_h"].total_ms : 0.0;
+    const double alenold = alen_;
+    niter_ = 0;
+    int rc = AMCFD_I_MAXIT;
+    while (niter_ < p_.maxit) {
+        ++niter_;
+        ++itertot_;
+        SourceParams sp = source_params();
+        int slot;
+        launch_bc_enthalpy(st_, d_, g_, f_, pr_, p_);
+        tic("assemble_h", &slot);
+        launch_energy_assemble(st_, d_, g_, f_, pr_, p_, sp);
+        toc("assemble_h", slot);
+        launch_props(st_, d_, f_, p_);
+        launch_block_correction(st_, d_, f_, pr_, sc_);
+        int st = sweep(f_.enthalpy, plan_dom_, 1, d_.nim2, 1, d_.njm2, 1, d_.nkm2, 0, "sweep_h");
+        if (st) return st;
+        tic("h2t", &slot);
+        launch_h2t(st_, d_, g_, f_, pr_, p_);
+        toc("h2t", slot);
+        launch_melt_pool(st_, d_, g_, f_, pr_, p_, istart_, jstart_, sc_);
+        HIPCALL(hipGetLastError());
+        if ((st = read_scalars())) return st;
+        alen_ = sc_host_->alen;
+        int flow = 0;
+        if (sc_host_->tpeak > p_.tsolid && p_.navierstokes) {
+            if ((st = flow_iteration())) return st;
+            flow = 1;
+        }
+        launch_finalize(st_, d_, pr_, sc_, p_, flow, p_.steady, sc_host_->width);
+        HIPCALL(hipGetLastError());
+        if ((st = read_scalars())) return st;
+        if (profile_) collect_timers();
+        const DevScalars &s = *sc_host_;
+        // std::max over {resorm, resoru, resorv, resorw, resorc}: first largest wins
+        double res[5] = {s.resorm, s.resoru, s.resorv, s.resorw, 0.0};
+        double amaxres = res[0];
+        for (int r = 1; r < 5; ++r)
+            if (amaxres < res[r]) amaxres = res[r];
+        const double ratio = s.ratio, tol = p_.nonlintol;
+        bool done;
+        if (p_.steady) done = ratio <= 1.001 && ratio >= 0.999 && amaxres < tol;
+        else if (powerindicator_ == 1) done = ratio <= 1.01 && ratio >= 0.99 && amaxres < tol;
+        else done = s.resorh < 1e-6 && amaxres < tol;
+        if (done) {
+            rc = AMCFD_OK;
+            break;
+        }
+    }
+    if (rc == AMCFD_I_MAXIT && std::fabs(alen_ - alenold) < 1.0e-6) steadylength_ = true;
+    hipEventRecord(e1, st_);
+    hipEventSynchronize(e1);
+    float ms = 0.f;
+    hipEventElapsedTime(&ms, e0, e1);
+    last_total_ms = ms;
+    last_tdma_ms = profile_ ? (timers["sweep_h"].total_ms - tdma_before) : -1.0;
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    return rc;
+}
+
+int Solver::end_step() {
+    if (p_.steady) return AMCFD_OK;
+    double trackid = 0.0;
+    for (int r = p_.n_toolpath; r > 0; --r)
+        if (timet_ <= tool_[6 * (r - 1)]) trackid = (double)(int)tool_[6 * (r - 1) + 5];
+    launch_end_step(st_, d_, f_, p_, (double)trackindx_, trackid);
+    HIPCALL(hipGetLastError());
+    return AMCFD_OK;
+}
+
+// ---------------------------------------------------------------------------
+// field access
+// ---------------------------------------------------------------------------
+double *Solver::field_ptr(const char *n) {
+    struct {
+        const char *name;
+        double *p;
+    } t[] = {{"enthalpy", f_.enthalpy}, {"temp", f_.temp}, {"fracl", f_.fracl}, {"uvel", f_.uvel},
+             {"vvel", f_.vvel}, {"wvel", f_.wvel}, {"pressure", f_.pressure}, {"pp", f_.pp}, {"rho", f_.rho},
+             {"diff", f_.diff}, {"vis", f_.vis}, {"hnot", f_.hnot}, {"fraclnot", f_.fraclnot},
+             {"unot", f_.unot}, {"vnot", f_.vnot}, {"wnot", f_.wnot}, {"sourceinput", f_.sourceinput},
+             {"dux", f_.dux}, {"dvy", f_.dvy}, {"dwz", f_.dwz}, {"csfrac", f_.csfrac}, {"solfrac", f_.solfrac},
+             {"mpPhase", f_.mpphase}, {"mpLiqPhase", f_.mpliqphase}, {"ap", f_.ap}, {"an", f_.an},
+             {"as", f_.as}, {"ae", f_.ae}, {"aw", f_.aw}, {"at", f_.at}, {"ab", f_.ab}, {"su", f_.su}};
+    for (auto &e : t)
+        if (std::strcmp(e.name, n) == 0) return e.p;
+    return nullptr;
+}
+
+int Solver::get_field(const char *name, double *host, size_t n) {
+    double *p = field_ptr(name);
+    if (!p) return set_error(AMCFD_E_NAME, std::string("unknown field ") + name);
+    size_t nn = (size_t)d_.ni * d_.nj * d_.nkf;
+    if (n != nn) return set_error(AMCFD_E_ARG, "field buffer must hold ni*nj*nk values");
+    HIPCALL(hipMemcpyAsync(host, p, nn * sizeof(double), hipMemcpyDeviceToHost, st_));
+    HIPCALL(hipStreamSynchronize(st_));
+    return AMCFD_OK;
+}
+
+int Solver::set_field(const char *name, const double *host, size_t n) {
+    double *p = field_ptr(name);
+    if (!p) return set_error(AMCFD_E_NAME, std::string("unknown field ") + name);
+    size_t nn = (size_t)d_.ni * d_.nj * d_.nkf;
+    if (n != nn) return set_error(AMCFD_E_ARG, "field buffer must hold ni*nj*nk values");
+    HIPCALL(hipMemcpyAsync(p, host, nn * sizeof(double), hipMemcpyHostToDevice, st_));
+    HIPCALL(hipStreamSynchronize(st_));
+    return AMCFD_OK;
+}
+
+int Solver::get_geometry(const char *name, double *host, size_t n) {
+    struct {
+        const char *name;
+        const std::vector<double> *v;
+        size_t len;
+    } t[] = {{"x", &hg_.x, (size_t)d_.ni}, {"y", &hg_.y, (size_t)d_.nj}, {"z", &hg_.z, (size_t)d_.nkf},
+             {"xu", &hg_.xu, (size_t)d_.ni}, {"yv", &hg_.yv, (size_t)d_.nj}, {"zw", &hg_.zw, (size_t)d_.nkf},
+             {"dxpwinv", &hg_.dxpwinv, (size_t)d_.ni}, {"dypsinv", &hg_.dypsinv, (size_t)d_.nj},
+             {"dzpbinv", &hg_.dzpbinv, (size_t)d_.nkf}, {"fracx", &hg_.fracx, (size_t)d_.ni},
+             {"fracy", &hg_.fracy, (size_t)d_.nj}, {"fracz", &hg_.fracz, (size_t)d_.nkf},
+             {"areaij", &hg_.areaij, hg_.areaij.size()}, {"areajk", &hg_.areajk, hg_.areajk.size()},
+             {"areaik", &hg_.areaik, hg_.areaik.size()}};
+    for (auto &e : t)
+        if (std::strcmp(e.name, name) == 0) {
+            if (n != e.len) return set_error(AMCFD_E_ARG, "geometry buffer has the wrong length");
+            std::copy(e.v->begin(), e.v->begin() + e.len, host);
+            return AMCFD_OK;
+        }
+    if (std::strcmp(name, "volume") == 0) {
+        size_t nn = (size_t)d_.ni * d_.nj * d_.nkf;
+        if (n != nn) return set_error(AMCFD_E_ARG, "geometry buffer has the wrong length");
+        std::fill(host, host + nn, 0.0);
+        for (int k = 1; k < d_.nkf - 1; ++k)
+            for (int j = 1; j < d_.nj - 1; ++j)
+                for (int i = 1; i < d_.ni - 1; ++i) host[((size_t)k * d_.nj + j) * d_.ni + i] = hg_.volume(i, j, k);
+        return AMCFD_OK;
+    }
+    return set_error(AMCFD_E_NAME, std::string("unknown geometry array ") + name);
+}
+
+int Solver::get_scalars(amcfd_scalars *o) {
+    HIPCALL(hipMemcpyAsync(sc_host_, sc_, sizeof(DevScalars), hipMemcpyDeviceToHost, st_));
+    HIPCALL(hipStreamSynchronize(st_));
+    const DevScalars &s = *sc_host_;
+    std::memset(o, 0, sizeof(*o));
+    o->timet = timet_;
+    o->niter = niter_;
+    o->itertot = itertot_;
+    o->resorh = s.resorh;
+    o->resorm = s.resorm;
+    o->resoru = s.resoru;
+    o->resorv = s.resorv;
+    o->resorw = s.resorw;
+    o->tpeak = s.tpeak;
+    o->alen = s.alen;
+    o->depth = s.depth;
+    o->width = s.width;
+    o->ratio = s.ratio;
+    o->heatinlaser = s.heatinlaser;
+    o->heatout = s.heatout;
+    o->accul = s.accul;
+    o->heatvol = s.heatvol;
+    o->ahtoploss = s.ahtoploss;
+    o->beamposx = beamx_;
+    o->beamposy = beamy_;
+    o->scanvel = scanvel_;
+    o->powerindicator = powerindicator_;
+    o->trackindx = trackindx_;
+    o->istat = s.istat;
+    o->iend = s.iend;
+    o->jstat = s.jstat;
+    o->jend = s.jend;
+    o->kstat = s.kstat;
+    o->istart = istart_;
+    o->jstart = jstart_;
+    o->nk = d_.nk;
+    o->step = step_;
+    return AMCFD_OK;
+}
+
+}  // namespace amcfd

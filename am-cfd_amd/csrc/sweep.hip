@@ -1,0 +1,210 @@
+// Gauss-Seidel-ordered x-line sweeps on the GPU.
+//
+// The reference sweeps  for ks in {0,1}: for k = kend..kbeg: for js in {0,1}:
+// for j = jbeg..jend: Thomas along i  (CFDSolverManager.cpp:2415-2453 on the
+// melt-pool box, 2492-2525 on the whole domain).  Each line's right-hand side
+// reads the lines (k+-1, j) and (k, j+-1) as they stand at that point of the
+// sequence, so the sweep is Gauss-Seidel in j and k.  Run single-threaded
+// that order is the reference's definition of the result.
+//
+// Here every line solve is a task; task (pass p = 2ks+js, k, j) may start once
+//   line (k, j-1) has finished pass p,
+//   line (k, j+1) has finished pass p-1            (p > 0),
+//   line (k+1, j) has finished pass 2ks+1,
+//   line (k-1, j) has finished pass 1              (ks = 1),
+// which is exactly the set of read-after-write and write-after-read
+// constraints of the sequential loop nest.  Tasks are claimed from an atomic
+// counter in increasing  tau = 6ks + 3(kend-k) + 2js + (j-jbeg)  (every
+// dependency has a smaller tau), so a claimed task only ever waits on tasks
+// already claimed by running workgroups: the schedule cannot deadlock.  Each
+// line keeps a version word (epoch + passes done); waits are bounded and raise
+// an error flag instead of hanging.
+//
+// A line is solved by the whole workgroup with parallel cyclic reduction in
+// LDS.  On the melt-pool box the reference guards |pivot| <= 1e-12; when a
+// line is not provably free of such pivots the workgroup falls back to the
+// reference's sequential Thomas recurrence with the guard.
+
+#include "fv.h"
+
+namespace amcfd {
+
+namespace {
+
+constexpr int SW_THREADS = 256;
+constexpr int SW_MAXN = 1024;                 // longest x-line handled
+constexpr int SW_MAXR = SW_MAXN / SW_THREADS;
+constexpr long long SPIN_LIMIT = 1LL << 20;   // bounded wait (~ a second)
+
+struct Coefs {
+    const double *an, *as, *ae, *aw, *at, *ab, *ap, *su;
+};
+
+__device__ __forceinline__ int load_version(const int *v) {
+    return __hip_atomic_load(v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Wait until *v >= need.  Gives up after SPIN_LIMIT polls, or at once when
+// another wait of this launch has already given up (error flag set).
+__device__ bool wait_version(const int *v, int need, const int *error) {
+    long long spins = 0;
+    while (load_version(v) < need) {
+        __builtin_amdgcn_s_sleep(1);
+        if (++spins > SPIN_LIMIT) return false;
+        if ((spins & 255) == 0 && load_version(error) != 0) return false;
+    }
+    return true;
+}
+
+__global__ __launch_bounds__(SW_THREADS) void k_line_sweeps(Dims d, Coefs cf, double *phi, const int *__restrict__ tasks,
+                                                            int ntasks, int ibeg, int iend, int jbeg, int jend,
+                                                            int kbeg, int kend, int guard,
+                                                            unsigned long long *counter,
+                                                            unsigned long long base, int *versions, int epoch,
+                                                            DevScalars *sc) {
+    __shared__ double A[SW_MAXN], B[SW_MAXN], C[SW_MAXN], D[SW_MAXN];
+    __shared__ int s_task, s_ok;
+    const int tid = threadIdx.x;
+    const int n = iend - ibeg + 1;
+    for (;;) {
+        if (tid == 0) {
+            unsigned long long t = atomicAdd(counter, 1ULL) - base;
+            s_task = (t < (unsigned long long)ntasks) ? (int)t : -1;
+        }
+        __syncthreads();
+        const int t = s_task;
+        __syncthreads();
+        if (t < 0) return;
+        const int packed = tasks[t];
+        const int pass = packed >> 28, k = (packed >> 14) & 0x3fff, j = packed & 0x3fff;
+        const int ks = pass >> 1;
+        const size_t row = (size_t)k * d.sk + (size_t)j * d.sj;
+        // coefficients do not depend on other lines: load before waiting
+        double rat[SW_MAXR], rab[SW_MAXR], ran[SW_MAXR], ras[SW_MAXR], rsu[SW_MAXR];
+#pragma unroll
+        for (int r = 0; r < SW_MAXR; ++r) {
+            const int ii = tid + r * SW_THREADS;
+            if (ii < n) {
+                const size_t c = row + ibeg + ii;
+                rat[r] = cf.at[c]; rab[r] = cf.ab[c]; ran[r] = cf.an[c]; ras[r] = cf.as[c]; rsu[r] = cf.su[c];
+                A[ii] = -cf.aw[c];
+                B[ii] = cf.ap[c];
+                C[ii] = -cf.ae[c];
+            }
+        }
+        if (tid == 0) {
+            bool ok = true;
+            int *vb = versions;
+            const int *err = &sc->error;
+            if (j > jbeg) ok &= wait_version(vb + k * d.nj + (j - 1), epoch + pass + 1, err);
+            if (j < jend && pass > 0) ok &= wait_version(vb + k * d.nj + (j + 1), epoch + pass, err);
+            if (k < kend) ok &= wait_version(vb + (k + 1) * d.nj + j, epoch + 2 * ks + 2, err);
+            if (k > kbeg && ks > 0) ok &= wait_version(vb + (k - 1) * d.nj + j, epoch + 2, err);
+            if (!ok) atomicOr(&sc->error, 1);
+        }
+        __syncthreads();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        // right-hand side from the neighbouring lines, in the reference's order
+#pragma unroll
+        for (int r = 0; r < SW_MAXR; ++r) {
+            const int ii = tid + r * SW_THREADS;
+            if (ii < n) {
+                const size_t c = row + ibeg + ii;
+                D[ii] = rat[r] * phi[c + d.sk] + rab[r] * phi[c - d.sk] + ran[r] * phi[c + d.sj] +
+                        ras[r] * phi[c - d.sj] + rsu[r];
+            }
+        }
+        if (tid == 0) s_ok = 1;
+        __syncthreads();
+        if (guard) {
+            // PCR is used only when no Thomas pivot can fall to 1e-12:
+            // nonnegative neighbours, ap >= aw + ae and ap - aw > 2e-12
+            bool ok = true;
+#pragma unroll
+            for (int r = 0; r < SW_MAXR; ++r) {
+                const int ii = tid + r * SW_THREADS;
+                if (ii < n) {
+                    double aw = -A[ii], ae = -C[ii], ap = B[ii];
+                    ok &= (aw >= 0.0) && (ae >= 0.0) && (ap - aw > 2e-12) && (ap >= (aw + ae) * (1.0 - 1e-12));
+                }
+            }
+            if (!ok) s_ok = 0;
+            __syncthreads();
+        }
+        if (s_ok) {
+            // fixed end values fold into the first / last rows
+            if (tid == 0) {
+                D[0] -= A[0] * phi[row + ibeg - 1];
+                D[n - 1] -= C[n - 1] * phi[row + iend + 1];
+                A[0] = 0.0;
+                C[n - 1] = 0.0;
+            }
+            __syncthreads();
+            pcr_solve<SW_MAXR>(n, A, B, C, D);
+#pragma unroll
+            for (int r = 0; r < SW_MAXR; ++r) {
+                const int ii = tid + r * SW_THREADS;
+                if (ii < n) phi[row + ibeg + ii] = D[ii];
+            }
+        } else if (tid == 0) {
+            // reference recurrence (CFDSolverManager.cpp:2424-2449), pr in A, qr in C
+            double prv = 0.0, qrv = phi[row + ibeg - 1];
+            for (int ii = 0; ii < n; ++ii) {
+                double aw = -A[ii];
+                double den = B[ii] - aw * prv;
+                if (fabs(den) <= 1e-12) den += AMCFD_SMALL;
+                prv = -C[ii] / den;
+                qrv = (D[ii] + aw * qrv) / den;
+                A[ii] = prv;
+                C[ii] = qrv;
+            }
+            double nxt = phi[row + iend + 1];
+            for (int ii = n - 1; ii >= 0; --ii) {
+                nxt = A[ii] * nxt + C[ii];
+                phi[row + ibeg + ii] = nxt;
+            }
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        __syncthreads();
+        if (tid == 0)
+            __hip_atomic_store(versions + k * d.nj + j, epoch + pass + 1, __ATOMIC_RELEASE,
+                               __HIP_MEMORY_SCOPE_AGENT);
+    }
+}
+
+}  // namespace
+
+void build_sweep_tasks(int kbeg, int kend, int jbeg, int jend, std::vector<int> &tasks, int &width) {
+    tasks.clear();
+    width = 0;
+    if (kend < kbeg || jend < jbeg) return;
+    const int nk = kend - kbeg + 1, nj = jend - jbeg + 1;
+    const int tmax = 6 + 3 * (nk - 1) + 2 + (nj - 1);
+    for (int tau = 0; tau <= tmax; ++tau) {
+        int w = 0;
+        for (int ks = 0; ks < 2; ++ks)
+            for (int kk = 0; kk < nk; ++kk)
+                for (int js = 0; js < 2; ++js) {
+                    int jj = tau - 6 * ks - 3 * kk - 2 * js;
+                    if (jj < 0 || jj >= nj) continue;
+                    int k = kend - kk, j = jbeg + jj, pass = 2 * ks + js;
+                    tasks.push_back((pass << 28) | (k << 14) | j);
+                    ++w;
+                }
+        if (w > width) width = w;
+    }
+}
+
+int sweep_max_line() { return SW_MAXN; }
+
+void launch_line_sweeps(hipStream_t st, const Dims &d, DevFields f, double *phi, const SweepPlan &plan, int ibeg,
+                        int iend, int guard, unsigned long long *counter, unsigned long long counter_base,
+                        int *versions, int epoch, DevScalars *sc, int nblocks) {
+    if (plan.ntasks == 0 || iend < ibeg) return;
+    Coefs cf{f.an, f.as, f.ae, f.aw, f.at, f.ab, f.ap, f.su};
+    hipLaunchKernelGGL(k_line_sweeps, dim3(nblocks), dim3(SW_THREADS), 0, st, d, cf, phi, plan.tasks, plan.ntasks,
+                       ibeg, iend, plan.jbeg, plan.jend, plan.kbeg, plan.kend, guard, counter, counter_base,
+                       versions, epoch, sc);
+}
+
+}  // namespace amcfd

@@ -1,0 +1,181 @@
+/*
+ * amcfd.h - C ABI of the MI355X SIMPLE solve (AM-CFD per-time-step path).
+ *
+ * The reference exposes this path only as C++ member functions on heap
+ * objects created by its executable:
+ *   Mesh::getDomainInfo / readMeshFile        src/sources/Mesh.cpp:39-550
+ *   DomainManager::assignInputValues          src/sources/DomainManager.cpp:29-263
+ *   CFDSolverManager::generateGrid            src/sources/CFDSolverManager.cpp:34-302
+ *   CFDSolverManager::initializeSolver        src/sources/CFDSolverManager.cpp:402-590
+ *   BoundCondManager::updateLaser             src/sources/BoundCondManager.cpp:68-170
+ *   CFDSolverManager::iterativeLoopSolve      src/sources/CFDSolverManager.cpp:595-722
+ *   CFDSolverManager::updateSolutions         src/sources/CFDSolverManager.cpp:2625-2687
+ *   SolutionVariableManager name->field map   src/headers/SolutionVariableManager.h:18-19
+ * called in that order by src/main.cpp:56-162.  Each entry point below names
+ * the member function it replaces.
+ *
+ * Conventions
+ *   - status: 0 ok; > 0 informational bits; < 0 errors (AMCFD_E_*).
+ *   - fields use the reference layout: index (k*nj + j)*ni + i, i fastest,
+ *     FP64, ni*nj*nk values (nk = allocated node count in z).
+ *   - caller owns host buffers, the library owns device memory.
+ *   - one handle per host thread; calls are stream ordered and synchronous
+ *     where they return host data.
+ */
+#ifndef AMCFD_H
+#define AMCFD_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define AMCFD_MAX_ZONES 32
+
+/* status codes */
+#define AMCFD_OK              0
+#define AMCFD_I_MAXIT         1    /* solve ended at maxit without converging  */
+#define AMCFD_E_ARG          -1    /* bad argument                             */
+#define AMCFD_E_IO           -2    /* deck / mesh / table file not readable    */
+#define AMCFD_E_PARSE        -3    /* deck grammar error                       */
+#define AMCFD_E_UNSUPPORTED  -4    /* keyword outside this path                */
+#define AMCFD_E_HIP          -5    /* HIP runtime failure                      */
+#define AMCFD_E_NAN          -6    /* non-finite value detected on device      */
+#define AMCFD_E_CAPACITY     -7    /* problem too large for the device         */
+#define AMCFD_E_NAME         -8    /* unknown field name                       */
+
+/* discretisation schemes, Mesh.cpp:378-392 (default upwind, Mesh.h:59) */
+#define AMCFD_DISC_UPWIND      0
+#define AMCFD_DISC_POWERLAW    1
+#define AMCFD_DISC_EXPONENTIAL 2
+
+/* solver precision modes */
+#define AMCFD_MODE_FP64        0   /* plain FP64, reference program order     */
+
+/*
+ * Every value the path reads from the reference's Mesh and DomainManager
+ * members (src/headers/Mesh.h:41-68, src/headers/DomainManager.h:26-72).
+ * Tables are borrowed pointers owned by whoever filled the struct.
+ */
+typedef struct amcfd_params {
+    /* grid zones: DomainManager.cpp:33-46 */
+    int    nzx, nzy, nzz;
+    double xzone[AMCFD_MAX_ZONES], ncvx[AMCFD_MAX_ZONES], powrx[AMCFD_MAX_ZONES];
+    double yzone[AMCFD_MAX_ZONES], ncvy[AMCFD_MAX_ZONES], powry[AMCFD_MAX_ZONES];
+    double zzone[AMCFD_MAX_ZONES], ncvz[AMCFD_MAX_ZONES], powrz[AMCFD_MAX_ZONES];
+    /* control: Mesh.cpp:284-299 */
+    double finaltime, maxit, nonlintol, delt, outtime;
+    /* *PARAMETERS: Mesh.cpp:440-470 */
+    double grav, sigm, tempamb, pressamb, gasconst;
+    /* feature flags */
+    int    navierstokes;       /* 0 if *FLUID_FLOW_OFF                     */
+    int    discretization;     /* AMCFD_DISC_*                             */
+    int    steady;             /* delt >= 100, DomainManager.cpp:203-210   */
+    int    isevap, powderbed, udtoolpath, gaussheatsource;
+    int    volheatsource, volheatsourcetable;
+    /* single track: DomainManager.cpp:51-54 */
+    double lasertime, xstart, ystart, scanvel;
+    /* material: DomainManager.cpp:59-73 */
+    double dens, denl, viscos, tsolid, tliquid, hsmelt, hlfriz, acpa, acpb, acpl;
+    double thconsa, thconsb, thconsc, thconla, thconlb;
+    /* relaxation: DomainManager.cpp:76-80 */
+    double urfu, urfv, urfw, urfp, urfh;
+    /* energy / temperature / momentum BCs: DomainManager.cpp:83-100 */
+    double htci, htcj, htck1, htckn, emiss;
+    double tempwest, tempeast, tempnorth, tempbottom, temppreheat;
+    double beta, dgdtp, darcyKo, a_oxygen;
+    /* Gaussian surface laser: DomainManager.cpp:103-110 */
+    double alaspow, alaseta, alasrb, alasfact;
+    /* evaporation: DomainManager.cpp:113-120 */
+    double evaplatent, evaptemp, evapfact, molarmass;
+    /* volumetric source: DomainManager.cpp:165-186 */
+    double avolpow, apowseta, heatthick, heatrb, avolfact;
+    /* powder bed: DomainManager.cpp:147-162 (zzone/ncvz already extended) */
+    int    numlayers, ncvzlayer;
+    double layerheight, powderden, powdercpa, powdercpb, powderthcona, powderthconb;
+    /* tables (row-major) */
+    int    n_toolpath;   const double *toolpath;    /* t x y z on track, 6/row */
+    int    n_dt_table;   const double *dt_table;    /* t_until dt outtime, 3/row */
+    int    n_heat_table; const double *heat_table;  /* t P eta thick rb fact, 6/row */
+    /* derived: DomainManager.cpp:213-254 */
+    int    ni, nj, nk;         /* node counts incl. boundary nodes (nk = full) */
+    int    nk_active0;         /* active nk after powder-layer deactivation    */
+    double hlcal, hlatnt;
+} amcfd_params;
+
+/* scalars the reference keeps as members (CFDSolverManager.h:81-83,
+ * BoundCondManager.h:62-80) after a solve / step */
+typedef struct amcfd_scalars {
+    double timet;
+    int    niter, itertot;
+    double resorh, resorm, resoru, resorv, resorw;
+    double tpeak, alen, depth, width;
+    double ratio, heatinlaser, heatout, accul, heatvol, ahtoploss;
+    double beamposx, beamposy, scanvel;
+    int    powerindicator, trackindx;
+    int    istat, iend, jstat, jend, kstat, istart, jstart, nk;
+    int    step;
+} amcfd_scalars;
+
+typedef struct amcfd_deck amcfd_deck;      /* parsed input deck          */
+typedef struct amcfd_solver amcfd_solver;  /* device-resident solver     */
+
+/* Parse a keyword deck and its mesh / toolpath / table files.
+ * Replaces Mesh::Mesh + getDomainInfo + assignParameters + readMeshFile
+ * (Mesh.cpp:23-26,39-550) and DomainManager::assignInputValues /
+ * getToolpath / getHeatSourceParameters (DomainManager.cpp:29-312).
+ * Relative file names resolve against the current directory, as in the
+ * reference. */
+int  amcfd_deck_load(const char *deck_path, amcfd_deck **out);
+const amcfd_params *amcfd_deck_params(const amcfd_deck *deck);
+void amcfd_deck_free(amcfd_deck *deck);
+
+/* Create the device solver: generateGrid + initializeSolver +
+ * initializeBoundaries (+ powder layer deactivation), PrintManager.cpp:49-74,
+ * and the steady-state laser set-up of main.cpp:105-106. */
+int  amcfd_create(const amcfd_params *params, int mode, amcfd_solver **out);
+void amcfd_destroy(amcfd_solver *s);
+
+/* One time step, split like src/main.cpp:125-162:
+ *   begin_step = variable-dt table + timet += delt + updateLaser
+ *   solve_step = CFDSolverManager::iterativeLoopSolve
+ *   end_step   = CFDSolverManager::updateSolutions (transient only)
+ * solve_step returns AMCFD_I_MAXIT when maxit was reached. */
+int  amcfd_begin_step(amcfd_solver *s);
+int  amcfd_solve_step(amcfd_solver *s);
+int  amcfd_end_step(amcfd_solver *s);
+/* begin + solve + end; returns 1 once timet >= finaltime (main.cpp:123) */
+int  amcfd_step(amcfd_solver *s, int *finished);
+
+/* Field access by registry name (CFDSolverManager.cpp:289-302,528-560):
+ * enthalpy temp fracl uvel vvel wvel pressure pp rho diff vis hnot fraclnot
+ * unot vnot wnot sourceinput dux dvy dwz csfrac solfrac.  n must equal
+ * ni*nj*nk (full z extent). */
+int  amcfd_get_field(amcfd_solver *s, const char *name, double *host, size_t n);
+int  amcfd_set_field(amcfd_solver *s, const char *name, const double *host, size_t n);
+/* 1-D / 2-D geometry: x y z xu yv zw dxpwinv dypsinv dzpbinv fracx fracy fracz
+ * volume areaij areajk areaik */
+int  amcfd_get_geometry(amcfd_solver *s, const char *name, double *host, size_t n);
+int  amcfd_get_scalars(amcfd_solver *s, amcfd_scalars *out);
+int  amcfd_dims(amcfd_solver *s, int *ni, int *nj, int *nk);
+
+/* Device time of the last solve_step in milliseconds, split into the line
+ * sweeps (x-line TDMA) and everything else; -1 if unavailable.
+ * No reference counterpart: the reference prints only wall-clock totals
+ * (PrintManager.cpp:713-737). */
+int  amcfd_last_timing(amcfd_solver *s, double *total_ms, double *tdma_ms);
+/* Record HIP events around the kernel families ("sweep_h", "sweep_u",
+ * "sweep_v", "sweep_w", "sweep_p", "assemble_h", "h2t") and report the
+ * accumulated launches and device milliseconds.  No reference counterpart. */
+int  amcfd_set_profiling(amcfd_solver *s, int on);
+int  amcfd_kernel_stats(amcfd_solver *s, const char *kernel, long *launches, double *total_ms);
+
+/* Thread-local description of the last error. */
+const char *amcfd_last_error(void);
+const char *amcfd_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* AMCFD_H */

@@ -1,0 +1,147 @@
+"""Writers for keyword decks and mesh files in the reference's input grammar.
+
+The grammar is the one parsed by the reference's ``Mesh::getDomainInfo``
+(src/sources/Mesh.cpp:39-435) and ``Mesh::readMeshFile`` (Mesh.cpp:475-550):
+whole-line ``*KEYWORD`` markers, ``$`` comment lines, whitespace-separated
+values; no blank lines (the reference throws on one, Mesh.cpp:50).
+
+Every deck written here carries a zero-power ``*VOLUMETRIC_HEAT_SOURCE``:
+without one the reference indexes an empty heat-source table
+(src/sources/CFDSolverManager.cpp:1707-1716).
+
+The named cases are the workloads the solver is measured and checked on:
+
+* ``tiny``      22x14x14 nodes, 40 steps: full-field fixtures every step.
+* ``tiny_cond`` same grid, upwind, ``*FLUID_FLOW_OFF``: conduction-only path.
+* ``tiny_evap`` same grid, evaporation loss and the Fe-O surfactant
+                surface-tension model (a_oxygen > 0).
+* ``plate120``  120x48x32 cells (2.0x0.6x0.4 mm), 400 steps, P=200 W,
+                r=50 um, 1 m/s: the CPU-runnable single-track case.
+* ``track400``  400x160x80 cells (4.0x1.2x0.6 mm, ~5.1M cells), P=300 W,
+                r=60 um, 0.8 m/s, dt=2e-6 s: the single-GPU throughput case.
+"""
+import os
+
+# Synthetic alloy (rho 4000, cp 700, k 25, Ts 1878, Tl 1928, L 2.9e5, mu 5e-3).
+ACPA = 1.0e-6
+ACPB = 700.0
+TS = 1878.0
+TL = 1928.0
+LATENT = 2.9e5
+HSMELT = 0.5 * ACPA * TS * TS + ACPB * TS
+HLCAL = HSMELT + 0.5 * (ACPA * TS + ACPB + ACPB) * (TL - TS)
+HLFRIZ = HLCAL + LATENT
+
+
+def _fmt(v):
+    return repr(float(v)) if not isinstance(v, str) else v
+
+
+def write_mesh(path, x, y, z):
+    """x/y/z: (lengths, ncv, exponents) per axis, one entry per zone."""
+    lines = []
+    for tag, (lens, ncv, expo) in (("X", x), ("Y", y), ("Z", z)):
+        lines.append("*%s_GRID_PARAMETERS" % tag)
+        lines.append("$ number of zones")
+        lines.append("  %d" % len(lens))
+        lines.append("$ zone lengths (m)")
+        lines.append("  " + " ".join(_fmt(v) for v in lens))
+        lines.append("$ control volumes per zone")
+        lines.append("  " + " ".join(str(int(v)) for v in ncv))
+        lines.append("$ exponents")
+        lines.append("  " + " ".join(_fmt(v) for v in expo))
+    with open(path, "w") as f:
+        f.write("\n".join(lines) + "\n")
+
+
+def write_deck(path, mesh_path, *, keyword="case", finaltime, maxit=100, tol=1e-6,
+               dt, lasertime, xstart, ystart, speed, power, eta=0.3, radius,
+               factor=2.0, discretization="power_law", flow=True, urf=(0.8,) * 5,
+               tamb=300.0, tpre=300.0, htc=(10.0, 10.0, 10.0, 10.0), emiss=0.4,
+               momentum=(5e-5, -2.6e-4, 1e7, -1.0), material=None, evaporation=None,
+               extra=()):
+    if material is None:
+        material = (4000.0, 4000.0, 5e-3, TS, TL, HSMELT, HLFRIZ, ACPA, ACPB, ACPB,
+                    0.0, 0.0, 25.0, 0.0, 25.0)
+    L = []
+    L += ["*KEYWORD_ID", keyword, "*DIRECTORY", keyword]
+    L += ["*PARAMETERS", "accelGravity 9.8", "radBoltzmann 5.67e-8",
+          "ambientTemp %s" % _fmt(tamb), "ambientPres 101.0e3", "gasConstant 8.314"]
+    L += ["*SCALAR_OUT", "temp"]
+    L += ["*CONTROL_TERMINATION", "$ total_time  max_iter  nonlin_tolerance",
+          "  %s %d %s" % (_fmt(finaltime), maxit, _fmt(tol))]
+    L += ["*CONTROL_TIMESTEP", "$ sim_timestep  output_timestep",
+          "  %s %s" % (_fmt(dt), _fmt(finaltime))]
+    L += ["*SINGLE_TRACK_TOOL", "$ lasertime  xstart  ystart  vel",
+          "  %s %s %s %s" % tuple(_fmt(v) for v in (lasertime, xstart, ystart, speed))]
+    L += ["*GAUSS_LASER_PARAMETERS", "$ power  absorb  radius  factor",
+          "  %s %s %s %s" % tuple(_fmt(v) for v in (power, eta, radius, factor))]
+    L += ["*VOLUMETRIC_HEAT_SOURCE", "$ power apowseta heatthick heatrb factor",
+          "  0.0 0.0 1e-5 5e-5 2.0"]
+    m = material
+    L += ["*MATERIAL_PROPERTIES", "$ dens denl viscos tsolid tliquid hsmelt hlfriz",
+          "  " + " ".join(_fmt(v) for v in m[0:7]), "$ acpa acpb acpl",
+          "  " + " ".join(_fmt(v) for v in m[7:10]), "$ thconsa thconsb thconsc thconla thconlb",
+          "  " + " ".join(_fmt(v) for v in m[10:15])]
+    L += ["*UNDER_RELAXATION", "  " + " ".join(_fmt(v) for v in urf)]
+    L += ["*BOUNDARY_TEMPERATURES", "  " + " ".join(_fmt(tpre) for _ in range(5))]
+    L += ["*BOUNDARY_ENERGY_EQUATION", "  " + " ".join(_fmt(v) for v in htc) + " " + _fmt(emiss)]
+    L += ["*BOUNDARY_MOMENTUM_EQUATION", "  " + " ".join(_fmt(v) for v in momentum)]
+    if discretization:
+        L += ["*SELECT_DISCRETIZATION", discretization]
+    if not flow:
+        L += ["*FLUID_FLOW_OFF"]
+    if evaporation is not None:
+        L += ["*BOUNDARY_EVAPORATION", "  " + " ".join(_fmt(v) for v in evaporation)]
+    L += list(extra)
+    L += ["*MESH_FILE", mesh_path]
+    with open(path, "w") as f:
+        f.write("\n".join(L) + "\n")
+
+
+CASES = {
+    "tiny": dict(
+        mesh=(([0.1e-3, 0.3e-3, 0.1e-3], [4, 12, 4], [-1.2, 1.0, 1.2]),
+              ([0.15e-3, 0.15e-3], [6, 6], [1.0, 1.0]),
+              ([0.15e-3, 0.1e-3], [4, 8], [1.0, 1.0])),
+        deck=dict(finaltime=0.2e-3 - 1e-9, maxit=100, dt=5e-6, lasertime=0.15e-3,
+                  xstart=0.15e-3, ystart=0.15e-3, speed=1.0, power=200.0, radius=50e-6)),
+    "tiny_cond": dict(
+        mesh=(([0.1e-3, 0.3e-3, 0.1e-3], [4, 12, 4], [-1.2, 1.0, 1.2]),
+              ([0.15e-3, 0.15e-3], [6, 6], [1.0, 1.0]),
+              ([0.15e-3, 0.1e-3], [4, 8], [1.0, 1.0])),
+        deck=dict(finaltime=0.1e-3 - 1e-9, maxit=40, dt=5e-6, lasertime=0.08e-3,
+                  xstart=0.15e-3, ystart=0.15e-3, speed=1.0, power=200.0, radius=50e-6,
+                  discretization=None, flow=False)),
+    "tiny_evap": dict(
+        mesh=(([0.1e-3, 0.3e-3, 0.1e-3], [4, 12, 4], [-1.2, 1.0, 1.2]),
+              ([0.15e-3, 0.15e-3], [6, 6], [1.0, 1.0]),
+              ([0.15e-3, 0.1e-3], [4, 8], [1.0, 1.0])),
+        deck=dict(finaltime=0.1e-3 - 1e-9, maxit=60, dt=5e-6, lasertime=0.08e-3,
+                  xstart=0.15e-3, ystart=0.15e-3, speed=1.0, power=250.0, radius=50e-6,
+                  momentum=(5e-5, -2.6e-4, 1e7, 0.02),
+                  evaporation=(6.34e6, 3133.0, 0.001, 0.0558))),
+    "plate120": dict(
+        mesh=(([0.5e-3, 1.0e-3, 0.5e-3], [20, 80, 20], [-1.2, 1.0, 1.2]),
+              ([0.2e-3, 0.2e-3, 0.2e-3], [16, 16, 16], [1.0, 1.0, 1.0]),
+              ([0.2e-3, 0.2e-3], [16, 16], [1.0, 1.0])),
+        deck=dict(finaltime=2.0e-3, maxit=100, dt=5e-6, lasertime=1.0e-3,
+                  xstart=0.5e-3, ystart=0.3e-3, speed=1.0, power=200.0, radius=50e-6)),
+    "track400": dict(
+        mesh=(([4.0e-3], [400], [1.0]), ([1.2e-3], [160], [1.0]), ([0.6e-3], [80], [1.0])),
+        deck=dict(finaltime=3.999e-3, maxit=100, dt=2e-6, lasertime=3.0e-3,
+                  xstart=0.4e-3, ystart=0.6e-3, speed=0.8, power=300.0, radius=60e-6)),
+}
+
+
+def write_case(name, outdir, **overrides):
+    """Write <outdir>/<name>.k and <outdir>/<name>_mesh.k; return the deck path."""
+    spec = CASES[name]
+    os.makedirs(outdir, exist_ok=True)
+    mesh_path = os.path.abspath(os.path.join(outdir, name + "_mesh.k"))
+    write_mesh(mesh_path, *spec["mesh"])
+    deck_path = os.path.join(outdir, name + ".k")
+    kw = dict(spec["deck"])
+    kw.update(overrides)
+    write_deck(deck_path, mesh_path, keyword=name, **kw)
+    return deck_path

@@ -1,0 +1,93 @@
+"""Write the golden fixtures under tests/golden/ from the compiled reference.
+
+Runs oracle/_ref/ref_harness (the reference solver classes compiled from
+their own sources by oracle/Makefile, driven in the reference executable's
+call order, OMP_NUM_THREADS=1) on the decks of decks.py and stores, per case:
+
+  <case>_scalars.csv     every per-step scalar the harness prints
+  <case>_digests.json    per step and field: SHA-256 of the raw float64
+                         bytes and the L2 norm
+  <case>_fields.npz      full fields at a few steps (tiny cases only)
+  <case>_deck.txt        the deck text (mesh path replaced by @MESH@)
+  <case>_mesh.txt        the mesh text
+
+Only data is stored; no reference source text is copied.  Run from the repo
+root:  python tests/golden/make_fixtures.py [case ...]
+"""
+import hashlib
+import json
+import os
+import shutil
+import subprocess
+import sys
+import tempfile
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, HERE)
+import decks  # noqa: E402
+
+HARNESS = os.path.join(ROOT, "oracle", "_ref", "ref_harness")
+FIELDS = ["enthalpy", "temp", "fracl", "uvel", "vvel", "wvel", "pressure", "pp", "rho", "diff",
+          "vis", "hnot", "fraclnot", "unot", "vnot", "wnot", "sourceinput", "dux", "dvy", "dwz"]
+# case -> (fields_every, max_steps, steps whose full fields are kept)
+PLAN = {
+    "tiny": (1, -1, [2, 40]),
+    "tiny_cond": (1, -1, []),
+    "tiny_evap": (1, -1, []),
+    "plate120": (50, -1, []),
+}
+
+
+def digest(a):
+    return hashlib.sha256(np.ascontiguousarray(a, dtype="<f8").tobytes()).hexdigest()
+
+
+def run_case(case, workdir=None):
+    every, max_steps, keep = PLAN[case]
+    tmp = workdir or tempfile.mkdtemp(prefix="amcfd_ref_")
+    deck = decks.write_case(case, tmp)
+    env = dict(os.environ, OMP_NUM_THREADS="1")
+    out = subprocess.run([HARNESS, os.path.basename(deck), ".", str(every), str(max_steps)],
+                         cwd=tmp, env=env, check=True, capture_output=True, text=True)
+    info = json.loads(out.stdout.strip().splitlines()[-1])
+    with open(os.path.join(tmp, "grid.txt")) as f:
+        ni, nj, nk = (int(v) for v in f.readline().split())
+    shape = (nk, nj, ni)
+    steps = sorted({int(n[1:6]) for n in os.listdir(tmp) if n.startswith("f") and n.endswith(".bin")})
+    dig = {"shape": list(shape), "steps": {}}
+    kept = {}
+    for s in steps:
+        dig["steps"][str(s)] = {}
+        for fld in FIELDS:
+            a = np.fromfile(os.path.join(tmp, "f%05d_%s.bin" % (s, fld)), dtype="<f8").reshape(shape)
+            dig["steps"][str(s)][fld] = {"sha256": digest(a), "l2": float(np.linalg.norm(a))}
+            if s in keep:
+                kept["s%d_%s" % (s, fld)] = a
+    shutil.copy(os.path.join(tmp, "scalars.csv"), os.path.join(HERE, case + "_scalars.csv"))
+    with open(os.path.join(HERE, case + "_digests.json"), "w") as f:
+        json.dump(dig, f, indent=0, sort_keys=True)
+    if kept:
+        np.savez_compressed(os.path.join(HERE, case + "_fields.npz"), **kept)
+    mesh = os.path.abspath(os.path.join(tmp, case + "_mesh.k"))
+    with open(deck) as f:
+        text = f.read().replace(mesh, "@MESH@")
+    with open(os.path.join(HERE, case + "_deck.txt"), "w") as f:
+        f.write(text)
+    shutil.copy(mesh, os.path.join(HERE, case + "_mesh.txt"))
+    info["case"] = case
+    info["harness"] = "oracle/_ref/ref_harness (OMP_NUM_THREADS=1)"
+    with open(os.path.join(HERE, case + "_run.json"), "w") as f:
+        json.dump(info, f, indent=1, sort_keys=True)
+    if workdir is None:
+        shutil.rmtree(tmp)
+    return info
+
+
+if __name__ == "__main__":
+    if not os.path.exists(HARNESS):
+        subprocess.run(["make", "-C", os.path.join(ROOT, "oracle"), "ref"], check=True)
+    for c in sys.argv[1:] or ["tiny", "tiny_cond", "tiny_evap"]:
+        print(run_case(c))
