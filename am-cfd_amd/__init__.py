@@ -148,7 +148,10 @@ class Deck:
         lib = load_library()
         self._h = ctypes.c_void_p()
         _check(lib.amcfd_deck_load(os.fsencode(path), ctypes.byref(self._h)))
-        self.params = lib.amcfd_deck_params(self._h).contents
+        # a copy of the struct; its table pointers still refer to this deck,
+        # so the copy keeps the deck alive
+        self.params = Params.from_buffer_copy(lib.amcfd_deck_params(self._h).contents)
+        self.params._deck = self
 
     def close(self):
         if self._h:

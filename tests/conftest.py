@@ -17,6 +17,15 @@ def pytest_configure(config):
     config.addinivalue_line("markers", "slow: long CPU run, excluded from the default suite")
 
 
+def pytest_collection_modifyitems(config, items):
+    if os.environ.get("AMCFD_SLOW") == "1":
+        return
+    skip = pytest.mark.skip(reason="long CPU run; set AMCFD_SLOW=1 to include")
+    for item in items:
+        if "slow" in item.keywords:
+            item.add_marker(skip)
+
+
 def amcfd_module():
     return importlib.import_module("am-cfd_amd")
 

@@ -45,14 +45,21 @@ def digest(a):
     return hashlib.sha256(np.ascontiguousarray(a, dtype="<f8").tobytes()).hexdigest()
 
 
-def run_case(case, workdir=None):
+def run_case(case, workdir=None, reuse=False):
+    """Run the harness for `case` (or, with reuse=True, ingest an earlier
+    harness run of the same deck found in `workdir`)."""
     every, max_steps, keep = PLAN[case]
     tmp = workdir or tempfile.mkdtemp(prefix="amcfd_ref_")
-    deck = decks.write_case(case, tmp)
-    env = dict(os.environ, OMP_NUM_THREADS="1")
-    out = subprocess.run([HARNESS, os.path.basename(deck), ".", str(every), str(max_steps)],
-                         cwd=tmp, env=env, check=True, capture_output=True, text=True)
-    info = json.loads(out.stdout.strip().splitlines()[-1])
+    if reuse:
+        deck = os.path.join(tmp, case + ".k")
+        with open(os.path.join(tmp, "run1.log")) as f:
+            info = json.loads(f.read().strip().splitlines()[-1])
+    else:
+        deck = decks.write_case(case, tmp)
+        env = dict(os.environ, OMP_NUM_THREADS="1")
+        out = subprocess.run([HARNESS, os.path.basename(deck), ".", str(every), str(max_steps)],
+                             cwd=tmp, env=env, check=True, capture_output=True, text=True)
+        info = json.loads(out.stdout.strip().splitlines()[-1])
     with open(os.path.join(tmp, "grid.txt")) as f:
         ni, nj, nk = (int(v) for v in f.readline().split())
     shape = (nk, nj, ni)
@@ -89,5 +96,10 @@ def run_case(case, workdir=None):
 if __name__ == "__main__":
     if not os.path.exists(HARNESS):
         subprocess.run(["make", "-C", os.path.join(ROOT, "oracle"), "ref"], check=True)
-    for c in sys.argv[1:] or ["tiny", "tiny_cond", "tiny_evap"]:
-        print(run_case(c))
+    args = sys.argv[1:]
+    if args[:1] == ["--reuse"]:
+        # --reuse <case> <dir>: ingest a finished harness run (run1.log, scalars.csv, f*.bin)
+        print(run_case(args[1], os.path.abspath(args[2]), reuse=True))
+    else:
+        for c in args or ["tiny", "tiny_cond", "tiny_evap"]:
+            print(run_case(c))
