@@ -19,34 +19,6 @@ __device__ __forceinline__ size_t cix(const Dims &d, int i, int j, int k) {
 }
 
 // ------------------------------------------------------------------------
-// Gaussian surface flux map + total absorbed power (one workgroup),
-// BoundCondManager.cpp:128-159
-// ------------------------------------------------------------------------
-__global__ __launch_bounds__(1024) void k_laser(Dims d, const double *__restrict__ x,
-                                                const double *__restrict__ y,
-                                                const double *__restrict__ areaij, double *heatin,
-                                                double beamx, double beamy, double peakhin, double alasfact,
-                                                double rb2, double varlas, int pi, DevScalars *sc) {
-    __shared__ double scratch[16];
-    const int tid = threadIdx.x, nt = blockDim.x;
-    const int n = d.ni * d.nj;
-    double acc = 0.0;
-    for (int c = tid; c < n; c += nt) {
-        int j = c / d.ni, i = c - j * d.ni;
-        double yd = beamy - y[j];
-        double xd = beamx - x[i];
-        double d2 = xd * xd + yd * yd;
-        double q = peakhin * exp(-alasfact * d2 / rb2) * (double)pi;
-        if (alasfact < 1.0e-6 && d2 < rb2) q = varlas / (M_PI * rb2) * (double)pi;
-        heatin[c] = q;
-        acc += areaij[c] * q;
-    }
-    double v[1] = {acc};
-    block_sum<1>(v, scratch);
-    if (tid == 0) sc->heatinlaser = v[0];
-}
-
-// ------------------------------------------------------------------------
 // enthalpy ghost values: top (laser - 2*radiation - evaporation), east/west
 // and north/south convection, BoundCondManager.cpp:565-651
 // ------------------------------------------------------------------------
@@ -735,14 +707,6 @@ __global__ __launch_bounds__(256) void k_end_step(size_t n, double tsolid, doubl
 // ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
-void launch_laser(hipStream_t st, const Dims &d, const DevGeom &g, DevFields f, Partials pr, double beamx,
-                  double beamy, double peakhin, double alasfact, double rb2, double varlas, int powerindicator,
-                  DevScalars *sc) {
-    (void)pr;
-    hipLaunchKernelGGL(k_laser, dim3(1), dim3(1024), 0, st, d, g.x, g.y, g.areaij, f.heatin, beamx, beamy,
-                       peakhin, alasfact, rb2, varlas, powerindicator, sc);
-}
-
 void launch_bc_enthalpy(hipStream_t st, const Dims &d, const DevGeom &g, DevFields f, Partials pr,
                         const amcfd_params &p) {
     BcConst k{p.sigm, p.emiss, p.htci, p.htcj, p.tempamb, p.pressamb, p.gasconst, p.evaplatent,

@@ -60,6 +60,7 @@ class Solver {
     Partials pr_{};
     DevScalars *sc_ = nullptr;       // device
     DevScalars *sc_host_ = nullptr;  // pinned mirror
+    double *heatin_host_ = nullptr;  // pinned staging of the laser map (+ total)
     hipStream_t st_ = nullptr;
     std::vector<void *> allocs_;
     SweepPlan plan_dom_, plan_box_;

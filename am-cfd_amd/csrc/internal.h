@@ -104,9 +104,6 @@ struct SweepPlan {
 // ---------------------------------------------------------------------------
 // kernel launchers (defined in the .hip files)
 // ---------------------------------------------------------------------------
-void launch_laser(hipStream_t st, const Dims &d, const DevGeom &g, DevFields f, Partials pr,
-                  double beamx, double beamy, double peakhin, double alasfact, double rb2, double varlas,
-                  int powerindicator, DevScalars *sc);
 void launch_bc_enthalpy(hipStream_t st, const Dims &d, const DevGeom &g, DevFields f, Partials pr,
                         const amcfd_params &p);
 void launch_energy_assemble(hipStream_t st, const Dims &d, const DevGeom &g, DevFields f, Partials pr,

@@ -25,7 +25,8 @@ namespace amcfd {
 Solver::~Solver() {
     if (st_) hipStreamSynchronize(st_);
     for (void *p : allocs_) hipFree(p);
-    if (sc_host_) hipHostFree(sc_host_);
+    if (sc_host_) (void)hipHostFree(sc_host_);
+    if (heatin_host_) (void)hipHostFree(heatin_host_);
     for (auto &e : ev_pool_) hipEventDestroy(e);
     if (st_) hipStreamDestroy(st_);
 }
@@ -136,6 +137,8 @@ int Solver::create(const amcfd_params &p, int mode) {
     HIPCALL(hipMemsetAsync(sc_, 0, sizeof(DevScalars), st_));
     HIPCALL(hipHostMalloc((void **)&sc_host_, sizeof(DevScalars), hipHostMallocDefault));
     std::memset(sc_host_, 0, sizeof(DevScalars));
+    HIPCALL(hipHostMalloc((void **)&heatin_host_, sizeof(double) * ((size_t)d_.ni * d_.nj + 1),
+                          hipHostMallocDefault));
     HIPCALL(hipMalloc(&v, sizeof(unsigned long long)));
     allocs_.push_back(v);
     counter_ = (unsigned long long *)v;
@@ -224,14 +227,33 @@ static int nearest(const std::vector<double> &c, int nlast, double pos) {
     return out;
 }
 
+// Gaussian surface flux map, BoundCondManager.cpp:128-159.  Evaluated on the
+// host once per step with the C library's exp (the same function the
+// reference calls, so the map is bitwise the reference's) and uploaded: the
+// map is ni*nj values per step, negligible next to the iterations.
 int Solver::launch_laser_map() {
     istart_ = nearest(hg_.x, d_.nim1, beamx_);
     jstart_ = nearest(hg_.y, d_.njm1, beamy_);
     const double rb2 = p_.alasrb * p_.alasrb;
     const double varlas = p_.alaspow * p_.alaseta;
     const double peakhin = p_.alasfact * varlas / (M_PI * rb2);
-    launch_laser(st_, d_, g_, f_, pr_, beamx_, beamy_, peakhin, p_.alasfact, rb2, varlas, powerindicator_, sc_);
-    HIPCALL(hipGetLastError());
+    const double pi = (double)powerindicator_;
+    double total = 0.0;
+    for (int j = 0; j < d_.nj; ++j) {
+        const double yd = beamy_ - hg_.y[j];
+        for (int i = 0; i < d_.ni; ++i) {
+            const double xd = beamx_ - hg_.x[i];
+            const double d2 = xd * xd + yd * yd;
+            double q = peakhin * std::exp(-p_.alasfact * d2 / rb2) * pi;
+            if (p_.alasfact < 1.0e-6 && d2 < rb2) q = varlas / (M_PI * rb2) * pi;
+            heatin_host_[j * d_.ni + i] = q;
+            total += hg_.areaij[j * d_.ni + i] * q;
+        }
+    }
+    heatin_host_[(size_t)d_.ni * d_.nj] = total;
+    HIPCALL(hipMemcpyAsync(f_.heatin, heatin_host_, sizeof(double) * d_.ni * d_.nj, hipMemcpyHostToDevice, st_));
+    HIPCALL(hipMemcpyAsync(&sc_->heatinlaser, heatin_host_ + (size_t)d_.ni * d_.nj, sizeof(double),
+                           hipMemcpyHostToDevice, st_));
     return AMCFD_OK;
 }
 

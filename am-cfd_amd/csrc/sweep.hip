@@ -20,10 +20,16 @@
 // line keeps a version word (epoch + passes done); waits are bounded and raise
 // an error flag instead of hanging.
 //
-// A line is solved by the whole workgroup with parallel cyclic reduction in
-// LDS.  On the melt-pool box the reference guards |pivot| <= 1e-12; when a
-// line is not provably free of such pivots the workgroup falls back to the
-// reference's sequential Thomas recurrence with the guard.
+// Line solvers:
+//   EXACT  the reference's pr/qr recurrence, run by one lane from LDS after
+//          the wave has staged the line; bitwise the reference's arithmetic
+//          (the SIMPLE iteration amplifies last-bit differences by ~1e7 per
+//          step, so the plain FP64 mode keeps every bit).
+//   PCR    parallel cyclic reduction by the whole workgroup; same system,
+//          different rounding (for modes that do not promise bitwise
+//          agreement).  On the melt-pool box the reference guards
+//          |pivot| <= 1e-12; a line that is not provably free of such pivots
+//          falls back to the exact recurrence.
 
 #include "fv.h"
 
@@ -31,9 +37,7 @@ namespace amcfd {
 
 namespace {
 
-constexpr int SW_THREADS = 256;
 constexpr int SW_MAXN = 1024;                 // longest x-line handled
-constexpr int SW_MAXR = SW_MAXN / SW_THREADS;
 constexpr long long SPIN_LIMIT = 1LL << 20;   // bounded wait (~ a second)
 
 struct Coefs {
@@ -56,12 +60,34 @@ __device__ bool wait_version(const int *v, int need, const int *error) {
     return true;
 }
 
-__global__ __launch_bounds__(SW_THREADS) void k_line_sweeps(Dims d, Coefs cf, double *phi, const int *__restrict__ tasks,
-                                                            int ntasks, int ibeg, int iend, int jbeg, int jend,
-                                                            int kbeg, int kend, int guard,
-                                                            unsigned long long *counter,
-                                                            unsigned long long base, int *versions, int epoch,
-                                                            DevScalars *sc) {
+// Reference recurrence on LDS rows (A = -aw, B = ap, C = -ae, D = rhs);
+// the solution overwrites D.  Run by a single lane.
+__device__ void thomas_exact(int n, double *A, const double *B, double *C, double *D, double lo, double hi,
+                             int guard) {
+    double prv = 0.0, qrv = lo;
+    for (int ii = 0; ii < n; ++ii) {
+        const double aw = -A[ii];
+        double den = B[ii] - aw * prv;
+        if (guard && fabs(den) <= 1e-12) den += AMCFD_SMALL;
+        prv = -C[ii] / den;
+        qrv = (D[ii] + aw * qrv) / den;
+        A[ii] = prv;
+        C[ii] = qrv;
+    }
+    double nxt = hi;
+    for (int ii = n - 1; ii >= 0; --ii) {
+        nxt = A[ii] * nxt + C[ii];
+        D[ii] = nxt;
+    }
+}
+
+template <int NT, bool PCR>
+__global__ __launch_bounds__(NT) void k_line_sweeps(Dims d, Coefs cf, double *phi, const int *__restrict__ tasks,
+                                                    int ntasks, int ibeg, int iend, int jbeg, int jend, int kbeg,
+                                                    int kend, int guard, unsigned long long *counter,
+                                                    unsigned long long base, int *versions, int epoch,
+                                                    DevScalars *sc) {
+    constexpr int MAXR = SW_MAXN / NT;
     __shared__ double A[SW_MAXN], B[SW_MAXN], C[SW_MAXN], D[SW_MAXN];
     __shared__ int s_task, s_ok;
     const int tid = threadIdx.x;
@@ -79,11 +105,11 @@ __global__ __launch_bounds__(SW_THREADS) void k_line_sweeps(Dims d, Coefs cf, do
         const int pass = packed >> 28, k = (packed >> 14) & 0x3fff, j = packed & 0x3fff;
         const int ks = pass >> 1;
         const size_t row = (size_t)k * d.sk + (size_t)j * d.sj;
-        // coefficients do not depend on other lines: load before waiting
-        double rat[SW_MAXR], rab[SW_MAXR], ran[SW_MAXR], ras[SW_MAXR], rsu[SW_MAXR];
+        // coefficients do not depend on other lines: stage them before waiting
+        double rat[MAXR], rab[MAXR], ran[MAXR], ras[MAXR], rsu[MAXR];
 #pragma unroll
-        for (int r = 0; r < SW_MAXR; ++r) {
-            const int ii = tid + r * SW_THREADS;
+        for (int r = 0; r < MAXR; ++r) {
+            const int ii = tid + r * NT;
             if (ii < n) {
                 const size_t c = row + ibeg + ii;
                 rat[r] = cf.at[c]; rab[r] = cf.ab[c]; ran[r] = cf.an[c]; ras[r] = cf.as[c]; rsu[r] = cf.su[c];
@@ -94,8 +120,8 @@ __global__ __launch_bounds__(SW_THREADS) void k_line_sweeps(Dims d, Coefs cf, do
         }
         if (tid == 0) {
             bool ok = true;
-            int *vb = versions;
             const int *err = &sc->error;
+            const int *vb = versions;
             if (j > jbeg) ok &= wait_version(vb + k * d.nj + (j - 1), epoch + pass + 1, err);
             if (j < jend && pass > 0) ok &= wait_version(vb + k * d.nj + (j + 1), epoch + pass, err);
             if (k < kend) ok &= wait_version(vb + (k + 1) * d.nj + j, epoch + 2 * ks + 2, err);
@@ -106,23 +132,24 @@ __global__ __launch_bounds__(SW_THREADS) void k_line_sweeps(Dims d, Coefs cf, do
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
         // right-hand side from the neighbouring lines, in the reference's order
 #pragma unroll
-        for (int r = 0; r < SW_MAXR; ++r) {
-            const int ii = tid + r * SW_THREADS;
+        for (int r = 0; r < MAXR; ++r) {
+            const int ii = tid + r * NT;
             if (ii < n) {
                 const size_t c = row + ibeg + ii;
                 D[ii] = rat[r] * phi[c + d.sk] + rab[r] * phi[c - d.sk] + ran[r] * phi[c + d.sj] +
                         ras[r] * phi[c - d.sj] + rsu[r];
             }
         }
-        if (tid == 0) s_ok = 1;
+        const double lo = phi[row + ibeg - 1], hi = phi[row + iend + 1];
+        if (tid == 0) s_ok = PCR ? 1 : 0;
         __syncthreads();
-        if (guard) {
-            // PCR is used only when no Thomas pivot can fall to 1e-12:
-            // nonnegative neighbours, ap >= aw + ae and ap - aw > 2e-12
+        if (PCR && guard) {
+            // PCR only when no Thomas pivot can fall to 1e-12: nonnegative
+            // neighbours, ap >= aw + ae and ap - aw > 2e-12
             bool ok = true;
 #pragma unroll
-            for (int r = 0; r < SW_MAXR; ++r) {
-                const int ii = tid + r * SW_THREADS;
+            for (int r = 0; r < MAXR; ++r) {
+                const int ii = tid + r * NT;
                 if (ii < n) {
                     double aw = -A[ii], ae = -C[ii], ap = B[ii];
                     ok &= (aw >= 0.0) && (ae >= 0.0) && (ap - aw > 2e-12) && (ap >= (aw + ae) * (1.0 - 1e-12));
@@ -131,38 +158,23 @@ __global__ __launch_bounds__(SW_THREADS) void k_line_sweeps(Dims d, Coefs cf, do
             if (!ok) s_ok = 0;
             __syncthreads();
         }
-        if (s_ok) {
-            // fixed end values fold into the first / last rows
-            if (tid == 0) {
-                D[0] -= A[0] * phi[row + ibeg - 1];
-                D[n - 1] -= C[n - 1] * phi[row + iend + 1];
+        if (PCR && s_ok) {
+            if (tid == 0) {   // fixed end values fold into the first / last rows
+                D[0] -= A[0] * lo;
+                D[n - 1] -= C[n - 1] * hi;
                 A[0] = 0.0;
                 C[n - 1] = 0.0;
             }
             __syncthreads();
-            pcr_solve<SW_MAXR>(n, A, B, C, D);
+            pcr_solve<MAXR>(n, A, B, C, D);
+        } else {
+            if (tid == 0) thomas_exact(n, A, B, C, D, lo, hi, guard);
+            __syncthreads();
+        }
 #pragma unroll
-            for (int r = 0; r < SW_MAXR; ++r) {
-                const int ii = tid + r * SW_THREADS;
-                if (ii < n) phi[row + ibeg + ii] = D[ii];
-            }
-        } else if (tid == 0) {
-            // reference recurrence (CFDSolverManager.cpp:2424-2449), pr in A, qr in C
-            double prv = 0.0, qrv = phi[row + ibeg - 1];
-            for (int ii = 0; ii < n; ++ii) {
-                double aw = -A[ii];
-                double den = B[ii] - aw * prv;
-                if (fabs(den) <= 1e-12) den += AMCFD_SMALL;
-                prv = -C[ii] / den;
-                qrv = (D[ii] + aw * qrv) / den;
-                A[ii] = prv;
-                C[ii] = qrv;
-            }
-            double nxt = phi[row + iend + 1];
-            for (int ii = n - 1; ii >= 0; --ii) {
-                nxt = A[ii] * nxt + C[ii];
-                phi[row + ibeg + ii] = nxt;
-            }
+        for (int r = 0; r < MAXR; ++r) {
+            const int ii = tid + r * NT;
+            if (ii < n) phi[row + ibeg + ii] = D[ii];
         }
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
         __syncthreads();
@@ -202,9 +214,20 @@ void launch_line_sweeps(hipStream_t st, const Dims &d, DevFields f, double *phi,
                         int *versions, int epoch, DevScalars *sc, int nblocks) {
     if (plan.ntasks == 0 || iend < ibeg) return;
     Coefs cf{f.an, f.as, f.ae, f.aw, f.at, f.ab, f.ap, f.su};
-    hipLaunchKernelGGL(k_line_sweeps, dim3(nblocks), dim3(SW_THREADS), 0, st, d, cf, phi, plan.tasks, plan.ntasks,
-                       ibeg, iend, plan.jbeg, plan.jend, plan.kbeg, plan.kend, guard, counter, counter_base,
-                       versions, epoch, sc);
+    hipLaunchKernelGGL((k_line_sweeps<64, false>), dim3(nblocks), dim3(64), 0, st, d, cf, phi, plan.tasks,
+                       plan.ntasks, ibeg, iend, plan.jbeg, plan.jend, plan.kbeg, plan.kend, guard, counter,
+                       counter_base, versions, epoch, sc);
+}
+
+void launch_line_sweeps_pcr(hipStream_t st, const Dims &d, DevFields f, double *phi, const SweepPlan &plan,
+                            int ibeg, int iend, int guard, unsigned long long *counter,
+                            unsigned long long counter_base, int *versions, int epoch, DevScalars *sc,
+                            int nblocks) {
+    if (plan.ntasks == 0 || iend < ibeg) return;
+    Coefs cf{f.an, f.as, f.ae, f.aw, f.at, f.ab, f.ap, f.su};
+    hipLaunchKernelGGL((k_line_sweeps<256, true>), dim3(nblocks), dim3(256), 0, st, d, cf, phi, plan.tasks,
+                       plan.ntasks, ibeg, iend, plan.jbeg, plan.jend, plan.kbeg, plan.kend, guard, counter,
+                       counter_base, versions, epoch, sc);
 }
 
 }  // namespace amcfd
