@@ -109,14 +109,14 @@ __global__ __launch_bounds__(TX *TY) void k_energy_assemble(
     const double *__restrict__ wvel, const double *__restrict__ rho, const double *__restrict__ diff,
     const double *__restrict__ hnot, const double *__restrict__ fracl, const double *__restrict__ fraclnot,
     const double *__restrict__ h, double *an_o, double *as_o, double *ae_o, double *aw_o, double *at_o,
-    double *ab_o, double *ap_o, double *su_o, double *src_o, double *part_res, double *part_blk) {
-    __shared__ double red[4][TY][TX];
+    double *ab_o, double *ap_o, double *su_o, double *src_o, double *part_res, double *term_bl,
+    double *term_blc) {
     __shared__ double scratch[2 * 4];
     const int tx = threadIdx.x, ty = threadIdx.y;
     const int i = 1 + blockIdx.x * TX + tx;
     const int j = 1 + blockIdx.y * TY + ty;
     const bool col_ok = (i <= d.nim2) && (j <= d.njm2);
-    double sumd = 0.0, sumh = 0.0, bl = 0.0, blp = 0.0, blm = 0.0, blc = 0.0;
+    double sumd = 0.0, sumh = 0.0;
     const int sch = q.scheme;
     const double ztop = g.z[d.nkm1];
     if (col_ok) {
@@ -242,27 +242,14 @@ __global__ __launch_bounds__(TX *TY) void k_energy_assemble(
             double r = (an * hN + as * hS + ae * hE + aw * hW + at * hT + ab * hB + su) / ap - hP;
             sumd += fabs(r);
             sumh += fabs(hP);
-            // block-correction column sums
-            bl += ap - an - as - at - ab;
-            blp += ae;
-            blm += aw;
-            blc += ae * hE + aw * hW + an * hN + as * hS + at * hT + ab * hB + su - ap * hP;
+            // per-cell terms of the block-correction column sums (summed per
+            // x-column in the reference's (k, j) order by k_column_sums)
+            term_bl[c] = ap - an - as - at - ab;
+            term_blc[c] = ae * hE + aw * hW + an * hN + as * hS + at * hT + ab * hB + su - ap * hP;
         }
     }
-    red[0][ty][tx] = bl;
-    red[1][ty][tx] = blp;
-    red[2][ty][tx] = blm;
-    red[3][ty][tx] = blc;
     double v[2] = {sumd, sumh};
     block_sum<2>(v, scratch);   // contains __syncthreads
-    const int row = blockIdx.y + gridDim.y * blockIdx.z;
-    if (ty == 0 && i <= d.nim2) {
-#pragma unroll
-        for (int qq = 0; qq < 4; ++qq) {
-            double s = red[qq][0][tx] + red[qq][1][tx] + red[qq][2][tx] + red[qq][3][tx];
-            part_blk[((size_t)row * 4 + qq) * d.ni + i] = s;
-        }
-    }
     if (tx == 0 && ty == 0) {
         const int b = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
         part_res[2 * b] = v[0];
@@ -321,37 +308,79 @@ __global__ __launch_bounds__(256) void k_props(size_t n, PropConst c, const doub
 // (second kernel) add the correction, CFDSolverManager.cpp:2270-2362.
 // Also folds the energy residual, CFDSolverManager.cpp:2099.
 // ------------------------------------------------------------------------
-constexpr int BC_THREADS = 1024, BC_MAXN = 4096;
-
-__global__ __launch_bounds__(BC_THREADS) void k_block_solve(Dims d, int nrow, int nres,
-                                                            const double *__restrict__ part_blk,
-                                                            const double *__restrict__ part_res,
-                                                            double *delphi, DevScalars *sc) {
-    extern __shared__ double lds[];
-    const int n = d.nim2;   // unknowns i = 1..nim2
-    double *a = lds, *b = lds + n, *c = lds + 2 * n, *r = lds + 3 * n;
-    const int tid = threadIdx.x;
-    for (int ii = tid; ii < n; ii += blockDim.x) {
-        const int i = ii + 1;
-        double bl = 0, blp = 0, blm = 0, blc = 0;
-        for (int row = 0; row < nrow; ++row) {
-            const double *p = part_blk + (size_t)row * 4 * d.ni;
-            bl += p[i];
-            blp += p[d.ni + i];
-            blm += p[2 * d.ni + i];
-            blc += p[3 * d.ni + i];
+// Column sums bl, blp, blm, blc for every x index i, each accumulated by one
+// lane in the reference's order (k outer, j inner).  Floating-point sums are
+// order dependent and the SIMPLE iteration is chaotic enough that a
+// reordered sum changes the trajectory after a few dozen steps, so the order
+// is kept; the lanes of a wave read consecutive i, so every load is a
+// coalesced row segment.  Output: blk[q*ni + i], q = bl, blp, blm, blc.
+__global__ __launch_bounds__(64) void k_column_sums(Dims d, const double *__restrict__ term_bl,
+                                                    const double *__restrict__ ae,
+                                                    const double *__restrict__ aw,
+                                                    const double *__restrict__ term_blc, double *blk) {
+    const int i = 1 + blockIdx.x * 64 + threadIdx.x;
+    if (i > d.nim2) return;
+    double bl = 0.0, blp = 0.0, blm = 0.0, blc = 0.0;
+    constexpr int U = 8;
+    for (int k = 1; k <= d.nkm2; ++k) {
+        const size_t base = (size_t)k * d.sk + (size_t)i;
+        int j = 1;
+        for (; j + U - 1 <= d.njm2; j += U) {
+            double t1[U], t2[U], t3[U], t4[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const size_t c = base + (size_t)(j + u) * d.sj;
+                t1[u] = term_bl[c];
+                t2[u] = ae[c];
+                t3[u] = aw[c];
+                t4[u] = term_blc[c];
+            }
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                bl += t1[u];
+                blp += t2[u];
+                blm += t3[u];
+                blc += t4[u];
+            }
         }
-        // bl*dphi[i] - blm*dphi[i-1] - blp*dphi[i+1] = blc; the reference
-        // recurrence drops blm at i=1 and blp at i=nim2
-        a[ii] = (ii == 0) ? 0.0 : -blm;
-        b[ii] = bl;
-        c[ii] = (ii == n - 1) ? 0.0 : -blp;
-        r[ii] = blc;
+        for (; j <= d.njm2; ++j) {
+            const size_t c = base + (size_t)j * d.sj;
+            bl += term_bl[c];
+            blp += ae[c];
+            blm += aw[c];
+            blc += term_blc[c];
+        }
     }
-    __syncthreads();
-    pcr_solve<BC_MAXN / BC_THREADS>(n, a, b, c, r);
-    for (int ii = tid; ii < n; ii += blockDim.x) delphi[ii + 1] = r[ii];
+    blk[i] = bl;
+    blk[d.ni + i] = blp;
+    blk[2 * d.ni + i] = blm;
+    blk[3 * d.ni + i] = blc;
+}
+
+// 1-D correction along x by the reference's recurrence (one lane, exact
+// order), CFDSolverManager.cpp:2334-2349; also folds the energy residual
+// partials, CFDSolverManager.cpp:2099.
+__global__ __launch_bounds__(64) void k_block_solve(Dims d, int nres, const double *__restrict__ blk,
+                                                    const double *__restrict__ part_res, double *work,
+                                                    double *delphi, DevScalars *sc) {
+    const int tid = threadIdx.x;
     if (tid == 0) {
+        const double *bl = blk, *blp = blk + d.ni, *blm = blk + 2 * d.ni, *blc = blk + 3 * d.ni;
+        double *pib = work, *qib = work + d.ni;
+        pib[1] = blp[1] / bl[1];
+        qib[1] = blc[1] / bl[1];
+        for (int i = 2; i < d.nim1; ++i) {
+            double den = bl[i] - blm[i] * pib[i - 1];
+            pib[i] = blp[i] / den;
+            qib[i] = (blc[i] + blm[i] * qib[i - 1]) / den;
+        }
+        double nxt = qib[d.nim2];
+        delphi[d.nim2] = nxt;
+        for (int i = d.nim2 - 1; i > 0; --i) {
+            nxt = pib[i] * nxt + qib[i];
+            delphi[i] = nxt;
+        }
+    } else if (tid == 32) {
         double sd = 0.0, sh = 0.0;
         for (int bidx = 0; bidx < nres; ++bidx) {
             sd += part_res[2 * bidx];
@@ -727,7 +756,7 @@ void launch_energy_assemble(hipStream_t st, const Dims &d, const DevGeom &g, Dev
                s.beamposx, s.beamposy, s.powerindicator, p.discretization};
     hipLaunchKernelGGL(k_energy_assemble, energy_grid(d), dim3(TX, TY), 0, st, d, q, g, f.uvel, f.vvel, f.wvel,
                        f.rho, f.diff, f.hnot, f.fracl, f.fraclnot, f.enthalpy, f.an, f.as, f.ae, f.aw, f.at,
-                       f.ab, f.ap, f.su, f.sourceinput, pr.energy, pr.blk);
+                       f.ab, f.ap, f.su, f.sourceinput, pr.energy, f.sp, f.apnot);
 }
 
 void launch_props(hipStream_t st, const Dims &d, DevFields f, const amcfd_params &p) {
@@ -740,10 +769,11 @@ void launch_props(hipStream_t st, const Dims &d, DevFields f, const amcfd_params
 }
 
 void launch_block_correction(hipStream_t st, const Dims &d, DevFields f, Partials pr, DevScalars *sc) {
-    size_t lds = sizeof(double) * 4 * (size_t)d.nim2;
-    const dim3 e = energy_grid(d);   // partial counts of the active grid
-    hipLaunchKernelGGL(k_block_solve, dim3(1), dim3(BC_THREADS), lds, st, d, (int)(e.y * e.z),
-                       (int)(e.x * e.y * e.z), pr.blk, pr.energy, pr.delphi, sc);
+    const dim3 e = energy_grid(d);   // residual partials of the active grid
+    hipLaunchKernelGGL(k_column_sums, dim3((d.nim2 + 63) / 64), dim3(64), 0, st, d, f.sp, f.ae, f.aw, f.apnot,
+                       pr.blk);
+    hipLaunchKernelGGL(k_block_solve, dim3(1), dim3(64), 0, st, d, (int)(e.x * e.y * e.z), pr.blk,
+                       pr.energy, pr.blk + 4 * (size_t)d.ni, pr.delphi, sc);
     hipLaunchKernelGGL(k_add_delphi, energy_grid(d), dim3(64, 4), 0, st, d, pr.delphi, f.enthalpy);
 }
 

@@ -59,6 +59,7 @@ struct DevFields {
     double *hnot, *fraclnot, *unot, *vnot, *wnot, *sourceinput, *dux, *dvy, *dwz;
     double *csfrac, *solfrac, *mpphase, *mpliqphase;
     double *ap, *an, *as, *ae, *aw, *at, *ab, *su, *sp, *apnot;
+    double *lpr, *lden, *ly;   // line-sweep pivots: pr, pivot, 1/pivot
     double *heatin;   // [nj*ni]
 };
 

@@ -108,7 +108,7 @@ int Solver::create(const amcfd_params &p, int mode) {
                      &f_.rho, &f_.diff, &f_.vis, &f_.hnot, &f_.fraclnot, &f_.unot, &f_.vnot, &f_.wnot,
                      &f_.sourceinput, &f_.dux, &f_.dvy, &f_.dwz, &f_.csfrac, &f_.solfrac, &f_.mpphase,
                      &f_.mpliqphase, &f_.ap, &f_.an, &f_.as, &f_.ae, &f_.aw, &f_.at, &f_.ab, &f_.su, &f_.sp,
-                     &f_.apnot};
+                     &f_.apnot, &f_.lpr, &f_.lden, &f_.ly};
     for (auto *q : fl)
         if ((st = alloc(q, nn))) return st;
     if ((st = alloc(&f_.heatin, (size_t)d_.ni * d_.nj))) return st;
@@ -123,7 +123,8 @@ int Solver::create(const amcfd_params &p, int mode) {
     Box whole{1, d_.nim2, 1, d_.njm2, 2};
     int nf = flow_partial_blocks(whole, d_) + 64;
     if ((st = alloc(&pr_.energy, (size_t)2 * pr_.nblk_e))) return st;
-    if ((st = alloc(&pr_.blk, (size_t)pr_.nrow_e * 4 * d_.ni))) return st;
+    // block correction: 4 column sums + 2 recurrence work rows, each ni long
+    if ((st = alloc(&pr_.blk, (size_t)6 * d_.ni + 8))) return st;
     if ((st = alloc(&pr_.h2t, (size_t)10 * pr_.nblk_t))) return st;
     if ((st = alloc(&pr_.flow, (size_t)4 * nf * 3))) return st;
     pr_.nblk_b = (d_.njm2 * d_.nim2 + d_.nkm2 * d_.njm2 + d_.nkm2 * d_.nim2 + 255) / 256;
