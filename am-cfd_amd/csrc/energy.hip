@@ -318,75 +318,59 @@ __global__ __launch_bounds__(64) void k_column_sums(Dims d, const double *__rest
                                                     const double *__restrict__ ae,
                                                     const double *__restrict__ aw,
                                                     const double *__restrict__ term_blc, double *blk) {
+    // one wave per (64 columns, quantity): four independent streams, each
+    // with U row loads in flight
+    constexpr int U = 32;
+    const int q = blockIdx.y;
+    const double *src = q == 0 ? term_bl : q == 1 ? ae : q == 2 ? aw : term_blc;
     const int i = 1 + blockIdx.x * 64 + threadIdx.x;
-    if (i > d.nim2) return;
-    double bl = 0.0, blp = 0.0, blm = 0.0, blc = 0.0;
-    constexpr int U = 8;
-    for (int k = 1; k <= d.nkm2; ++k) {
-        const size_t base = (size_t)k * d.sk + (size_t)i;
-        int j = 1;
-        for (; j + U - 1 <= d.njm2; j += U) {
-            double t1[U], t2[U], t3[U], t4[U];
+    const bool ok = i <= d.nim2;
+    const int R = d.njm2 * d.nkm2;
+    double s = 0.0;
+    int kc = 1, jc = 1;   // (k, j) of the chunk's first row
+    for (int r0 = 0; r0 < R; r0 += U) {
+        double t[U];
+        int k = kc, j = jc;
 #pragma unroll
-            for (int u = 0; u < U; ++u) {
-                const size_t c = base + (size_t)(j + u) * d.sj;
-                t1[u] = term_bl[c];
-                t2[u] = ae[c];
-                t3[u] = aw[c];
-                t4[u] = term_blc[c];
-            }
-#pragma unroll
-            for (int u = 0; u < U; ++u) {
-                bl += t1[u];
-                blp += t2[u];
-                blm += t3[u];
-                blc += t4[u];
+        for (int u = 0; u < U; ++u) {
+            t[u] = (ok && r0 + u < R) ? src[(size_t)k * d.sk + (size_t)j * d.sj + i] : 0.0;
+            if (++j > d.njm2) {
+                j = 1;
+                ++k;
             }
         }
-        for (; j <= d.njm2; ++j) {
-            const size_t c = base + (size_t)j * d.sj;
-            bl += term_bl[c];
-            blp += ae[c];
-            blm += aw[c];
-            blc += term_blc[c];
-        }
+        kc = k;
+        jc = j;
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+            if (r0 + u < R) s += t[u];
     }
-    blk[i] = bl;
-    blk[d.ni + i] = blp;
-    blk[2 * d.ni + i] = blm;
-    blk[3 * d.ni + i] = blc;
+    if (ok) blk[(size_t)q * d.ni + i] = s;
 }
 
 // 1-D correction along x by the reference's recurrence (one lane, exact
 // order), CFDSolverManager.cpp:2334-2349; also folds the energy residual
 // partials, CFDSolverManager.cpp:2099.
-__global__ __launch_bounds__(64) void k_block_solve(Dims d, int nres, const double *__restrict__ blk,
-                                                    const double *__restrict__ part_res, double *work,
-                                                    double *delphi, DevScalars *sc) {
-    const int tid = threadIdx.x;
-    if (tid == 0) {
-        const double *bl = blk, *blp = blk + d.ni, *blm = blk + 2 * d.ni, *blc = blk + 3 * d.ni;
-        double *pib = work, *qib = work + d.ni;
-        pib[1] = blp[1] / bl[1];
-        qib[1] = blc[1] / bl[1];
-        for (int i = 2; i < d.nim1; ++i) {
-            double den = bl[i] - blm[i] * pib[i - 1];
-            pib[i] = blp[i] / den;
-            qib[i] = (blc[i] + blm[i] * qib[i - 1]) / den;
-        }
-        double nxt = qib[d.nim2];
-        delphi[d.nim2] = nxt;
-        for (int i = d.nim2 - 1; i > 0; --i) {
-            nxt = pib[i] * nxt + qib[i];
-            delphi[i] = nxt;
-        }
-    } else if (tid == 32) {
-        double sd = 0.0, sh = 0.0;
-        for (int bidx = 0; bidx < nres; ++bidx) {
-            sd += part_res[2 * bidx];
-            sh += part_res[2 * bidx + 1];
-        }
-        sc->resorh = sd / (sh + AMCFD_SMALL);
+__global__ __launch_bounds__(64) void k_block_solve(Dims d, const double *__restrict__ blk, double *work,
+                                                    double *delphi) {
+    if (threadIdx.x != 0) return;
+    const double *bl = blk, *blp = blk + d.ni, *blm = blk + 2 * d.ni, *blc = blk + 3 * d.ni;
+    double *pib = work, *qib = work + d.ni;
+    double pv = blp[1] / bl[1], qv = blc[1] / bl[1];
+    pib[1] = pv;
+    qib[1] = qv;
+    for (int i = 2; i < d.nim1; ++i) {   // the running values stay in registers
+        const double den = bl[i] - blm[i] * pv;
+        pv = blp[i] / den;
+        qv = (blc[i] + blm[i] * qv) / den;
+        pib[i] = pv;
+        qib[i] = qv;
+    }
+    double nxt = qv;
+    delphi[d.nim2] = nxt;
+    for (int i = d.nim2 - 1; i > 0; --i) {
+        nxt = pib[i] * nxt + qib[i];
+        delphi[i] = nxt;
     }
 }
 
@@ -650,53 +634,79 @@ __global__ __launch_bounds__(1024) void k_melt_pool(Dims d, int nblk, const doub
 // end-of-iteration scalars: residual folds, heat balance, energy ratio
 // (CFDSolverManager.cpp:1989-2071, BoundCondManager.cpp:846-857)
 // ------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_finalize(int nblk_t, int nblk_b, int nblk_f, const double *__restrict__ part_t,
+__global__ __launch_bounds__(256) void k_finalize(int nblk_t, int nblk_b, int nblk_f, int nblk_e,
+                                                  const double *__restrict__ part_t,
                                                   const double *__restrict__ part_b,
-                                                  const double *__restrict__ part_f, int flow_done, int steady,
+                                                  const double *__restrict__ part_f,
+                                                  const double *__restrict__ part_e, int flow_done, int steady,
                                                   double dens, double width, DevScalars *sc) {
-    __shared__ double s_acc[9];
-    const int tid = threadIdx.x;
-    if (tid < 8) {
-        double s = 0.0;
-        for (int b = 0; b < nblk_t; ++b) s += part_t[(size_t)b * 10 + 1 + tid];
-        s_acc[tid] = s;
-    } else if (tid == 8) {
-        double s = 0.0;
-        for (int b = 0; b < nblk_b; ++b) s += part_b[b];
-        s_acc[8] = s;
+    // every quantity here only feeds the convergence test, so the partials
+    // are folded by a parallel tree (fixed order: still reproducible)
+    constexpr int NQ = 15;   // 8 heat-balance, toploss, 2 energy residual, 4 flow (+ umax)
+    __shared__ double s_red[NQ + 1][4];
+    const int tid = threadIdx.x, nt = blockDim.x;
+    double a[NQ + 1];
+#pragma unroll
+    for (int q = 0; q <= NQ; ++q) a[q] = 0.0;
+    for (int b = tid; b < nblk_t; b += nt)
+#pragma unroll
+        for (int q = 0; q < 8; ++q) a[q] += part_t[(size_t)b * 10 + 1 + q];
+    for (int b = tid; b < nblk_b; b += nt) a[8] += part_b[b];
+    for (int b = tid; b < nblk_e; b += nt) {
+        a[9] += part_e[2 * b];
+        a[10] += part_e[2 * b + 1];
+    }
+    if (flow_done) {
+        for (int b = tid; b < nblk_f; b += nt) {
+            const double *pu = part_f + (size_t)b * 3, *pv = part_f + ((size_t)nblk_f + b) * 3;
+            const double *pw = part_f + ((size_t)2 * nblk_f + b) * 3, *pp = part_f + ((size_t)3 * nblk_f + b) * 3;
+            a[11] += pu[0];
+            a[15] = dmax(a[15], pu[1]);
+            a[12] += pv[0];
+            a[13] += pw[0];
+            a[14] += pp[1];           // continuity normaliser
+        }
+    }
+    double mass = 0.0;   // sum of |mass imbalance| (pressure-correction partials)
+    if (flow_done)
+        for (int b = tid; b < nblk_f; b += nt) mass += part_f[((size_t)3 * nblk_f + b) * 3 + 2];
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) a[q] = wave_sum(a[q]);
+    a[NQ] = wave_max(a[NQ]);
+    mass = wave_sum(mass);
+    __shared__ double s_mass[4];
+    if ((tid & 63) == 0) {
+#pragma unroll
+        for (int q = 0; q <= NQ; ++q) s_red[q][tid >> 6] = a[q];
+        s_mass[tid >> 6] = mass;
     }
     __syncthreads();
     if (tid != 0) return;
-    sc->accul = steady ? 0.0 : s_acc[0];
-    sc->heatvol = steady ? 0.0 : s_acc[1];
-    sc->fluxwest = s_acc[2];
-    sc->fluxeast = s_acc[3];
-    sc->fluxbottom = s_acc[4];
-    sc->fluxtop = s_acc[5];
-    sc->fluxsouth = s_acc[6];
-    sc->fluxnorth = s_acc[7];
-    sc->ahtoploss = s_acc[8];
+    double r[NQ + 1];
+    for (int q = 0; q < NQ; ++q) r[q] = s_red[q][0] + s_red[q][1] + s_red[q][2] + s_red[q][3];
+    r[NQ] = dmax(dmax(s_red[NQ][0], s_red[NQ][1]), dmax(s_red[NQ][2], s_red[NQ][3]));
+    const double msum = s_mass[0] + s_mass[1] + s_mass[2] + s_mass[3];
+    sc->resorh = r[9] / (r[10] + AMCFD_SMALL);
+    sc->accul = steady ? 0.0 : r[0];
+    sc->heatvol = steady ? 0.0 : r[1];
+    sc->fluxwest = r[2];
+    sc->fluxeast = r[3];
+    sc->fluxbottom = r[4];
+    sc->fluxtop = r[5];
+    sc->fluxsouth = r[6];
+    sc->fluxnorth = r[7];
+    sc->ahtoploss = r[8];
     double heatout = (sc->fluxnorth + sc->fluxbottom + sc->fluxwest + sc->fluxeast + sc->fluxsouth) - sc->ahtoploss;
     sc->heatout = heatout;
     if (steady) sc->ratio = -heatout / sc->heatinlaser;
     else sc->ratio = (sc->heatvol + sc->heatinlaser) / (sc->accul - heatout);
     if (flow_done) {
-        // per variable: [sumd, umax (u only) / denom (p only), resorm sum (p only)]
-        double sum[4] = {0, 0, 0, 0}, aux[4] = {0, 0, 0, 0}, aux2 = 0.0;
-        for (int v = 0; v < 4; ++v)
-            for (int b = 0; b < nblk_f; ++b) {
-                const double *p = part_f + ((size_t)v * nblk_f + b) * 3;
-                sum[v] += p[0];
-                if (v == 0) aux[0] = dmax(aux[0], p[1]);
-                if (v == 3) { aux[3] += p[1]; aux2 += p[2]; }
-            }
-        double refmom = 0.25 * M_PI * (width * width) * dens * (aux[0] * aux[0]);
+        double refmom = 0.25 * M_PI * (width * width) * dens * (r[NQ] * r[NQ]);
         sc->refmom = refmom;
-        sc->resoru = sum[0] / refmom;
-        sc->resorv = sum[1] / refmom;
-        sc->resorw = sum[2] / refmom;
-        double denom = aux[3] * dens;
-        sc->resorm = aux2 / (denom + AMCFD_SMALL);
+        sc->resoru = r[11] / refmom;
+        sc->resorv = r[12] / refmom;
+        sc->resorw = r[13] / refmom;
+        sc->resorm = msum / (r[14] * dens + AMCFD_SMALL);
     }
 }
 
@@ -769,11 +779,9 @@ void launch_props(hipStream_t st, const Dims &d, DevFields f, const amcfd_params
 }
 
 void launch_block_correction(hipStream_t st, const Dims &d, DevFields f, Partials pr, DevScalars *sc) {
-    const dim3 e = energy_grid(d);   // residual partials of the active grid
-    hipLaunchKernelGGL(k_column_sums, dim3((d.nim2 + 63) / 64), dim3(64), 0, st, d, f.sp, f.ae, f.aw, f.apnot,
-                       pr.blk);
-    hipLaunchKernelGGL(k_block_solve, dim3(1), dim3(64), 0, st, d, (int)(e.x * e.y * e.z), pr.blk,
-                       pr.energy, pr.blk + 4 * (size_t)d.ni, pr.delphi, sc);
+    hipLaunchKernelGGL(k_column_sums, dim3((d.nim2 + 63) / 64, 4), dim3(64), 0, st, d, f.sp, f.ae, f.aw,
+                       f.apnot, pr.blk);
+    hipLaunchKernelGGL(k_block_solve, dim3(1), dim3(64), 0, st, d, pr.blk, pr.blk + 4 * (size_t)d.ni, pr.delphi);
     hipLaunchKernelGGL(k_add_delphi, energy_grid(d), dim3(64, 4), 0, st, d, pr.delphi, f.enthalpy);
 }
 
@@ -800,8 +808,9 @@ void launch_finalize(hipStream_t st, const Dims &d, Partials pr, DevScalars *sc,
     dim3 gr = node_grid(d);
     int nblk_t = gr.x * gr.y * gr.z;
     const int nblk_b = (d.njm2 * d.nim2 + d.nkm2 * d.njm2 + d.nkm2 * d.nim2 + 255) / 256;
-    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, st, nblk_t, nblk_b, pr.nblk_f, pr.h2t, pr.bc, pr.flow,
-                       flow_done, steady, p.dens, width, sc);
+    const dim3 e = energy_grid(d);
+    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, st, nblk_t, nblk_b, pr.nblk_f, (int)(e.x * e.y * e.z),
+                       pr.h2t, pr.bc, pr.flow, pr.energy, flow_done, steady, p.dens, width, sc);
 }
 
 void launch_end_step(hipStream_t st, const Dims &d, DevFields f, const amcfd_params &p, double track,
