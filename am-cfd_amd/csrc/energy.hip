@@ -46,7 +46,7 @@ __global__ __launch_bounds__(256) void k_bc_enthalpy(Dims d, BcConst k, const do
         double T = temp[top];
         double hevap = 0.0;
         if (k.isevap) {
-            double psat = k.pressamb * exp(-k.evaplatent * k.molarmass / k.gasconst * (1.0 / T - 1.0 / k.evaptemp));
+            double psat = k.pressamb * libm_exp(-k.evaplatent * k.molarmass / k.gasconst * (1.0 / T - 1.0 / k.evaptemp));
             double mdot = k.evapfact * psat * sqrt(k.molarmass / (2.0 * M_PI * k.gasconst * T));
             hevap = mdot * k.evaplatent;
         }
@@ -204,7 +204,7 @@ __global__ __launch_bounds__(TX *TY) void k_energy_assemble(
             double src = 0.0;
             if (ztop - g.z[k] <= q.heatthick) {
                 src = q.avolpow * q.avolfact / M_PI / (q.heatrb * q.heatrb) / q.heatthick * q.apowseta *
-                      exp(-q.avolfact / (q.heatrb * q.heatrb) * ((xb * xb) + (yb * yb))) * (double)q.pi;
+                      libm_exp(-q.avolfact / (q.heatrb * q.heatrb) * ((xb * xb) + (yb * yb))) * (double)q.pi;
                 su += vol * src;
             }
             // latent heat: transient and advection terms
@@ -754,6 +754,51 @@ void launch_bc_enthalpy(hipStream_t st, const Dims &d, const DevGeom &g, DevFiel
     int nb = (n + 255) / 256;
     hipLaunchKernelGGL(k_bc_enthalpy, dim3(nb), dim3(256), 0, st, d, k, g.dxpwinv, g.dypsinv, g.dzpbinv,
                        g.areaij, f.heatin, f.temp, f.diff, f.enthalpy, pr.bc);
+}
+
+// Gaussian surface flux map, BoundCondManager.cpp:128-159: one thread per
+// (i, j); also writes areaij * heatin for the total below
+__global__ __launch_bounds__(256) void k_laser_map(int ni, int n, const double *__restrict__ x,
+                                                   const double *__restrict__ y,
+                                                   const double *__restrict__ areaij, LaserParams q,
+                                                   double *__restrict__ heatin, double *__restrict__ contrib) {
+    const int t = blockIdx.x * 256 + threadIdx.x;
+    if (t >= n) return;
+    const int j = t / ni, i = t - j * ni;
+    const double ydist = q.beamy - y[j];
+    const double xdist = q.beamx - x[i];
+    const double dist2 = (xdist * xdist) + (ydist * ydist);
+    double h = q.peakhin * libm_exp(-q.alasfact * dist2 / q.rb2) * q.pind;
+    if (q.alasfact < 1.0e-6 && dist2 < q.rb2) h = q.varlas / (M_PI * q.rb2) * q.pind;
+    heatin[t] = h;
+    contrib[t] = areaij[t] * h;
+}
+
+// heatinlaser = sum of areaij * heatin in the reference's (j, i) order
+// (BoundCondManager.cpp:158): one dependent add per cell, loads batched
+__global__ __launch_bounds__(64) void k_laser_total(int n, const double *__restrict__ contrib,
+                                                    DevScalars *sc) {
+    if (threadIdx.x != 0) return;
+    constexpr int U = 32;
+    double total = 0.0;
+    int c = 0;
+    for (; c + U <= n; c += U) {
+        double v[U];
+#pragma unroll
+        for (int r = 0; r < U; ++r) v[r] = contrib[c + r];
+#pragma unroll
+        for (int r = 0; r < U; ++r) total += v[r];
+    }
+    for (; c < n; ++c) total += contrib[c];
+    sc->heatinlaser = total;
+}
+
+void launch_laser_map(hipStream_t st, const Dims &d, const DevGeom &g, DevFields f, Partials pr,
+                      const LaserParams &q, DevScalars *sc) {
+    const int n = d.ni * d.nj;
+    hipLaunchKernelGGL(k_laser_map, dim3((n + 255) / 256), dim3(256), 0, st, d.ni, n, g.x, g.y, g.areaij, q,
+                       f.heatin, pr.laser);
+    hipLaunchKernelGGL(k_laser_total, dim3(1), dim3(64), 0, st, n, pr.laser, sc);
 }
 
 static dim3 energy_grid(const Dims &d) {

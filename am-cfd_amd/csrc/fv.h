@@ -4,6 +4,7 @@
 #pragma once
 
 #include "internal.h"
+#include "libm_exp.h"
 
 namespace amcfd {
 
@@ -24,7 +25,7 @@ __device__ __forceinline__ double weight_fn(int scheme, double pe) {
         r = r * b;
         return dmax(0.0, r);
     }
-    if (scheme == AMCFD_DISC_EXPONENTIAL) return pe / (exp(pe) - 1.0);
+    if (scheme == AMCFD_DISC_EXPONENTIAL) return pe / (libm_exp(pe) - 1.0);
     return 1.0;
 }
 // neighbour coefficient D*A(|F|/D) + max(0, F) (upstream on the + side)

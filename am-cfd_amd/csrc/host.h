@@ -60,8 +60,8 @@ class Solver {
     Partials pr_{};
     DevScalars *sc_ = nullptr;       // device
     DevScalars *sc_host_ = nullptr;  // pinned mirror
-    double *heatin_host_ = nullptr;  // pinned staging of the laser map (+ total)
     hipStream_t st_ = nullptr;
+    hipEvent_t step_ev_[2] = {nullptr, nullptr};   // brackets one solve_step
     std::vector<void *> allocs_;
     SweepPlan plan_dom_, plan_box_;
     unsigned long long *counter_ = nullptr;

@@ -83,6 +83,11 @@ struct SourceParams {
     int powerindicator;
 };
 
+// Gaussian surface flux of one laser position (BoundCondManager.cpp:128-131)
+struct LaserParams {
+    double beamx, beamy, alasfact, rb2, varlas, peakhin, pind;
+};
+
 // partial-sum buffers (deterministic two-pass reductions)
 struct Partials {
     double *energy;     // [nblk_e][2]       resid sumd, sumh
@@ -90,7 +95,7 @@ struct Partials {
     double *h2t;        // [nblk_t][10]      tpeak + fluxes + accul/heatvol
     double *flow;       // [4][nblk_f][3]    per variable: sumd / umax / denom
     double *bc;         // [nblk_b]          ahtoploss
-    double *laser;      // [nblk_l]          heatinlaser
+    double *laser;      // [nj*ni]           areaij * heatin, summed in order
     double *delphi;     // [ni]
     int nblk_e, nrow_e, nblk_t, nblk_f, nblk_b, nblk_l;
 };
@@ -109,6 +114,8 @@ void launch_bc_enthalpy(hipStream_t st, const Dims &d, const DevGeom &g, DevFiel
                         const amcfd_params &p);
 void launch_energy_assemble(hipStream_t st, const Dims &d, const DevGeom &g, DevFields f, Partials pr,
                             const amcfd_params &p, const SourceParams &sp);
+void launch_laser_map(hipStream_t st, const Dims &d, const DevGeom &g, DevFields f, Partials pr,
+                      const LaserParams &q, DevScalars *sc);
 void launch_props(hipStream_t st, const Dims &d, DevFields f, const amcfd_params &p);
 void launch_block_correction(hipStream_t st, const Dims &d, DevFields f, Partials pr, DevScalars *sc);
 void launch_h2t(hipStream_t st, const Dims &d, const DevGeom &g, DevFields f, Partials pr,

@@ -23,12 +23,14 @@ namespace amcfd {
     } while (0)
 
 Solver::~Solver() {
-    if (st_) hipStreamSynchronize(st_);
-    for (void *p : allocs_) hipFree(p);
+    // teardown: errors here have no caller to go to
+    if (st_) (void)hipStreamSynchronize(st_);
+    for (void *p : allocs_) (void)hipFree(p);
     if (sc_host_) (void)hipHostFree(sc_host_);
-    if (heatin_host_) (void)hipHostFree(heatin_host_);
-    for (auto &e : ev_pool_) hipEventDestroy(e);
-    if (st_) hipStreamDestroy(st_);
+    for (auto &e : ev_pool_) (void)hipEventDestroy(e);
+    for (auto &e : step_ev_)
+        if (e) (void)hipEventDestroy(e);
+    if (st_) (void)hipStreamDestroy(st_);
 }
 
 int Solver::alloc(double **ptr, size_t n) {
@@ -70,6 +72,8 @@ int Solver::create(const amcfd_params &p, int mode) {
     int dev = 0;
     HIPCALL(hipGetDevice(&dev));
     HIPCALL(hipStreamCreateWithFlags(&st_, hipStreamNonBlocking));
+    HIPCALL(hipEventCreate(&step_ev_[0]));
+    HIPCALL(hipEventCreate(&step_ev_[1]));
 
     d_.ni = p.ni;
     d_.nj = p.nj;
@@ -130,7 +134,7 @@ int Solver::create(const amcfd_params &p, int mode) {
     pr_.nblk_b = (d_.njm2 * d_.nim2 + d_.nkm2 * d_.njm2 + d_.nkm2 * d_.nim2 + 255) / 256;
     if ((st = alloc(&pr_.bc, (size_t)pr_.nblk_b))) return st;
     if ((st = alloc(&pr_.delphi, (size_t)d_.ni + 2))) return st;
-    pr_.laser = nullptr;
+    if ((st = alloc(&pr_.laser, (size_t)d_.ni * d_.nj))) return st;
     void *v = nullptr;
     HIPCALL(hipMalloc(&v, sizeof(DevScalars)));
     allocs_.push_back(v);
@@ -138,8 +142,6 @@ int Solver::create(const amcfd_params &p, int mode) {
     HIPCALL(hipMemsetAsync(sc_, 0, sizeof(DevScalars), st_));
     HIPCALL(hipHostMalloc((void **)&sc_host_, sizeof(DevScalars), hipHostMallocDefault));
     std::memset(sc_host_, 0, sizeof(DevScalars));
-    HIPCALL(hipHostMalloc((void **)&heatin_host_, sizeof(double) * ((size_t)d_.ni * d_.nj + 1),
-                          hipHostMallocDefault));
     HIPCALL(hipMalloc(&v, sizeof(unsigned long long)));
     allocs_.push_back(v);
     counter_ = (unsigned long long *)v;
@@ -228,33 +230,21 @@ static int nearest(const std::vector<double> &c, int nlast, double pos) {
     return out;
 }
 
-// Gaussian surface flux map, BoundCondManager.cpp:128-159.  Evaluated on the
-// host once per step with the C library's exp (the same function the
-// reference calls, so the map is bitwise the reference's) and uploaded: the
-// map is ni*nj values per step, negligible next to the iterations.
+// Gaussian surface flux map, BoundCondManager.cpp:128-159, evaluated on the
+// device (k_laser_map); the nearest grid indices stay on the host.
 int Solver::launch_laser_map() {
     istart_ = nearest(hg_.x, d_.nim1, beamx_);
     jstart_ = nearest(hg_.y, d_.njm1, beamy_);
-    const double rb2 = p_.alasrb * p_.alasrb;
-    const double varlas = p_.alaspow * p_.alaseta;
-    const double peakhin = p_.alasfact * varlas / (M_PI * rb2);
-    const double pi = (double)powerindicator_;
-    double total = 0.0;
-    for (int j = 0; j < d_.nj; ++j) {
-        const double yd = beamy_ - hg_.y[j];
-        for (int i = 0; i < d_.ni; ++i) {
-            const double xd = beamx_ - hg_.x[i];
-            const double d2 = xd * xd + yd * yd;
-            double q = peakhin * std::exp(-p_.alasfact * d2 / rb2) * pi;
-            if (p_.alasfact < 1.0e-6 && d2 < rb2) q = varlas / (M_PI * rb2) * pi;
-            heatin_host_[j * d_.ni + i] = q;
-            total += hg_.areaij[j * d_.ni + i] * q;
-        }
-    }
-    heatin_host_[(size_t)d_.ni * d_.nj] = total;
-    HIPCALL(hipMemcpyAsync(f_.heatin, heatin_host_, sizeof(double) * d_.ni * d_.nj, hipMemcpyHostToDevice, st_));
-    HIPCALL(hipMemcpyAsync(&sc_->heatinlaser, heatin_host_ + (size_t)d_.ni * d_.nj, sizeof(double),
-                           hipMemcpyHostToDevice, st_));
+    LaserParams q;
+    q.beamx = beamx_;
+    q.beamy = beamy_;
+    q.alasfact = p_.alasfact;
+    q.rb2 = p_.alasrb * p_.alasrb;
+    q.varlas = p_.alaspow * p_.alaseta;
+    q.peakhin = p_.alasfact * q.varlas / (M_PI * q.rb2);
+    q.pind = (double)powerindicator_;
+    amcfd::launch_laser_map(st_, d_, g_, f_, pr_, q, sc_);
+    HIPCALL(hipGetLastError());
     return AMCFD_OK;
 }
 
@@ -307,14 +297,14 @@ void Solver::tic(const char *name, int *slot) {
             ev_pool_.push_back(e);
         }
     }
+    if (hipEventRecord(ev_pool_[ev_next_], st_) != hipSuccess) return;
     *slot = (int)ev_next_;
     ev_next_ += 2;
-    hipEventRecord(ev_pool_[*slot], st_);
 }
 
 void Solver::toc(const char *name, int slot) {
     if (slot < 0) return;
-    hipEventRecord(ev_pool_[slot + 1], st_);
+    if (hipEventRecord(ev_pool_[slot + 1], st_) != hipSuccess) return;
     ev_pending_.push_back({name, {slot, slot + 1}});
 }
 
@@ -411,10 +401,7 @@ int Solver::flow_iteration() {
 }
 
 int Solver::solve_step() {
-    hipEvent_t e0 = nullptr, e1 = nullptr;
-    hipEventCreate(&e0);
-    hipEventCreate(&e1);
-    hipEventRecord(e0, st_);
+    HIPCALL(hipEventRecord(step_ev_[0], st_));
     double tdma_before = timers.count("sweep_h") ? timers["sweep_h"].total_ms : 0.0;
     const double alenold = alen_;
     niter_ = 0;
@@ -465,14 +452,12 @@ int Solver::solve_step() {
         }
     }
     if (rc == AMCFD_I_MAXIT && std::fabs(alen_ - alenold) < 1.0e-6) steadylength_ = true;
-    hipEventRecord(e1, st_);
-    hipEventSynchronize(e1);
+    HIPCALL(hipEventRecord(step_ev_[1], st_));
+    HIPCALL(hipEventSynchronize(step_ev_[1]));
     float ms = 0.f;
-    hipEventElapsedTime(&ms, e0, e1);
+    HIPCALL(hipEventElapsedTime(&ms, step_ev_[0], step_ev_[1]));
     last_total_ms = ms;
     last_tdma_ms = profile_ ? (timers["sweep_h"].total_ms - tdma_before) : -1.0;
-    hipEventDestroy(e0);
-    hipEventDestroy(e1);
     return rc;
 }
 

@@ -15,6 +15,8 @@ The named cases are the workloads the solver is measured and checked on:
 * ``tiny_cond`` same grid, upwind, ``*FLUID_FLOW_OFF``: conduction-only path.
 * ``tiny_evap`` same grid, evaporation loss and the Fe-O surfactant
                 surface-tension model (a_oxygen > 0).
+* ``tiny_vap``  same grid, evaporation loss only (constant dgamma/dT).
+* ``tiny_vol``  same grid, Gaussian laser plus a 100 W volumetric source.
 * ``plate120``  120x48x32 cells (2.0x0.6x0.4 mm), 400 steps, P=200 W,
                 r=50 um, 1 m/s: the CPU-runnable single-track case.
 * ``track400``  400x160x80 cells (4.0x1.2x0.6 mm, ~5.1M cells), P=300 W,
@@ -59,7 +61,7 @@ def write_deck(path, mesh_path, *, keyword="case", finaltime, maxit=100, tol=1e-
                factor=2.0, discretization="power_law", flow=True, urf=(0.8,) * 5,
                tamb=300.0, tpre=300.0, htc=(10.0, 10.0, 10.0, 10.0), emiss=0.4,
                momentum=(5e-5, -2.6e-4, 1e7, -1.0), material=None, evaporation=None,
-               extra=()):
+               volumetric=None, extra=()):
     if material is None:
         material = (4000.0, 4000.0, 5e-3, TS, TL, HSMELT, HLFRIZ, ACPA, ACPB, ACPB,
                     0.0, 0.0, 25.0, 0.0, 25.0)
@@ -77,7 +79,8 @@ def write_deck(path, mesh_path, *, keyword="case", finaltime, maxit=100, tol=1e-
     L += ["*GAUSS_LASER_PARAMETERS", "$ power  absorb  radius  factor",
           "  %s %s %s %s" % tuple(_fmt(v) for v in (power, eta, radius, factor))]
     L += ["*VOLUMETRIC_HEAT_SOURCE", "$ power apowseta heatthick heatrb factor",
-          "  0.0 0.0 1e-5 5e-5 2.0"]
+          "  0.0 0.0 1e-5 5e-5 2.0" if volumetric is None else
+          "  " + " ".join(_fmt(v) for v in volumetric)]
     m = material
     L += ["*MATERIAL_PROPERTIES", "$ dens denl viscos tsolid tliquid hsmelt hlfriz",
           "  " + " ".join(_fmt(v) for v in m[0:7]), "$ acpa acpb acpl",
@@ -121,6 +124,20 @@ CASES = {
                   xstart=0.15e-3, ystart=0.15e-3, speed=1.0, power=250.0, radius=50e-6,
                   momentum=(5e-5, -2.6e-4, 1e7, 0.02),
                   evaporation=(6.34e6, 3133.0, 0.001, 0.0558))),
+    "tiny_vap": dict(
+        mesh=(([0.1e-3, 0.3e-3, 0.1e-3], [4, 12, 4], [-1.2, 1.0, 1.2]),
+              ([0.15e-3, 0.15e-3], [6, 6], [1.0, 1.0]),
+              ([0.15e-3, 0.1e-3], [4, 8], [1.0, 1.0])),
+        deck=dict(finaltime=0.1e-3 - 1e-9, maxit=60, dt=5e-6, lasertime=0.08e-3,
+                  xstart=0.15e-3, ystart=0.15e-3, speed=1.0, power=250.0, radius=50e-6,
+                  evaporation=(6.34e6, 3133.0, 0.001, 0.0558))),
+    "tiny_vol": dict(
+        mesh=(([0.1e-3, 0.3e-3, 0.1e-3], [4, 12, 4], [-1.2, 1.0, 1.2]),
+              ([0.15e-3, 0.15e-3], [6, 6], [1.0, 1.0]),
+              ([0.15e-3, 0.1e-3], [4, 8], [1.0, 1.0])),
+        deck=dict(finaltime=0.1e-3 - 1e-9, maxit=60, dt=5e-6, lasertime=0.08e-3,
+                  xstart=0.15e-3, ystart=0.15e-3, speed=1.0, power=150.0, radius=50e-6,
+                  volumetric=(100.0, 0.3, 5e-5, 5e-5, 2.0))),
     "plate120": dict(
         mesh=(([0.5e-3, 1.0e-3, 0.5e-3], [20, 80, 20], [-1.2, 1.0, 1.2]),
               ([0.2e-3, 0.2e-3, 0.2e-3], [16, 16, 16], [1.0, 1.0, 1.0]),

@@ -37,6 +37,8 @@ PLAN = {
     "tiny": (1, -1, [2, 40]),
     "tiny_cond": (1, -1, []),
     "tiny_evap": (1, -1, []),
+    "tiny_vap": (1, -1, []),
+    "tiny_vol": (1, -1, []),
     "plate120": (50, -1, []),
 }
 

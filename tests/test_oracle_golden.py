@@ -11,7 +11,7 @@ import pytest
 
 from conftest import golden_path
 
-CASES = ["tiny", "tiny_cond", "tiny_evap"]
+CASES = ["tiny", "tiny_cond", "tiny_evap", "tiny_vap", "tiny_vol"]
 INT_KEYS = ["niter", "itertot", "powerindicator", "istat", "iend", "jstat", "jend", "kstat", "jstart",
             "istart", "nk"]
 REAL_KEYS = ["timet", "resorh", "resorm", "resoru", "resorv", "resorw", "tpeak", "alen", "depth",
