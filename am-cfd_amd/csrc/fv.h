@@ -74,47 +74,6 @@ __device__ __forceinline__ void block_sum(double (&v)[NV], double *scratch) {
     __syncthreads();
 }
 
-// Parallel cyclic reduction on an n-row tridiagonal system held in LDS:
-//   a[i] x[i-1] + b[i] x[i] + c[i] x[i+1] = d[i],  a[0] = c[n-1] = 0.
-// The solution overwrites d.  Every thread of the block must call it.
-// Each thread owns rows tid, tid+nt, ... (at most MAXR rows).
-template <int MAXR>
-__device__ void pcr_solve(int n, double *a, double *b, double *c, double *d) {
-    const int nt = blockDim.x * blockDim.y;
-    const int tid = threadIdx.x + threadIdx.y * blockDim.x;
-    for (int s = 1; s < n; s <<= 1) {
-        double na[MAXR], nb[MAXR], nc[MAXR], nd[MAXR];
-#pragma unroll
-        for (int r = 0; r < MAXR; ++r) {
-            int i = tid + r * nt;
-            if (i < n) {
-                double ai = a[i], bi = b[i], ci = c[i], di = d[i];
-                double k1 = 0.0, k2 = 0.0;
-                double am = 0.0, cm = 0.0, dm = 0.0, ap = 0.0, cp = 0.0, dp = 0.0;
-                if (i - s >= 0) { k1 = ai / b[i - s]; am = a[i - s]; cm = c[i - s]; dm = d[i - s]; }
-                if (i + s < n) { k2 = ci / b[i + s]; ap = a[i + s]; cp = c[i + s]; dp = d[i + s]; }
-                na[r] = -am * k1;
-                nb[r] = bi - cm * k1 - ap * k2;
-                nc[r] = -cp * k2;
-                nd[r] = di - dm * k1 - dp * k2;
-            }
-        }
-        __syncthreads();
-#pragma unroll
-        for (int r = 0; r < MAXR; ++r) {
-            int i = tid + r * nt;
-            if (i < n) { a[i] = na[r]; b[i] = nb[r]; c[i] = nc[r]; d[i] = nd[r]; }
-        }
-        __syncthreads();
-    }
-#pragma unroll
-    for (int r = 0; r < MAXR; ++r) {
-        int i = tid + r * nt;
-        if (i < n) d[i] = d[i] / b[i];
-    }
-    __syncthreads();
-}
-
 #define HIP_CHECK_LAUNCH() (void)hipGetLastError()
 
 }  // namespace amcfd

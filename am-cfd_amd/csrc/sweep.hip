@@ -20,16 +20,12 @@
 // line keeps a version word (epoch + passes done); waits are bounded and raise
 // an error flag instead of hanging.
 //
-// Line solvers:
-//   EXACT  the reference's pr/qr recurrence, run by one lane from LDS after
-//          the wave has staged the line; bitwise the reference's arithmetic
-//          (the SIMPLE iteration amplifies last-bit differences by ~1e7 per
-//          step, so the plain FP64 mode keeps every bit).
-//   PCR    parallel cyclic reduction by the whole workgroup; same system,
-//          different rounding (for modes that do not promise bitwise
-//          agreement).  On the melt-pool box the reference guards
-//          |pivot| <= 1e-12; a line that is not provably free of such pivots
-//          falls back to the exact recurrence.
+// Line solver: the reference's pr/qr recurrence with every rounding of the
+// reference (the SIMPLE iteration amplifies last-bit differences by ~1e7 per
+// step, so the plain FP64 mode keeps every bit).  The pivots depend only on
+// the coefficients and are computed for all lines up front (k_line_prep);
+// the wavefront carries the right-hand-side recurrence, one wave per line,
+// rows held in registers.
 
 #include "fv.h"
 
@@ -38,7 +34,10 @@ namespace amcfd {
 namespace {
 
 constexpr int SW_MAXN = 1024;                 // longest x-line handled
+constexpr int SW_M = 20;                      // rows per lane in the recurrence (64 * SW_M >= SW_MAXN)
+constexpr int SW_M_SHORT = 6;                 // rows per lane for lines of at most 384 cells
 constexpr long long SPIN_LIMIT = 1LL << 20;   // bounded wait (~ a second)
+static_assert(64 * SW_M >= SW_MAXN, "one wave must hold a whole line");
 
 struct Coefs {
     const double *an, *as, *ae, *aw, *at, *ab, *ap, *su;
@@ -61,54 +60,34 @@ __device__ bool wait_version(const int *v, int need, const int *error) {
     return true;
 }
 
-// Reference recurrence on LDS rows (A = -aw, B = ap, C = -ae, D = rhs);
-// the solution overwrites D.  Run by a single lane.
-__device__ void thomas_exact(int n, double *A, const double *B, double *C, double *D, double lo, double hi,
-                             int guard) {
-    double prv = 0.0, qrv = lo;
-    for (int ii = 0; ii < n; ++ii) {
-        const double aw = -A[ii];
-        double den = B[ii] - aw * prv;
-        if (guard && fabs(den) <= 1e-12) den += AMCFD_SMALL;
-        prv = -C[ii] / den;
-        qrv = (D[ii] + aw * qrv) / den;
-        A[ii] = prv;
-        C[ii] = qrv;
-    }
-    double nxt = hi;
-    for (int ii = n - 1; ii >= 0; --ii) {
-        nxt = A[ii] * nxt + C[ii];
-        D[ii] = nxt;
-    }
-}
-
 __device__ __forceinline__ double lane_bcast(double v, int lane) {
     int lo = __builtin_amdgcn_readlane(__double2loint(v), lane);
     int hi = __builtin_amdgcn_readlane(__double2hiint(v), lane);
     return __hiloint2double(hi, lo);
 }
 
-// Correctly rounded x / b given y = RN(1/b): a faithful first quotient and
+// Correctly rounded x / b for b > 0, given y = RN(1/b): a first quotient and
 // two exact-remainder corrections (Markstein's theorem: with y the rounded
-// reciprocal and q faithful, RN(q + (x - b q) y) = RN(x / b)).  Five dependent
-// operations instead of the eleven of a full division; valid for the normal,
-// finite operands of the line solves (pivots ~1e-7 .. 1e20).  A zero x keeps
-// the sign IEEE division gives it.
-__device__ __forceinline__ double div_by_recip(double x, double b, double y) {
+// reciprocal and q within an ulp of x/b, RN(q + (x - b q) y) = RN(x / b); the
+// first correction brings x*y within an ulp).  The remainders are formed
+// negated, b q - x, which for b > 0 leaves a zero x with the sign IEEE
+// division gives it, so no select is needed.  Valid when b is positive and
+// finite and x is zero or finite with |x| >= 2^-900 (the remainders are then
+// exact); the caller checks that after the pass.  Five dependent operations
+// against the eleven of a full division.
+__device__ __forceinline__ double div_pos(double x, double b, double y) {
     const double q0 = x * y;
-    const double e0 = __fma_rn(-b, q0, x);
-    const double q1 = __fma_rn(e0, y, q0);
-    const double e1 = __fma_rn(-b, q1, x);
-    const double q2 = __fma_rn(e1, y, q1);
-    return (x == 0.0) ? q0 : q2;
+    const double e0 = __fma_rn(b, q0, -x);
+    const double q1 = __fma_rn(-e0, y, q0);
+    const double e1 = __fma_rn(b, q1, -x);
+    return __fma_rn(-e1, y, q1);
 }
 
 // Per-line pivots of the reference recurrence.  den[i] = ap - aw*pr[i-1]
 // (plus the melt-pool guard) and pr[i] = ae/den[i] depend only on the
 // coefficients, which are fixed for the whole sweep, so they are computed
 // once for every line of the sweep in parallel (one lane per line), with
-// y[i] = RN(1/den[i]).  The wavefront then only carries the right-hand-side
-// recurrence.
+// y[i] = RN(1/den[i]).
 __global__ __launch_bounds__(256) void k_line_prep(Dims d, const double *__restrict__ aw,
                                                    const double *__restrict__ ap, const double *__restrict__ ae,
                                                    double *pr_o, double *den_o, double *y_o, int ibeg, int iend,
@@ -131,19 +110,20 @@ __global__ __launch_bounds__(256) void k_line_prep(Dims d, const double *__restr
 }
 
 // Right-hand-side recurrence and back substitution of the reference
-// (CFDSolverManager.cpp:2512-2521) for one line by one wave, rows in
-// registers.  Rows are padded at the front to NL*M (pass-through rows:
-// aw = 1, rhs = 0, den = y = 1, pr = 0) so lane l owns rows [l*M, l*M + M)
-// with no per-row bounds; the running qr and the back-substitution value
-// travel from lane to lane with readlane.  LDS inputs (index = real row):
-// AW, DEN, Y, PR, D = rhs; the solution overwrites D.
+// (CFDSolverManager.cpp:2512-2521) for one line by one wave.  Lane l owns rows
+// [l*M - pad, l*M - pad + M); rows are padded at the front to a multiple of M
+// with pass-through rows (aw = 1, rhs = -0, den = y = 1, pr = 0: x = -0 + q
+// is q for either sign of zero, and q / 1 is q), so there are no per-row
+// bounds.  The running qr and the back-substitution value travel from lane to
+// lane with readlane.  LDS inputs (index = real row): AW, DEN, Y, PR, D = rhs;
+// the solution overwrites D.
 template <int M>
-__device__ void rhs_wave(int n, const double *AW, const double *DEN, const double *Y, const double *PR, double *D,
+__device__ __forceinline__ void rhs_wave(int n, const double *AW, const double *DEN, const double *Y, const double *PR, double *D,
                          double lo, double hi) {
     const int lane = threadIdx.x;
     const int nl = (n + M - 1) / M;
     const int pad = nl * M - n;
-    double raw[M], rden[M], ry[M], rpr[M], rd[M];
+    double raw[M], rden[M], ry[M], rd[M], rx[M];
 #pragma unroll
     for (int r = 0; r < M; ++r) {
         const int p = lane * M + r - pad;
@@ -151,26 +131,57 @@ __device__ void rhs_wave(int n, const double *AW, const double *DEN, const doubl
         raw[r] = real ? AW[p] : 1.0;
         rden[r] = real ? DEN[p] : 1.0;
         ry[r] = real ? Y[p] : 1.0;
-        rpr[r] = real ? PR[p] : 0.0;
-        rd[r] = real ? D[p] : 0.0;
+        rd[r] = real ? D[p] : -0.0;
+        rx[r] = 0.0;
     }
-    double qrv = lo;
+    double q = lo;
     for (int L = 0; L < nl; ++L) {
         if (lane == L) {
 #pragma unroll
             for (int r = 0; r < M; ++r) {
-                qrv = div_by_recip(rd[r] + raw[r] * qrv, rden[r], ry[r]);
-                rd[r] = qrv;
+                const double x = rd[r] + raw[r] * q;
+                rx[r] = x;
+                q = div_pos(x, rden[r], ry[r]);
+                rd[r] = q;
             }
         }
-        qrv = lane_bcast(qrv, L);
+        q = lane_bcast(q, L);
+    }
+    // the fast division's preconditions, checked for every row in parallel;
+    // a line that misses one is solved again with IEEE division
+    bool ok = true;
+#pragma unroll
+    for (int r = 0; r < M; ++r) {
+        const double ax = fabs(rx[r]);
+        ok = ok && (rden[r] > 0.0) && (rden[r] < __builtin_inf()) &&
+             ((ax == 0.0) || ((ax >= 0x1p-900) && (ax < __builtin_inf())));
+    }
+    if (!__all(ok)) {
+        q = lo;
+        for (int L = 0; L < nl; ++L) {
+            if (lane == L) {
+#pragma unroll
+                for (int r = 0; r < M; ++r) {
+                    const int p = lane * M + r - pad;
+                    const double dr = (p >= 0) ? D[p] : -0.0;
+                    q = (dr + raw[r] * q) / rden[r];
+                    rd[r] = q;
+                }
+            }
+            q = lane_bcast(q, L);
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < M; ++r) {
+        const int p = lane * M + r - pad;
+        raw[r] = ((p >= 0) && (lane < nl)) ? PR[p] : 0.0;
     }
     double nxt = hi;
     for (int L = nl - 1; L >= 0; --L) {
         if (lane == L) {
 #pragma unroll
             for (int r = M - 1; r >= 0; --r) {
-                nxt = rpr[r] * nxt + rd[r];
+                nxt = raw[r] * nxt + rd[r];
                 rd[r] = nxt;
             }
         }
@@ -181,18 +192,6 @@ __device__ void rhs_wave(int n, const double *AW, const double *DEN, const doubl
         const int p = lane * M + r - pad;
         if (p >= 0 && lane < nl) D[p] = rd[r];
     }
-}
-
-template <int M>
-__device__ void rhs_wave_dispatch(int n, const double *AW, const double *DEN, const double *Y, const double *PR,
-                                  double *D, double lo, double hi) {
-    if constexpr (M < 16) {
-        if (n > 64 * M) {
-            rhs_wave_dispatch<M + 1>(n, AW, DEN, Y, PR, D, lo, hi);
-            return;
-        }
-    }
-    rhs_wave<M>(n, AW, DEN, Y, PR, D, lo, hi);
 }
 
 // Line data move between workgroups (and XCDs) with device-coherent
@@ -207,15 +206,17 @@ __device__ __forceinline__ void st_line(double *p, double v) {
     __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-template <int NT, bool PCR>
+// One x-line solve per task; M rows per lane in the recurrence (64 M >= n).
+// DOMAIN marks the instantiation for the whole-domain enthalpy sweep, so it
+// shows up as its own kernel in a profile; the code is the same.
+template <int NT, int M, bool DOMAIN>
 __global__ __launch_bounds__(NT) void k_line_sweeps(Dims d, Coefs cf, double *phi, const int *__restrict__ tasks,
                                                     int ntasks, int ibeg, int iend, int jbeg, int jend, int kbeg,
-                                                    int kend, int guard, unsigned long long *counter,
-                                                    unsigned long long base, int *versions, int epoch,
-                                                    DevScalars *sc) {
+                                                    int kend, unsigned long long *counter, unsigned long long base,
+                                                    int *versions, int epoch, DevScalars *sc) {
     constexpr int MAXR = SW_MAXN / NT;
-    __shared__ double A[SW_MAXN], B[SW_MAXN], C[SW_MAXN], D[SW_MAXN], E[PCR ? 1 : SW_MAXN];
-    __shared__ int s_task, s_ok;
+    __shared__ double A[SW_MAXN], B[SW_MAXN], C[SW_MAXN], D[SW_MAXN], E[SW_MAXN];
+    __shared__ int s_task;
     const int tid = threadIdx.x;
     const int n = iend - ibeg + 1;
     for (;;) {
@@ -232,32 +233,17 @@ __global__ __launch_bounds__(NT) void k_line_sweeps(Dims d, Coefs cf, double *ph
         const int ks = pass >> 1;
         const size_t row = (size_t)k * d.sk + (size_t)j * d.sj;
         // coefficients do not depend on other lines: stage them before waiting
-        double rat[MAXR], rab[MAXR], ran[MAXR], ras[MAXR], rsu[MAXR], raw[MAXR], rap[MAXR], rae[MAXR];
+        double rat[MAXR], rab[MAXR], ran[MAXR], ras[MAXR], rsu[MAXR];
 #pragma unroll
         for (int r = 0; r < MAXR; ++r) {
             const int ii = tid + r * NT;
             if (ii < n) {
                 const size_t c = row + ibeg + ii;
                 rat[r] = cf.at[c]; rab[r] = cf.ab[c]; ran[r] = cf.an[c]; ras[r] = cf.as[c]; rsu[r] = cf.su[c];
-                raw[r] = cf.aw[c];
-                if (PCR) { rap[r] = cf.ap[c]; rae[r] = cf.ae[c]; }
-                else { rap[r] = cf.den[c]; rae[r] = cf.y[c]; }
-            }
-        }
-#pragma unroll
-        for (int r = 0; r < MAXR; ++r) {
-            const int ii = tid + r * NT;
-            if (ii < n) {
-                if (PCR) {   // a x[i-1] + b x[i] + c x[i+1] = d with a = -aw, c = -ae
-                    A[ii] = -raw[r];
-                    B[ii] = rap[r];
-                    C[ii] = -rae[r];
-                } else {     // exact recurrence: aw, pivot, 1/pivot, pr
-                    A[ii] = raw[r];
-                    B[ii] = rap[r];
-                    C[ii] = rae[r];
-                    E[ii] = cf.pr[row + ibeg + ii];
-                }
+                A[ii] = cf.aw[c];
+                B[ii] = cf.den[c];
+                C[ii] = cf.y[c];
+                E[ii] = cf.pr[c];
             }
         }
         const double lo = phi[row + ibeg - 1], hi = phi[row + iend + 1];   // fixed end values
@@ -270,7 +256,6 @@ __global__ __launch_bounds__(NT) void k_line_sweeps(Dims d, Coefs cf, double *ph
             if (k < kend) ok &= wait_version(vb + (k + 1) * d.nj + j, epoch + 2 * ks + 2, err);
             if (k > kbeg && ks > 0) ok &= wait_version(vb + (k - 1) * d.nj + j, epoch + 2, err);
             if (!ok) atomicOr(&sc->error, 1);
-            s_ok = PCR ? 1 : 0;
         }
         __syncthreads();
         // right-hand side from the neighbouring lines, in the reference's order
@@ -283,37 +268,9 @@ __global__ __launch_bounds__(NT) void k_line_sweeps(Dims d, Coefs cf, double *ph
                         ran[r] * ld_line(phi + c + d.sj) + ras[r] * ld_line(phi + c - d.sj) + rsu[r];
             }
         }
-        if (PCR && guard) {
-            // PCR only when no Thomas pivot can fall to 1e-12: nonnegative
-            // neighbours, ap >= aw + ae and ap - aw > 2e-12
-            bool ok = true;
-#pragma unroll
-            for (int r = 0; r < MAXR; ++r) {
-                const int ii = tid + r * NT;
-                if (ii < n) {
-                    const double aw = raw[r], ae = rae[r], ap = rap[r];
-                    ok &= (aw >= 0.0) && (ae >= 0.0) && (ap - aw > 2e-12) && (ap >= (aw + ae) * (1.0 - 1e-12));
-                }
-            }
-            if (!ok) s_ok = 0;
-        }
         __syncthreads();
-        if (PCR && s_ok) {
-            if (tid == 0) {   // fixed end values fold into the first / last rows
-                D[0] -= A[0] * lo;
-                D[n - 1] -= C[n - 1] * hi;
-                A[0] = 0.0;
-                C[n - 1] = 0.0;
-            }
-            __syncthreads();
-            pcr_solve<MAXR>(n, A, B, C, D);
-        } else if (!PCR) {
-            if (tid < 64) rhs_wave_dispatch<1>(n, A, B, C, E, D, lo, hi);
-            __syncthreads();
-        } else {
-            if (tid == 0) thomas_exact(n, A, B, C, D, lo, hi, guard);
-            __syncthreads();
-        }
+        if (tid < 64) rhs_wave<M>(n, A, B, C, E, D, lo, hi);
+        __syncthreads();
 #pragma unroll
         for (int r = 0; r < MAXR; ++r) {
             const int ii = tid + r * NT;
@@ -360,20 +317,21 @@ void launch_line_sweeps(hipStream_t st, const Dims &d, DevFields f, double *phi,
     const int nlines = (plan.kend - plan.kbeg + 1) * (plan.jend - plan.jbeg + 1);
     hipLaunchKernelGGL(k_line_prep, dim3((nlines + 255) / 256), dim3(256), 0, st, d, f.aw, f.ap, f.ae, f.lpr, f.lden,
                        f.ly, ibeg, iend, plan.jbeg, plan.jend, plan.kbeg, plan.kend, guard);
-    hipLaunchKernelGGL((k_line_sweeps<256, false>), dim3(nblocks), dim3(256), 0, st, d, cf, phi, plan.tasks,
-                       plan.ntasks, ibeg, iend, plan.jbeg, plan.jend, plan.kbeg, plan.kend, guard, counter,
-                       counter_base, versions, epoch, sc);
-}
-
-void launch_line_sweeps_pcr(hipStream_t st, const Dims &d, DevFields f, double *phi, const SweepPlan &plan,
-                            int ibeg, int iend, int guard, unsigned long long *counter,
-                            unsigned long long counter_base, int *versions, int epoch, DevScalars *sc,
-                            int nblocks) {
-    if (plan.ntasks == 0 || iend < ibeg) return;
-    Coefs cf{f.an, f.as, f.ae, f.aw, f.at, f.ab, f.ap, f.su, nullptr, nullptr, nullptr};
-    hipLaunchKernelGGL((k_line_sweeps<256, true>), dim3(nblocks), dim3(256), 0, st, d, cf, phi, plan.tasks,
-                       plan.ntasks, ibeg, iend, plan.jbeg, plan.jend, plan.kbeg, plan.kend, guard, counter,
-                       counter_base, versions, epoch, sc);
+    // short lines (the melt-pool box) get a lighter instantiation: fewer
+    // registers, two workgroups per CU
+    const int n = iend - ibeg + 1;
+    if (!guard)
+        hipLaunchKernelGGL((k_line_sweeps<256, SW_M, true>), dim3(nblocks), dim3(256), 0, st, d, cf, phi, plan.tasks,
+                           plan.ntasks, ibeg, iend, plan.jbeg, plan.jend, plan.kbeg, plan.kend, counter, counter_base,
+                           versions, epoch, sc);
+    else if (n <= 64 * SW_M_SHORT)
+        hipLaunchKernelGGL((k_line_sweeps<256, SW_M_SHORT, false>), dim3(nblocks), dim3(256), 0, st, d, cf, phi,
+                           plan.tasks, plan.ntasks, ibeg, iend, plan.jbeg, plan.jend, plan.kbeg, plan.kend, counter,
+                           counter_base, versions, epoch, sc);
+    else
+        hipLaunchKernelGGL((k_line_sweeps<256, SW_M, false>), dim3(nblocks), dim3(256), 0, st, d, cf, phi, plan.tasks,
+                           plan.ntasks, ibeg, iend, plan.jbeg, plan.jend, plan.kbeg, plan.kend, counter, counter_base,
+                           versions, epoch, sc);
 }
 
 }  // namespace amcfd
