@@ -308,70 +308,108 @@ __global__ __launch_bounds__(256) void k_props(size_t n, PropConst c, const doub
 // (second kernel) add the correction, CFDSolverManager.cpp:2270-2362.
 // Also folds the energy residual, CFDSolverManager.cpp:2099.
 // ------------------------------------------------------------------------
-// Column sums bl, blp, blm, blc for every x index i, each accumulated by one
-// lane in the reference's order (k outer, j inner).  Floating-point sums are
-// order dependent and the SIMPLE iteration is chaotic enough that a
-// reordered sum changes the trajectory after a few dozen steps, so the order
-// is kept; the lanes of a wave read consecutive i, so every load is a
-// coalesced row segment.  Output: blk[q*ni + i], q = bl, blp, blm, blc.
-__global__ __launch_bounds__(64) void k_column_sums(Dims d, const double *__restrict__ term_bl,
-                                                    const double *__restrict__ ae,
-                                                    const double *__restrict__ aw,
-                                                    const double *__restrict__ term_blc, double *blk) {
-    // one wave per (64 columns, quantity): four independent streams, each
-    // with U row loads in flight
-    constexpr int U = 32;
+// Column sums bl, blp, blm, blc for every x index i, each accumulated in the
+// reference's order (k outer, j inner).  Floating-point sums are order
+// dependent and the SIMPLE iteration is chaotic enough that a reordered sum
+// changes the trajectory after a few dozen steps, so the order is kept.  Each
+// sum is one dependent chain of (nj-2)(nk-2) adds; to keep the chains fed,
+// one workgroup serves CS_COLS columns of one quantity: lanes 0..CS_COLS-1 of
+// wave 0 hold the running sums while the other waves stream the next CS_CH
+// rows into an LDS double buffer.  Output: blk[q*ni + i], q = bl, blp, blm,
+// blc.
+constexpr int CS_COLS = 16, CS_CH = 512, CS_NT = 1024;
+
+__global__ __launch_bounds__(CS_NT) void k_column_sums(Dims d, const double *__restrict__ term_bl,
+                                                       const double *__restrict__ ae,
+                                                       const double *__restrict__ aw,
+                                                       const double *__restrict__ term_blc, double *blk) {
+    __shared__ double buf[2][CS_CH * CS_COLS];
+    constexpr int NL = CS_NT - 64;                          // loader threads
+    constexpr int PER = (CS_CH * CS_COLS + NL - 1) / NL;    // elements per loader per chunk
     const int q = blockIdx.y;
     const double *src = q == 0 ? term_bl : q == 1 ? ae : q == 2 ? aw : term_blc;
-    const int i = 1 + blockIdx.x * 64 + threadIdx.x;
-    const bool ok = i <= d.nim2;
+    const int i0 = 1 + blockIdx.x * CS_COLS;
     const int R = d.njm2 * d.nkm2;
-    double s = 0.0;
-    int kc = 1, jc = 1;   // (k, j) of the chunk's first row
-    for (int r0 = 0; r0 < R; r0 += U) {
-        double t[U];
-        int k = kc, j = jc;
+    const int nch = (R + CS_CH - 1) / CS_CH;
+    const int tid = threadIdx.x;
+    const bool loader = tid >= 64;
+    auto load_chunk = [&](int c, double *dst) {
+        const int lt = tid - 64;
+        double v[PER];
 #pragma unroll
-        for (int u = 0; u < U; ++u) {
-            t[u] = (ok && r0 + u < R) ? src[(size_t)k * d.sk + (size_t)j * d.sj + i] : 0.0;
-            if (++j > d.njm2) {
-                j = 1;
-                ++k;
+        for (int u = 0; u < PER; ++u) {
+            const int e = lt + u * NL;
+            const int r = c * CS_CH + e / CS_COLS, i = i0 + e % CS_COLS;
+            v[u] = 0.0;
+            if (e < CS_CH * CS_COLS && r < R && i <= d.nim2) {
+                const int k = 1 + r / d.njm2, j = 1 + r % d.njm2;
+                v[u] = src[(size_t)k * d.sk + (size_t)j * d.sj + i];
             }
         }
-        kc = k;
-        jc = j;
 #pragma unroll
-        for (int u = 0; u < U; ++u)
-            if (r0 + u < R) s += t[u];
+        for (int u = 0; u < PER; ++u) {
+            const int e = lt + u * NL;
+            if (e < CS_CH * CS_COLS) dst[e] = v[u];
+        }
+    };
+    if (loader) load_chunk(0, buf[0]);
+    __syncthreads();
+    double s = 0.0;
+    for (int c = 0; c < nch; ++c) {
+        if (loader) {
+            if (c + 1 < nch) load_chunk(c + 1, buf[(c + 1) & 1]);
+        } else if (tid < CS_COLS) {
+            const double *b = buf[c & 1] + tid;
+            const int rows = min(CS_CH, R - c * CS_CH);
+            int r = 0;
+            for (; r + 16 <= rows; r += 16) {
+                double t[16];
+#pragma unroll
+                for (int u = 0; u < 16; ++u) t[u] = b[(r + u) * CS_COLS];
+#pragma unroll
+                for (int u = 0; u < 16; ++u) s += t[u];
+            }
+            for (; r < rows; ++r) s += b[r * CS_COLS];
+        }
+        __syncthreads();
     }
-    if (ok) blk[(size_t)q * d.ni + i] = s;
+    if (tid < CS_COLS && i0 + tid <= d.nim2) blk[(size_t)q * d.ni + i0 + tid] = s;
 }
 
 // 1-D correction along x by the reference's recurrence (one lane, exact
-// order), CFDSolverManager.cpp:2334-2349; also folds the energy residual
-// partials, CFDSolverManager.cpp:2099.
-__global__ __launch_bounds__(64) void k_block_solve(Dims d, const double *__restrict__ blk, double *work,
-                                                    double *delphi) {
-    if (threadIdx.x != 0) return;
-    const double *bl = blk, *blp = blk + d.ni, *blm = blk + 2 * d.ni, *blc = blk + 3 * d.ni;
-    double *pib = work, *qib = work + d.ni;
-    double pv = blp[1] / bl[1], qv = blc[1] / bl[1];
-    pib[1] = pv;
-    qib[1] = qv;
-    for (int i = 2; i < d.nim1; ++i) {   // the running values stay in registers
-        const double den = bl[i] - blm[i] * pv;
-        pv = blp[i] / den;
-        qv = (blc[i] + blm[i] * qv) / den;
-        pib[i] = pv;
-        qib[i] = qv;
+// order), CFDSolverManager.cpp:2334-2349.  The column sums are staged in LDS
+// first so the recurrence never waits on global memory.
+constexpr int BS_MAX = 1032;   // ni of the longest x-line handled, rounded up
+
+__global__ __launch_bounds__(256) void k_block_solve(Dims d, const double *__restrict__ blk, double *delphi) {
+    __shared__ double bl[BS_MAX], blp[BS_MAX], blm[BS_MAX], blc[BS_MAX], pib[BS_MAX], qib[BS_MAX];
+    for (int i = threadIdx.x; i < d.ni; i += 256) {
+        bl[i] = blk[i];
+        blp[i] = blk[d.ni + i];
+        blm[i] = blk[2 * d.ni + i];
+        blc[i] = blk[3 * d.ni + i];
     }
-    double nxt = qv;
-    delphi[d.nim2] = nxt;
-    for (int i = d.nim2 - 1; i > 0; --i) {
-        nxt = pib[i] * nxt + qib[i];
-        delphi[i] = nxt;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double pv = blp[1] / bl[1], qv = blc[1] / bl[1];
+        pib[1] = pv;
+        qib[1] = qv;
+        for (int i = 2; i < d.nim1; ++i) {   // the running values stay in registers
+            const double den = bl[i] - blm[i] * pv;
+            pv = blp[i] / den;
+            qv = (blc[i] + blm[i] * qv) / den;
+            pib[i] = pv;
+            qib[i] = qv;
+        }
+        double nxt = qv;
+        qib[d.nim2] = nxt;
+        for (int i = d.nim2 - 1; i > 0; --i) {
+            nxt = pib[i] * nxt + qib[i];
+            qib[i] = nxt;
+        }
     }
+    __syncthreads();
+    for (int i = 1 + threadIdx.x; i <= d.nim2; i += 256) delphi[i] = qib[i];
 }
 
 __global__ __launch_bounds__(256) void k_add_delphi(Dims d, const double *__restrict__ delphi, double *phi) {
@@ -824,9 +862,9 @@ void launch_props(hipStream_t st, const Dims &d, DevFields f, const amcfd_params
 }
 
 void launch_block_correction(hipStream_t st, const Dims &d, DevFields f, Partials pr, DevScalars *sc) {
-    hipLaunchKernelGGL(k_column_sums, dim3((d.nim2 + 63) / 64, 4), dim3(64), 0, st, d, f.sp, f.ae, f.aw,
+    hipLaunchKernelGGL(k_column_sums, dim3((d.nim2 + CS_COLS - 1) / CS_COLS, 4), dim3(CS_NT), 0, st, d, f.sp, f.ae, f.aw,
                        f.apnot, pr.blk);
-    hipLaunchKernelGGL(k_block_solve, dim3(1), dim3(64), 0, st, d, pr.blk, pr.blk + 4 * (size_t)d.ni, pr.delphi);
+    hipLaunchKernelGGL(k_block_solve, dim3(1), dim3(256), 0, st, d, pr.blk, pr.delphi);
     hipLaunchKernelGGL(k_add_delphi, energy_grid(d), dim3(64, 4), 0, st, d, pr.delphi, f.enthalpy);
 }
 

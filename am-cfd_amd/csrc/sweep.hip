@@ -86,26 +86,78 @@ __device__ __forceinline__ double div_pos(double x, double b, double y) {
 // Per-line pivots of the reference recurrence.  den[i] = ap - aw*pr[i-1]
 // (plus the melt-pool guard) and pr[i] = ae/den[i] depend only on the
 // coefficients, which are fixed for the whole sweep, so they are computed
-// once for every line of the sweep in parallel (one lane per line), with
-// y[i] = RN(1/den[i]).
-__global__ __launch_bounds__(256) void k_line_prep(Dims d, const double *__restrict__ aw,
-                                                   const double *__restrict__ ap, const double *__restrict__ ae,
-                                                   double *pr_o, double *den_o, double *y_o, int ibeg, int iend,
-                                                   int jbeg, int jend, int kbeg, int kend, int guard) {
+// once for every line of the sweep in parallel, with y[i] = RN(1/den[i]).
+// One wave serves 64 lines (one lane each); the rows move through LDS in
+// tiles of LP_CH so that global loads and stores are coalesced row segments,
+// and the next tile is in flight while the lanes work through the current one.
+constexpr int LP_CH = 16;
+
+__global__ __launch_bounds__(64) void k_line_prep(Dims d, const double *__restrict__ aw,
+                                                  const double *__restrict__ ap, const double *__restrict__ ae,
+                                                  double *pr_o, double *den_o, double *y_o, int ibeg, int iend,
+                                                  int jbeg, int jend, int kbeg, int kend, int guard) {
+    constexpr int LD = LP_CH + 1;   // padded row stride of a line in the tiles
+    __shared__ double T[3][64 * LD], O[3][64 * LD];
+    const int lane = threadIdx.x;
     const int nj = jend - jbeg + 1;
-    const int line = blockIdx.x * blockDim.x + threadIdx.x;
-    if (line >= nj * (kend - kbeg + 1)) return;
-    const int k = kbeg + line / nj, j = jbeg + line % nj;
-    const size_t row = (size_t)k * d.sk + (size_t)j * d.sj;
+    const int nlines = nj * (kend - kbeg + 1);
+    const int line0 = blockIdx.x * 64;
+    const int n = iend - ibeg + 1;
+    const int nch = (n + LP_CH - 1) / LP_CH;
+    // element e of a tile: line e / LP_CH of the wave, row e % LP_CH
+    auto cell = [&](int e, int c, size_t &addr) -> bool {
+        const int line = line0 + e / LP_CH, i = ibeg + c * LP_CH + e % LP_CH;
+        if (line >= nlines || i > iend) return false;
+        addr = (size_t)(kbeg + line / nj) * d.sk + (size_t)(jbeg + line % nj) * d.sj + i;
+        return true;
+    };
+    double v0[LP_CH], v1[LP_CH], v2[LP_CH];
+    auto fetch = [&](int c) {
+#pragma unroll
+        for (int u = 0; u < LP_CH; ++u) {
+            size_t a;
+            const bool ok = cell(lane + 64 * u, c, a);
+            v0[u] = ok ? ap[a] : 1.0;
+            v1[u] = ok ? aw[a] : 0.0;
+            v2[u] = ok ? ae[a] : 0.0;
+        }
+    };
+    fetch(0);
     double prv = 0.0;
-    for (int i = ibeg; i <= iend; ++i) {
-        const size_t c = row + i;
-        double den = ap[c] - aw[c] * prv;
-        if (guard && fabs(den) <= 1e-12) den += AMCFD_SMALL;
-        prv = ae[c] / den;
-        pr_o[c] = prv;
-        den_o[c] = den;
-        y_o[c] = 1.0 / den;
+    for (int c = 0; c < nch; ++c) {
+#pragma unroll
+        for (int u = 0; u < LP_CH; ++u) {
+            const int e = lane + 64 * u;
+            const int o = (e / LP_CH) * LD + e % LP_CH;
+            T[0][o] = v0[u];
+            T[1][o] = v1[u];
+            T[2][o] = v2[u];
+        }
+        __syncthreads();
+        if (c + 1 < nch) fetch(c + 1);
+        const int rows = min(LP_CH, n - c * LP_CH);
+        for (int r = 0; r < rows; ++r) {
+            const int o = lane * LD + r;
+            double den = T[0][o] - T[1][o] * prv;
+            if (guard && fabs(den) <= 1e-12) den += AMCFD_SMALL;
+            prv = T[2][o] / den;
+            O[0][o] = prv;
+            O[1][o] = den;
+            O[2][o] = 1.0 / den;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int u = 0; u < LP_CH; ++u) {
+            const int e = lane + 64 * u;
+            size_t a;
+            if (cell(e, c, a)) {
+                const int o = (e / LP_CH) * LD + e % LP_CH;
+                pr_o[a] = O[0][o];
+                den_o[a] = O[1][o];
+                y_o[a] = O[2][o];
+            }
+        }
+        __syncthreads();
     }
 }
 
@@ -315,7 +367,7 @@ void launch_line_sweeps(hipStream_t st, const Dims &d, DevFields f, double *phi,
     if (plan.ntasks == 0 || iend < ibeg) return;
     Coefs cf{f.an, f.as, f.ae, f.aw, f.at, f.ab, f.ap, f.su, f.lpr, f.lden, f.ly};
     const int nlines = (plan.kend - plan.kbeg + 1) * (plan.jend - plan.jbeg + 1);
-    hipLaunchKernelGGL(k_line_prep, dim3((nlines + 255) / 256), dim3(256), 0, st, d, f.aw, f.ap, f.ae, f.lpr, f.lden,
+    hipLaunchKernelGGL(k_line_prep, dim3((nlines + 63) / 64), dim3(64), 0, st, d, f.aw, f.ap, f.ae, f.lpr, f.lden,
                        f.ly, ibeg, iend, plan.jbeg, plan.jend, plan.kbeg, plan.kend, guard);
     // short lines (the melt-pool box) get a lighter instantiation: fewer
     // registers, two workgroups per CU
