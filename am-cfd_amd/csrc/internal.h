@@ -91,7 +91,7 @@ struct LaserParams {
 // partial-sum buffers (deterministic two-pass reductions)
 struct Partials {
     double *energy;     // [nblk_e][2]       resid sumd, sumh
-    double *blk;        // [nrow_e][4][ni]   block-correction column sums
+    double *blk;        // [4][ni]           block-correction column sums
     double *h2t;        // [nblk_t][10]      tpeak + fluxes + accul/heatvol
     double *flow;       // [4][nblk_f][3]    per variable: sumd / umax / denom
     double *bc;         // [nblk_b]          ahtoploss
@@ -134,6 +134,9 @@ void launch_finalize(hipStream_t st, const Dims &d, Partials pr, DevScalars *sc,
                      int flow_done, int steady, double width);
 void launch_end_step(hipStream_t st, const Dims &d, DevFields f, const amcfd_params &p, double track,
                      double trackid);
+// pivots of every line of a sweep (must precede launch_line_sweeps on the stream)
+void launch_line_prep(hipStream_t st, const Dims &d, DevFields f, const SweepPlan &plan, int ibeg, int iend,
+                      int guard);
 void launch_line_sweeps(hipStream_t st, const Dims &d, DevFields f, double *phi, const SweepPlan &plan,
                         int ibeg, int iend, int guard, unsigned long long *counter,
                         unsigned long long counter_base, int *versions, int epoch, DevScalars *sc,

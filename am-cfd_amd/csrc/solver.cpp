@@ -127,8 +127,8 @@ int Solver::create(const amcfd_params &p, int mode) {
     Box whole{1, d_.nim2, 1, d_.njm2, 2};
     int nf = flow_partial_blocks(whole, d_) + 64;
     if ((st = alloc(&pr_.energy, (size_t)2 * pr_.nblk_e))) return st;
-    // block correction: 4 column sums + 2 recurrence work rows, each ni long
-    if ((st = alloc(&pr_.blk, (size_t)6 * d_.ni + 8))) return st;
+    // block correction: 4 column sums, each ni long
+    if ((st = alloc(&pr_.blk, (size_t)4 * d_.ni + 8))) return st;
     if ((st = alloc(&pr_.h2t, (size_t)10 * pr_.nblk_t))) return st;
     if ((st = alloc(&pr_.flow, (size_t)4 * nf * 3))) return st;
     pr_.nblk_b = (d_.njm2 * d_.nim2 + d_.nkm2 * d_.njm2 + d_.nkm2 * d_.nim2 + 255) / 256;
@@ -360,6 +360,7 @@ int Solver::sweep(double *phi, SweepPlan &plan, int ibeg, int iend, int jbeg, in
     if (plan.ntasks == 0 || iend < ibeg) return 0;
     int nblocks = std::min(plan.ntasks, std::max(2 * plan.width, 1));
     nblocks = std::min(nblocks, 2048);
+    launch_line_prep(st_, d_, f_, plan, ibeg, iend, guard);
     int slot;
     tic(timer, &slot);
     launch_line_sweeps(st_, d_, f_, phi, plan, ibeg, iend, guard, counter_, counter_base_, versions_, epoch_, sc_,
@@ -468,6 +469,8 @@ int Solver::end_step() {
         if (timet_ <= tool_[6 * (r - 1)]) trackid = (double)(int)tool_[6 * (r - 1) + 5];
     launch_end_step(st_, d_, f_, p_, (double)trackindx_, trackid);
     HIPCALL(hipGetLastError());
+    // the step is complete when this returns, as the reference's is
+    HIPCALL(hipStreamSynchronize(st_));
     return AMCFD_OK;
 }
 

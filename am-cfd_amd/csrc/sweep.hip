@@ -361,14 +361,19 @@ void build_sweep_tasks(int kbeg, int kend, int jbeg, int jend, std::vector<int> 
 
 int sweep_max_line() { return SW_MAXN; }
 
+void launch_line_prep(hipStream_t st, const Dims &d, DevFields f, const SweepPlan &plan, int ibeg, int iend,
+                      int guard) {
+    if (plan.ntasks == 0 || iend < ibeg) return;
+    const int nlines = (plan.kend - plan.kbeg + 1) * (plan.jend - plan.jbeg + 1);
+    hipLaunchKernelGGL(k_line_prep, dim3((nlines + 63) / 64), dim3(64), 0, st, d, f.aw, f.ap, f.ae, f.lpr, f.lden,
+                       f.ly, ibeg, iend, plan.jbeg, plan.jend, plan.kbeg, plan.kend, guard);
+}
+
 void launch_line_sweeps(hipStream_t st, const Dims &d, DevFields f, double *phi, const SweepPlan &plan, int ibeg,
                         int iend, int guard, unsigned long long *counter, unsigned long long counter_base,
                         int *versions, int epoch, DevScalars *sc, int nblocks) {
     if (plan.ntasks == 0 || iend < ibeg) return;
     Coefs cf{f.an, f.as, f.ae, f.aw, f.at, f.ab, f.ap, f.su, f.lpr, f.lden, f.ly};
-    const int nlines = (plan.kend - plan.kbeg + 1) * (plan.jend - plan.jbeg + 1);
-    hipLaunchKernelGGL(k_line_prep, dim3((nlines + 63) / 64), dim3(64), 0, st, d, f.aw, f.ap, f.ae, f.lpr, f.lden,
-                       f.ly, ibeg, iend, plan.jbeg, plan.jend, plan.kbeg, plan.kend, guard);
     // short lines (the melt-pool box) get a lighter instantiation: fewer
     // registers, two workgroups per CU
     const int n = iend - ibeg + 1;

@@ -17,6 +17,7 @@ import argparse
 import importlib
 import json
 import os
+import subprocess
 import sys
 import tempfile
 import time
@@ -26,6 +27,10 @@ sys.path.insert(0, ROOT)
 sys.path.insert(0, os.path.join(ROOT, "tests", "golden"))
 
 METRIC = "cell-steps/s on 1 GPU, mixed+warm-start vs plain FP64, ≤1% melt-pool dims"
+# HBM bytes per launch of the enthalpy sweep on the 400x160x80 track, from two
+# rocprofv3 --pmc runs (FETCH_SIZE x 2.00, WRITE_SIZE x 1.00, factors from a
+# plain 512 MB device copy): profiles/k_line_sweeps_track400_hbm_traffic.md
+SWEEP_TRAFFIC_BYTES = {"track400": 1.966e9 + 0.179e9}
 HBM_PEAK_GBS = 8000.0
 CASE = "track400"
 
@@ -36,9 +41,36 @@ def sweep_bytes_per_launch(dom, elem=8):
     return 4 * (8 + 1 + 1) * elem * dom
 
 
+def cpu_baseline_reference(case, dom, gpu_iters_per_step, threads, steps):
+    """Time the reference's own solver classes (oracle/_ref/ref_harness, built
+    from the reference sources with -O3 -fopenmp) on the first `steps` time
+    steps of the same deck with `threads` OpenMP threads; scale the measured
+    seconds per SIMPLE iteration to cell-steps/s with the GPU's iterations per
+    step.  Returns None when the harness is not built."""
+    import decks
+    exe = os.path.join(ROOT, "oracle", "_ref", "ref_harness")
+    if not os.path.exists(exe):
+        return None
+    tmp = tempfile.mkdtemp(prefix="amcfd_refcpu_")
+    deck = decks.write_case(case, tmp)
+    env = dict(os.environ, OMP_NUM_THREADS=str(threads))
+    out = subprocess.run([exe, os.path.basename(deck), ".", "0", str(steps)], cwd=tmp, env=env,
+                         capture_output=True, text=True, timeout=900, check=True)
+    info = json.loads(out.stdout.strip().splitlines()[-1])
+    per_iter = info["compute_s"] / max(info["itertot"], 1)
+    value = dom / (per_iter * max(gpu_iters_per_step, 1.0))
+    return {"value": value, "unit": "cell-steps/s", "cores": threads, "kind": "reference",
+            "sample": "reference solver classes (CFDSolverManager etc., g++ -O3 -fopenmp), OMP_NUM_THREADS=%d, "
+                      "first %d step(s) = %d SIMPLE iterations of the 400x160x80 track in %.1f s "
+                      "(%.4f s/iteration), scaled by the GPU's %.1f iterations/step"
+                      % (threads, info["steps"], info["itertot"], info["compute_s"], per_iter,
+                         gpu_iters_per_step)}
+
+
 def cpu_baseline(deck_params, dom, gpu_iters_per_step, budget_s=20.0):
     """Time the C restatement (1 thread) on the first SIMPLE iterations of the
-    same workload; scale to cell-steps/s with the GPU's iterations per step."""
+    same workload; scale to cell-steps/s with the GPU's iterations per step.
+    Used when the reference harness is not built."""
     import oracle
     orc = oracle.Oracle(deck_params)
     t0 = time.perf_counter()
@@ -66,6 +98,9 @@ def main():
     ap.add_argument("--case", default=CASE)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
+    ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1),
+                    help="OpenMP threads of the reference CPU baseline (the host cores used)")
+    ap.add_argument("--cpu-steps", type=int, default=2, help="time steps in the CPU baseline sample")
     ap.add_argument("--copy-probe-mb", type=float, default=0.0,
                     help="only run one device copy of this size (counter calibration)")
     args = ap.parse_args()
@@ -137,15 +172,19 @@ def main():
                    "iters_per_step": iters / args.steps, "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "kernel": "k_line_sweeps (enthalpy, whole domain)",
                      "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": (achieved / HBM_PEAK_GBS) if achieved else None, "traffic": None,
+                     "frac": (achieved / HBM_PEAK_GBS) if achieved else None,
+                     "traffic": SWEEP_TRAFFIC_BYTES.get(args.case),
                      "launches": launches, "avg_launch_ms": avg_ms,
                      "algorithmic_bytes_per_launch": algo},
         "kernels": other,
     }
+    result["cpu_baseline"] = None
     if not args.no_cpu:
-        result["cpu_baseline"] = cpu_baseline(p, dom, iters / args.steps, args.cpu_budget)
-    else:
-        result["cpu_baseline"] = None
+        print("cpu baseline: reference harness, %d threads" % args.cpu_threads, file=sys.stderr, flush=True)
+        result["cpu_baseline"] = cpu_baseline_reference(args.case, dom, iters / args.steps, args.cpu_threads,
+                                                        args.cpu_steps)
+        if result["cpu_baseline"] is None:
+            result["cpu_baseline"] = cpu_baseline(p, dom, iters / args.steps, args.cpu_budget)
     solver.close()
     print(json.dumps(result))
 
