@@ -167,84 +167,99 @@ __global__ __launch_bounds__(64) void k_line_prep(Dims d, const double *__restri
 // with pass-through rows (aw = 1, rhs = -0, den = y = 1, pr = 0: x = -0 + q
 // is q for either sign of zero, and q / 1 is q), so there are no per-row
 // bounds.  The running qr and the back-substitution value travel from lane to
-// lane with readlane.  LDS inputs (index = real row): AW, DEN, Y, PR, D = rhs;
+// lane with readlane.  The coefficient registers are filled (load_coefs)
+// before the line's dependencies are awaited; solve() then needs only the
+// right-hand side.  LDS inputs (index = real row): AW, DEN, Y, PR, D = rhs;
 // the solution overwrites D.
 template <int M>
-__device__ __forceinline__ void rhs_wave(int n, const double *AW, const double *DEN, const double *Y, const double *PR, double *D,
-                         double lo, double hi) {
-    const int lane = threadIdx.x;
-    const int nl = (n + M - 1) / M;
-    const int pad = nl * M - n;
-    double raw[M], rden[M], ry[M], rd[M], rx[M];
+struct LineRegs {
+    double raw[M], rden[M], ry[M];
+
+    __device__ __forceinline__ void load_coefs(int n, const double *AW, const double *DEN, const double *Y) {
+        const int lane = threadIdx.x;
+        const int nl = (n + M - 1) / M, pad = nl * M - n;
 #pragma unroll
-    for (int r = 0; r < M; ++r) {
-        const int p = lane * M + r - pad;
-        const bool real = (p >= 0) && (lane < nl);
-        raw[r] = real ? AW[p] : 1.0;
-        rden[r] = real ? DEN[p] : 1.0;
-        ry[r] = real ? Y[p] : 1.0;
-        rd[r] = real ? D[p] : -0.0;
-        rx[r] = 0.0;
-    }
-    double q = lo;
-    for (int L = 0; L < nl; ++L) {
-        if (lane == L) {
-#pragma unroll
-            for (int r = 0; r < M; ++r) {
-                const double x = rd[r] + raw[r] * q;
-                rx[r] = x;
-                q = div_pos(x, rden[r], ry[r]);
-                rd[r] = q;
-            }
+        for (int r = 0; r < M; ++r) {
+            const int p = lane * M + r - pad;
+            const bool real = (p >= 0) && (lane < nl);
+            raw[r] = real ? AW[p] : 1.0;
+            rden[r] = real ? DEN[p] : 1.0;
+            ry[r] = real ? Y[p] : 1.0;
         }
-        q = lane_bcast(q, L);
     }
-    // the fast division's preconditions, checked for every row in parallel;
-    // a line that misses one is solved again with IEEE division
-    bool ok = true;
+
+    __device__ __forceinline__ void solve(int n, const double *PR, double *D, double lo, double hi) {
+        const int lane = threadIdx.x;
+        const int nl = (n + M - 1) / M, pad = nl * M - n;
+        double rd[M], rx[M];
 #pragma unroll
-    for (int r = 0; r < M; ++r) {
-        const double ax = fabs(rx[r]);
-        ok = ok && (rden[r] > 0.0) && (rden[r] < __builtin_inf()) &&
-             ((ax == 0.0) || ((ax >= 0x1p-900) && (ax < __builtin_inf())));
-    }
-    if (!__all(ok)) {
-        q = lo;
+        for (int r = 0; r < M; ++r) {
+            const int p = lane * M + r - pad;
+            rd[r] = ((p >= 0) && (lane < nl)) ? D[p] : -0.0;
+            rx[r] = 0.0;
+        }
+        double q = lo;
         for (int L = 0; L < nl; ++L) {
             if (lane == L) {
 #pragma unroll
                 for (int r = 0; r < M; ++r) {
-                    const int p = lane * M + r - pad;
-                    const double dr = (p >= 0) ? D[p] : -0.0;
-                    q = (dr + raw[r] * q) / rden[r];
+                    const double x = rd[r] + raw[r] * q;
+                    rx[r] = x;
+                    q = div_pos(x, rden[r], ry[r]);
                     rd[r] = q;
                 }
             }
             q = lane_bcast(q, L);
         }
-    }
+        // the fast division's preconditions, for every row at once (bitwise
+        // ops, no branches); a line that misses one is solved again with IEEE
+        // division
+        int bad = 0;
 #pragma unroll
-    for (int r = 0; r < M; ++r) {
-        const int p = lane * M + r - pad;
-        raw[r] = ((p >= 0) && (lane < nl)) ? PR[p] : 0.0;
-    }
-    double nxt = hi;
-    for (int L = nl - 1; L >= 0; --L) {
-        if (lane == L) {
+        for (int r = 0; r < M; ++r) {
+            const double ax = fabs(rx[r]);
+            bad |= (int)!(rden[r] > 0.0) | (int)!(rden[r] < __builtin_inf()) |
+                   ((int)(ax != 0.0) & ((int)!(ax >= 0x1p-900) | (int)!(ax < __builtin_inf())));
+        }
+        if (__any(bad)) {
+            q = lo;
+            for (int L = 0; L < nl; ++L) {
+                if (lane == L) {
 #pragma unroll
-            for (int r = M - 1; r >= 0; --r) {
-                nxt = raw[r] * nxt + rd[r];
-                rd[r] = nxt;
+                    for (int r = 0; r < M; ++r) {
+                        const int p = lane * M + r - pad;
+                        const double dr = (p >= 0) ? D[p] : -0.0;
+                        q = (dr + raw[r] * q) / rden[r];
+                        rd[r] = q;
+                    }
+                }
+                q = lane_bcast(q, L);
             }
         }
-        nxt = lane_bcast(nxt, L);
-    }
+        double rpr[M];
 #pragma unroll
-    for (int r = 0; r < M; ++r) {
-        const int p = lane * M + r - pad;
-        if (p >= 0 && lane < nl) D[p] = rd[r];
+        for (int r = 0; r < M; ++r) {
+            const int p = lane * M + r - pad;
+            rpr[r] = ((p >= 0) && (lane < nl)) ? PR[p] : 0.0;
+        }
+        double nxt = hi;
+        for (int L = nl - 1; L >= 0; --L) {
+            if (lane == L) {
+#pragma unroll
+                for (int r = M - 1; r >= 0; --r) {
+                    nxt = rpr[r] * nxt + rd[r];
+                    rd[r] = nxt;
+                }
+            }
+            nxt = lane_bcast(nxt, L);
+        }
+#pragma unroll
+        for (int r = 0; r < M; ++r) {
+            const int p = lane * M + r - pad;
+            if (p >= 0 && lane < nl) D[p] = rd[r];
+        }
     }
-}
+};
 
 // Line data move between workgroups (and XCDs) with device-coherent
 // relaxed atomic loads and stores, so no L2-wide writeback / invalidate is
@@ -271,6 +286,7 @@ __global__ __launch_bounds__(NT) void k_line_sweeps(Dims d, Coefs cf, double *ph
     __shared__ int s_task;
     const int tid = threadIdx.x;
     const int n = iend - ibeg + 1;
+    LineRegs<M> regs;
     for (;;) {
         if (tid == 0) {
             unsigned long long t = atomicAdd(counter, 1ULL) - base;
@@ -299,6 +315,10 @@ __global__ __launch_bounds__(NT) void k_line_sweeps(Dims d, Coefs cf, double *ph
             }
         }
         const double lo = phi[row + ibeg - 1], hi = phi[row + iend + 1];   // fixed end values
+        __syncthreads();
+        // wave 0 takes its coefficient registers while lane 0 awaits the
+        // neighbour lines
+        if (tid < 64) regs.load_coefs(n, A, B, C);
         if (tid == 0) {
             bool ok = true;
             const int *err = &sc->error;
@@ -321,7 +341,7 @@ __global__ __launch_bounds__(NT) void k_line_sweeps(Dims d, Coefs cf, double *ph
             }
         }
         __syncthreads();
-        if (tid < 64) rhs_wave<M>(n, A, B, C, E, D, lo, hi);
+        if (tid < 64) regs.solve(n, E, D, lo, hi);
         __syncthreads();
 #pragma unroll
         for (int r = 0; r < MAXR; ++r) {
