@@ -66,8 +66,7 @@ class Solver {
     SweepPlan plan_dom_, plan_box_;
     unsigned long long *counter_ = nullptr;
     unsigned long long counter_base_ = 0;
-    int *versions_ = nullptr;
-    int epoch_ = 4;
+    double *pass_buf_ = nullptr;   // four field-sized pass buffers of the line sweeps
     int profile_ = 0;
     std::vector<hipEvent_t> ev_pool_;
     std::vector<std::pair<std::string, std::pair<int, int>>> ev_pending_;

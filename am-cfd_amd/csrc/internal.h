@@ -134,13 +134,15 @@ void launch_finalize(hipStream_t st, const Dims &d, Partials pr, DevScalars *sc,
                      int flow_done, int steady, double width);
 void launch_end_step(hipStream_t st, const Dims &d, DevFields f, const amcfd_params &p, double track,
                      double trackid);
-// pivots of every line of a sweep (must precede launch_line_sweeps on the stream)
-void launch_line_prep(hipStream_t st, const Dims &d, DevFields f, const SweepPlan &plan, int ibeg, int iend,
-                      int guard);
-void launch_line_sweeps(hipStream_t st, const Dims &d, DevFields f, double *phi, const SweepPlan &plan,
-                        int ibeg, int iend, int guard, unsigned long long *counter,
-                        unsigned long long counter_base, int *versions, int epoch, DevScalars *sc,
-                        int nblocks);
+// pivots of every line of a sweep and the pass buffers' "pending" fill (must
+// precede launch_line_sweeps on the stream)
+void launch_line_prep(hipStream_t st, const Dims &d, DevFields f, double *pb, size_t pstride, const SweepPlan &plan,
+                      int ibeg, int iend, int guard);
+// one Gauss-Seidel x-line sweep; pb holds four pass buffers of pstride
+// doubles each (the field's size)
+void launch_line_sweeps(hipStream_t st, const Dims &d, DevFields f, double *phi, double *pb, size_t pstride,
+                        const SweepPlan &plan, int ibeg, int iend, int guard, unsigned long long *counter,
+                        unsigned long long counter_base, DevScalars *sc, int nblocks);
 
 // wavefront task list for the Gauss-Seidel line sweeps (host side)
 void build_sweep_tasks(int kbeg, int kend, int jbeg, int jend, std::vector<int> &tasks, int &width);

@@ -146,10 +146,8 @@ int Solver::create(const amcfd_params &p, int mode) {
     allocs_.push_back(v);
     counter_ = (unsigned long long *)v;
     HIPCALL(hipMemsetAsync(counter_, 0, sizeof(unsigned long long), st_));
-    HIPCALL(hipMalloc(&v, sizeof(int) * (size_t)d_.nkf * d_.nj));
-    allocs_.push_back(v);
-    versions_ = (int *)v;
-    HIPCALL(hipMemsetAsync(versions_, 0, sizeof(int) * (size_t)d_.nkf * d_.nj, st_));
+    // four pass buffers for the line sweeps
+    if ((st = alloc(&pass_buf_, 4 * nn))) return st;
 
     scanvel_ = p_.scanvel;
     set_active_nk(p_.nk_active0);
@@ -360,15 +358,14 @@ int Solver::sweep(double *phi, SweepPlan &plan, int ibeg, int iend, int jbeg, in
     if (plan.ntasks == 0 || iend < ibeg) return 0;
     int nblocks = std::min(plan.ntasks, std::max(2 * plan.width, 1));
     nblocks = std::min(nblocks, 2048);
-    launch_line_prep(st_, d_, f_, plan, ibeg, iend, guard);
+    launch_line_prep(st_, d_, f_, pass_buf_, (size_t)d_.ni * d_.nj * d_.nkf, plan, ibeg, iend, guard);
     int slot;
     tic(timer, &slot);
-    launch_line_sweeps(st_, d_, f_, phi, plan, ibeg, iend, guard, counter_, counter_base_, versions_, epoch_, sc_,
-                       nblocks);
+    launch_line_sweeps(st_, d_, f_, phi, pass_buf_, (size_t)d_.ni * d_.nj * d_.nkf, plan, ibeg, iend, guard,
+                       counter_, counter_base_, sc_, nblocks);
     HIPCALL(hipGetLastError());
     toc(timer, slot);
     counter_base_ += (unsigned long long)plan.ntasks + (unsigned long long)nblocks;
-    epoch_ += 4;
     return 0;
 }
 

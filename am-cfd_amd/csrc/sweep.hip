@@ -7,18 +7,24 @@
 // sequence, so the sweep is Gauss-Seidel in j and k.  Run single-threaded
 // that order is the reference's definition of the result.
 //
-// Here every line solve is a task; task (pass p = 2ks+js, k, j) may start once
-//   line (k, j-1) has finished pass p,
-//   line (k, j+1) has finished pass p-1            (p > 0),
-//   line (k+1, j) has finished pass 2ks+1,
-//   line (k-1, j) has finished pass 1              (ks = 1),
-// which is exactly the set of read-after-write and write-after-read
-// constraints of the sequential loop nest.  Tasks are claimed from an atomic
-// counter in increasing  tau = 6ks + 3(kend-k) + 2js + (j-jbeg)  (every
-// dependency has a smaller tau), so a claimed task only ever waits on tasks
-// already claimed by running workgroups: the schedule cannot deadlock.  Each
-// line keeps a version word (epoch + passes done); waits are bounded and raise
-// an error flag instead of hanging.
+// Here every line solve is a task; task (pass p = 2ks+js, k, j) needs
+//   line (k, j-1) as left by pass p,
+//   line (k, j+1) as left by pass p-1               (the sweep's input if p = 0),
+//   line (k+1, j) as left by pass 2ks+1,
+//   line (k-1, j) as left by pass 1                 (the input if ks = 0),
+// which is exactly what the sequential loop nest reads.  Each pass writes its
+// lines into a buffer of its own (pass[p]), so every value a task reads is
+// written once per sweep and nothing is overwritten while it may still be
+// read; pass 3 also writes the field in place, which is safe because every
+// task that reads a line's input value precedes that line's pass-3 task in
+// the dependency order.  The pass buffers are filled with a signalling-NaN
+// pattern that no arithmetic produces before the sweep, and a task polls its
+// neighbour values until the pattern is gone: the data is its own flag.
+// Tasks are claimed from an atomic counter in increasing
+//   tau = 6ks + 3(kend-k) + 2js + (j-jbeg)
+// (every dependency has a smaller tau), so a claimed task only ever waits on
+// tasks already claimed by running workgroups: the schedule cannot deadlock.
+// Waits are bounded and raise an error flag instead of hanging.
 //
 // Line solver: the reference's pr/qr recurrence with every rounding of the
 // reference (the SIMPLE iteration amplifies last-bit differences by ~1e7 per
@@ -26,6 +32,8 @@
 // the coefficients and are computed for all lines up front (k_line_prep);
 // the wavefront carries the right-hand-side recurrence, one wave per line,
 // rows held in registers.
+
+#include <algorithm>
 
 #include "fv.h"
 
@@ -44,21 +52,9 @@ struct Coefs {
     const double *pr, *den, *y;   // per-line pivots from k_line_prep (exact path)
 };
 
-__device__ __forceinline__ int load_version(const int *v) {
-    return __hip_atomic_load(v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-// Wait until *v >= need.  Gives up after SPIN_LIMIT polls, or at once when
-// another wait of this launch has already given up (error flag set).
-__device__ bool wait_version(const int *v, int need, const int *error) {
-    long long spins = 0;
-    while (load_version(v) < need) {
-        __builtin_amdgcn_s_sleep(1);
-        if (++spins > SPIN_LIMIT) return false;
-        if ((spins & 255) == 0 && load_version(error) != 0) return false;
-    }
-    return true;
-}
+// "not written yet" marker of the pass buffers: a signalling NaN with a
+// payload; arithmetic on the GPU only ever yields the canonical quiet NaN
+constexpr unsigned long long SW_PENDING = 0x7ff4dead0000beefull;
 
 __device__ __forceinline__ double lane_bcast(double v, int lane) {
     int lo = __builtin_amdgcn_readlane(__double2loint(v), lane);
@@ -262,25 +258,56 @@ struct LineRegs {
 };
 
 // Line data move between workgroups (and XCDs) with device-coherent
-// relaxed atomic loads and stores, so no L2-wide writeback / invalidate is
-// needed: a producer stores its line, waits for the stores to complete
-// (s_waitcnt) and only then publishes the line's version word; a consumer
-// reads the neighbour lines only after it has seen their versions.
+// relaxed atomic loads and stores (no L2-wide writeback or invalidate).  A
+// consumer polls a pass-buffer value until it is no longer SW_PENDING.
 __device__ __forceinline__ double ld_line(const double *p) {
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 __device__ __forceinline__ void st_line(double *p, double v) {
     __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
+__device__ __forceinline__ bool pending(double v) {
+    return __builtin_bit_cast(unsigned long long, v) == SW_PENDING;
+}
+// Re-read v from p until it has been written; false if the bounded wait
+// expired (or another wait of this launch already gave up).
+__device__ bool await_value(const double *p, double &v, const int *error) {
+    long long spins = 0;
+    while (pending(v)) {
+        __builtin_amdgcn_s_sleep(1);
+        v = ld_line(p);
+        if (++spins > SPIN_LIMIT) return false;
+        if ((spins & 255) == 0 && __hip_atomic_load(error, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0)
+            return false;
+    }
+    return true;
+}
+
+// pass-buffer fill before a sweep: every cell the sweep's tasks write
+__global__ __launch_bounds__(256) void k_fill_pending(Dims d, double *pb, size_t pstride, int ibeg, int iend,
+                                                      int jbeg, int jend, int kbeg, int kend) {
+    const int n = iend - ibeg + 1, nj = jend - jbeg + 1;
+    const size_t total = (size_t)n * nj * (kend - kbeg + 1);
+    const double v = __builtin_bit_cast(double, SW_PENDING);
+    for (size_t e = (size_t)blockIdx.x * 256 + threadIdx.x; e < total; e += (size_t)gridDim.x * 256) {
+        const int i = ibeg + (int)(e % n);
+        const size_t line = e / n;
+        const int j = jbeg + (int)(line % nj), k = kbeg + (int)(line / nj);
+        const size_t c = (size_t)k * d.sk + (size_t)j * d.sj + i;
+#pragma unroll
+        for (int p = 0; p < 4; ++p) pb[p * pstride + c] = v;
+    }
+}
 
 // One x-line solve per task; M rows per lane in the recurrence (64 M >= n).
 // DOMAIN marks the instantiation for the whole-domain enthalpy sweep, so it
 // shows up as its own kernel in a profile; the code is the same.
 template <int NT, int M, bool DOMAIN>
-__global__ __launch_bounds__(NT) void k_line_sweeps(Dims d, Coefs cf, double *phi, const int *__restrict__ tasks,
-                                                    int ntasks, int ibeg, int iend, int jbeg, int jend, int kbeg,
-                                                    int kend, unsigned long long *counter, unsigned long long base,
-                                                    int *versions, int epoch, DevScalars *sc) {
+__global__ __launch_bounds__(NT) void k_line_sweeps(Dims d, Coefs cf, double *phi, double *pb, size_t pstride,
+                                                    const int *__restrict__ tasks, int ntasks, int ibeg, int iend,
+                                                    int jbeg, int jend, int kbeg, int kend,
+                                                    unsigned long long *counter, unsigned long long base,
+                                                    DevScalars *sc) {
     constexpr int MAXR = SW_MAXN / NT;
     __shared__ double A[SW_MAXN], B[SW_MAXN], C[SW_MAXN], D[SW_MAXN], E[SW_MAXN];
     __shared__ int s_task;
@@ -316,43 +343,53 @@ __global__ __launch_bounds__(NT) void k_line_sweeps(Dims d, Coefs cf, double *ph
         }
         const double lo = phi[row + ibeg - 1], hi = phi[row + iend + 1];   // fixed end values
         __syncthreads();
-        // wave 0 takes its coefficient registers while lane 0 awaits the
-        // neighbour lines
         if (tid < 64) regs.load_coefs(n, A, B, C);
-        if (tid == 0) {
-            bool ok = true;
-            const int *err = &sc->error;
-            const int *vb = versions;
-            if (j > jbeg) ok &= wait_version(vb + k * d.nj + (j - 1), epoch + pass + 1, err);
-            if (j < jend && pass > 0) ok &= wait_version(vb + k * d.nj + (j + 1), epoch + pass, err);
-            if (k < kend) ok &= wait_version(vb + (k + 1) * d.nj + j, epoch + 2 * ks + 2, err);
-            if (k > kbeg && ks > 0) ok &= wait_version(vb + (k - 1) * d.nj + j, epoch + 2, err);
-            if (!ok) atomicOr(&sc->error, 1);
-        }
-        __syncthreads();
-        // right-hand side from the neighbouring lines, in the reference's order
+        // neighbour lines as the sequential loop nest sees them: a pass buffer
+        // inside the sweep's range, the sweep's input outside it
+        const double *src_t = (k < kend) ? pb + (size_t)(2 * ks + 1) * pstride : phi;
+        const double *src_b = (k > kbeg && ks == 1) ? pb + pstride : phi;
+        const double *src_n = (j < jend && pass > 0) ? pb + (size_t)(pass - 1) * pstride : phi;
+        const double *src_s = (j > jbeg) ? pb + (size_t)pass * pstride : phi;
+        // right-hand side in the reference's order; every load is issued
+        // first, then values still pending are polled
+        double vt[MAXR], vb[MAXR], vn[MAXR], vs[MAXR];
 #pragma unroll
         for (int r = 0; r < MAXR; ++r) {
             const int ii = tid + r * NT;
             if (ii < n) {
                 const size_t c = row + ibeg + ii;
-                D[ii] = rat[r] * ld_line(phi + c + d.sk) + rab[r] * ld_line(phi + c - d.sk) +
-                        ran[r] * ld_line(phi + c + d.sj) + ras[r] * ld_line(phi + c - d.sj) + rsu[r];
+                vt[r] = ld_line(src_t + c + d.sk);
+                vb[r] = ld_line(src_b + c - d.sk);
+                vn[r] = ld_line(src_n + c + d.sj);
+                vs[r] = ld_line(src_s + c - d.sj);
             }
         }
-        __syncthreads();
-        if (tid < 64) regs.solve(n, E, D, lo, hi);
-        __syncthreads();
+        bool ok = true;
 #pragma unroll
         for (int r = 0; r < MAXR; ++r) {
             const int ii = tid + r * NT;
-            if (ii < n) st_line(phi + row + ibeg + ii, D[ii]);
+            if (ii < n) {
+                const size_t c = row + ibeg + ii;
+                ok &= await_value(src_t + c + d.sk, vt[r], &sc->error);
+                ok &= await_value(src_b + c - d.sk, vb[r], &sc->error);
+                ok &= await_value(src_n + c + d.sj, vn[r], &sc->error);
+                ok &= await_value(src_s + c - d.sj, vs[r], &sc->error);
+                D[ii] = rat[r] * vt[r] + rab[r] * vb[r] + ran[r] * vn[r] + ras[r] * vs[r] + rsu[r];
+            }
         }
-        __builtin_amdgcn_s_waitcnt(0);   // this thread's line stores have completed
+        if (!ok) atomicOr(&sc->error, 1);
         __syncthreads();
-        if (tid == 0)
-            __hip_atomic_store(versions + k * d.nj + j, epoch + pass + 1, __ATOMIC_RELAXED,
-                               __HIP_MEMORY_SCOPE_AGENT);
+        if (tid < 64) regs.solve(n, E, D, lo, hi);
+        __syncthreads();
+        double *out = pb + (size_t)pass * pstride + row + ibeg;
+#pragma unroll
+        for (int r = 0; r < MAXR; ++r) {
+            const int ii = tid + r * NT;
+            if (ii < n) {
+                st_line(out + ii, D[ii]);
+                if (pass == 3) phi[row + ibeg + ii] = D[ii];
+            }
+        }
     }
 }
 
@@ -381,34 +418,38 @@ void build_sweep_tasks(int kbeg, int kend, int jbeg, int jend, std::vector<int> 
 
 int sweep_max_line() { return SW_MAXN; }
 
-void launch_line_prep(hipStream_t st, const Dims &d, DevFields f, const SweepPlan &plan, int ibeg, int iend,
-                      int guard) {
+void launch_line_prep(hipStream_t st, const Dims &d, DevFields f, double *pb, size_t pstride, const SweepPlan &plan,
+                      int ibeg, int iend, int guard) {
     if (plan.ntasks == 0 || iend < ibeg) return;
     const int nlines = (plan.kend - plan.kbeg + 1) * (plan.jend - plan.jbeg + 1);
     hipLaunchKernelGGL(k_line_prep, dim3((nlines + 63) / 64), dim3(64), 0, st, d, f.aw, f.ap, f.ae, f.lpr, f.lden,
                        f.ly, ibeg, iend, plan.jbeg, plan.jend, plan.kbeg, plan.kend, guard);
+    const size_t cells = (size_t)(iend - ibeg + 1) * nlines;
+    const int fill_blocks = (int)std::min<size_t>((cells + 255) / 256, 4096);
+    hipLaunchKernelGGL(k_fill_pending, dim3(fill_blocks), dim3(256), 0, st, d, pb, pstride, ibeg, iend, plan.jbeg,
+                       plan.jend, plan.kbeg, plan.kend);
 }
 
-void launch_line_sweeps(hipStream_t st, const Dims &d, DevFields f, double *phi, const SweepPlan &plan, int ibeg,
-                        int iend, int guard, unsigned long long *counter, unsigned long long counter_base,
-                        int *versions, int epoch, DevScalars *sc, int nblocks) {
+void launch_line_sweeps(hipStream_t st, const Dims &d, DevFields f, double *phi, double *pb, size_t pstride,
+                        const SweepPlan &plan, int ibeg, int iend, int guard, unsigned long long *counter,
+                        unsigned long long counter_base, DevScalars *sc, int nblocks) {
     if (plan.ntasks == 0 || iend < ibeg) return;
     Coefs cf{f.an, f.as, f.ae, f.aw, f.at, f.ab, f.ap, f.su, f.lpr, f.lden, f.ly};
-    // short lines (the melt-pool box) get a lighter instantiation: fewer
-    // registers, two workgroups per CU
     const int n = iend - ibeg + 1;
+    // short lines (the melt-pool box) get a lighter instantiation: fewer
+    // registers, three waves per SIMD
     if (!guard)
-        hipLaunchKernelGGL((k_line_sweeps<256, SW_M, true>), dim3(nblocks), dim3(256), 0, st, d, cf, phi, plan.tasks,
-                           plan.ntasks, ibeg, iend, plan.jbeg, plan.jend, plan.kbeg, plan.kend, counter, counter_base,
-                           versions, epoch, sc);
-    else if (n <= 64 * SW_M_SHORT)
-        hipLaunchKernelGGL((k_line_sweeps<256, SW_M_SHORT, false>), dim3(nblocks), dim3(256), 0, st, d, cf, phi,
+        hipLaunchKernelGGL((k_line_sweeps<256, SW_M, true>), dim3(nblocks), dim3(256), 0, st, d, cf, phi, pb, pstride,
                            plan.tasks, plan.ntasks, ibeg, iend, plan.jbeg, plan.jend, plan.kbeg, plan.kend, counter,
-                           counter_base, versions, epoch, sc);
+                           counter_base, sc);
+    else if (n <= 64 * SW_M_SHORT)
+        hipLaunchKernelGGL((k_line_sweeps<256, SW_M_SHORT, false>), dim3(nblocks), dim3(256), 0, st, d, cf, phi, pb,
+                           pstride, plan.tasks, plan.ntasks, ibeg, iend, plan.jbeg, plan.jend, plan.kbeg, plan.kend,
+                           counter, counter_base, sc);
     else
-        hipLaunchKernelGGL((k_line_sweeps<256, SW_M, false>), dim3(nblocks), dim3(256), 0, st, d, cf, phi, plan.tasks,
-                           plan.ntasks, ibeg, iend, plan.jbeg, plan.jend, plan.kbeg, plan.kend, counter, counter_base,
-                           versions, epoch, sc);
+        hipLaunchKernelGGL((k_line_sweeps<256, SW_M, false>), dim3(nblocks), dim3(256), 0, st, d, cf, phi, pb,
+                           pstride, plan.tasks, plan.ntasks, ibeg, iend, plan.jbeg, plan.jend, plan.kbeg, plan.kend,
+                           counter, counter_base, sc);
 }
 
 }  // namespace amcfd
