@@ -25,6 +25,7 @@ CLI_PATH = os.path.join(_HERE, "amcfd")
 MAX_ZONES = 32
 OK, I_MAXIT = 0, 1
 MODE_FP64 = 0
+MODE_SCAN = 1
 
 FIELD_NAMES = ("enthalpy", "temp", "fracl", "uvel", "vvel", "wvel", "pressure", "pp", "rho",
                "diff", "vis", "hnot", "fraclnot", "unot", "vnot", "wnot", "sourceinput", "dux",

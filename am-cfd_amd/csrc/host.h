@@ -61,6 +61,7 @@ class Solver {
     DevScalars *sc_ = nullptr;       // device
     DevScalars *sc_host_ = nullptr;  // pinned mirror
     hipStream_t st_ = nullptr;
+    int mode_ = AMCFD_MODE_FP64;
     hipEvent_t step_ev_[2] = {nullptr, nullptr};   // brackets one solve_step
     std::vector<void *> allocs_;
     SweepPlan plan_dom_, plan_box_;

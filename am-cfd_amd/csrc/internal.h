@@ -139,9 +139,9 @@ void launch_end_step(hipStream_t st, const Dims &d, DevFields f, const amcfd_par
 void launch_line_prep(hipStream_t st, const Dims &d, DevFields f, double *pb, size_t pstride, const SweepPlan &plan,
                       int ibeg, int iend, int guard);
 // one Gauss-Seidel x-line sweep; pb holds four pass buffers of pstride
-// doubles each (the field's size)
+// doubles each (the field's size); scan selects the fast-mode line solve
 void launch_line_sweeps(hipStream_t st, const Dims &d, DevFields f, double *phi, double *pb, size_t pstride,
-                        const SweepPlan &plan, int ibeg, int iend, int guard, unsigned long long *counter,
+                        const SweepPlan &plan, int ibeg, int iend, int guard, int scan, unsigned long long *counter,
                         unsigned long long counter_base, DevScalars *sc, int nblocks);
 
 // wavefront task list for the Gauss-Seidel line sweeps (host side)

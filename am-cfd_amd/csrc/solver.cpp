@@ -54,7 +54,8 @@ void Solver::set_active_nk(int nk) {
 }
 
 int Solver::create(const amcfd_params &p, int mode) {
-    if (mode != AMCFD_MODE_FP64) return set_error(AMCFD_E_ARG, "unknown solver mode");
+    if (mode != AMCFD_MODE_FP64 && mode != AMCFD_MODE_SCAN) return set_error(AMCFD_E_ARG, "unknown solver mode");
+    mode_ = mode;
     p_ = p;
     if (p.n_toolpath > 0) tool_.assign(p.toolpath, p.toolpath + 6 * p.n_toolpath);
     if (p.n_dt_table > 0) dtt_.assign(p.dt_table, p.dt_table + 3 * p.n_dt_table);
@@ -362,7 +363,7 @@ int Solver::sweep(double *phi, SweepPlan &plan, int ibeg, int iend, int jbeg, in
     int slot;
     tic(timer, &slot);
     launch_line_sweeps(st_, d_, f_, phi, pass_buf_, (size_t)d_.ni * d_.nj * d_.nkf, plan, ibeg, iend, guard,
-                       counter_, counter_base_, sc_, nblocks);
+                       mode_ == AMCFD_MODE_SCAN, counter_, counter_base_, sc_, nblocks);
     HIPCALL(hipGetLastError());
     toc(timer, slot);
     counter_base_ += (unsigned long long)plan.ntasks + (unsigned long long)nblocks;

@@ -257,6 +257,103 @@ struct LineRegs {
     }
 };
 
+// Fast-mode line solve.  With the pivots fixed for the sweep, both halves of
+// the Thomas algorithm are first-order linear recurrences,
+//   forward   q_i   = (aw_i y_i) q_{i-1} + d_i y_i,   q_{-1} = lo,
+//   backward  phi_i = pr_i phi_{i+1} + q_i,           phi_n  = hi,
+// so each is an inclusive scan of affine maps x -> a x + b: every lane
+// composes the maps of its M consecutive rows, the lanes combine their maps
+// with log2(64) shuffle steps, and each lane then replays its rows from the
+// incoming value.  The rounding differs from the sequential recurrence.
+// Rows past n are identity maps.
+template <int M>
+struct ScanLine {
+    __device__ __forceinline__ static double shfl_up(double v, int off) { return __shfl_up(v, off, 64); }
+    __device__ __forceinline__ static double shfl_down(double v, int off) { return __shfl_down(v, off, 64); }
+
+    // D = rhs in, solution out; AW, Y, PR as in LineRegs
+    __device__ static void solve(int n, const double *AW, const double *Y, const double *PR, double *D, double lo,
+                                 double hi) {
+        const int lane = threadIdx.x;
+        double a[M], b[M];
+#pragma unroll
+        for (int r = 0; r < M; ++r) {
+            const int p = lane * M + r;
+            const bool real = p < n;
+            const double y = real ? Y[p] : 1.0;
+            a[r] = real ? AW[p] * y : 1.0;
+            b[r] = real ? D[p] * y : 0.0;
+        }
+        // forward: compose this lane's rows, scan over lanes, replay
+        double A = 1.0, B = 0.0;
+#pragma unroll
+        for (int r = 0; r < M; ++r) {
+            B = a[r] * B + b[r];
+            A = a[r] * A;
+        }
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) {
+            const double Ap = shfl_up(A, off), Bp = shfl_up(B, off);
+            if (lane >= off) {
+                B = A * Bp + B;
+                A = A * Ap;
+            }
+        }
+        double Ae = shfl_up(A, 1), Be = shfl_up(B, 1);
+        double x = (lane == 0) ? lo : Ae * lo + Be;
+#pragma unroll
+        for (int r = 0; r < M; ++r) {
+            x = a[r] * x + b[r];
+            b[r] = x;
+        }
+        // backward: the same from the top row down
+#pragma unroll
+        for (int r = 0; r < M; ++r) {
+            const int p = lane * M + r;
+            a[r] = (p < n) ? PR[p] : 1.0;
+            if (p >= n) b[r] = 0.0;
+        }
+        A = 1.0;
+        B = 0.0;
+#pragma unroll
+        for (int r = M - 1; r >= 0; --r) {
+            B = a[r] * B + b[r];
+            A = a[r] * A;
+        }
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) {
+            const double Ap = shfl_down(A, off), Bp = shfl_down(B, off);
+            if (lane + off < 64) {
+                B = A * Bp + B;
+                A = A * Ap;
+            }
+        }
+        Ae = shfl_down(A, 1);
+        Be = shfl_down(B, 1);
+        x = (lane == 63) ? hi : Ae * hi + Be;
+#pragma unroll
+        for (int r = M - 1; r >= 0; --r) {
+            x = a[r] * x + b[r];
+            b[r] = x;
+        }
+#pragma unroll
+        for (int r = 0; r < M; ++r) {
+            const int p = lane * M + r;
+            if (p < n) D[p] = b[r];
+        }
+    }
+};
+
+// fewest rows per lane that fit the line
+__device__ __forceinline__ void scan_solve(int n, const double *AW, const double *Y, const double *PR, double *D,
+                                           double lo, double hi) {
+    if (n <= 128) ScanLine<2>::solve(n, AW, Y, PR, D, lo, hi);
+    else if (n <= 256) ScanLine<4>::solve(n, AW, Y, PR, D, lo, hi);
+    else if (n <= 448) ScanLine<7>::solve(n, AW, Y, PR, D, lo, hi);
+    else if (n <= 640) ScanLine<10>::solve(n, AW, Y, PR, D, lo, hi);
+    else ScanLine<(SW_MAXN + 63) / 64>::solve(n, AW, Y, PR, D, lo, hi);
+}
+
 // Line data move between workgroups (and XCDs) with device-coherent
 // relaxed atomic loads and stores (no L2-wide writeback or invalidate).  A
 // consumer polls a pass-buffer value until it is no longer SW_PENDING.
@@ -301,8 +398,10 @@ __global__ __launch_bounds__(256) void k_fill_pending(Dims d, double *pb, size_t
 
 // One x-line solve per task; M rows per lane in the recurrence (64 M >= n).
 // DOMAIN marks the instantiation for the whole-domain enthalpy sweep, so it
-// shows up as its own kernel in a profile; the code is the same.
-template <int NT, int M, bool DOMAIN>
+// shows up as its own kernel in a profile; the code is the same.  SCAN
+// selects the fast-mode line solve (ScanLine) instead of the reference's
+// recurrence.
+template <int NT, int M, bool DOMAIN, bool SCAN>
 __global__ __launch_bounds__(NT) void k_line_sweeps(Dims d, Coefs cf, double *phi, double *pb, size_t pstride,
                                                     const int *__restrict__ tasks, int ntasks, int ibeg, int iend,
                                                     int jbeg, int jend, int kbeg, int kend,
@@ -343,7 +442,7 @@ __global__ __launch_bounds__(NT) void k_line_sweeps(Dims d, Coefs cf, double *ph
         }
         const double lo = phi[row + ibeg - 1], hi = phi[row + iend + 1];   // fixed end values
         __syncthreads();
-        if (tid < 64) regs.load_coefs(n, A, B, C);
+        if (!SCAN && tid < 64) regs.load_coefs(n, A, B, C);
         // neighbour lines as the sequential loop nest sees them: a pass buffer
         // inside the sweep's range, the sweep's input outside it
         const double *src_t = (k < kend) ? pb + (size_t)(2 * ks + 1) * pstride : phi;
@@ -379,7 +478,12 @@ __global__ __launch_bounds__(NT) void k_line_sweeps(Dims d, Coefs cf, double *ph
         }
         if (!ok) atomicOr(&sc->error, 1);
         __syncthreads();
-        if (tid < 64) regs.solve(n, E, D, lo, hi);
+        if (tid < 64) {
+            if constexpr (SCAN)
+                scan_solve(n, A, C, E, D, lo, hi);
+            else
+                regs.solve(n, E, D, lo, hi);
+        }
         __syncthreads();
         double *out = pb + (size_t)pass * pstride + row + ibeg;
 #pragma unroll
@@ -430,26 +534,38 @@ void launch_line_prep(hipStream_t st, const Dims &d, DevFields f, double *pb, si
                        plan.jend, plan.kbeg, plan.kend);
 }
 
+template <bool SCAN>
+static void launch_sweeps_t(hipStream_t st, const Dims &d, const Coefs &cf, double *phi, double *pb, size_t pstride,
+                            const SweepPlan &plan, int ibeg, int iend, int guard, unsigned long long *counter,
+                            unsigned long long counter_base, DevScalars *sc, int nblocks) {
+    const int n = iend - ibeg + 1;
+    // short lines (the melt-pool box) get a lighter instantiation: fewer
+    // registers, more waves per SIMD
+    if (!guard)
+        hipLaunchKernelGGL((k_line_sweeps<256, SW_M, true, SCAN>), dim3(nblocks), dim3(256), 0, st, d, cf, phi, pb,
+                           pstride, plan.tasks, plan.ntasks, ibeg, iend, plan.jbeg, plan.jend, plan.kbeg, plan.kend,
+                           counter, counter_base, sc);
+    else if (n <= 64 * SW_M_SHORT)
+        hipLaunchKernelGGL((k_line_sweeps<256, SW_M_SHORT, false, SCAN>), dim3(nblocks), dim3(256), 0, st, d, cf, phi,
+                           pb, pstride, plan.tasks, plan.ntasks, ibeg, iend, plan.jbeg, plan.jend, plan.kbeg, plan.kend,
+                           counter, counter_base, sc);
+    else
+        hipLaunchKernelGGL((k_line_sweeps<256, SW_M, false, SCAN>), dim3(nblocks), dim3(256), 0, st, d, cf, phi, pb,
+                           pstride, plan.tasks, plan.ntasks, ibeg, iend, plan.jbeg, plan.jend, plan.kbeg, plan.kend,
+                           counter, counter_base, sc);
+}
+
 void launch_line_sweeps(hipStream_t st, const Dims &d, DevFields f, double *phi, double *pb, size_t pstride,
-                        const SweepPlan &plan, int ibeg, int iend, int guard, unsigned long long *counter,
+                        const SweepPlan &plan, int ibeg, int iend, int guard, int scan, unsigned long long *counter,
                         unsigned long long counter_base, DevScalars *sc, int nblocks) {
     if (plan.ntasks == 0 || iend < ibeg) return;
     Coefs cf{f.an, f.as, f.ae, f.aw, f.at, f.ab, f.ap, f.su, f.lpr, f.lden, f.ly};
-    const int n = iend - ibeg + 1;
-    // short lines (the melt-pool box) get a lighter instantiation: fewer
-    // registers, three waves per SIMD
-    if (!guard)
-        hipLaunchKernelGGL((k_line_sweeps<256, SW_M, true>), dim3(nblocks), dim3(256), 0, st, d, cf, phi, pb, pstride,
-                           plan.tasks, plan.ntasks, ibeg, iend, plan.jbeg, plan.jend, plan.kbeg, plan.kend, counter,
-                           counter_base, sc);
-    else if (n <= 64 * SW_M_SHORT)
-        hipLaunchKernelGGL((k_line_sweeps<256, SW_M_SHORT, false>), dim3(nblocks), dim3(256), 0, st, d, cf, phi, pb,
-                           pstride, plan.tasks, plan.ntasks, ibeg, iend, plan.jbeg, plan.jend, plan.kbeg, plan.kend,
-                           counter, counter_base, sc);
+    if (scan)
+        launch_sweeps_t<true>(st, d, cf, phi, pb, pstride, plan, ibeg, iend, guard, counter, counter_base, sc,
+                              nblocks);
     else
-        hipLaunchKernelGGL((k_line_sweeps<256, SW_M, false>), dim3(nblocks), dim3(256), 0, st, d, cf, phi, pb,
-                           pstride, plan.tasks, plan.ntasks, ibeg, iend, plan.jbeg, plan.jend, plan.kbeg, plan.kend,
-                           counter, counter_base, sc);
+        launch_sweeps_t<false>(st, d, cf, phi, pb, pstride, plan, ibeg, iend, guard, counter, counter_base, sc,
+                               nblocks);
 }
 
 }  // namespace amcfd

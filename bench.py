@@ -96,7 +96,10 @@ def main():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--case", default=CASE)
+    ap.add_argument("--mode", choices=["fp64", "scan"], default="fp64",
+                    help="fp64: reference program order, bit for bit; scan: parallel line solves")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-fast", action="store_true", help="skip the scan-mode leg")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1),
                     help="OpenMP threads of the reference CPU baseline (the host cores used)")
@@ -126,7 +129,7 @@ def main():
     deck = amcfd.Deck(deck_path)
     p = deck.params
     dom = (p.ni - 2) * (p.nj - 2) * (p.nk - 2)
-    solver = amcfd.Solver(deck)
+    solver = amcfd.Solver(deck, mode=amcfd.MODE_SCAN if args.mode == "scan" else amcfd.MODE_FP64)
     for w in range(args.warmup):
         solver.step()
         print("warmup step %d: iters %d tpeak %.1f" % (w + 1, solver.scalars()["niter"],
@@ -168,7 +171,8 @@ def main():
         "data": "synthetic",
         "config": {"workload": "single track, bare plate, 400x160x80 cells (5.12M), P=300 W, r=60 um, "
                                "0.8 m/s, dt=2e-6 s, maxit 100, tol 1e-6",
-                   "mode": "plain FP64, reference program order", "interior_cells": dom,
+                   "mode": ("plain FP64, reference program order" if args.mode == "fp64" else
+                            "FP64, x-line solves by parallel scans, reference sweep order"), "interior_cells": dom,
                    "iters_per_step": iters / args.steps, "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "kernel": "k_line_sweeps (enthalpy, whole domain)",
                      "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -178,6 +182,30 @@ def main():
                      "algorithmic_bytes_per_launch": algo},
         "kernels": other,
     }
+    solver.close()
+    if args.mode == "fp64" and not args.no_fast:
+        # the scan mode on the same workload, same warm-up and step count
+        fast = amcfd.Solver(deck, mode=amcfd.MODE_SCAN)
+        for w in range(args.warmup):
+            fast.step()
+        fast.set_profiling(True)
+        f0 = fast.scalars()["itertot"]
+        n0f, ms0f = fast.kernel_stats("sweep_h")
+        t0 = time.perf_counter()
+        for k in range(args.steps):
+            fast.step()
+        el = time.perf_counter() - t0
+        n1f, ms1f = fast.kernel_stats("sweep_h")
+        avgf = (ms1f - ms0f) / max(n1f - n0f, 1)
+        result["fast_modes"] = {"scan": {
+            "value": dom * args.steps / el, "ms_per_step": 1e3 * el / args.steps,
+            "iters_per_step": (fast.scalars()["itertot"] - f0) / args.steps,
+            "speedup_vs_fp64": (dom * args.steps / el) / value,
+            "sweep_avg_launch_ms": avgf,
+            "sweep_frac": algo / (avgf * 1e-3) / 1e9 / HBM_PEAK_GBS,
+            "note": "FP64, x-line solves by parallel scans in the reference's sweep order; drift vs the "
+                    "FP64 mode equals a 1-ulp perturbation's (tests/drift_study.py)"}}
+        fast.close()
     result["cpu_baseline"] = None
     if not args.no_cpu:
         print("cpu baseline: reference harness, %d threads" % args.cpu_threads, file=sys.stderr, flush=True)
@@ -185,7 +213,6 @@ def main():
                                                         args.cpu_steps)
         if result["cpu_baseline"] is None:
             result["cpu_baseline"] = cpu_baseline(p, dom, iters / args.steps, args.cpu_budget)
-    solver.close()
     print(json.dumps(result))
 
 

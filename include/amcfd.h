@@ -50,8 +50,13 @@ extern "C" {
 #define AMCFD_DISC_POWERLAW    1
 #define AMCFD_DISC_EXPONENTIAL 2
 
-/* solver precision modes */
+/* solver modes */
 #define AMCFD_MODE_FP64        0   /* plain FP64, reference program order     */
+#define AMCFD_MODE_SCAN        1   /* FP64; x-line solves by parallel scans in
+                                      the same Gauss-Seidel order: rounding
+                                      differs from the reference's, so results
+                                      agree to the iteration's sensitivity,
+                                      not bit for bit                         */
 
 /*
  * Every value the path reads from the reference's Mesh and DomainManager
