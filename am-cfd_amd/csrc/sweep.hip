@@ -172,7 +172,7 @@ struct LineRegs {
     double raw[M], rden[M], ry[M];
 
     __device__ __forceinline__ void load_coefs(int n, const double *AW, const double *DEN, const double *Y) {
-        const int lane = threadIdx.x;
+        const int lane = threadIdx.x & 63;
         const int nl = (n + M - 1) / M, pad = nl * M - n;
 #pragma unroll
         for (int r = 0; r < M; ++r) {
@@ -185,7 +185,7 @@ struct LineRegs {
     }
 
     __device__ __forceinline__ void solve(int n, const double *PR, double *D, double lo, double hi) {
-        const int lane = threadIdx.x;
+        const int lane = threadIdx.x & 63;
         const int nl = (n + M - 1) / M, pad = nl * M - n;
         double rd[M], rx[M];
 #pragma unroll
@@ -274,7 +274,7 @@ struct ScanLine {
     // D = rhs in, solution out; AW, Y, PR as in LineRegs
     __device__ static void solve(int n, const double *AW, const double *Y, const double *PR, double *D, double lo,
                                  double hi) {
-        const int lane = threadIdx.x;
+        const int lane = threadIdx.x & 63;
         double a[M], b[M];
 #pragma unroll
         for (int r = 0; r < M; ++r) {
