@@ -435,7 +435,7 @@ __global__ __launch_bounds__(NT) void k_line_sweeps(Dims d, Coefs cf, double *ph
                 const size_t c = row + ibeg + ii;
                 rat[r] = cf.at[c]; rab[r] = cf.ab[c]; ran[r] = cf.an[c]; ras[r] = cf.as[c]; rsu[r] = cf.su[c];
                 A[ii] = cf.aw[c];
-                B[ii] = cf.den[c];
+                if (!SCAN) B[ii] = cf.den[c];   // the scan solve needs only 1/den
                 C[ii] = cf.y[c];
                 E[ii] = cf.pr[c];
             }
