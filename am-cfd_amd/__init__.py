@@ -36,7 +36,8 @@ EXPORTS = ("amcfd_deck_load", "amcfd_deck_params", "amcfd_deck_free", "amcfd_cre
            "amcfd_destroy", "amcfd_begin_step", "amcfd_solve_step", "amcfd_end_step",
            "amcfd_step", "amcfd_get_field", "amcfd_set_field", "amcfd_get_geometry",
            "amcfd_get_scalars", "amcfd_dims", "amcfd_last_timing", "amcfd_set_profiling",
-           "amcfd_kernel_stats", "amcfd_last_error", "amcfd_version")
+           "amcfd_kernel_stats", "amcfd_write_vts", "amcfd_write_pvd", "amcfd_append_residuals",
+           "amcfd_append_final_geometry", "amcfd_deck_string", "amcfd_last_error", "amcfd_version")
 
 
 class Params(ctypes.Structure):
@@ -83,7 +84,7 @@ class Scalars(ctypes.Structure):
             "beamposy", "scanvel")] + [
         (n, ctypes.c_int) for n in (
             "powerindicator", "trackindx", "istat", "iend", "jstat", "jend", "kstat", "istart",
-            "jstart", "nk", "step")]
+            "jstart", "nk", "step")] + [("delt", ctypes.c_double), ("outtime", ctypes.c_double)]
 
     def as_dict(self):
         return {n: getattr(self, n) for n, _ in self._fields_}
@@ -125,6 +126,11 @@ def load_library(path=LIB_PATH):
         "amcfd_last_timing": (ip, [vp, dp, dp]),
         "amcfd_set_profiling": (ip, [vp, ip]),
         "amcfd_kernel_stats": (ip, [vp, cp, ctypes.POINTER(ctypes.c_long), dp]),
+        "amcfd_write_vts": (ip, [vp, cp, ctypes.POINTER(cp), ip]),
+        "amcfd_write_pvd": (ip, [cp, ip, dp, ctypes.POINTER(cp)]),
+        "amcfd_append_residuals": (ip, [vp, cp]),
+        "amcfd_append_final_geometry": (ip, [vp, cp]),
+        "amcfd_deck_string": (cp, [vp, cp, ip]),
         "amcfd_last_error": (cp, []),
         "amcfd_version": (cp, []),
     }

@@ -158,6 +158,44 @@ int amcfd_kernel_stats(amcfd_solver *s, const char *kernel, long *launches, doub
     return AMCFD_OK;
 }
 
+int amcfd_write_vts(amcfd_solver *s, const char *path, const char *const *scalar_names, int n_names) {
+    if (!s || !path || (n_names > 0 && !scalar_names)) return amcfd::set_error(AMCFD_E_ARG, "null argument");
+    GUARD(amcfd::write_vts(s->s, path, scalar_names, n_names));
+}
+
+int amcfd_write_pvd(const char *path, int n, const double *times, const char *const *files) {
+    if (!path || n < 0 || (n > 0 && (!times || !files))) return amcfd::set_error(AMCFD_E_ARG, "null argument");
+    GUARD(amcfd::write_pvd(path, n, times, files));
+}
+
+int amcfd_append_residuals(amcfd_solver *s, const char *path) {
+    if (!s || !path) return amcfd::set_error(AMCFD_E_ARG, "null argument");
+    GUARD(amcfd::append_residuals(s->s, path));
+}
+
+int amcfd_append_final_geometry(amcfd_solver *s, const char *path) {
+    if (!s || !path) return amcfd::set_error(AMCFD_E_ARG, "null argument");
+    GUARD(amcfd::append_final_geometry(s->s, path));
+}
+
+const char *amcfd_deck_string(const amcfd_deck *deck, const char *key, int index) {
+    if (!deck || !key) return nullptr;
+    const Deck &d = deck->d;
+    const std::string k = key;
+    const std::string *v = nullptr;
+    if (k == "keyword_id") v = &d.keyword_id;
+    else if (k == "directory") v = &d.directory;
+    else if (k == "final_geometry") v = &d.geometry_out;
+    else if (k == "mesh_file") v = &d.mesh_file;
+    else if (k == "scalar_out" || k == "vector_out") {
+        const std::vector<std::string> &list = k == "scalar_out" ? d.scalar_out : d.vector_out;
+        if (index < 0 || index >= (int)list.size()) return nullptr;
+        return list[index].c_str();
+    }
+    if (!v || v->empty()) return nullptr;
+    return v->c_str();
+}
+
 const char *amcfd_last_error(void) { return amcfd::g_last_error.c_str(); }
 
 const char *amcfd_version(void) { return "amcfd-mi355x 0.1 (gfx950, FP64 reference-order mode)"; }

@@ -12,7 +12,8 @@
 // values, short toolpath rows) this reader returns AMCFD_E_PARSE instead.
 // Keywords whose subsystems are outside the per-step solve and would change
 // its results (species, moving mesh, energy free surface, G/R) are refused
-// with AMCFD_E_UNSUPPORTED; output-only keywords are accepted and ignored.
+// with AMCFD_E_UNSUPPORTED; output keywords are kept for the output writers
+// (output.cpp), other output-only keywords are accepted and ignored.
 
 #include "internal.h"
 
@@ -111,6 +112,7 @@ int Deck::load(const std::string &path) {
             case B_EVAP: flags.isevap = 1; break;
             case B_TOOL: flags.udtoolpath = 1; break;
             case B_POWDER: flags.powderbed = 1; break;
+            case B_TEC: tecplot = 1; break;
             case B_VOLHEAT: flags.volheatsource = 1; break;
             case B_HEATTABLE: flags.volheatsource = 1; flags.volheatsourcetable = 1; break;
             case B_SPECIES:
@@ -185,7 +187,11 @@ int Deck::load(const std::string &path) {
             have_dt = true;
             break;
         }
-        default: break;  // output-only blocks
+        case B_SCALOUT: w = words(line); if (!w.empty()) scalar_out.push_back(w[0]); break;
+        case B_VECOUT: w = words(line); if (!w.empty()) vector_out.push_back(w[0]); break;
+        case B_DIR: w = words(line); if (!w.empty()) directory = w[0]; break;
+        case B_GEOM: w = words(line); if (!w.empty()) geometry_out = w[0]; break;
+        default: break;  // other output-only blocks
         }
     }
 

@@ -46,6 +46,11 @@ class Solver {
     int get_geometry(const char *name, double *host, size_t n);
     int get_scalars(amcfd_scalars *out);
     void dims(int *ni, int *nj, int *nk) const { *ni = d_.ni; *nj = d_.nj; *nk = d_.nkf; }
+    // read-only views for the output writers (output.cpp)
+    const DevScalars &host_scalars() const { return *sc_host_; }
+    const amcfd_params &params() const { return p_; }
+    const HostGrid &grid() const { return hg_; }
+    int active_nk() const { return d_.nk; }
     double last_total_ms = -1.0, last_tdma_ms = -1.0;
     void set_profiling(int on) { profile_ = on; }
     std::map<std::string, KernelTimer> timers;
@@ -97,5 +102,11 @@ class Solver {
     void collect_timers();
     double *field_ptr(const char *name);
 };
+
+// output writers (output.cpp)
+int write_vts(Solver &s, const char *path, const char *const *names, int nnames);
+int write_pvd(const char *path, int n, const double *times, const char *const *files);
+int append_residuals(Solver &s, const char *path);
+int append_final_geometry(Solver &s, const char *path);
 
 }  // namespace amcfd

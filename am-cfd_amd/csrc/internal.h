@@ -25,6 +25,10 @@ struct Deck {
     amcfd_params p{};
     Flags flags;
     std::string keyword_id, mesh_file, tool_file, heat_file;
+    // output settings (Mesh.cpp:263-268, 314-336, 407-411)
+    std::string directory, geometry_out;
+    std::vector<std::string> scalar_out, vector_out;
+    int tecplot = 0;
     std::vector<std::vector<double>> tool_rows, dt_rows, heat_rows;
     std::vector<double> flat_tool, flat_dt, flat_heat;
     int load(const std::string &path);

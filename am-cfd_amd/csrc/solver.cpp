@@ -547,6 +547,8 @@ int Solver::get_scalars(amcfd_scalars *o) {
     const DevScalars &s = *sc_host_;
     std::memset(o, 0, sizeof(*o));
     o->timet = timet_;
+    o->delt = p_.delt;
+    o->outtime = p_.outtime;
     o->niter = niter_;
     o->itertot = itertot_;
     o->resorh = s.resorh;

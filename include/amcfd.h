@@ -121,6 +121,7 @@ typedef struct amcfd_scalars {
     int    powerindicator, trackindx;
     int    istat, iend, jstat, jend, kstat, istart, jstart, nk;
     int    step;
+    double delt, outtime;   /* time step and output interval in force (main.cpp:125-133) */
 } amcfd_scalars;
 
 typedef struct amcfd_deck amcfd_deck;      /* parsed input deck          */
@@ -177,6 +178,27 @@ int  amcfd_set_profiling(amcfd_solver *s, int on);
 int  amcfd_kernel_stats(amcfd_solver *s, const char *kernel, long *launches, double *total_ms);
 
 /* Thread-local description of the last error. */
+/* Output files in the reference's formats, so a run leaves the files the
+ * reference executable writes (src/main.cpp:115-241):
+ *   write_vts       one StructuredGrid file, vtsBinWriter::visualOut
+ *                   (vtsBinWriter.cpp:50-139); scalar_names are the deck's
+ *                   *SCALAR_OUT names (uvel/vvel/wvel written cell-centred as
+ *                   auvel/avvel/awvel; names the reference does not hold are
+ *                   skipped, as it does, PrintManager.cpp:314-378)
+ *   write_pvd       the ParaView collection of all files so far
+ *                   (vtsBinWriter.cpp:420-468)
+ *   append_residuals / append_final_geometry
+ *                   PrintManager::outputResiduals (PrintManager.cpp:380-533)
+ *                   and ::outputFinalGeometry (:658-677), appending to path */
+int  amcfd_write_vts(amcfd_solver *s, const char *path, const char *const *scalar_names, int n_names);
+int  amcfd_write_pvd(const char *path, int n, const double *times, const char *const *files);
+int  amcfd_append_residuals(amcfd_solver *s, const char *path);
+int  amcfd_append_final_geometry(amcfd_solver *s, const char *path);
+/* Deck strings the output layer needs (Mesh.cpp:263-268, 314-336, 407-411):
+ * key "keyword_id", "directory", "final_geometry", "mesh_file", or
+ * "scalar_out" / "vector_out" with an index; NULL when absent. */
+const char *amcfd_deck_string(const amcfd_deck *deck, const char *key, int index);
+
 const char *amcfd_last_error(void);
 const char *amcfd_version(void);
 

@@ -17,6 +17,8 @@ The named cases are the workloads the solver is measured and checked on:
                 surface-tension model (a_oxygen > 0).
 * ``tiny_vap``  same grid, evaporation loss only (constant dgamma/dT).
 * ``tiny_vol``  same grid, Gaussian laser plus a 100 W volumetric source.
+* ``tiny_out``  same grid, 20 steps with the laser on throughout, output every
+                5 steps (VTS/PVD, residual log, final geometry).
 * ``plate120``  120x48x32 cells (2.0x0.6x0.4 mm), 400 steps, P=200 W,
                 r=50 um, 1 m/s: the CPU-runnable single-track case.
 * ``track400``  400x160x80 cells (4.0x1.2x0.6 mm, ~5.1M cells), P=300 W,
@@ -61,7 +63,7 @@ def write_deck(path, mesh_path, *, keyword="case", finaltime, maxit=100, tol=1e-
                factor=2.0, discretization="power_law", flow=True, urf=(0.8,) * 5,
                tamb=300.0, tpre=300.0, htc=(10.0, 10.0, 10.0, 10.0), emiss=0.4,
                momentum=(5e-5, -2.6e-4, 1e7, -1.0), material=None, evaporation=None,
-               volumetric=None, extra=()):
+               volumetric=None, outtime=None, scalar_out=("temp",), final_geometry=None, extra=()):
     if material is None:
         material = (4000.0, 4000.0, 5e-3, TS, TL, HSMELT, HLFRIZ, ACPA, ACPB, ACPB,
                     0.0, 0.0, 25.0, 0.0, 25.0)
@@ -69,11 +71,12 @@ def write_deck(path, mesh_path, *, keyword="case", finaltime, maxit=100, tol=1e-
     L += ["*KEYWORD_ID", keyword, "*DIRECTORY", keyword]
     L += ["*PARAMETERS", "accelGravity 9.8", "radBoltzmann 5.67e-8",
           "ambientTemp %s" % _fmt(tamb), "ambientPres 101.0e3", "gasConstant 8.314"]
-    L += ["*SCALAR_OUT", "temp"]
+    for name in scalar_out:
+        L += ["*SCALAR_OUT", name]
     L += ["*CONTROL_TERMINATION", "$ total_time  max_iter  nonlin_tolerance",
           "  %s %d %s" % (_fmt(finaltime), maxit, _fmt(tol))]
     L += ["*CONTROL_TIMESTEP", "$ sim_timestep  output_timestep",
-          "  %s %s" % (_fmt(dt), _fmt(finaltime))]
+          "  %s %s" % (_fmt(dt), _fmt(finaltime if outtime is None else outtime))]
     L += ["*SINGLE_TRACK_TOOL", "$ lasertime  xstart  ystart  vel",
           "  %s %s %s %s" % tuple(_fmt(v) for v in (lasertime, xstart, ystart, speed))]
     L += ["*GAUSS_LASER_PARAMETERS", "$ power  absorb  radius  factor",
@@ -96,6 +99,8 @@ def write_deck(path, mesh_path, *, keyword="case", finaltime, maxit=100, tol=1e-
         L += ["*FLUID_FLOW_OFF"]
     if evaporation is not None:
         L += ["*BOUNDARY_EVAPORATION", "  " + " ".join(_fmt(v) for v in evaporation)]
+    if final_geometry is not None:
+        L += ["*OUTPUT_FINAL_GEOMETRY", final_geometry]
     L += list(extra)
     L += ["*MESH_FILE", mesh_path]
     with open(path, "w") as f:
@@ -138,6 +143,14 @@ CASES = {
         deck=dict(finaltime=0.1e-3 - 1e-9, maxit=60, dt=5e-6, lasertime=0.08e-3,
                   xstart=0.15e-3, ystart=0.15e-3, speed=1.0, power=150.0, radius=50e-6,
                   volumetric=(100.0, 0.3, 5e-5, 5e-5, 2.0))),
+    "tiny_out": dict(
+        mesh=(([0.1e-3, 0.3e-3, 0.1e-3], [4, 12, 4], [-1.2, 1.0, 1.2]),
+              ([0.15e-3, 0.15e-3], [6, 6], [1.0, 1.0]),
+              ([0.15e-3, 0.1e-3], [4, 8], [1.0, 1.0])),
+        deck=dict(finaltime=0.1e-3 - 1e-9, maxit=100, dt=5e-6, lasertime=0.15e-3,
+                  xstart=0.15e-3, ystart=0.15e-3, speed=1.0, power=200.0, radius=50e-6,
+                  outtime=2.5e-5, scalar_out=("temp", "uvel", "vvel", "wvel", "fracl", "enthalpy"),
+                  final_geometry="tiny_out_geometry.txt")),
     "plate120": dict(
         mesh=(([0.5e-3, 1.0e-3, 0.5e-3], [20, 80, 20], [-1.2, 1.0, 1.2]),
               ([0.2e-3, 0.2e-3, 0.2e-3], [16, 16, 16], [1.0, 1.0, 1.0]),
