@@ -72,7 +72,8 @@ class Params(ctypes.Structure):
         ("n_dt_table", ctypes.c_int), ("dt_table", ctypes.POINTER(ctypes.c_double)),
         ("n_heat_table", ctypes.c_int), ("heat_table", ctypes.POINTER(ctypes.c_double)),
         ("ni", ctypes.c_int), ("nj", ctypes.c_int), ("nk", ctypes.c_int), ("nk_active0", ctypes.c_int),
-        ("hlcal", ctypes.c_double), ("hlatnt", ctypes.c_double)]
+        ("hlcal", ctypes.c_double), ("hlatnt", ctypes.c_double),
+        ("movingmesh", ctypes.c_int), ("xmoveindex", ctypes.c_int), ("ymoveindex", ctypes.c_int)]
 
 
 class Scalars(ctypes.Structure):

@@ -11,7 +11,7 @@
 // Where the reference crashes or reads out of bounds (blank line, missing
 // values, short toolpath rows) this reader returns AMCFD_E_PARSE instead.
 // Keywords whose subsystems are outside the per-step solve and would change
-// its results (species, moving mesh, energy free surface, G/R) are refused
+// its results (species, energy free surface, G/R) are refused
 // with AMCFD_E_UNSUPPORTED; output keywords are kept for the output writers
 // (output.cpp), other output-only keywords are accepted and ignored.
 
@@ -117,8 +117,7 @@ int Deck::load(const std::string &path) {
             case B_HEATTABLE: flags.volheatsource = 1; flags.volheatsourcetable = 1; break;
             case B_SPECIES:
                 return set_error(AMCFD_E_UNSUPPORTED, "*SOLVE_SPECIES is not supported");
-            case B_MOVING:
-                return set_error(AMCFD_E_UNSUPPORTED, "*ACTIVATE_MOVING_MESH is not supported");
+            case B_MOVING: flags.movingmesh = 1; break;
             case B_FREESURF:
                 return set_error(AMCFD_E_UNSUPPORTED, "*TRACK_ENERGY_BASED_FREE_SURFACE is not supported");
             default: break;
@@ -140,6 +139,13 @@ int Deck::load(const std::string &path) {
         case B_EVAP: v = numbers(line); evap.insert(evap.end(), v.begin(), v.end()); break;
         case B_POWDER: v = numbers(line); powder.insert(powder.end(), v.begin(), v.end()); break;
         case B_VOLHEAT: v = numbers(line); volheat.insert(volheat.end(), v.begin(), v.end()); break;
+        case B_MOVING:   // each line sets both 1-based zone indices, Mesh.cpp:364-369
+            v = numbers(line);
+            if (v.size() < 2)
+                return set_error(AMCFD_E_PARSE, path + ":" + std::to_string(lineno) + ": *ACTIVATE_MOVING_MESH needs two zone indices");
+            p.xmoveindex = (int)v[0] - 1;
+            p.ymoveindex = (int)v[1] - 1;
+            break;
         case B_MESH:
             w = words(line);
             if (!w.empty()) { mesh_file = w[0]; have_mesh = true; }
@@ -269,6 +275,16 @@ int Deck::load(const std::string &path) {
         if (tool_rows.size() < 2) return set_error(AMCFD_E_PARSE, "toolpath needs at least two rows: " + tool_file);
     }
     flags.steady = p.delt < 100.0 ? 0 : 1;
+    if (flags.movingmesh) {
+        // generateGrid shifts xzone[ix-1]/xzone[ix+1] and, until the pool length
+        // is steady, yzone[ix-1]/yzone[ix+1] (with the x index), afterwards
+        // yzone[iy-1]/yzone[iy+1] (CFDSolverManager.cpp:102-118)
+        const int ix = p.xmoveindex, iy = p.ymoveindex;
+        if (ix < 1 || ix > p.nzx - 2 || iy < 1 || iy > p.nzy - 2 || ix > p.nzy - 2)
+            return set_error(AMCFD_E_PARSE, "*ACTIVATE_MOVING_MESH zone indices need a zone on either side in x and y "
+                                            "(x index " + std::to_string(ix + 1) + ", y index " +
+                                            std::to_string(iy + 1) + ")");
+    }
 
     double sx = 0, sy = 0, sz = 0;
     for (int z = 0; z < p.nzx; ++z) sx += p.ncvx[z];
@@ -350,6 +366,7 @@ void Deck::finalize_tables() {
     p.isevap = flags.isevap;
     p.powderbed = flags.powderbed;
     p.udtoolpath = flags.udtoolpath;
+    p.movingmesh = flags.movingmesh;
     p.gaussheatsource = flags.gaussheatsource;
     p.volheatsource = flags.volheatsource;
     p.volheatsourcetable = flags.volheatsourcetable;

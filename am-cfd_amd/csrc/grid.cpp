@@ -5,9 +5,10 @@
 //   face areas                            CFDSolverManager.cpp:258-286
 //   initial fields                        CFDSolverManager.cpp:429-525
 //
-// The grid is computed once on the host in FP64 with the same expressions
-// (so it is bitwise the reference grid) and uploaded; cell volumes are
-// recomputed on the fly in the kernels from the 1-D face coordinates.
+// The grid is computed on the host in FP64 with the same expressions (so it
+// is bitwise the reference grid) and uploaded: once, or every step when the
+// mesh moves; cell volumes are recomputed on the fly in the kernels from the
+// 1-D face coordinates.
 
 #include "host.h"
 
@@ -15,10 +16,13 @@
 
 namespace amcfd {
 
+// Faces of every zone are placed; nodes only up to n-1 (the active extent):
+// with a powder bed the reference regenerates the grid with its current nk,
+// leaving the entries above it as they were.
 static void place_axis(int nzones, const double *len, const double *ncv, const double *expo, int n,
                        std::vector<double> &face, std::vector<double> &node) {
-    face.assign(n + 2, 0.0);
-    node.assign(n + 2, 0.0);
+    face[0] = 0.0;
+    face[1] = 0.0;
     int base = 2;
     double origin = 0.0;
     for (int z = 0; z < nzones; ++z) {
@@ -39,45 +43,44 @@ void HostGrid::build(const amcfd_params &p) {
     ni = p.ni;
     nj = p.nj;
     nk = p.nk;
+    for (auto *v : {&x, &xu, &dxpwinv, &fracx}) v->assign(ni + 2, 0.0);
+    for (auto *v : {&y, &yv, &dypsinv, &fracy}) v->assign(nj + 2, 0.0);
+    for (auto *v : {&z, &zw, &dzpbinv, &fracz}) v->assign(nk + 2, 0.0);
+    for (auto *v : {&areaij, &areauij, &areavij}) v->assign((size_t)nj * ni, 0.0);
+    for (auto *v : {&areaik, &areawik, &areauik}) v->assign((size_t)nk * ni, 0.0);
+    for (auto *v : {&areajk, &areavjk, &areawjk}) v->assign((size_t)nk * nj, 0.0);
+    regenerate(p, nk);
+}
+
+void HostGrid::regenerate(const amcfd_params &p, int nka) {
     place_axis(p.nzx, p.xzone, p.ncvx, p.powrx, ni, xu, x);
     place_axis(p.nzy, p.yzone, p.ncvy, p.powry, nj, yv, y);
-    place_axis(p.nzz, p.zzone, p.ncvz, p.powrz, nk, zw, z);
+    place_axis(p.nzz, p.zzone, p.ncvz, p.powrz, nka, zw, z);
     auto inv = [](const std::vector<double> &c, int n, std::vector<double> &out) {
-        out.assign(n + 2, 0.0);
         for (int i = 1; i < n; ++i) out[i] = 1.0 / (c[i] - c[i - 1]);
     };
     inv(x, ni, dxpwinv);
     inv(y, nj, dypsinv);
-    inv(z, nk, dzpbinv);
+    inv(z, nka, dzpbinv);
     auto frac = [](const std::vector<double> &c, const std::vector<double> &f, int n, std::vector<double> &out) {
-        out.assign(n + 2, 0.0);
         for (int i = 0; i < n - 1; ++i) out[i] = (c[i + 1] - f[i + 1]) / (c[i + 1] - c[i]);
     };
     frac(x, xu, ni, fracx);
     frac(y, yv, nj, fracy);
-    frac(z, zw, nk, fracz);
-    areaij.assign((size_t)nj * ni, 0.0);
-    areauij = areaij;
-    areavij = areaij;
+    frac(z, zw, nka, fracz);
     for (int j = 1; j < nj - 1; ++j)
         for (int i = 1; i < ni - 1; ++i) {
             areaij[j * ni + i] = (xu[i + 1] - xu[i]) * (yv[j + 1] - yv[j]);
             areauij[j * ni + i] = (x[i] - x[i - 1]) * (yv[j + 1] - yv[j]);
             areavij[j * ni + i] = (xu[i + 1] - xu[i]) * (y[j] - y[j - 1]);
         }
-    areaik.assign((size_t)nk * ni, 0.0);
-    areawik = areaik;
-    areauik = areaik;
-    for (int k = 1; k < nk - 1; ++k)
+    for (int k = 1; k < nka - 1; ++k)
         for (int i = 1; i < ni - 1; ++i) {
             areaik[k * ni + i] = (xu[i + 1] - xu[i]) * (zw[k + 1] - zw[k]);
             areawik[k * ni + i] = (xu[i + 1] - xu[i]) * (z[k] - z[k - 1]);
             areauik[k * ni + i] = (x[i] - x[i - 1]) * (zw[k + 1] - zw[k]);
         }
-    areajk.assign((size_t)nk * nj, 0.0);
-    areavjk = areajk;
-    areawjk = areajk;
-    for (int k = 1; k < nk - 1; ++k)
+    for (int k = 1; k < nka - 1; ++k)
         for (int j = 1; j < nj - 1; ++j) {
             areajk[k * nj + j] = (yv[j + 1] - yv[j]) * (zw[k + 1] - zw[k]);
             areavjk[k * nj + j] = (y[j] - y[j - 1]) * (zw[k + 1] - zw[k]);

@@ -14,6 +14,8 @@ struct HostGrid {
     std::vector<double> x, y, z, xu, yv, zw, dxpwinv, dypsinv, dzpbinv, fracx, fracy, fracz;
     std::vector<double> areaij, areauij, areavij, areaik, areawik, areauik, areajk, areavjk, areawjk;
     void build(const amcfd_params &p);
+    // CFDSolverManager::generateGrid with the z extent nka (the active nk)
+    void regenerate(const amcfd_params &p, int nka);
     double volume(int i, int j, int k) const;
 };
 
@@ -84,6 +86,8 @@ class Solver {
     int heat_it_ = 0, dt_state_ = 0, step_ = 0, niter_ = 0, itertot_ = 0;
     double alen_ = 0.0;
     bool steadylength_ = false;
+    bool layer_overflow_ = false;            // a powder layer beyond the grid was requested
+    double *xold_ = nullptr, *yold_ = nullptr;   // node coordinates before a mesh move
     Box box_{};
 
     void set_active_nk(int nk);
@@ -91,6 +95,8 @@ class Solver {
     int upload(double *dst, const std::vector<double> &src);
     void laser_position();
     int launch_laser_map();
+    int move_mesh();
+    int upload_geometry();
     SourceParams source_params();
     int sweep(double *phi, SweepPlan &plan, int ibeg, int iend, int jbeg, int jend, int kbeg, int kend, int guard,
               const char *timer);

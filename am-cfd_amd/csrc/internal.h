@@ -17,7 +17,7 @@ int set_error(int code, const std::string &msg);
 // parsed input deck (Mesh + DomainManager inputs)
 // ---------------------------------------------------------------------------
 struct Flags {
-    int navierstokes = 1, steady = 0, isevap = 0, powderbed = 0, udtoolpath = 0;
+    int navierstokes = 1, steady = 0, isevap = 0, powderbed = 0, udtoolpath = 0, movingmesh = 0;
     int gaussheatsource = 0, volheatsource = 0, volheatsourcetable = 0;
 };
 
@@ -147,6 +147,11 @@ void launch_line_prep(hipStream_t st, const Dims &d, DevFields f, double *pb, si
 void launch_line_sweeps(hipStream_t st, const Dims &d, DevFields f, double *phi, double *pb, size_t pstride,
                         const SweepPlan &plan, int ibeg, int iend, int guard, int scan, unsigned long long *counter,
                         unsigned long long counter_base, DevScalars *sc, int nblocks);
+
+// moving mesh: remap the twelve solution fields from the node coordinates
+// xold/yold onto g.x/g.y (x pass, then y pass), dirx/diry = sign of the move
+void launch_remap(hipStream_t st, const Dims &d, DevFields f, const DevGeom &g, const double *xold,
+                  const double *yold, int dirx, int diry);
 
 // wavefront task list for the Gauss-Seidel line sweeps (host side)
 void build_sweep_tasks(int kbeg, int kend, int jbeg, int jend, std::vector<int> &tasks, int &width);

@@ -31,8 +31,6 @@ namespace amcfd {
 
 namespace {
 
-bool velocity_name(const std::string &n) { return n == "uvel" || n == "vvel" || n == "wvel"; }
-
 // the names the reference's solution registry holds for this run
 // (CFDSolverManager.cpp:528-560, PrintManager.cpp:283-285,
 // FreeSurfaceManager.cpp:64-65 register tempr/zr, whose writer is not part

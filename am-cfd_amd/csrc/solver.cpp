@@ -100,6 +100,10 @@ int Solver::create(const amcfd_params &p, int mode) {
         if ((st = alloc(g1[a], n1))) return st;
         if ((st = upload(*g1[a], *h1[a]))) return st;
     }
+    if (p_.movingmesh) {
+        if ((st = alloc(&xold_, n1))) return st;
+        if ((st = alloc(&yold_, n1))) return st;
+    }
     double **g2[] = {&g_.areaij, &g_.areauij, &g_.areavij, &g_.areaik, &g_.areawik, &g_.areauik,
                      &g_.areajk, &g_.areavjk, &g_.areawjk};
     const std::vector<double> *h2[] = {&hg_.areaij, &hg_.areauij, &hg_.areavij, &hg_.areaik, &hg_.areawik,
@@ -215,7 +219,8 @@ void Solver::laser_position() {
     if (oldpi - powerindicator_ > 0 && dty > small) trackindx_++;
     if (powerindicator_ - oldpi > 0 && p_.powderbed) {
         trackindx_++;
-        set_active_nk(d_.nk + p_.ncvzlayer);
+        if (d_.nk + p_.ncvzlayer <= d_.nkf) set_active_nk(d_.nk + p_.ncvzlayer);
+        else layer_overflow_ = true;
     }
 }
 
@@ -255,9 +260,58 @@ int Solver::begin_step() {
     }
     timet_ += p_.delt;
     ++step_;
-    if (p_.steady) return AMCFD_OK;
-    laser_position();
-    return launch_laser_map();
+    if (!p_.steady) {
+        laser_position();
+        if (layer_overflow_)
+            return set_error(AMCFD_E_CAPACITY, "the toolpath starts more powder layers than *POWDER_BED_MODE declares");
+        int st = launch_laser_map();
+        if (st) return st;
+    }
+    if (p_.movingmesh) return move_mesh();   // main.cpp:144-145
+    return AMCFD_OK;
+}
+
+// all grid arrays, host -> device (the z extent may be the active one only,
+// the rest keeps its earlier values on both sides)
+int Solver::upload_geometry() {
+    double *g1[] = {g_.x, g_.y, g_.z, g_.xu, g_.yv, g_.zw, g_.dxpwinv, g_.dypsinv, g_.dzpbinv,
+                    g_.fracx, g_.fracy, g_.fracz, g_.areaij, g_.areauij, g_.areavij, g_.areaik,
+                    g_.areawik, g_.areauik, g_.areajk, g_.areavjk, g_.areawjk};
+    const std::vector<double> *h1[] = {&hg_.x, &hg_.y, &hg_.z, &hg_.xu, &hg_.yv, &hg_.zw, &hg_.dxpwinv,
+                                       &hg_.dypsinv, &hg_.dzpbinv, &hg_.fracx, &hg_.fracy, &hg_.fracz,
+                                       &hg_.areaij, &hg_.areauij, &hg_.areavij, &hg_.areaik, &hg_.areawik,
+                                       &hg_.areauik, &hg_.areajk, &hg_.areavjk, &hg_.areawjk};
+    int st;
+    for (int a = 0; a < 21; ++a)
+        if ((st = upload(g1[a], *h1[a]))) return st;
+    return AMCFD_OK;
+}
+
+// CFDSolverManager::generateGrid after set-up (CFDSolverManager.cpp:60-124,
+// 345-346): the zones either side of the moving one absorb the beam's
+// displacement since the last step, the grid is rebuilt over the active
+// extent, and the solution is remapped onto it on the device.  Until the
+// pool length is steady the reference picks the y zones with the x index.
+int Solver::move_mesh() {
+    const std::vector<double> px = hg_.x, py = hg_.y;
+    const double dx = beamx_ - oldx_, dy = beamy_ - oldy_;
+    const int dirx = (dx < 0) ? -1 : 1, diry = (dy < 0) ? -1 : 1;
+    const int ix = p_.xmoveindex, iy = p_.ymoveindex;
+    p_.xzone[ix - 1] += dx;
+    p_.xzone[ix + 1] -= dx;
+    const int iyz = steadylength_ ? iy : ix;
+    p_.yzone[iyz - 1] += dy;
+    p_.yzone[iyz + 1] -= dy;
+    hg_.regenerate(p_, d_.nk);
+    // the laser map queued above still reads the old grid
+    HIPCALL(hipStreamSynchronize(st_));
+    int st;
+    if ((st = upload_geometry()) || (st = upload(xold_, px)) || (st = upload(yold_, py))) return st;
+    launch_remap(st_, d_, f_, g_, xold_, yold_, dirx, diry);
+    HIPCALL(hipGetLastError());
+    // the host vectors are rewritten at the next move
+    HIPCALL(hipStreamSynchronize(st_));
+    return AMCFD_OK;
 }
 
 // volumetric-source parameters, CFDSolverManager.cpp:1686-1718 (the table

@@ -107,6 +107,10 @@ typedef struct amcfd_params {
     int    ni, nj, nk;         /* node counts incl. boundary nodes (nk = full) */
     int    nk_active0;         /* active nk after powder-layer deactivation    */
     double hlcal, hlatnt;
+    /* *ACTIVATE_MOVING_MESH: Mesh.cpp:364-369 (0-based zone indices); the
+     * zones either side of them stretch and shrink with the beam every step
+     * (CFDSolverManager.cpp:72-124) */
+    int    movingmesh, xmoveindex, ymoveindex;
 } amcfd_params;
 
 /* scalars the reference keeps as members (CFDSolverManager.h:81-83,

@@ -113,9 +113,12 @@ static void grid_axis(int nz, const double *zone, const double *ncv, const doubl
     node[n - 1] = face[n - 1];
 }
 
+/* The z extent is the active nk: in powder mode the reference regenerates the
+ * grid (moving mesh) with the current nk_, leaving entries above it as they
+ * were; at set-up nk is the full extent. */
 static void generate_grid(orc *s) {
     const amcfd_params *p = &s->p;
-    const int ni = s->ni, nj = s->nj, nk = s->nkf;
+    const int ni = s->ni, nj = s->nj, nk = s->nk;
     grid_axis(p->nzx, p->xzone, p->ncvx, p->powrx, ni, s->xu, s->x);
     grid_axis(p->nzy, p->yzone, p->ncvy, p->powry, nj, s->yv, s->y);
     grid_axis(p->nzz, p->zzone, p->ncvz, p->powrz, nk, s->zw, s->z);
@@ -163,6 +166,58 @@ static void set_active_nk(orc *s, int nk) {
     s->nk = nk;
     s->nkm1 = nk - 1;
     s->nkm2 = nk - 2;
+}
+
+/* linear interpolation, CFDSolverManager.cpp:3057-3066 */
+static double lerp_ref(double x0, double x1, double y0, double y1, double x) {
+    double dy = y1 - y0, dx = x1 - x0;
+    return y0 + (x - x0) * (dy / dx);
+}
+
+/* Moving mesh, CFDSolverManager.cpp:60-124 (zone shift) + 345-346 and
+ * 2809-3051 (remap).  The zones either side of the moving one take up the
+ * beam's displacement since the last step; the grid is rebuilt; then twelve
+ * fields are re-interpolated in place, first along x (node i from i and
+ * i+dir), then along y (j and j+dir), wherever a node moved by more than
+ * 1e-6.  Sweeping i (j) upwards in place makes a backward move (dir = -1) a
+ * recurrence on the already-remapped neighbour. */
+static void move_mesh(orc *s) {
+    amcfd_params *p = &s->p;
+    const int ni = s->ni, nj = s->nj, nk = s->nk;
+    double *oldx = (double *)malloc(sizeof(double) * ni), *oldy = (double *)malloc(sizeof(double) * nj);
+    memcpy(oldx, s->x, sizeof(double) * ni);
+    memcpy(oldy, s->y, sizeof(double) * nj);
+    double dx = s->beamposx - s->oldbeamposx, dy = s->beamposy - s->oldbeamposy;
+    int dirx = (dx < 0) ? -1 : 1, diry = (dy < 0) ? -1 : 1;
+    int ix = p->xmoveindex, iy = p->ymoveindex;
+    p->xzone[ix - 1] += dx;
+    p->xzone[ix + 1] -= dx;
+    /* until the pool length is steady the y zones are picked by the x index */
+    int iyz = s->steadylength ? iy : ix;
+    p->yzone[iyz - 1] += dy;
+    p->yzone[iyz + 1] -= dy;
+    generate_grid(s);
+    double *f[12] = {s->enthalpy, s->hnot, s->temp, s->fracl, s->uvel, s->vvel,
+                     s->wvel, s->unot, s->vnot, s->wnot, s->pressure, s->pp};
+    for (int k = 0; k < nk; ++k)
+        for (int j = 0; j < nj - 1; ++j)
+            for (int i = 1; i < ni - 1; ++i) {
+                if (!(fabs(oldx[i] - s->x[i]) > 1e-6)) continue;
+                size_t c = IX(s, i, j, k);
+                for (int a = 0; a < 12; ++a)
+                    f[a][c] = lerp_ref(oldx[i], oldx[i + dirx], f[a][c], f[a][c + dirx], s->x[i]);
+            }
+    const long sy = (long)diry * ni;
+    for (int k = 0; k < nk; ++k)
+        for (int j = 0; j < nj - 1; ++j)
+            for (int i = 1; i < ni - 1; ++i) {
+                if (!(fabs(oldy[j] - s->y[j]) > 1e-6)) continue;
+                size_t c = IX(s, i, j, k);
+                for (int a = 0; a < 12; ++a)
+                    f[a][c] = lerp_ref(oldy[j], oldy[j + diry], f[a][c], f[a][(size_t)((long)c + sy)], s->y[j]);
+            }
+    free(oldx);
+    free(oldy);
 }
 
 /* ------------------------------------------------------------------------ */
@@ -259,7 +314,8 @@ static void laser_coords(orc *s) {
     if (oldpi - s->powerindicator > 0 && dty > small) s->trackindx++;
     if (s->powerindicator - oldpi > 0 && p->powderbed) {
         s->trackindx++;
-        set_active_nk(s, s->nk + p->ncvzlayer);
+        /* more layers than *POWDER_BED_MODE declares would run past the grid */
+        if (s->nk + p->ncvzlayer <= s->nkf) set_active_nk(s, s->nk + p->ncvzlayer);
     }
 }
 
@@ -1354,6 +1410,7 @@ void orc_begin_step(orc *s) {
     }
     s->timet += s->p.delt;
     if (!p->steady) update_laser(s);
+    if (p->movingmesh) move_mesh(s);   /* main.cpp:144-145 */
     s->step++;
 }
 

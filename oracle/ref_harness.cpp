@@ -5,8 +5,9 @@
 //   src/main.cpp:53-74      construct Mesh / managers
 //   src/sources/PrintManager.cpp:49-74   init order (assignInputValues,
 //                                        generateGrid, initializeSolver, ...)
-//   src/main.cpp:123-162    per-step order (dt, timet, updateLaser, solve,
-//                                           updateSolutions)
+//   src/main.cpp:123-162    per-step order (dt, timet, updateLaser,
+//                           generateGrid when the mesh moves, solve,
+//                           updateSolutions)
 // without any of the output writers, and dumps per-step scalars and (on
 // request) full fields as raw little-endian float64 so tests can compare the
 // C restatement and the GPU path against the reference program order.
@@ -138,6 +139,7 @@ int main(int argc, char **argv) {
         }
         dm->timet_ += meshObj.delt_;
         if (!dm->steady_) bc->updateLaser();
+        if (meshObj.movingmesh_) cfd->generateGrid();   // main.cpp:144-145
         cfd->iterativeLoopSolve();
         if (!dm->steady_) cfd->updateSolutions();
         ++step;
@@ -150,6 +152,15 @@ int main(int argc, char **argv) {
                      bc->ahtoploss_, bc->beamposx_, bc->beamposy_, bc->powerindicator_, bc->istat_,
                      bc->iend_, bc->jstat_, bc->jend_, bc->kstat_, bc->jstart_, bc->istart_, dm->nk_);
         std::fflush(sc);
+        if (meshObj.movingmesh_) {   // the regenerated x/y node coordinates of this step
+            FILE *gs = std::fopen((outdir + "/grid_steps.txt").c_str(), step == 1 ? "w" : "a");
+            std::fprintf(gs, "%d x", step);
+            for (int i = 0; i < ni; ++i) std::fprintf(gs, " %.17g", cfd->x_[i]);
+            std::fprintf(gs, " y");
+            for (int j = 0; j < nj; ++j) std::fprintf(gs, " %.17g", cfd->y_[j]);
+            std::fprintf(gs, "\n");
+            std::fclose(gs);
+        }
         if (fields_every > 0 && step % fields_every == 0) dump_fields(step);
     }
     double t1 = omp_get_wtime();

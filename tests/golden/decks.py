@@ -19,6 +19,13 @@ The named cases are the workloads the solver is measured and checked on:
 * ``tiny_vol``  same grid, Gaussian laser plus a 100 W volumetric source.
 * ``tiny_out``  same grid, 20 steps with the laser on throughout, output every
                 5 steps (VTS/PVD, residual log, final geometry).
+* ``tiny_tool`` same grid, a two-track toolpath file with a laser-off hatch.
+* ``tiny_powder`` powder-bed mode, two layers; the toolpath's off->on
+                transition adds the second layer mid-run.
+* ``tiny_move``  24x16x14 nodes, ``*ACTIVATE_MOVING_MESH 2 2``: the middle x
+                and y zones follow the beam (+x, +y hatch, -x).
+* ``tiny_move_powder`` the moving mesh over a two-layer powder bed (the
+                configuration of the reference's multi-layer examples).
 * ``plate120``  120x48x32 cells (2.0x0.6x0.4 mm), 400 steps, P=200 W,
                 r=50 um, 1 m/s: the CPU-runnable single-track case.
 * ``track400``  400x160x80 cells (4.0x1.2x0.6 mm, ~5.1M cells), P=300 W,
@@ -63,7 +70,11 @@ def write_deck(path, mesh_path, *, keyword="case", finaltime, maxit=100, tol=1e-
                factor=2.0, discretization="power_law", flow=True, urf=(0.8,) * 5,
                tamb=300.0, tpre=300.0, htc=(10.0, 10.0, 10.0, 10.0), emiss=0.4,
                momentum=(5e-5, -2.6e-4, 1e7, -1.0), material=None, evaporation=None,
-               volumetric=None, outtime=None, scalar_out=("temp",), final_geometry=None, extra=()):
+               volumetric=None, outtime=None, scalar_out=("temp",), final_geometry=None,
+               toolpath=None, powder=None, moving=None, extra=()):
+    """toolpath: rows of (t, x, y, z, laser_on, track) written to <deck>_tool.crs;
+    powder: the 8 *POWDER_BED_MODE values; moving: the two 1-based zone
+    indices of *ACTIVATE_MOVING_MESH."""
     if material is None:
         material = (4000.0, 4000.0, 5e-3, TS, TL, HSMELT, HLFRIZ, ACPA, ACPB, ACPB,
                     0.0, 0.0, 25.0, 0.0, 25.0)
@@ -101,11 +112,42 @@ def write_deck(path, mesh_path, *, keyword="case", finaltime, maxit=100, tol=1e-
         L += ["*BOUNDARY_EVAPORATION", "  " + " ".join(_fmt(v) for v in evaporation)]
     if final_geometry is not None:
         L += ["*OUTPUT_FINAL_GEOMETRY", final_geometry]
+    if powder is not None:
+        L += ["*POWDER_BED_MODE", "$ numlayers ncvlayer layerheight density cpa cpb thcona thconb",
+              "  " + " ".join(_fmt(v) for v in powder)]
+    if moving is not None:
+        L += ["*ACTIVATE_MOVING_MESH", "  %d %d" % tuple(moving)]
     L += list(extra)
     L += ["*MESH_FILE", mesh_path]
+    if toolpath is not None:
+        tool_path = os.path.splitext(os.path.abspath(path))[0] + "_tool.crs"
+        with open(tool_path, "w") as f:
+            for row in toolpath:
+                f.write(" ".join(_fmt(v) for v in row[:4]) + " %d %d\n" % (row[4], row[5]))
+        L += ["*TOOLPATH_FILE", tool_path]
     with open(path, "w") as f:
         f.write("\n".join(L) + "\n")
 
+
+# Toolpaths, rows (t, x, y, z, laser_on, track); a row's laser state holds on
+# the interval that ends at it (BoundCondManager.cpp:216-235).
+ZTOP = 0.25e-3
+# two +x/-x passes joined by a laser-off hatch jump in +y
+TOOL_TWO_TRACKS = [(0.0, 0.15e-3, 0.12e-3, ZTOP, 1, 1), (60e-6, 0.25e-3, 0.12e-3, ZTOP, 1, 1),
+                   (70e-6, 0.25e-3, 0.18e-3, ZTOP, 0, 1), (130e-6, 0.17e-3, 0.18e-3, ZTOP, 1, 2),
+                   (200e-6, 0.17e-3, 0.18e-3, ZTOP, 0, 2)]
+# the same, with the laser-off -> on transition that adds a powder layer
+TOOL_TWO_LAYERS = [(0.0, 0.15e-3, 0.15e-3, ZTOP, 1, 1), (60e-6, 0.25e-3, 0.15e-3, ZTOP, 1, 1),
+                   (75e-6, 0.15e-3, 0.15e-3, ZTOP + 2e-5, 0, 1), (135e-6, 0.25e-3, 0.15e-3, ZTOP + 2e-5, 1, 2),
+                   (200e-6, 0.25e-3, 0.15e-3, ZTOP + 2e-5, 0, 2)]
+# moving-mesh path: +x at 1 m/s, a +y hatch, then -x (the remap's backward recurrence)
+TOOL_MOVING = [(0.0, 0.25e-3, 0.15e-3, ZTOP, 1, 1), (60e-6, 0.31e-3, 0.15e-3, ZTOP, 1, 1),
+               (70e-6, 0.31e-3, 0.19e-3, ZTOP, 0, 1), (130e-6, 0.25e-3, 0.19e-3, ZTOP, 1, 2),
+               (200e-6, 0.25e-3, 0.19e-3, ZTOP, 0, 2)]
+TOOL_MOVING_LAYERS = [(0.0, 0.25e-3, 0.15e-3, ZTOP, 1, 1), (60e-6, 0.31e-3, 0.15e-3, ZTOP, 1, 1),
+                      (75e-6, 0.25e-3, 0.13e-3, ZTOP + 2e-5, 0, 1),
+                      (135e-6, 0.31e-3, 0.13e-3, ZTOP + 2e-5, 1, 2),
+                      (200e-6, 0.31e-3, 0.13e-3, ZTOP + 2e-5, 0, 2)]
 
 CASES = {
     "tiny": dict(
@@ -151,6 +193,36 @@ CASES = {
                   xstart=0.15e-3, ystart=0.15e-3, speed=1.0, power=200.0, radius=50e-6,
                   outtime=2.5e-5, scalar_out=("temp", "uvel", "vvel", "wvel", "fracl", "enthalpy"),
                   final_geometry="tiny_out_geometry.txt")),
+    "tiny_tool": dict(
+        mesh=(([0.1e-3, 0.3e-3, 0.1e-3], [4, 12, 4], [-1.2, 1.0, 1.2]),
+              ([0.15e-3, 0.15e-3], [6, 6], [1.0, 1.0]),
+              ([0.15e-3, 0.1e-3], [4, 8], [1.0, 1.0])),
+        deck=dict(finaltime=0.15e-3 - 1e-9, maxit=60, dt=5e-6, lasertime=0.15e-3,
+                  xstart=0.15e-3, ystart=0.15e-3, speed=1.0, power=200.0, radius=50e-6,
+                  toolpath=TOOL_TWO_TRACKS)),
+    "tiny_powder": dict(
+        mesh=(([0.1e-3, 0.3e-3, 0.1e-3], [4, 12, 4], [-1.2, 1.0, 1.2]),
+              ([0.15e-3, 0.15e-3], [6, 6], [1.0, 1.0]),
+              ([0.15e-3, 0.1e-3], [4, 6], [1.0, 1.0])),
+        deck=dict(finaltime=0.15e-3 - 1e-9, maxit=60, dt=5e-6, lasertime=0.15e-3,
+                  xstart=0.15e-3, ystart=0.15e-3, speed=1.0, power=200.0, radius=50e-6,
+                  powder=(2, 2, 2.0e-5, 2400.0, 1.0e-6, 700.0, 0.0, 7.5),
+                  toolpath=TOOL_TWO_LAYERS)),
+    "tiny_move": dict(
+        mesh=(([0.15e-3, 0.2e-3, 0.15e-3], [6, 10, 6], [1.0, 1.0, 1.0]),
+              ([0.1e-3, 0.1e-3, 0.1e-3], [4, 6, 4], [1.0, 1.0, 1.0]),
+              ([0.15e-3, 0.1e-3], [4, 8], [1.0, 1.0])),
+        deck=dict(finaltime=0.15e-3 - 1e-9, maxit=60, dt=5e-6, lasertime=0.15e-3,
+                  xstart=0.25e-3, ystart=0.15e-3, speed=1.0, power=200.0, radius=40e-6,
+                  toolpath=TOOL_MOVING, moving=(2, 2))),
+    "tiny_move_powder": dict(
+        mesh=(([0.15e-3, 0.2e-3, 0.15e-3], [6, 10, 6], [1.0, 1.0, 1.0]),
+              ([0.1e-3, 0.1e-3, 0.1e-3], [4, 6, 4], [1.0, 1.0, 1.0]),
+              ([0.15e-3, 0.1e-3], [4, 6], [1.0, 1.0])),
+        deck=dict(finaltime=0.15e-3 - 1e-9, maxit=60, dt=5e-6, lasertime=0.15e-3,
+                  xstart=0.25e-3, ystart=0.15e-3, speed=1.0, power=200.0, radius=40e-6,
+                  powder=(2, 2, 2.0e-5, 2400.0, 1.0e-6, 700.0, 0.0, 7.5),
+                  toolpath=TOOL_MOVING_LAYERS, moving=(2, 2))),
     "plate120": dict(
         mesh=(([0.5e-3, 1.0e-3, 0.5e-3], [20, 80, 20], [-1.2, 1.0, 1.2]),
               ([0.2e-3, 0.2e-3, 0.2e-3], [16, 16, 16], [1.0, 1.0, 1.0]),

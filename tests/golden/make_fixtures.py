@@ -10,6 +10,9 @@ call order, OMP_NUM_THREADS=1) on the decks of decks.py and stores, per case:
   <case>_fields.npz      full fields at a few steps (tiny cases only)
   <case>_deck.txt        the deck text (mesh path replaced by @MESH@)
   <case>_mesh.txt        the mesh text
+  <case>_tool.txt        the toolpath file, when the deck has one
+  <case>_grid_steps.txt  per step, the regenerated x and y node coordinates
+                         (moving-mesh decks)
 
 Only data is stored; no reference source text is copied.  Run from the repo
 root:  python tests/golden/make_fixtures.py [case ...]
@@ -39,6 +42,10 @@ PLAN = {
     "tiny_evap": (1, -1, []),
     "tiny_vap": (1, -1, []),
     "tiny_vol": (1, -1, []),
+    "tiny_tool": (1, -1, []),
+    "tiny_powder": (1, -1, []),
+    "tiny_move": (1, -1, []),
+    "tiny_move_powder": (1, -1, []),
     "plate120": (50, -1, []),
 }
 
@@ -81,8 +88,13 @@ def run_case(case, workdir=None, reuse=False):
     if kept:
         np.savez_compressed(os.path.join(HERE, case + "_fields.npz"), **kept)
     mesh = os.path.abspath(os.path.join(tmp, case + "_mesh.k"))
+    tool = os.path.abspath(os.path.join(tmp, case + "_tool.crs"))
     with open(deck) as f:
-        text = f.read().replace(mesh, "@MESH@")
+        text = f.read().replace(mesh, "@MESH@").replace(tool, "@TOOL@")
+    if os.path.exists(tool):
+        shutil.copy(tool, os.path.join(HERE, case + "_tool.txt"))
+    if os.path.exists(os.path.join(tmp, "grid_steps.txt")):
+        shutil.copy(os.path.join(tmp, "grid_steps.txt"), os.path.join(HERE, case + "_grid_steps.txt"))
     with open(os.path.join(HERE, case + "_deck.txt"), "w") as f:
         f.write(text)
     shutil.copy(mesh, os.path.join(HERE, case + "_mesh.txt"))

@@ -69,11 +69,25 @@ def test_deck_errors(amcfd, tmp_path, case_dir):
         amcfd.Deck(missing)
     assert e.value.code == -2
     # subsystems outside the per-step solve are refused, not ignored
-    for kw in ("*SOLVE_SPECIES\n1 1 1 1", "*ACTIVATE_MOVING_MESH\n2 2"):
+    for kw in ("*SOLVE_SPECIES\n1 1 1 1",):
         t = _write(tmp_path, "unsup.k", good.replace("*MESH_FILE", kw + "\n*MESH_FILE"))
         with pytest.raises(amcfd.AmcfdError) as e:
             amcfd.Deck(t)
         assert e.value.code == -4
+    # a moving zone needs a zone on either side in x and y (the reference
+    # writes past its zone arrays otherwise); this grid has two y zones
+    for kw in ("*ACTIVATE_MOVING_MESH\n2 2", "*ACTIVATE_MOVING_MESH\n2"):
+        t = _write(tmp_path, "mm.k", good.replace("*MESH_FILE", kw + "\n*MESH_FILE"))
+        with pytest.raises(amcfd.AmcfdError) as e:
+            amcfd.Deck(t)
+        assert e.value.code == -3
+
+
+def test_moving_mesh_keyword(amcfd, case_dir):
+    p = amcfd.Deck(case_dir("tiny_move")).params
+    assert (p.movingmesh, p.xmoveindex, p.ymoveindex) == (1, 1, 1)
+    assert p.udtoolpath == 1 and p.n_toolpath == 5
+    assert amcfd.Deck(case_dir("tiny")).params.movingmesh == 0
 
 
 def test_deck_grammar_details(amcfd, tmp_path, case_dir):

@@ -23,7 +23,7 @@ def rel_l2(a, b):
     return float(np.linalg.norm(a - b) / nb)
 
 
-def compare_run(amcfd, oracle_mod, deck_path, max_steps=None, fields=FIELDS, tol=TOL):
+def compare_run(amcfd, oracle_mod, deck_path, max_steps=None, fields=FIELDS, tol=TOL, geometry=()):
     deck = amcfd.Deck(deck_path)
     gpu = amcfd.Solver(deck)
     orc = oracle_mod.Oracle(deck.params)
@@ -47,10 +47,14 @@ def compare_run(amcfd, oracle_mod, deck_path, max_steps=None, fields=FIELDS, tol
                           ("heatinlaser", tol), ("ratio", 1e-6), ("resorh", 1e-6)):
             a, b = sg[key], so[key]
             assert abs(a - b) <= rtol * max(abs(b), 1e-30) or (np.isnan(a) and np.isnan(b)), (step, key, a, b)
+        assert sg["nk"] == so["nk"], (step, sg["nk"], so["nk"])
         for f in fields:
             e = rel_l2(gpu.field(f), orc.field(f))
             worst[f] = max(worst.get(f, 0.0), e)
             assert e <= tol, (step, f, e)
+        for g in geometry:
+            a = gpu.geometry(g)
+            assert np.array_equal(a, orc.field(g, shape=a.shape)), (step, g)
     return step, worst
 
 
@@ -59,6 +63,21 @@ def test_gpu_matches_oracle_tiny(case, case_dir, amcfd, oracle_mod):
     steps, worst = compare_run(amcfd, oracle_mod, case_dir(case))
     print(case, steps, {k: "%.1e" % v for k, v in worst.items()})
     assert steps > 0
+
+
+POWDER_FIELDS = FIELDS + ["csfrac", "solfrac", "mpPhase", "mpLiqPhase"]
+
+
+@pytest.mark.parametrize("case", ["tiny_tool", "tiny_powder", "tiny_move", "tiny_move_powder"])
+def test_gpu_matches_oracle_toolpath_powder_moving(case, case_dir, amcfd, oracle_mod):
+    """Toolpath file, powder-bed layers and the moving mesh (grid rebuilt and
+    the solution remapped every step): the fields, the powder history fields
+    and the regenerated grid after every step."""
+    geometry = ("x", "y", "z", "xu", "yv", "dxpwinv", "dypsinv", "dzpbinv", "fracx", "fracy", "fracz",
+                "areaij", "areajk", "areaik")
+    steps, worst = compare_run(amcfd, oracle_mod, case_dir(case), fields=POWDER_FIELDS, geometry=geometry)
+    print(case, steps, {k: "%.1e" % v for k, v in worst.items()})
+    assert steps == 30
 
 
 def test_gpu_matches_oracle_surfactant_until_first_log_difference(case_dir, amcfd, oracle_mod):
