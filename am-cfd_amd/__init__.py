@@ -73,7 +73,12 @@ class Params(ctypes.Structure):
         ("n_heat_table", ctypes.c_int), ("heat_table", ctypes.POINTER(ctypes.c_double)),
         ("ni", ctypes.c_int), ("nj", ctypes.c_int), ("nk", ctypes.c_int), ("nk_active0", ctypes.c_int),
         ("hlcal", ctypes.c_double), ("hlatnt", ctypes.c_double),
-        ("movingmesh", ctypes.c_int), ("xmoveindex", ctypes.c_int), ("ymoveindex", ctypes.c_int)]
+        ("movingmesh", ctypes.c_int), ("xmoveindex", ctypes.c_int), ("ymoveindex", ctypes.c_int),
+        ("species", ctypes.c_int)] + [
+        (n, ctypes.c_double) for n in ("initconcentration", "massdiff", "kp", "urfc")] + [
+        ("solidification", ctypes.c_int), ("energyfreesurface", ctypes.c_int)] + [
+        (n, ctypes.c_double) for n in ("initlam1", "initlam2", "surftens", "maxiterenergy", "tolenergy",
+                                       "maxiterbisect", "tolbisect", "urfree")]
 
 
 class Scalars(ctypes.Structure):
@@ -85,7 +90,8 @@ class Scalars(ctypes.Structure):
             "beamposy", "scanvel")] + [
         (n, ctypes.c_int) for n in (
             "powerindicator", "trackindx", "istat", "iend", "jstat", "jend", "kstat", "istart",
-            "jstart", "nk", "step")] + [("delt", ctypes.c_double), ("outtime", ctypes.c_double)]
+            "jstart", "nk", "step")] + [("delt", ctypes.c_double), ("outtime", ctypes.c_double),
+                                        ("resorc", ctypes.c_double)]
 
     def as_dict(self):
         return {n: getattr(self, n) for n, _ in self._fields_}

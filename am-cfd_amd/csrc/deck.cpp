@@ -11,7 +11,7 @@
 // Where the reference crashes or reads out of bounds (blank line, missing
 // values, short toolpath rows) this reader returns AMCFD_E_PARSE instead.
 // Keywords whose subsystems are outside the per-step solve and would change
-// its results (species, energy free surface, G/R) are refused
+// its results (the energy-based free surface) are refused
 // with AMCFD_E_UNSUPPORTED; output keywords are kept for the output writers
 // (output.cpp), other output-only keywords are accepted and ignored.
 
@@ -91,7 +91,7 @@ int Deck::load(const std::string &path) {
         {"*OUTPUT_FINAL_GEOMETRY", B_GEOM}, {"*OUTPUT_SECTION", B_SECTION},
         {"*VOLUMETRIC_HEAT_SOURCE_TABLE", B_HEATTABLE}, {"*CONTROL_TIMESTEP_VARIABLE", B_DTVAR}};
 
-    std::vector<double> track, mat, relax, temps, energy, gauss, momentum, evap, powder, volheat;
+    std::vector<double> track, mat, relax, temps, energy, gauss, momentum, evap, powder, volheat, species;
     std::map<std::string, double> params;
     bool have_mesh = false, have_dt = false;
     int block = B_NONE, lineno = 0;
@@ -102,8 +102,8 @@ int Deck::load(const std::string &path) {
             return set_error(AMCFD_E_PARSE, path + ":" + std::to_string(lineno) + ": empty line");
         if (line[0] == '$') continue;
         if (line == "*FLUID_FLOW_OFF") { flags.navierstokes = 0; continue; }
-        if (line == "*CALCULATE_SOLIDIFICATION_PARAMETERS")
-            return set_error(AMCFD_E_UNSUPPORTED, "*CALCULATE_SOLIDIFICATION_PARAMETERS (G/R post-processing) is not supported");
+        // a flag line that, like *FLUID_FLOW_OFF, leaves the current block open (Mesh.cpp:53-62)
+        if (line == "*CALCULATE_SOLIDIFICATION_PARAMETERS") { flags.solidification = 1; continue; }
         auto it = kw.find(line);
         if (it != kw.end()) {
             block = it->second;
@@ -115,8 +115,7 @@ int Deck::load(const std::string &path) {
             case B_TEC: tecplot = 1; break;
             case B_VOLHEAT: flags.volheatsource = 1; break;
             case B_HEATTABLE: flags.volheatsource = 1; flags.volheatsourcetable = 1; break;
-            case B_SPECIES:
-                return set_error(AMCFD_E_UNSUPPORTED, "*SOLVE_SPECIES is not supported");
+            case B_SPECIES: flags.species = 1; break;
             case B_MOVING: flags.movingmesh = 1; break;
             case B_FREESURF:
                 return set_error(AMCFD_E_UNSUPPORTED, "*TRACK_ENERGY_BASED_FREE_SURFACE is not supported");
@@ -139,6 +138,7 @@ int Deck::load(const std::string &path) {
         case B_EVAP: v = numbers(line); evap.insert(evap.end(), v.begin(), v.end()); break;
         case B_POWDER: v = numbers(line); powder.insert(powder.end(), v.begin(), v.end()); break;
         case B_VOLHEAT: v = numbers(line); volheat.insert(volheat.end(), v.begin(), v.end()); break;
+        case B_SPECIES: v = numbers(line); species.insert(species.end(), v.begin(), v.end()); break;
         case B_MOVING:   // each line sets both 1-based zone indices, Mesh.cpp:364-369
             v = numbers(line);
             if (v.size() < 2)
@@ -248,6 +248,10 @@ int Deck::load(const std::string &path) {
     if (flags.isevap) {
         if ((st = need(evap, 4, "*BOUNDARY_EVAPORATION"))) return st;
         p.evaplatent = evap[0]; p.evaptemp = evap[1]; p.evapfact = evap[2]; p.molarmass = evap[3];
+    }
+    if (flags.species) {   // DomainManager.cpp:123-130
+        if ((st = need(species, 4, "*SOLVE_SPECIES"))) return st;
+        p.initconcentration = species[0]; p.massdiff = species[1]; p.kp = species[2]; p.urfc = species[3];
     }
     if (flags.powderbed) {
         if ((st = need(powder, 8, "*POWDER_BED_MODE"))) return st;
@@ -367,6 +371,8 @@ void Deck::finalize_tables() {
     p.powderbed = flags.powderbed;
     p.udtoolpath = flags.udtoolpath;
     p.movingmesh = flags.movingmesh;
+    p.species = flags.species;
+    p.solidification = flags.solidification;
     p.gaussheatsource = flags.gaussheatsource;
     p.volheatsource = flags.volheatsource;
     p.volheatsourcetable = flags.volheatsourcetable;

@@ -676,11 +676,12 @@ __global__ __launch_bounds__(256) void k_finalize(int nblk_t, int nblk_b, int nb
                                                   const double *__restrict__ part_t,
                                                   const double *__restrict__ part_b,
                                                   const double *__restrict__ part_f,
-                                                  const double *__restrict__ part_e, int flow_done, int steady,
-                                                  double dens, double width, DevScalars *sc) {
+                                                  const double *__restrict__ part_e,
+                                                  const double *__restrict__ part_c, int flow_done, int species_done,
+                                                  int steady, double dens, double width, DevScalars *sc) {
     // every quantity here only feeds the convergence test, so the partials
     // are folded by a parallel tree (fixed order: still reproducible)
-    constexpr int NQ = 15;   // 8 heat-balance, toploss, 2 energy residual, 4 flow (+ umax)
+    constexpr int NQ = 17;   // 8 heat-balance, toploss, 2 energy residual, 4 flow, 2 species (+ umax)
     __shared__ double s_red[NQ + 1][4];
     const int tid = threadIdx.x, nt = blockDim.x;
     double a[NQ + 1];
@@ -694,12 +695,17 @@ __global__ __launch_bounds__(256) void k_finalize(int nblk_t, int nblk_b, int nb
         a[9] += part_e[2 * b];
         a[10] += part_e[2 * b + 1];
     }
+    if (species_done)
+        for (int b = tid; b < nblk_e; b += nt) {
+            a[15] += part_c[2 * b];
+            a[16] += part_c[2 * b + 1];
+        }
     if (flow_done) {
         for (int b = tid; b < nblk_f; b += nt) {
             const double *pu = part_f + (size_t)b * 3, *pv = part_f + ((size_t)nblk_f + b) * 3;
             const double *pw = part_f + ((size_t)2 * nblk_f + b) * 3, *pp = part_f + ((size_t)3 * nblk_f + b) * 3;
             a[11] += pu[0];
-            a[15] = dmax(a[15], pu[1]);
+            a[NQ] = dmax(a[NQ], pu[1]);
             a[12] += pv[0];
             a[13] += pw[0];
             a[14] += pp[1];           // continuity normaliser
@@ -746,6 +752,7 @@ __global__ __launch_bounds__(256) void k_finalize(int nblk_t, int nblk_b, int nb
         sc->resorw = r[13] / refmom;
         sc->resorm = msum / (r[14] * dens + AMCFD_SMALL);
     }
+    if (species_done) sc->resorc = r[15] / (r[16] + AMCFD_SMALL);
 }
 
 // ------------------------------------------------------------------------
@@ -861,11 +868,11 @@ void launch_props(hipStream_t st, const Dims &d, DevFields f, const amcfd_params
                        f.csfrac, f.diff, f.rho, f.vis);
 }
 
-void launch_block_correction(hipStream_t st, const Dims &d, DevFields f, Partials pr, DevScalars *sc) {
+void launch_block_correction(hipStream_t st, const Dims &d, DevFields f, Partials pr, double *phi) {
     hipLaunchKernelGGL(k_column_sums, dim3((d.nim2 + CS_COLS - 1) / CS_COLS, 4), dim3(CS_NT), 0, st, d, f.sp, f.ae, f.aw,
                        f.apnot, pr.blk);
     hipLaunchKernelGGL(k_block_solve, dim3(1), dim3(256), 0, st, d, pr.blk, pr.delphi);
-    hipLaunchKernelGGL(k_add_delphi, energy_grid(d), dim3(64, 4), 0, st, d, pr.delphi, f.enthalpy);
+    hipLaunchKernelGGL(k_add_delphi, energy_grid(d), dim3(64, 4), 0, st, d, pr.delphi, phi);
 }
 
 static dim3 node_grid(const Dims &d) {
@@ -887,13 +894,14 @@ void launch_melt_pool(hipStream_t st, const Dims &d, const DevGeom &g, DevFields
 }
 
 void launch_finalize(hipStream_t st, const Dims &d, Partials pr, DevScalars *sc, const amcfd_params &p,
-                     int flow_done, int steady, double width) {
+                     int flow_done, int species_done, int steady, double width) {
     dim3 gr = node_grid(d);
     int nblk_t = gr.x * gr.y * gr.z;
     const int nblk_b = (d.njm2 * d.nim2 + d.nkm2 * d.njm2 + d.nkm2 * d.nim2 + 255) / 256;
     const dim3 e = energy_grid(d);
     hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, st, nblk_t, nblk_b, pr.nblk_f, (int)(e.x * e.y * e.z),
-                       pr.h2t, pr.bc, pr.flow, pr.energy, flow_done, steady, p.dens, width, sc);
+                       pr.h2t, pr.bc, pr.flow, pr.energy, pr.species, flow_done, species_done, steady, p.dens,
+                       width, sc);
 }
 
 void launch_end_step(hipStream_t st, const Dims &d, DevFields f, const amcfd_params &p, double track,

@@ -18,6 +18,7 @@ int set_error(int code, const std::string &msg);
 // ---------------------------------------------------------------------------
 struct Flags {
     int navierstokes = 1, steady = 0, isevap = 0, powderbed = 0, udtoolpath = 0, movingmesh = 0;
+    int species = 0, solidification = 0;
     int gaussheatsource = 0, volheatsource = 0, volheatsourcetable = 0;
 };
 
@@ -40,7 +41,7 @@ struct Deck {
 // device-side scalar block: everything the host reads once per iteration
 // ---------------------------------------------------------------------------
 struct DevScalars {
-    double resorh, resorm, resoru, resorv, resorw, refmom;
+    double resorh, resorm, resoru, resorv, resorw, refmom, resorc;
     double tpeak, alen, depth, width;
     double ratio, heatout, accul, heatvol, ahtoploss, heatinlaser;
     double fluxwest, fluxeast, fluxbottom, fluxtop, fluxsouth, fluxnorth;
@@ -65,6 +66,8 @@ struct DevFields {
     double *ap, *an, *as, *ae, *aw, *at, *ab, *su, *sp, *apnot;
     double *lpr, *lden, *ly;   // line-sweep pivots: pr, pivot, 1/pivot
     double *heatin;   // [nj*ni]
+    // species (with *SOLVE_SPECIES) and G/R (with *CALCULATE_SOLIDIFICATION_PARAMETERS), else null
+    double *conc, *concnot, *concl, *grad, *rate;
 };
 
 // dimensions and constants every kernel needs (passed by value)
@@ -95,6 +98,7 @@ struct LaserParams {
 // partial-sum buffers (deterministic two-pass reductions)
 struct Partials {
     double *energy;     // [nblk_e][2]       resid sumd, sumh
+    double *species;    // [nblk_e][2]       species resid sumd, sumc
     double *blk;        // [4][ni]           block-correction column sums
     double *h2t;        // [nblk_t][10]      tpeak + fluxes + accul/heatvol
     double *flow;       // [4][nblk_f][3]    per variable: sumd / umax / denom
@@ -121,7 +125,9 @@ void launch_energy_assemble(hipStream_t st, const Dims &d, const DevGeom &g, Dev
 void launch_laser_map(hipStream_t st, const Dims &d, const DevGeom &g, DevFields f, Partials pr,
                       const LaserParams &q, DevScalars *sc);
 void launch_props(hipStream_t st, const Dims &d, DevFields f, const amcfd_params &p);
-void launch_block_correction(hipStream_t st, const Dims &d, DevFields f, Partials pr, DevScalars *sc);
+// x-direction block correction of phi from the column-sum terms the assemble
+// kernel left in f.sp / f.apnot (enthalpy or concentration)
+void launch_block_correction(hipStream_t st, const Dims &d, DevFields f, Partials pr, double *phi);
 void launch_h2t(hipStream_t st, const Dims &d, const DevGeom &g, DevFields f, Partials pr,
                 const amcfd_params &p);
 void launch_melt_pool(hipStream_t st, const Dims &d, const DevGeom &g, DevFields f, Partials pr,
@@ -135,7 +141,7 @@ void launch_pressure(hipStream_t st, const Dims &d, const DevGeom &g, DevFields 
                      const Box &b, const amcfd_params &p);
 void launch_correct(hipStream_t st, const Dims &d, DevFields f, const Box &b, const amcfd_params &p);
 void launch_finalize(hipStream_t st, const Dims &d, Partials pr, DevScalars *sc, const amcfd_params &p,
-                     int flow_done, int steady, double width);
+                     int flow_done, int species_done, int steady, double width);
 void launch_end_step(hipStream_t st, const Dims &d, DevFields f, const amcfd_params &p, double track,
                      double trackid);
 // pivots of every line of a sweep and the pass buffers' "pending" fill (must
@@ -148,10 +154,18 @@ void launch_line_sweeps(hipStream_t st, const Dims &d, DevFields f, double *phi,
                         const SweepPlan &plan, int ibeg, int iend, int guard, int scan, unsigned long long *counter,
                         unsigned long long counter_base, DevScalars *sc, int nblocks);
 
-// moving mesh: remap the twelve solution fields from the node coordinates
+// species (species.hip)
+void launch_bc_species(hipStream_t st, const Dims &d, const DevGeom &g, DevFields f, const amcfd_params &p);
+void launch_species_assemble(hipStream_t st, const Dims &d, const DevGeom &g, DevFields f, Partials pr,
+                             const amcfd_params &p);
+void launch_species_end(hipStream_t st, const Dims &d, DevFields f, const amcfd_params &p);
+void launch_solidification(hipStream_t st, const Dims &d, const DevGeom &g, DevFields f, double tsolid,
+                           double scanvel);
+
+// moving mesh: remap the twelve (fourteen with species) solution fields from the node coordinates
 // xold/yold onto g.x/g.y (x pass, then y pass), dirx/diry = sign of the move
 void launch_remap(hipStream_t st, const Dims &d, DevFields f, const DevGeom &g, const double *xold,
-                  const double *yold, int dirx, int diry);
+                  const double *yold, int dirx, int diry, int species);
 
 // wavefront task list for the Gauss-Seidel line sweeps (host side)
 void build_sweep_tasks(int kbeg, int kend, int jbeg, int jend, std::vector<int> &tasks, int &width);

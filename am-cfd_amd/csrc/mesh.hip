@@ -3,10 +3,10 @@
 // CFDSolverManager::interpolate (:3057-3066).
 //
 // The reference walks k, j (< nj-1), i (1..ni-2) over the active extent and,
-// where a node moved by more than 1e-6, replaces each of twelve fields by the
-// linear interpolant between the node and its neighbour in the direction of
-// the move, evaluated at the node's new coordinate: first the whole x pass,
-// then the whole y pass.  Walking upwards in place, a backward move
+// where a node moved by more than 1e-6, replaces each of twelve fields
+// (fourteen with species) by the linear interpolant between the node and its
+// neighbour in the direction of the move, evaluated at the node's new
+// coordinate: first the whole x pass, then the whole y pass.  Walking upwards in place, a backward move
 // (dir = -1) reads the neighbour that was already remapped, so each x-line
 // (x pass) or y-column (y pass) is a recurrence; a forward move reads only
 // values not yet touched.  One thread here owns one line or column of one
@@ -19,9 +19,9 @@
 
 namespace amcfd {
 
-// the twelve remapped fields, in the reference's order
+// the remapped fields, in the reference's order (the last two with species)
 struct RemapFields {
-    double *f[12];
+    double *f[14];
 };
 
 __device__ __forceinline__ double lerp_ref(double x0, double x1, double y0, double y1, double x) {
@@ -64,13 +64,14 @@ __global__ void k_remap_y(Dims d, RemapFields rf, const double *__restrict__ yne
 }
 
 void launch_remap(hipStream_t st, const Dims &d, DevFields f, const DevGeom &g, const double *xold,
-                  const double *yold, int dirx, int diry) {
+                  const double *yold, int dirx, int diry, int species) {
     RemapFields rf{{f.enthalpy, f.hnot, f.temp, f.fracl, f.uvel, f.vvel, f.wvel, f.unot, f.vnot, f.wnot,
-                    f.pressure, f.pp}};
+                    f.pressure, f.pp, f.conc, f.concnot}};
+    const unsigned nf = species ? 14 : 12;
     const int nt = 128;
     const int lines = (d.nj - 1) * d.nk, cols = (d.ni - 2) * d.nk;
-    hipLaunchKernelGGL(k_remap_x, dim3((lines + nt - 1) / nt, 12), dim3(nt), 0, st, d, rf, g.x, xold, dirx);
-    hipLaunchKernelGGL(k_remap_y, dim3((cols + nt - 1) / nt, 12), dim3(nt), 0, st, d, rf, g.y, yold, diry);
+    hipLaunchKernelGGL(k_remap_x, dim3((lines + nt - 1) / nt, nf), dim3(nt), 0, st, d, rf, g.x, xold, dirx);
+    hipLaunchKernelGGL(k_remap_y, dim3((cols + nt - 1) / nt, nf), dim3(nt), 0, st, d, rf, g.y, yold, diry);
 }
 
 }  // namespace amcfd

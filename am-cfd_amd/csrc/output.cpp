@@ -41,6 +41,8 @@ bool scalar_output_known(const std::string &n, const amcfd_params &p) {
     for (const char *b : base)
         if (n == b) return true;
     if (p.powderbed && (n == "csfrac" || n == "solfrac" || n == "mpPhase" || n == "mpLiqPhase")) return true;
+    if (p.species && (n == "avgconcentration" || n == "concentration")) return true;
+    if (p.solidification && (n == "rate" || n == "grad")) return true;
     return false;
 }
 
@@ -238,7 +240,7 @@ int append_residuals(Solver &s, const char *path) {
         }
     const double crs = (tso - tsob) / (xso - xsob) * sc.scanvel;
     const double crl = (tliq - tso) / (xliq - xso) * sc.scanvel;
-    const double resorc = 0.0;   // no species equation on this path
+    const double resorc = sc.resorc;   // 0 without species (the reference's member starts at zero)
 
     std::ofstream f(path, std::ios::out | std::ios::app);
     if (!f) return set_error(AMCFD_E_IO, std::string("cannot write ") + path);

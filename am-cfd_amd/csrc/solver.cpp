@@ -121,6 +121,17 @@ int Solver::create(const amcfd_params &p, int mode) {
     for (auto *q : fl)
         if ((st = alloc(q, nn))) return st;
     if ((st = alloc(&f_.heatin, (size_t)d_.ni * d_.nj))) return st;
+    if (p_.species) {   // CFDSolverManager.cpp:474-479
+        double **sp3[] = {&f_.conc, &f_.concnot, &f_.concl};
+        for (auto *q : sp3) {
+            if ((st = alloc(q, nn))) return st;
+            if ((st = upload(*q, std::vector<double>(nn, p_.initconcentration)))) return st;
+        }
+    }
+    if (p_.solidification) {   // zero until the first molten step (CFDSolverManager.cpp:469-473)
+        if ((st = alloc(&f_.grad, nn))) return st;
+        if ((st = alloc(&f_.rate, nn))) return st;
+    }
     InitialState is;
     initial_fields(p_, hg_, is);
     if ((st = upload(f_.enthalpy, is.enthalpy)) || (st = upload(f_.hnot, is.hnot)) ||
@@ -132,6 +143,7 @@ int Solver::create(const amcfd_params &p, int mode) {
     Box whole{1, d_.nim2, 1, d_.njm2, 2};
     int nf = flow_partial_blocks(whole, d_) + 64;
     if ((st = alloc(&pr_.energy, (size_t)2 * pr_.nblk_e))) return st;
+    if ((st = alloc(&pr_.species, (size_t)2 * pr_.nblk_e))) return st;
     // block correction: 4 column sums, each ni long
     if ((st = alloc(&pr_.blk, (size_t)4 * d_.ni + 8))) return st;
     if ((st = alloc(&pr_.h2t, (size_t)10 * pr_.nblk_t))) return st;
@@ -307,7 +319,7 @@ int Solver::move_mesh() {
     HIPCALL(hipStreamSynchronize(st_));
     int st;
     if ((st = upload_geometry()) || (st = upload(xold_, px)) || (st = upload(yold_, py))) return st;
-    launch_remap(st_, d_, f_, g_, xold_, yold_, dirx, diry);
+    launch_remap(st_, d_, f_, g_, xold_, yold_, dirx, diry, p_.species);
     HIPCALL(hipGetLastError());
     // the host vectors are rewritten at the next move
     HIPCALL(hipStreamSynchronize(st_));
@@ -469,7 +481,7 @@ int Solver::solve_step() {
         launch_energy_assemble(st_, d_, g_, f_, pr_, p_, sp);
         toc("assemble_h", slot);
         launch_props(st_, d_, f_, p_);
-        launch_block_correction(st_, d_, f_, pr_, sc_);
+        launch_block_correction(st_, d_, f_, pr_, f_.enthalpy);
         int st = sweep(f_.enthalpy, plan_dom_, 1, d_.nim2, 1, d_.njm2, 1, d_.nkm2, 0, "sweep_h");
         if (st) return st;
         tic("h2t", &slot);
@@ -484,13 +496,21 @@ int Solver::solve_step() {
             if ((st = flow_iteration())) return st;
             flow = 1;
         }
-        launch_finalize(st_, d_, pr_, sc_, p_, flow, p_.steady, sc_host_->width);
+        if (p_.species) {   // CFDSolverManager.cpp:677-686
+            launch_bc_species(st_, d_, g_, f_, p_);
+            tic("assemble_c", &slot);
+            launch_species_assemble(st_, d_, g_, f_, pr_, p_);
+            toc("assemble_c", slot);
+            launch_block_correction(st_, d_, f_, pr_, f_.conc);
+            if ((st = sweep(f_.conc, plan_dom_, 1, d_.nim2, 1, d_.njm2, 1, d_.nkm2, 0, "sweep_c"))) return st;
+        }
+        launch_finalize(st_, d_, pr_, sc_, p_, flow, p_.species, p_.steady, sc_host_->width);
         HIPCALL(hipGetLastError());
         if ((st = read_scalars())) return st;
         if (profile_) collect_timers();
         const DevScalars &s = *sc_host_;
         // std::max over {resorm, resoru, resorv, resorw, resorc}: first largest wins
-        double res[5] = {s.resorm, s.resoru, s.resorv, s.resorw, 0.0};
+        double res[5] = {s.resorm, s.resoru, s.resorv, s.resorw, s.resorc};
         double amaxres = res[0];
         for (int r = 1; r < 5; ++r)
             if (amaxres < res[r]) amaxres = res[r];
@@ -515,11 +535,21 @@ int Solver::solve_step() {
 }
 
 int Solver::end_step() {
-    if (p_.steady) return AMCFD_OK;
+    // getSolidificationParameters runs between the solve and the push-back
+    // (main.cpp:151-152), steady runs included; skipped while nothing melts
+    if (p_.solidification && sc_host_->tpeak > p_.tsolid) {
+        launch_solidification(st_, d_, g_, f_, p_.tsolid, scanvel_);
+        HIPCALL(hipGetLastError());
+    }
+    if (p_.steady) {
+        HIPCALL(hipStreamSynchronize(st_));
+        return AMCFD_OK;
+    }
     double trackid = 0.0;
     for (int r = p_.n_toolpath; r > 0; --r)
         if (timet_ <= tool_[6 * (r - 1)]) trackid = (double)(int)tool_[6 * (r - 1) + 5];
     launch_end_step(st_, d_, f_, p_, (double)trackindx_, trackid);
+    if (p_.species) launch_species_end(st_, d_, f_, p_);
     HIPCALL(hipGetLastError());
     // the step is complete when this returns, as the reference's is
     HIPCALL(hipStreamSynchronize(st_));
@@ -539,7 +569,10 @@ double *Solver::field_ptr(const char *n) {
              {"unot", f_.unot}, {"vnot", f_.vnot}, {"wnot", f_.wnot}, {"sourceinput", f_.sourceinput},
              {"dux", f_.dux}, {"dvy", f_.dvy}, {"dwz", f_.dwz}, {"csfrac", f_.csfrac}, {"solfrac", f_.solfrac},
              {"mpPhase", f_.mpphase}, {"mpLiqPhase", f_.mpliqphase}, {"ap", f_.ap}, {"an", f_.an},
-             {"as", f_.as}, {"ae", f_.ae}, {"aw", f_.aw}, {"at", f_.at}, {"ab", f_.ab}, {"su", f_.su}};
+             {"as", f_.as}, {"ae", f_.ae}, {"aw", f_.aw}, {"at", f_.at}, {"ab", f_.ab}, {"su", f_.su},
+             {"avgconcentration", f_.conc}, {"avgconcentrationnot", f_.concnot}, {"concentration", f_.concl},
+             {"grad", f_.grad}, {"rate", f_.rate}};
+    // species and G/R fields exist only with their keywords (null otherwise)
     for (auto &e : t)
         if (std::strcmp(e.name, n) == 0) return e.p;
     return nullptr;
@@ -610,6 +643,7 @@ int Solver::get_scalars(amcfd_scalars *o) {
     o->resoru = s.resoru;
     o->resorv = s.resorv;
     o->resorw = s.resorw;
+    o->resorc = s.resorc;
     o->tpeak = s.tpeak;
     o->alen = s.alen;
     o->depth = s.depth;

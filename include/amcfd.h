@@ -111,6 +111,18 @@ typedef struct amcfd_params {
      * zones either side of them stretch and shrink with the beam every step
      * (CFDSolverManager.cpp:72-124) */
     int    movingmesh, xmoveindex, ymoveindex;
+    /* *SOLVE_SPECIES: DomainManager.cpp:123-130 (initial concentration, mass
+     * diffusivity, partition coefficient, under-relaxation factor) */
+    int    species;
+    double initconcentration, massdiff, kp, urfc;
+    /* *CALCULATE_SOLIDIFICATION_PARAMETERS: temperature gradient G and growth
+     * rate R after every solve (CFDSolverManager.cpp:2763-2804) */
+    int    solidification;
+    /* *TRACK_ENERGY_BASED_FREE_SURFACE: DomainManager.cpp:133-144 (bisection
+     * bracket, surface tension, energy-iteration limit and tolerance,
+     * bisection limit and tolerance, relaxation); evaluated at output times */
+    int    energyfreesurface;
+    double initlam1, initlam2, surftens, maxiterenergy, tolenergy, maxiterbisect, tolbisect, urfree;
 } amcfd_params;
 
 /* scalars the reference keeps as members (CFDSolverManager.h:81-83,
@@ -126,6 +138,7 @@ typedef struct amcfd_scalars {
     int    istat, iend, jstat, jend, kstat, istart, jstart, nk;
     int    step;
     double delt, outtime;   /* time step and output interval in force (main.cpp:125-133) */
+    double resorc;          /* species residual (CFDSolverManager.cpp:2128); 0 without species */
 } amcfd_scalars;
 
 typedef struct amcfd_deck amcfd_deck;      /* parsed input deck          */

@@ -43,6 +43,14 @@ struct orc {
     double *mpphase, *mpliqphase, *unot, *vnot, *wnot, *pressure;
     double *ap, *an, *as, *ae, *aw, *at, *ab, *apnot, *dux, *dvy, *dwz, *su, *sp;
     double *heatin;
+    /* species (CFDSolverManager.h:87-89) and G/R (:68-69) */
+    double *conc, *concnot, *concl, *grad, *rate;
+    /* energy-based free surface (FreeSurfaceManager.h): surface shift fi,
+     * its previous iterate zzm, applied pressure apdis [nj*ni]; shifted
+     * node heights zr and temperatures tempr [nk*nj*ni]; last bisection */
+    double *fi, *zzm, *apdis, *zr, *tempr;
+    double fs_xmid, fs_fmid, fs_vd;
+    int fs_niter;
     /* solver scalars */
     double resoru, resorv, resorw, resorm, resorh, resorc, refmom;
     int itertot, niter, heat_it;
@@ -177,7 +185,7 @@ static double lerp_ref(double x0, double x1, double y0, double y1, double x) {
 /* Moving mesh, CFDSolverManager.cpp:60-124 (zone shift) + 345-346 and
  * 2809-3051 (remap).  The zones either side of the moving one take up the
  * beam's displacement since the last step; the grid is rebuilt; then twelve
- * fields are re-interpolated in place, first along x (node i from i and
+ * fields (fourteen with species) are re-interpolated in place, first along x (node i from i and
  * i+dir), then along y (j and j+dir), wherever a node moved by more than
  * 1e-6.  Sweeping i (j) upwards in place makes a backward move (dir = -1) a
  * recurrence on the already-remapped neighbour. */
@@ -197,14 +205,15 @@ static void move_mesh(orc *s) {
     p->yzone[iyz - 1] += dy;
     p->yzone[iyz + 1] -= dy;
     generate_grid(s);
-    double *f[12] = {s->enthalpy, s->hnot, s->temp, s->fracl, s->uvel, s->vvel,
-                     s->wvel, s->unot, s->vnot, s->wnot, s->pressure, s->pp};
+    double *f[14] = {s->enthalpy, s->hnot, s->temp, s->fracl, s->uvel, s->vvel,
+                     s->wvel, s->unot, s->vnot, s->wnot, s->pressure, s->pp, s->conc, s->concnot};
+    const int nf = p->species ? 14 : 12;
     for (int k = 0; k < nk; ++k)
         for (int j = 0; j < nj - 1; ++j)
             for (int i = 1; i < ni - 1; ++i) {
                 if (!(fabs(oldx[i] - s->x[i]) > 1e-6)) continue;
                 size_t c = IX(s, i, j, k);
-                for (int a = 0; a < 12; ++a)
+                for (int a = 0; a < nf; ++a)
                     f[a][c] = lerp_ref(oldx[i], oldx[i + dirx], f[a][c], f[a][c + dirx], s->x[i]);
             }
     const long sy = (long)diry * ni;
@@ -213,7 +222,7 @@ static void move_mesh(orc *s) {
             for (int i = 1; i < ni - 1; ++i) {
                 if (!(fabs(oldy[j] - s->y[j]) > 1e-6)) continue;
                 size_t c = IX(s, i, j, k);
-                for (int a = 0; a < 12; ++a)
+                for (int a = 0; a < nf; ++a)
                     f[a][c] = lerp_ref(oldy[j], oldy[j + diry], f[a][c], f[a][(size_t)((long)c + sy)], s->y[j]);
             }
     free(oldx);
@@ -241,6 +250,11 @@ static void initialize(orc *s) {
         s->enthalpy[c] = hpre;
         s->hnot[c] = hpre;
         s->temp[c] = tp;
+        if (p->species) {   /* CFDSolverManager.cpp:474-479 */
+            s->conc[c] = p->initconcentration;
+            s->concnot[c] = p->initconcentration;
+            s->concl[c] = p->initconcentration;
+        }
         if (p->powderbed) {
             int k = (int)(c / ((size_t)ni * nj));
             s->csfrac[c] = (s->z[k] < ztop) ? 1.0 : 0.0;
@@ -453,6 +467,33 @@ static void bc_box_sides(orc *s, double *phi) {
         }
 }
 
+/* species boundary values, BoundCondManager.cpp:655-699: top from the
+ * surface flux where the surface is liquid, fixed bottom, then zero-gradient
+ * north/south and east/west over whole planes (corners take the last copy) */
+static void bc_species(orc *s) {
+    const amcfd_params *p = &s->p;
+    const int ni = s->ni, nj = s->nj, nk = s->nk;
+    double *c = s->conc;
+    const double dl = p->dens * p->massdiff;
+    for (int j = 1; j < s->njm1; ++j)
+        for (int i = 1; i < s->nim1; ++i) {
+            size_t t = IX(s, i, j, s->nkm1), b = IX(s, i, j, s->nkm2);
+            if (s->temp[t] > p->tliquid) c[t] = 2.0 * (13.0 - c[b]) / s->dzpbinv[s->nkm1] / dl + c[b];
+            else c[t] = c[b];
+            c[IX(s, i, j, 0)] = p->initconcentration;
+        }
+    for (int k = 0; k < nk; ++k)
+        for (int i = 0; i < ni; ++i) {
+            c[IX(s, i, 0, k)] = c[IX(s, i, 1, k)];
+            c[IX(s, i, s->njm1, k)] = c[IX(s, i, s->njm1 - 1, k)];
+        }
+    for (int k = 0; k < nk; ++k)
+        for (int j = 0; j < nj; ++j) {
+            c[IX(s, 0, j, k)] = c[IX(s, 1, j, k)];
+            c[IX(s, s->nim1, j, k)] = c[IX(s, s->nim1 - 1, j, k)];
+        }
+}
+
 static void apply_bcs(orc *s, int ivar) {
     switch (ivar) {
     case 1: bc_tangential(s, 1); bc_box_sides(s, s->uvel); break;
@@ -625,6 +666,124 @@ static void source_energy(orc *s) {
                 s->ap[c] /= p->urfh;
                 s->su[c] += (1.0 - p->urfh) * s->ap[c] * s->enthalpy[c];
             }
+}
+
+/* species coefficients, CFDSolverManager.cpp:1265-1367: the cell's own
+ * diffusivity on every face (no harmonic mean; 1e-11 in solid), convection
+ * with twice the face velocity; each cell writes its north/east/top
+ * coefficient and its neighbour's south/west/bottom one, the first row,
+ * column and plane get theirs from a separate loop */
+static void discretize_species(orc *s) {
+    const amcfd_params *p = &s->p;
+    const int ni = s->ni, nj = s->nj;
+    const size_t sj = (size_t)ni, sk = (size_t)ni * nj, n = (size_t)ni * nj * s->nk;
+    double *ds = (double *)malloc(sizeof(double) * (n ? n : 1));
+    for (size_t c = 0; c < n; ++c) ds[c] = (s->temp[c] <= p->tsolid) ? 1e-11 : p->massdiff * s->rho[c];
+    for (int k = 1; k < s->nkm1; ++k)
+        for (int j = 1; j < s->njm1; ++j)
+            for (int i = 1; i < s->nim1; ++i) {
+                size_t c = IX(s, i, j, k);
+                double aik = s->areaik[k * ni + i], ajk = s->areajk[k * nj + j], aij = s->areaij[j * ni + i];
+                double vn = s->vvel[c + sj] * 2.0, ue = s->uvel[c + 1] * 2.0, wt = s->wvel[c + sk] * 2.0;
+                double fn = s->rho[c + sj] * vn * aik, fe = s->rho[c + 1] * ue * ajk, ft = s->rho[c + sk] * wt * aij;
+                double dn = ds[c] * aik * s->dypsinv[j + 1], de = ds[c] * ajk * s->dxpwinv[i + 1];
+                double dt = ds[c] * aij * s->dzpbinv[k + 1];
+                double densbydt = s->rho[c] / p->delt;
+                s->an[c] = coef_neg(s, dn, fn);
+                s->as[c + sj] = coef_pos(s, dn, fn);
+                s->ae[c] = coef_neg(s, de, fe);
+                s->aw[c + 1] = coef_pos(s, de, fe);
+                s->at[c] = coef_neg(s, dt, ft);
+                s->ab[c + sk] = coef_pos(s, dt, ft);
+                s->apnot[c] = densbydt * s->volume[c];
+                s->sp[c] = 0.0;
+                s->su[c] = s->apnot[c] * s->concnot[c];
+            }
+    for (int k = 1; k < s->nkm1; ++k)
+        for (int i = 1; i < s->nim1; ++i) {
+            size_t c = IX(s, i, 1, k);
+            double aik = s->areaik[k * ni + i];
+            double vs = s->vvel[c] * 2.0, fs = s->rho[c] * vs * aik, dd = ds[c] * aik * s->dypsinv[1];
+            s->as[c] = coef_pos(s, dd, fs);
+        }
+    for (int k = 1; k < s->nkm1; ++k)
+        for (int j = 1; j < s->njm1; ++j) {
+            size_t c = IX(s, 1, j, k);
+            double ajk = s->areajk[k * nj + j];
+            double uw = s->uvel[c] * 2.0, fw = s->rho[c] * uw * ajk, dd = ds[c] * ajk * s->dxpwinv[1];
+            s->aw[c] = coef_pos(s, dd, fw);
+        }
+    for (int j = 1; j < s->njm1; ++j)
+        for (int i = 1; i < s->nim1; ++i) {
+            size_t c = IX(s, i, j, 1);
+            double aij = s->areaij[j * ni + i];
+            double wb = s->wvel[c] * 2.0, fb = s->rho[c] * wb * aij, dd = ds[c] * aij * s->dzpbinv[1];
+            s->ab[c] = coef_pos(s, dd, fb);
+        }
+    free(ds);
+}
+
+/* boundary folding, ap and relaxation, CFDSolverManager.cpp:1844-1924 */
+static void source_species(orc *s) {
+    const amcfd_params *p = &s->p;
+    const double *c = s->conc;
+    for (int j = 1; j < s->njm1; ++j)
+        for (int i = 1; i < s->nim1; ++i) {
+            size_t b = IX(s, i, j, 1), t = IX(s, i, j, s->nkm2);
+            s->su[b] += s->ab[b] * c[IX(s, i, j, 0)];
+            s->sp[b] -= s->ab[b];
+            s->ab[b] = 0.0;
+            s->su[t] += s->at[t] * c[IX(s, i, j, s->nkm1)];
+            s->sp[t] -= s->at[t];
+            s->at[t] = 0.0;
+        }
+    for (int k = 1; k < s->nkm1; ++k)
+        for (int i = 1; i < s->nim1; ++i) {
+            size_t a = IX(s, i, 1, k), b = IX(s, i, s->njm2, k);
+            s->su[a] += s->as[a] * c[IX(s, i, 0, k)];
+            s->sp[a] -= s->as[a];
+            s->as[a] = 0.0;
+            s->su[b] += s->an[b] * c[IX(s, i, s->njm1, k)];
+            s->sp[b] -= s->an[b];
+            s->an[b] = 0.0;
+        }
+    for (int k = 1; k < s->nkm1; ++k)
+        for (int j = 1; j < s->njm1; ++j) {
+            size_t a = IX(s, 1, j, k), b = IX(s, s->nim2, j, k);
+            s->su[a] += s->aw[a] * c[IX(s, 0, j, k)];
+            s->sp[a] -= s->aw[a];
+            s->aw[a] = 0.0;
+            s->su[b] += s->ae[b] * c[IX(s, s->nim1, j, k)];
+            s->sp[b] -= s->ae[b];
+            s->ae[b] = 0.0;
+        }
+    for (int k = 1; k < s->nkm1; ++k)
+        for (int j = 1; j < s->njm1; ++j)
+            for (int i = 1; i < s->nim1; ++i) {
+                size_t q = IX(s, i, j, k);
+                double ap = s->an[q] + s->as[q] + s->ae[q] + s->aw[q] + s->at[q] + s->ab[q] + s->apnot[q] - s->sp[q];
+                ap /= p->urfc;
+                s->ap[q] = ap;
+                s->su[q] += (1.0 - p->urfc) * ap * c[q];
+            }
+}
+
+/* L1 species residual, CFDSolverManager.cpp:2102-2130 */
+static void residual_species(orc *s) {
+    const int ni = s->ni, nj = s->nj;
+    const size_t sj = (size_t)ni, sk = (size_t)ni * nj;
+    double sumc = 0.0, sumd = 0.0;
+    const double *h = s->conc;
+    for (int k = 1; k < s->nkm1; ++k)
+        for (int j = 1; j < s->njm1; ++j)
+            for (int i = 1; i < s->nim1; ++i) {
+                size_t c = IX(s, i, j, k);
+                double r = (s->an[c] * h[c + sj] + s->as[c] * h[c - sj] + s->ae[c] * h[c + 1] + s->aw[c] * h[c - 1] +
+                            s->at[c] * h[c + sk] + s->ab[c] * h[c - sk] + s->su[c]) / s->ap[c] - h[c];
+                sumd += fabs(r);
+                sumc += fabs(h[c]);
+            }
+    s->resorc = sumd / (sumc + SMALL);
 }
 
 /* L1 enthalpy residual, CFDSolverManager.cpp:2073-2101 */
@@ -1343,6 +1502,193 @@ static void flow_iteration(orc *s) {
     }
 }
 
+/* species conservation after the flow, CFDSolverManager.cpp:677-686 */
+static void species_iteration(orc *s) {
+    bc_species(s);
+    discretize_species(s);
+    source_species(s);
+    residual_species(s);
+    block_correction(s, s->conc);
+    line_sweeps(s, s->conc, 1, s->nim2, 1, s->njm2, 1, s->nkm2, 0);
+}
+
+/* Temperature gradient G and growth rate R, CFDSolverManager.cpp:2763-2804:
+ * in every x-row (k < nkm1, all j) at the first node i with T >= Ts, written
+ * to node i-1 from the flat-index neighbours i-2 / i, +-one row, +-one plane
+ * of i-1 (so rows at the edges read into the adjacent row or plane, as the
+ * reference's flat arrays do).  Where the reference reads its coordinate or
+ * temperature arrays outside their bounds (first such node at i = 0, or in
+ * row j = 0 or plane k = 0) G and R are set to NaN. */
+static void solidification(orc *s) {
+    const amcfd_params *p = &s->p;
+    if (s->tpeak <= p->tsolid) return;
+    const long ni = s->ni, nj = s->nj;
+    const long sj = ni, sk = ni * nj;
+    for (int k = 0; k < s->nkm1; ++k)
+        for (int j = 0; j < nj; ++j)
+            for (int i = 0; i < ni; ++i) {
+                long c = ((long)k * nj + j) * ni + i;
+                if (!(s->temp[c] >= p->tsolid)) continue;
+                long o = c - 1;
+                if (i == 0 || j == 0 || k == 0) {
+                    if (o >= 0) {
+                        s->grad[o] = NAN;
+                        s->rate[o] = NAN;
+                    }
+                    break;
+                }
+                double gx = (s->temp[c] - s->temp[c - 2]) / (s->x[i] - s->x[i - 1]) / 2;
+                double gy = (s->temp[o + sj] - s->temp[o - sj]) / (s->y[j] - s->y[j - 1]) / 2;
+                double gz = (s->temp[o + sk] - s->temp[o - sk]) / (s->z[k] - s->z[k - 1]) / 2;
+                s->grad[o] = sqrt(gx * gx + gy * gy + gz * gz);
+                s->rate[o] = s->scanvel * gx / s->grad[o];
+                break;
+            }
+}
+
+/* ------------------------------------------------------------------------ */
+/* energy-based free surface, FreeSurfaceManager.cpp:72-342 (output times)   */
+/* ------------------------------------------------------------------------ */
+/* surface shift for one Lagrange multiplier: relaxed fixed-point iteration
+ * of the surface energy balance over the molten top cells, :251-341.  The
+ * largest relative change is kept across iterations (it is not reset), so
+ * once it is infinite (first iteration from zero) the loop runs to its
+ * limit. */
+static void fs_ebal(orc *s, double alambda) {
+    const amcfd_params *p = &s->p;
+    const int ni = s->ni, nj = s->nj, nkm1 = s->nkm1;
+    const int nitermax = (int)p->maxiterenergy;
+    const double c1 = p->dens * p->grav, gamma = p->surftens, urfree = p->urfree, tol = p->tolenergy;
+    const double *xu = s->xu, *yv = s->yv, *x = s->x, *y = s->y;
+    double *fi = s->fi, *zzm = s->zzm;
+    int niter = 0;
+    double difmax = 0.0;
+    for (int c = 0; c < ni * nj; ++c) zzm[c] = 0.0;
+    while (niter < nitermax) {
+        niter++;
+        double pmax = -INFINITY;   /* the OpenMP max-reduction's private copy */
+        for (int j = 1; j < s->njm1; ++j)
+            for (int i = 1; i < s->nim1; ++i) {
+                if (s->temp[IX(s, i, j, nkm1)] < p->tsolid) continue;
+                int c = j * ni + i;
+                double dely = yv[j + 1] - yv[j - 1], dely1 = y[j + 1] - y[j - 1];
+                double delyp = y[j + 1] - y[j], delym = y[j] - y[j - 1];
+                double delx = xu[i + 1] - xu[i - 1], delx1 = x[i + 1] - x[i - 1];
+                double delxp = x[i + 1] - x[i], delxm = x[i] - x[i - 1];
+                double c2 = s->apdis[c] + alambda;
+                double dzdx = (zzm[c + 1] - zzm[c - 1]) / delx1;
+                double dzdy = (zzm[c + ni] - zzm[c - ni]) / dely1;
+                double dzdyp1 = (zzm[c + ni + 1] - zzm[c - ni + 1]) / dely1;
+                double dzdym1 = (zzm[c + ni - 1] - zzm[c - ni - 1]) / dely1;
+                double d2zdxdy = (dzdyp1 - dzdym1) / delx1;
+                double c3 = pow(1.0 + dzdx * dzdx + dzdy * dzdy, 1.5);
+                double c4 = 1.0 + dzdy * dzdy, c5 = 1.0 + dzdx * dzdx;
+                double c6 = 2.0 * dzdx * dzdy * d2zdxdy;
+                double c7 = zzm[c + 1] / (delxp * delx) + zzm[c - 1] / (delxm * delx);
+                double c8 = zzm[c + ni] / (delyp * dely) + zzm[c - ni] / (delym * dely);
+                double c9 = 1.0 / (delxp * delx) + 1.0 / (delxm * delx);
+                double c10 = 1.0 / (delyp * dely) + 1.0 / (delym * dely);
+                double c11 = gamma / c3;
+                double term1 = c2 + c11 * (c4 * c7 + c5 * c8 - c6);
+                double term2 = c1 + c11 * (c4 * c9 + c5 * c10);
+                fi[c] = urfree * term1 / term2 + (1.0 - urfree) * zzm[c];
+                double diff1 = fabs((fi[c] - zzm[c]) / zzm[c]);
+                pmax = mx(diff1, pmax);
+            }
+        difmax = (pmax > difmax) ? pmax : difmax;
+        if (difmax < tol) break;
+        for (int c = 0; c < ni * nj; ++c) zzm[c] = fi[c];
+    }
+}
+
+/* volume change for one multiplier, :225-246 */
+static double fs_delta_vol(orc *s, double alambda) {
+    fs_ebal(s, alambda);
+    double delv = 0.0;
+    for (int j = 1; j < s->njm1; ++j)
+        for (int i = 1; i < s->nim1; ++i) delv += s->areaij[j * s->ni + i] * s->fi[j * s->ni + i];
+    return delv;
+}
+
+/* FreeSurfaceManager::execute, :72-220: bisection on the multiplier so that
+ * the surface displaces the added droplet volume, then heights and
+ * temperatures of every node column re-interpolated along z.  Returns 1 if
+ * the reference would print its bracket warning. */
+int orc_free_surface(orc *s) {
+    const amcfd_params *p = &s->p;
+    const int ni = s->ni, nj = s->nj, nk = s->nk, nkm1 = s->nkm1;
+    const int nitermax = (int)p->maxiterbisect;
+    double alam1 = p->initlam1, alam2 = p->initlam2, tol = p->tolbisect;
+    const double delt = p->delt, startdeltn = 0.0;
+    double melttime = delt * (1.0 + startdeltn);
+    for (int c = 0; c < ni * nj; ++c) s->apdis[c] = 0.0;
+    const double vdp = 2e-9;
+    double vd;
+    if (s->tpeak <= p->tsolid) {
+        vd = 0.0;
+        melttime = 0.0;
+    } else {
+        vd = vdp * (s->timet - melttime - startdeltn * delt);
+        if (s->powerindicator < 1) vd = vdp * (p->lasertime - melttime);
+    }
+    double f = fs_delta_vol(s, alam1) + vd;
+    double fmid = fs_delta_vol(s, alam2) + vd;
+    int warn = (f * fmid >= 0.0 && f != vd);
+    double alamopt, dx, xmid = 0.0;
+    if (f < 0.0) {
+        alamopt = alam1;
+        dx = alam2 - alam1;
+    } else {
+        alamopt = alam2;
+        dx = alam1 - alam2;
+    }
+    int niter;
+    for (niter = 0; niter < nitermax; niter++) {
+        dx *= 0.5;
+        xmid = alamopt + dx;
+        fmid = fs_delta_vol(s, xmid) + vd;
+        if (fmid < 0.0) alamopt = xmid;
+        if (fabs(dx) < 1.0e-5 || (fabs(fmid) < tol && niter != 0)) break;
+    }
+    for (int k = 0; k < nk; ++k)
+        for (int j = 0; j < nj; ++j)
+            for (int i = 0; i < ni; ++i) {
+                size_t c = IX(s, i, j, k);
+                double factor = s->fi[j * ni + i] / s->z[nkm1];
+                s->zr[c] = s->z[k] * (1.0 - factor);
+                if (s->zr[c] > s->z[nkm1]) continue;
+                if (s->zr[c] == s->z[nkm1]) s->tempr[c] = s->temp[IX(s, i, j, nkm1)];
+                for (int l = nkm1 - 1; l >= 0; l--)
+                    if (s->z[l] <= s->zr[c]) {
+                        s->tempr[c] = (s->zr[c] - s->z[l]) / (s->z[l + 1] - s->z[l]) *
+                                          (s->temp[IX(s, i, j, l + 1)] - s->temp[IX(s, i, j, l)]) +
+                                      s->temp[IX(s, i, j, l)];
+                        break;
+                    }
+            }
+    s->fs_xmid = xmid;
+    s->fs_fmid = fmid;
+    s->fs_vd = vd;
+    s->fs_niter = niter;
+    return warn;
+}
+
+void orc_free_surface_monitor(const orc *s, double *xmid, double *fmid, double *fimin, double *fimax,
+                              double *vd, int *niter) {
+    const int n = s->ni * s->nj;
+    double lo = s->fi[0], hi = s->fi[0];
+    for (int c = 1; c < n; ++c) {   /* std::min_element / max_element: first extremum */
+        if (s->fi[c] < lo) lo = s->fi[c];
+        if (hi < s->fi[c]) hi = s->fi[c];
+    }
+    *xmid = s->fs_xmid;
+    *fmid = s->fs_fmid;
+    *fimin = lo;
+    *fimax = hi;
+    *vd = s->fs_vd;
+    *niter = s->fs_niter;
+}
+
 int orc_solve(orc *s) {
     const amcfd_params *p = &s->p;
     double alenold = s->alen;
@@ -1352,6 +1698,7 @@ int orc_solve(orc *s) {
         s->itertot++;
         orc_energy_iteration(s);
         if (s->tpeak > p->tsolid && p->navierstokes) flow_iteration(s);
+        if (p->species) species_iteration(s);
         /* max over the flow residuals with std::max_element semantics */
         double res[5] = {s->resorm, s->resoru, s->resorv, s->resorw, s->resorc};
         double amaxres = res[0];
@@ -1374,6 +1721,7 @@ int orc_solve(orc *s) {
 /* end-of-step push-back, CFDSolverManager.cpp:2625-2687 */
 void orc_end_step(orc *s) {
     const amcfd_params *p = &s->p;
+    if (p->solidification) solidification(s);   /* main.cpp:151-152 */
     if (p->steady) return;
     double track = (double)s->trackindx;
     double trackid = 0.0;
@@ -1391,6 +1739,10 @@ void orc_end_step(orc *s) {
         s->wnot[c] = s->wvel[c];
         s->hnot[c] = s->enthalpy[c];
         s->fraclnot[c] = s->fracl[c];
+        if (p->species) {
+            s->concnot[c] = s->conc[c];
+            s->concl[c] = s->conc[c] / (s->fracl[c] + p->kp * (1.0 - s->fracl[c]));
+        }
         if (p->powderbed) {
             s->csfrac[c] = mx(s->csfrac[c], s->fracl[c]);
             s->solfrac[c] = mx(s->solfrac[c], s->fracl[c] * track);
@@ -1461,7 +1813,8 @@ orc *orc_create(const amcfd_params *p) {
                         &s->fracl, &s->fraclnot, &s->sourceinput, &s->csfrac, &s->rho, &s->solfrac,
                         &s->mpphase, &s->mpliqphase, &s->unot, &s->vnot, &s->wnot, &s->pressure,
                         &s->ap, &s->an, &s->as, &s->ae, &s->aw, &s->at, &s->ab, &s->apnot,
-                        &s->dux, &s->dvy, &s->dwz, &s->su, &s->sp};
+                        &s->dux, &s->dvy, &s->dwz, &s->su, &s->sp, &s->conc, &s->concnot, &s->concl,
+                        &s->grad, &s->rate};
     for (size_t a = 0; a < sizeof(three) / sizeof(three[0]); ++a) *three[a] = zalloc(nn);
     s->areaij = zalloc((size_t)s->nj * s->ni);
     s->areauij = zalloc((size_t)s->nj * s->ni);
@@ -1471,6 +1824,11 @@ orc *orc_create(const amcfd_params *p) {
     s->areawik = zalloc((size_t)s->nkf * s->ni);
     s->areauik = zalloc((size_t)s->nkf * s->ni);
     s->areajk = zalloc((size_t)s->nkf * s->nj);
+    s->fi = zalloc((size_t)s->nj * s->ni);
+    s->zzm = zalloc((size_t)s->nj * s->ni);
+    s->apdis = zalloc((size_t)s->nj * s->ni);
+    s->zr = zalloc(nn);
+    s->tempr = zalloc(nn);
     s->areavjk = zalloc((size_t)s->nkf * s->nj);
     s->areawjk = zalloc((size_t)s->nkf * s->nj);
     s->powerindicator = 1;
@@ -1479,6 +1837,11 @@ orc *orc_create(const amcfd_params *p) {
     s->scanvel = p->scanvel;
     generate_grid(s);
     initialize(s);
+    /* FreeSurfaceManager::initializeFreeSurface, :35-67 (always run) */
+    for (size_t c = 0; c < nn; ++c) {
+        s->zr[c] = s->z[c / ((size_t)s->ni * s->nj)];
+        s->tempr[c] = s->temp[c];
+    }
     set_active_nk(s, p->nk_active0);
     if (p->steady) update_laser(s);
     return s;
@@ -1493,7 +1856,8 @@ void orc_destroy(orc *s) {
                      s->vnot, s->wnot, s->pressure, s->ap, s->an, s->as, s->ae, s->aw, s->at, s->ab,
                      s->apnot, s->dux, s->dvy, s->dwz, s->su, s->sp, s->areaij, s->areauij, s->areavij,
                      s->heatin, s->areaik, s->areawik, s->areauik, s->areajk, s->areavjk, s->areawjk,
-                     s->tool, s->dtt, s->heatt};
+                     s->tool, s->dtt, s->heatt, s->conc, s->concnot, s->concl, s->grad, s->rate,
+                     s->fi, s->zzm, s->apdis, s->zr, s->tempr};
     for (size_t a = 0; a < sizeof(all) / sizeof(all[0]); ++a) free(all[a]);
     free(s);
 }
@@ -1512,7 +1876,9 @@ double *orc_field(orc *s, const char *n) {
         {"xu", s->xu}, {"yv", s->yv}, {"zw", s->zw}, {"volume", s->volume}, {"areaij", s->areaij},
         {"areajk", s->areajk}, {"areaik", s->areaik}, {"dxpwinv", s->dxpwinv},
         {"dypsinv", s->dypsinv}, {"dzpbinv", s->dzpbinv}, {"fracx", s->fracx}, {"fracy", s->fracy},
-        {"fracz", s->fracz}};
+        {"fracz", s->fracz}, {"avgconcentration", s->conc}, {"avgconcentrationnot", s->concnot},
+        {"concentration", s->concl}, {"grad", s->grad}, {"rate", s->rate}, {"zr", s->zr},
+        {"tempr", s->tempr}, {"fi", s->fi}};
     for (size_t a = 0; a < sizeof(t) / sizeof(t[0]); ++a)
         if (strcmp(t[a].name, n) == 0) return t[a].ptr;
     return NULL;
@@ -1552,6 +1918,9 @@ void orc_get_scalars(const orc *s, amcfd_scalars *o) {
     o->jstart = s->jstart;
     o->nk = s->nk;
     o->step = s->step;
+    o->delt = s->p.delt;
+    o->outtime = s->p.outtime;
+    o->resorc = s->resorc;
 }
 
 void orc_dims(const orc *s, int *ni, int *nj, int *nk) {

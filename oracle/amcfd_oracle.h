@@ -49,6 +49,11 @@ void  orc_thomas(int n, const double *aw, const double *ap, const double *ae,
                  const double *d, double *x);
 /* run only the energy half of one SIMPLE iteration (BCs .. melt pool) */
 void  orc_energy_iteration(orc *s);
+/* energy-based free surface at an output time (FreeSurfaceManager::execute);
+ * returns 1 where the reference prints its bracket warning */
+int   orc_free_surface(orc *s);
+void  orc_free_surface_monitor(const orc *s, double *xmid, double *fmid, double *fimin,
+                               double *fimax, double *vd, int *niter);
 
 #ifdef __cplusplus
 }

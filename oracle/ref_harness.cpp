@@ -103,7 +103,7 @@ int main(int argc, char **argv) {
     }
 
     FILE *sc = std::fopen((outdir + "/scalars.csv").c_str(), "w");
-    std::fprintf(sc, "step,timet,niter,itertot,resorh,resorm,resoru,resorv,resorw,tpeak,alen,depth,width,"
+    std::fprintf(sc, "step,timet,niter,itertot,resorh,resorm,resoru,resorv,resorw,resorc,tpeak,alen,depth,width,"
                      "ratio,heatinlaser,heatout,accul,heatvol,ahtoploss,beamposx,beamposy,powerindicator,"
                      "istat,iend,jstat,jend,kstat,jstart,istart,nk\n");
 
@@ -119,6 +119,16 @@ int main(int argc, char **argv) {
             char buf[512];
             std::snprintf(buf, sizeof buf, "%s/f%05d_%s.bin", outdir.c_str(), step, names[f]);
             write_raw(buf, ptr[f], nn);
+        }
+        // species and G/R fields, present only with their keywords
+        const char *xnames[] = {"avgconcentration", "avgconcentrationnot", "concentration", "grad", "rate"};
+        const double *xptr[] = {cfd->avgconcentration_, cfd->avgconcentrationnot_, cfd->concentration_,
+                                cfd->grad_, cfd->rate_};
+        for (int f = 0; f < 5; ++f) {
+            if (f < 3 ? !meshObj.species_ : !meshObj.solidificationparams_) continue;
+            char buf[512];
+            std::snprintf(buf, sizeof buf, "%s/f%05d_%s.bin", outdir.c_str(), step, xnames[f]);
+            write_raw(buf, xptr[f], nn);
         }
     };
     if (fields_every > 0) dump_fields(0);
@@ -141,13 +151,14 @@ int main(int argc, char **argv) {
         if (!dm->steady_) bc->updateLaser();
         if (meshObj.movingmesh_) cfd->generateGrid();   // main.cpp:144-145
         cfd->iterativeLoopSolve();
+        if (meshObj.solidificationparams_) cfd->getSolidificationParameters();   // main.cpp:151-152
         if (!dm->steady_) cfd->updateSolutions();
         ++step;
         std::fprintf(sc,
-                     "%d,%.17g,%d,%d,%.17g,%.17g,%.17g,%.17g,%.17g,%.17g,%.17g,%.17g,%.17g,%.17g,%.17g,"
+                     "%d,%.17g,%d,%d,%.17g,%.17g,%.17g,%.17g,%.17g,%.17g,%.17g,%.17g,%.17g,%.17g,%.17g,%.17g,"
                      "%.17g,%.17g,%.17g,%.17g,%.17g,%.17g,%d,%d,%d,%d,%d,%d,%d,%d,%d\n",
                      step, dm->timet_, cfd->niter_, cfd->itertot_, cfd->resorh_, cfd->resorm_,
-                     cfd->resoru_, cfd->resorv_, cfd->resorw_, bc->tpeak_, bc->alen_, bc->depth_,
+                     cfd->resoru_, cfd->resorv_, cfd->resorw_, cfd->resorc_, bc->tpeak_, bc->alen_, bc->depth_,
                      bc->width_, bc->ratio_, bc->heatinlaser_, bc->heatout_, bc->accul_, bc->heatvol_,
                      bc->ahtoploss_, bc->beamposx_, bc->beamposy_, bc->powerindicator_, bc->istat_,
                      bc->iend_, bc->jstat_, bc->jend_, bc->kstat_, bc->jstart_, bc->istart_, dm->nk_);

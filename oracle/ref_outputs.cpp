@@ -4,13 +4,15 @@
 //
 // It follows the reference executable's sequence (src/main.cpp:53-241): set
 // up through PrintManager::initialization, write the initial VTS file, then
-// per step dt / timet / updateLaser / iterativeLoopSolve / updateSolutions
-// and, every output interval, outputResiduals + visualOut; at the end a last
+// per step dt / timet / updateLaser / generateGrid (moving mesh) /
+// iterativeLoopSolve / getSolidificationParameters / outputSection /
+// updateSolutions and, every output interval, the free surface (execute),
+// outputResiduals + visualOut; at the end the free surface, a last section,
 // outputResiduals + visualOut and outputFinalGeometry.  The files it leaves in
 // the working directory (<dir>/<id>_<n>.vts, <id>.pvd, <id>_Residuals.txt,
-// the final-geometry file) are the reference's own output for the deck, for
-// byte comparison with the MI355X CLI.  Moving mesh, free surface, sections
-// and solidification parameters are not part of the driven path.
+// the final-geometry file, <id>_FreeSurfaceMonitor.txt, the extracted
+// sections) are the reference's own output for the deck, for comparison with
+// the MI355X CLI.
 //
 // usage: ref_outputs <deck.k>      (run with OMP_NUM_THREADS=1)
 
@@ -76,12 +78,16 @@ int main(int argc, char **argv) {
         domainMgr->timet_ += meshObj.delt_;
         outtrack += meshObj.delt_;
         if (!domainMgr->steady_) bcMgr->updateLaser();
+        if (meshObj.movingmesh_) cfdSolvMgr->generateGrid();
         cfdSolvMgr->iterativeLoopSolve();
+        if (meshObj.solidificationparams_) cfdSolvMgr->getSolidificationParameters();
+        if (meshObj.outputsection_) printMgr->outputSection();
         if (!domainMgr->steady_) {
             cfdSolvMgr->updateSolutions();
             if (outtrack >= meshObj.outtime_) {
                 outCt++;
                 outFile = folderName + meshObj.outFileName_ + "_" + std::to_string(outCt) + extName;
+                if (meshObj.energyfreesurface_) freesurMgr->execute();
                 printMgr->outputResiduals();
                 printMgr->visualOut();
                 outtrack = 0.0;
@@ -89,6 +95,8 @@ int main(int argc, char **argv) {
         }
     }
     domainMgr->finish_ = true;
+    if (meshObj.energyfreesurface_) freesurMgr->execute();
+    if (meshObj.outputsection_) printMgr->outputSection();
     outCt++;
     outFile = folderName + meshObj.outFileName_ + "_" + std::to_string(outCt) + extName;
     printMgr->outputResiduals();

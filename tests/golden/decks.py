@@ -26,6 +26,12 @@ The named cases are the workloads the solver is measured and checked on:
                 and y zones follow the beam (+x, +y hatch, -x).
 * ``tiny_move_powder`` the moving mesh over a two-layer powder bed (the
                 configuration of the reference's multi-layer examples).
+* ``tiny_species`` the tiny grid with species transport (*SOLVE_SPECIES) and
+                the G/R post-processing (*CALCULATE_SOLIDIFICATION_PARAMETERS).
+* ``tiny_move_species`` the moving mesh with species and G/R (the remap then
+                carries the concentration fields too).
+* ``tiny_fs``   the tiny_out outputs plus the energy-based free surface (at
+                every output) and an extracted section every 10 us.
 * ``plate120``  120x48x32 cells (2.0x0.6x0.4 mm), 400 steps, P=200 W,
                 r=50 um, 1 m/s: the CPU-runnable single-track case.
 * ``track400``  400x160x80 cells (4.0x1.2x0.6 mm, ~5.1M cells), P=300 W,
@@ -71,10 +77,15 @@ def write_deck(path, mesh_path, *, keyword="case", finaltime, maxit=100, tol=1e-
                tamb=300.0, tpre=300.0, htc=(10.0, 10.0, 10.0, 10.0), emiss=0.4,
                momentum=(5e-5, -2.6e-4, 1e7, -1.0), material=None, evaporation=None,
                volumetric=None, outtime=None, scalar_out=("temp",), final_geometry=None,
-               toolpath=None, powder=None, moving=None, extra=()):
+               toolpath=None, powder=None, moving=None, species=None, solidification=False,
+               free_surface=None, section=None, extra=()):
     """toolpath: rows of (t, x, y, z, laser_on, track) written to <deck>_tool.crs;
     powder: the 8 *POWDER_BED_MODE values; moving: the two 1-based zone
-    indices of *ACTIVATE_MOVING_MESH."""
+    indices of *ACTIVATE_MOVING_MESH; species: the 4 *SOLVE_SPECIES values
+    (initial concentration, mass diffusivity, partition coefficient,
+    relaxation); solidification: add *CALCULATE_SOLIDIFICATION_PARAMETERS;
+    free_surface: the 8 *TRACK_ENERGY_BASED_FREE_SURFACE values; section: the
+    7 *OUTPUT_SECTION values (box min, box max, interval)."""
     if material is None:
         material = (4000.0, 4000.0, 5e-3, TS, TL, HSMELT, HLFRIZ, ACPA, ACPB, ACPB,
                     0.0, 0.0, 25.0, 0.0, 25.0)
@@ -117,6 +128,17 @@ def write_deck(path, mesh_path, *, keyword="case", finaltime, maxit=100, tol=1e-
               "  " + " ".join(_fmt(v) for v in powder)]
     if moving is not None:
         L += ["*ACTIVATE_MOVING_MESH", "  %d %d" % tuple(moving)]
+    if species is not None:
+        L += ["*SOLVE_SPECIES", "$ initconc massdiff kp urfc", "  " + " ".join(_fmt(v) for v in species)]
+    if solidification:
+        L += ["*CALCULATE_SOLIDIFICATION_PARAMETERS"]
+    if free_surface is not None:
+        L += ["*TRACK_ENERGY_BASED_FREE_SURFACE",
+              "$ lambda1 lambda2 surftens maxiter_energy tol_energy maxiter_bisect tol_bisect urf",
+              "  " + " ".join(_fmt(v) for v in free_surface)]
+    if section is not None:
+        L += ["*OUTPUT_SECTION", "$ xmin ymin zmin xmax ymax zmax interval",
+              "  " + " ".join(_fmt(v) for v in section)]
     L += list(extra)
     L += ["*MESH_FILE", mesh_path]
     if toolpath is not None:
@@ -223,6 +245,30 @@ CASES = {
                   xstart=0.25e-3, ystart=0.15e-3, speed=1.0, power=200.0, radius=40e-6,
                   powder=(2, 2, 2.0e-5, 2400.0, 1.0e-6, 700.0, 0.0, 7.5),
                   toolpath=TOOL_MOVING_LAYERS, moving=(2, 2))),
+    "tiny_species": dict(
+        mesh=(([0.1e-3, 0.3e-3, 0.1e-3], [4, 12, 4], [-1.2, 1.0, 1.2]),
+              ([0.15e-3, 0.15e-3], [6, 6], [1.0, 1.0]),
+              ([0.15e-3, 0.1e-3], [4, 8], [1.0, 1.0])),
+        deck=dict(finaltime=0.15e-3 - 1e-9, maxit=60, dt=5e-6, lasertime=0.1e-3,
+                  xstart=0.15e-3, ystart=0.15e-3, speed=1.0, power=200.0, radius=50e-6,
+                  species=(1.0, 5e-9, 0.5, 0.7), solidification=True)),
+    "tiny_move_species": dict(
+        mesh=(([0.15e-3, 0.2e-3, 0.15e-3], [6, 10, 6], [1.0, 1.0, 1.0]),
+              ([0.1e-3, 0.1e-3, 0.1e-3], [4, 6, 4], [1.0, 1.0, 1.0]),
+              ([0.15e-3, 0.1e-3], [4, 8], [1.0, 1.0])),
+        deck=dict(finaltime=0.15e-3 - 1e-9, maxit=60, dt=5e-6, lasertime=0.15e-3,
+                  xstart=0.25e-3, ystart=0.15e-3, speed=1.0, power=200.0, radius=40e-6,
+                  toolpath=TOOL_MOVING, moving=(2, 2), species=(1.0, 5e-9, 0.5, 0.7),
+                  solidification=True)),
+    "tiny_fs": dict(
+        mesh=(([0.1e-3, 0.3e-3, 0.1e-3], [4, 12, 4], [-1.2, 1.0, 1.2]),
+              ([0.15e-3, 0.15e-3], [6, 6], [1.0, 1.0]),
+              ([0.15e-3, 0.1e-3], [4, 8], [1.0, 1.0])),
+        deck=dict(finaltime=0.1e-3 - 1e-9, maxit=100, dt=5e-6, lasertime=0.08e-3,
+                  xstart=0.15e-3, ystart=0.15e-3, speed=1.0, power=200.0, radius=50e-6,
+                  outtime=2.5e-5, scalar_out=("temp", "tempr", "zr", "fracl"),
+                  free_surface=(-1.0e6, 1.0e6, 1.8, 20, 1e-6, 30, 1e-16, 0.5),
+                  section=(0.1e-3, 0.0, 0.1e-3, 0.4e-3, 0.15e-3, 0.25e-3, 1e-5))),
     "plate120": dict(
         mesh=(([0.5e-3, 1.0e-3, 0.5e-3], [20, 80, 20], [-1.2, 1.0, 1.2]),
               ([0.2e-3, 0.2e-3, 0.2e-3], [16, 16, 16], [1.0, 1.0, 1.0]),

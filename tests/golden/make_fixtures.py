@@ -35,6 +35,8 @@ import decks  # noqa: E402
 HARNESS = os.path.join(ROOT, "oracle", "_ref", "ref_harness")
 FIELDS = ["enthalpy", "temp", "fracl", "uvel", "vvel", "wvel", "pressure", "pp", "rho", "diff",
           "vis", "hnot", "fraclnot", "unot", "vnot", "wnot", "sourceinput", "dux", "dvy", "dwz"]
+# written by the harness only when the deck enables species / G/R
+OPTIONAL_FIELDS = ["avgconcentration", "avgconcentrationnot", "concentration", "grad", "rate"]
 # case -> (fields_every, max_steps, steps whose full fields are kept)
 PLAN = {
     "tiny": (1, -1, [2, 40]),
@@ -46,6 +48,8 @@ PLAN = {
     "tiny_powder": (1, -1, []),
     "tiny_move": (1, -1, []),
     "tiny_move_powder": (1, -1, []),
+    "tiny_species": (1, -1, []),
+    "tiny_move_species": (1, -1, []),
     "plate120": (50, -1, []),
 }
 
@@ -77,7 +81,8 @@ def run_case(case, workdir=None, reuse=False):
     kept = {}
     for s in steps:
         dig["steps"][str(s)] = {}
-        for fld in FIELDS:
+        for fld in FIELDS + [f for f in OPTIONAL_FIELDS
+                             if os.path.exists(os.path.join(tmp, "f%05d_%s.bin" % (s, f)))]:
             a = np.fromfile(os.path.join(tmp, "f%05d_%s.bin" % (s, fld)), dtype="<f8").reshape(shape)
             dig["steps"][str(s)][fld] = {"sha256": digest(a), "l2": float(np.linalg.norm(a))}
             if s in keep:

@@ -69,11 +69,16 @@ def test_deck_errors(amcfd, tmp_path, case_dir):
         amcfd.Deck(missing)
     assert e.value.code == -2
     # subsystems outside the per-step solve are refused, not ignored
-    for kw in ("*SOLVE_SPECIES\n1 1 1 1",):
+    for kw in ("*TRACK_ENERGY_BASED_FREE_SURFACE\n1 1 1 1 1 1 1 1",):
         t = _write(tmp_path, "unsup.k", good.replace("*MESH_FILE", kw + "\n*MESH_FILE"))
         with pytest.raises(amcfd.AmcfdError) as e:
             amcfd.Deck(t)
         assert e.value.code == -4
+    # *SOLVE_SPECIES needs its four values (the reference indexes past them)
+    t = _write(tmp_path, "sp.k", good.replace("*MESH_FILE", "*SOLVE_SPECIES\n1.0 5e-9 0.5\n*MESH_FILE"))
+    with pytest.raises(amcfd.AmcfdError) as e:
+        amcfd.Deck(t)
+    assert e.value.code == -3
     # a moving zone needs a zone on either side in x and y (the reference
     # writes past its zone arrays otherwise); this grid has two y zones
     for kw in ("*ACTIVATE_MOVING_MESH\n2 2", "*ACTIVATE_MOVING_MESH\n2"):
@@ -81,6 +86,19 @@ def test_deck_errors(amcfd, tmp_path, case_dir):
         with pytest.raises(amcfd.AmcfdError) as e:
             amcfd.Deck(t)
         assert e.value.code == -3
+
+
+def test_species_and_solidification_keywords(amcfd, case_dir, tmp_path):
+    p = amcfd.Deck(case_dir("tiny_species")).params
+    assert (p.species, p.solidification) == (1, 1)
+    assert (p.initconcentration, p.massdiff, p.kp, p.urfc) == (1.0, 5e-9, 0.5, 0.7)
+    # the solidification flag line, like *FLUID_FLOW_OFF, leaves the open
+    # block open: values after it still belong to the block (Mesh.cpp:53-62)
+    good = open(case_dir("tiny")).read()
+    t = good.replace("  0.8 0.8 0.8 0.8 0.8", "  0.7 0.6\n*CALCULATE_SOLIDIFICATION_PARAMETERS\n  0.5 0.4 0.3")
+    q = amcfd.Deck(_write(tmp_path, "s.k", t)).params
+    assert q.solidification == 1 and (q.urfu, q.urfv, q.urfw, q.urfp, q.urfh) == (0.7, 0.6, 0.5, 0.4, 0.3)
+    assert amcfd.Deck(case_dir("tiny")).params.species == 0
 
 
 def test_moving_mesh_keyword(amcfd, case_dir):

@@ -12,8 +12,8 @@ import pytest
 from conftest import golden_path
 
 CASES = ["tiny", "tiny_cond", "tiny_evap", "tiny_vap", "tiny_vol", "tiny_tool", "tiny_powder",
-         "tiny_move", "tiny_move_powder"]
-MOVING = ["tiny_move", "tiny_move_powder"]
+         "tiny_move", "tiny_move_powder", "tiny_species", "tiny_move_species"]
+MOVING = ["tiny_move", "tiny_move_powder", "tiny_move_species"]
 INT_KEYS = ["niter", "itertot", "powerindicator", "istat", "iend", "jstat", "jend", "kstat", "jstart",
             "istart", "nk"]
 REAL_KEYS = ["timet", "resorh", "resorm", "resoru", "resorv", "resorw", "tpeak", "alen", "depth",
@@ -60,6 +60,8 @@ def test_oracle_matches_reference_bitwise(case, case_dir, amcfd, oracle_mod):
             assert int(row[key]) == s[key], (case, step, key)
         for key in REAL_KEYS:
             assert _same(float(row[key]), s[key]), (case, step, key, row[key], s[key])
+        if "resorc" in row:
+            assert _same(float(row["resorc"]), s["resorc"]), (case, step, "resorc", row["resorc"], s["resorc"])
         for fld, rec in dig["steps"].get(str(step), {}).items():
             a = orc.field(fld)
             h = hashlib.sha256(np.ascontiguousarray(a, dtype="<f8").tobytes()).hexdigest()
