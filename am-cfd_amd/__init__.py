@@ -37,7 +37,8 @@ EXPORTS = ("amcfd_deck_load", "amcfd_deck_params", "amcfd_deck_free", "amcfd_cre
            "amcfd_step", "amcfd_get_field", "amcfd_set_field", "amcfd_get_geometry",
            "amcfd_get_scalars", "amcfd_dims", "amcfd_last_timing", "amcfd_set_profiling",
            "amcfd_kernel_stats", "amcfd_write_vts", "amcfd_write_pvd", "amcfd_append_residuals",
-           "amcfd_append_final_geometry", "amcfd_deck_string", "amcfd_last_error", "amcfd_version")
+           "amcfd_append_final_geometry", "amcfd_deck_string", "amcfd_free_surface",
+           "amcfd_output_section", "amcfd_run_batch", "amcfd_last_error", "amcfd_version")
 
 
 class Params(ctypes.Structure):
@@ -78,7 +79,9 @@ class Params(ctypes.Structure):
         (n, ctypes.c_double) for n in ("initconcentration", "massdiff", "kp", "urfc")] + [
         ("solidification", ctypes.c_int), ("energyfreesurface", ctypes.c_int)] + [
         (n, ctypes.c_double) for n in ("initlam1", "initlam2", "surftens", "maxiterenergy", "tolenergy",
-                                       "maxiterbisect", "tolbisect", "urfree")]
+                                       "maxiterbisect", "tolbisect", "urfree")] + [
+        ("outputsection", ctypes.c_int)] + [
+        (n, ctypes.c_double) for n in ("gxmin", "gymin", "gzmin", "gxmax", "gymax", "gzmax", "toutsect")]
 
 
 class Scalars(ctypes.Structure):
@@ -138,6 +141,9 @@ def load_library(path=LIB_PATH):
         "amcfd_append_residuals": (ip, [vp, cp]),
         "amcfd_append_final_geometry": (ip, [vp, cp]),
         "amcfd_deck_string": (cp, [vp, cp, ip]),
+        "amcfd_free_surface": (ip, [vp, cp]),
+        "amcfd_output_section": (ip, [vp, cp, ip, ctypes.POINTER(ip)]),
+        "amcfd_run_batch": (ip, [ctypes.POINTER(vp), ip, ip, ctypes.POINTER(ip)]),
         "amcfd_last_error": (cp, []),
         "amcfd_version": (cp, []),
     }
@@ -227,6 +233,16 @@ class Solver:
                 break
         return n
 
+    def free_surface(self, monitor_path=None):
+        """Energy-based free surface at an output time (updates "zr", "tempr")."""
+        _check(load_library().amcfd_free_surface(self._h, monitor_path.encode() if monitor_path else None))
+
+    def output_section(self, base, finish=False):
+        """Extracted-section cadence step; True if a section was written."""
+        w = ctypes.c_int(0)
+        _check(load_library().amcfd_output_section(self._h, base.encode(), int(finish), ctypes.byref(w)))
+        return bool(w.value)
+
     # -- data access ------------------------------------------------------------
     def field(self, name):
         out = np.empty(self.shape, dtype=np.float64)
@@ -278,6 +294,16 @@ class Solver:
             self.close()
         except Exception:
             pass
+
+
+def run_batch(solvers, max_steps=-1):
+    """Advance independent solvers concurrently on the one GPU (each on its
+    own host thread and stream); returns the steps each took."""
+    n = len(solvers)
+    handles = (ctypes.c_void_p * max(n, 1))(*[s._h.value for s in solvers])
+    done = (ctypes.c_int * max(n, 1))()
+    _check(load_library().amcfd_run_batch(handles, n, int(max_steps), done))
+    return [done[i] for i in range(n)]
 
 
 def version():

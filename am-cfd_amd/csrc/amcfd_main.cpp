@@ -7,8 +7,11 @@
 // a residual block in <id>_Residuals.txt and the next VTS file, and at the end
 // one more of each plus the final melt-pool geometry when the deck asks for
 // it; <id>.pvd is rewritten after every VTS file.  <dir> is the deck's
-// *DIRECTORY (default "Results"), <id> its *KEYWORD_ID.  Also prints one JSON
-// line per output and a summary line.
+// *DIRECTORY (default "Results"), <id> its *KEYWORD_ID.  With
+// *TRACK_ENERGY_BASED_FREE_SURFACE the free surface is evaluated before each
+// of those outputs (monitor in <id>_FreeSurfaceMonitor.txt); with
+// *OUTPUT_SECTION every step may append an extracted section, and the end
+// writes one more.  Also prints one JSON line per output and a summary line.
 
 #include <sys/stat.h>
 
@@ -53,6 +56,7 @@ int main(int argc, char **argv) {
     const char *geom_c = amcfd_deck_string(deck, "final_geometry", 0);
     const std::string dir = dir_c ? dir_c : "Results", id = id_c ? id_c : "";
     const std::string folder = dir + "/", pvd = id + ".pvd", resid = id + "_Residuals.txt";
+    const std::string monitor = id + "_FreeSurfaceMonitor.txt";
     std::vector<const char *> names;
     for (int q = 0; amcfd_deck_string(deck, "scalar_out", q); ++q) names.push_back(amcfd_deck_string(deck, "scalar_out", q));
     if (amcfd_deck_string(deck, "vector_out", 0))
@@ -96,13 +100,24 @@ int main(int argc, char **argv) {
             break;
         }
         outtrack += sc.delt;
+        int written = 0;
+        if (p->outputsection && amcfd_output_section(s, id.c_str(), 0, &written)) {   // main.cpp:155-156
+            rc = fail("section");
+            break;
+        }
         if (!p->steady && outtrack >= sc.outtime) {
             ++outct;
+            if (p->energyfreesurface && amcfd_free_surface(s, monitor.c_str())) { rc = fail("free surface"); break; }
             if (amcfd_append_residuals(s, resid.c_str())) { rc = fail("residuals"); break; }
             if ((rc = visual_out())) break;
             outtrack = 0.0;
             report();
         }
+    }
+    if (!rc) {   // main.cpp:204-225
+        int written = 0;
+        if (p->energyfreesurface && amcfd_free_surface(s, monitor.c_str())) rc = fail("free surface");
+        if (!rc && p->outputsection && amcfd_output_section(s, id.c_str(), 1, &written)) rc = fail("section");
     }
     if (!rc) {
         ++outct;

@@ -5,6 +5,8 @@
 #include <cstring>
 #include <exception>
 #include <new>
+#include <thread>
+#include <vector>
 
 namespace amcfd {
 
@@ -111,6 +113,46 @@ int amcfd_step(amcfd_solver *s, int *finished) {
     return rs;
 }
 
+int amcfd_run_batch(amcfd_solver *const *s, int n, int max_steps, int *steps_done) {
+    if (n < 0 || (n > 0 && (!s || !steps_done))) return amcfd::set_error(AMCFD_E_ARG, "null argument");
+    for (int i = 0; i < n; ++i)
+        if (!s[i]) return amcfd::set_error(AMCFD_E_ARG, "null solver in batch");
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return amcfd::set_error(AMCFD_E_HIP, "hipGetDevice failed");
+    std::vector<int> rc(n, AMCFD_OK);
+    std::vector<std::string> msg(n);
+    auto run = [&](int i) {
+        // the calling thread's device, for the allocations a plan change makes
+        if (hipSetDevice(dev) != hipSuccess) {
+            rc[i] = AMCFD_E_HIP;
+            msg[i] = "hipSetDevice failed";
+            return;
+        }
+        int k = 0, fin = 0;
+        while ((max_steps < 0 || k < max_steps) && !s[i]->s.finished()) {
+            const int st = amcfd_step(s[i], &fin);
+            if (st < 0) {
+                rc[i] = st;
+                msg[i] = amcfd_last_error();
+                break;
+            }
+            ++k;
+        }
+        steps_done[i] = k;
+    };
+    try {
+        std::vector<std::thread> pool;
+        pool.reserve(n);
+        for (int i = 0; i < n; ++i) pool.emplace_back(run, i);
+        for (auto &t : pool) t.join();
+    } catch (const std::exception &e) {
+        return amcfd::set_error(AMCFD_E_ARG, std::string("cannot start batch threads: ") + e.what());
+    }
+    for (int i = 0; i < n; ++i)
+        if (rc[i] < 0) return amcfd::set_error(rc[i], "batch instance " + std::to_string(i) + ": " + msg[i]);
+    return AMCFD_OK;
+}
+
 int amcfd_get_field(amcfd_solver *s, const char *name, double *host, size_t n) {
     if (!s || !name || !host) return amcfd::set_error(AMCFD_E_ARG, "null argument");
     GUARD(s->s.get_field(name, host, n));
@@ -176,6 +218,16 @@ int amcfd_append_residuals(amcfd_solver *s, const char *path) {
 int amcfd_append_final_geometry(amcfd_solver *s, const char *path) {
     if (!s || !path) return amcfd::set_error(AMCFD_E_ARG, "null argument");
     GUARD(amcfd::append_final_geometry(s->s, path));
+}
+
+int amcfd_free_surface(amcfd_solver *s, const char *monitor_path) {
+    if (!s) return amcfd::set_error(AMCFD_E_ARG, "null argument");
+    GUARD(s->s.free_surface(monitor_path));
+}
+
+int amcfd_output_section(amcfd_solver *s, const char *base, int finish, int *written) {
+    if (!s || !base || !written) return amcfd::set_error(AMCFD_E_ARG, "null argument");
+    GUARD(s->s.output_section(base, finish, written));
 }
 
 const char *amcfd_deck_string(const amcfd_deck *deck, const char *key, int index) {

@@ -10,10 +10,8 @@
 //
 // Where the reference crashes or reads out of bounds (blank line, missing
 // values, short toolpath rows) this reader returns AMCFD_E_PARSE instead.
-// Keywords whose subsystems are outside the per-step solve and would change
-// its results (the energy-based free surface) are refused
-// with AMCFD_E_UNSUPPORTED; output keywords are kept for the output writers
-// (output.cpp), other output-only keywords are accepted and ignored.
+// Output keywords are kept for the output writers (output.cpp); other
+// output-only keywords (*OUTPUT_TEC, *CELL_OUT) are accepted and ignored.
 
 #include "internal.h"
 
@@ -91,7 +89,8 @@ int Deck::load(const std::string &path) {
         {"*OUTPUT_FINAL_GEOMETRY", B_GEOM}, {"*OUTPUT_SECTION", B_SECTION},
         {"*VOLUMETRIC_HEAT_SOURCE_TABLE", B_HEATTABLE}, {"*CONTROL_TIMESTEP_VARIABLE", B_DTVAR}};
 
-    std::vector<double> track, mat, relax, temps, energy, gauss, momentum, evap, powder, volheat, species;
+    std::vector<double> track, mat, relax, temps, energy, gauss, momentum, evap, powder, volheat, species, freesurf,
+        section;
     std::map<std::string, double> params;
     bool have_mesh = false, have_dt = false;
     int block = B_NONE, lineno = 0;
@@ -117,8 +116,8 @@ int Deck::load(const std::string &path) {
             case B_HEATTABLE: flags.volheatsource = 1; flags.volheatsourcetable = 1; break;
             case B_SPECIES: flags.species = 1; break;
             case B_MOVING: flags.movingmesh = 1; break;
-            case B_FREESURF:
-                return set_error(AMCFD_E_UNSUPPORTED, "*TRACK_ENERGY_BASED_FREE_SURFACE is not supported");
+            case B_FREESURF: flags.energyfreesurface = 1; break;
+            case B_SECTION: flags.outputsection = 1; break;
             default: break;
             }
             continue;
@@ -139,6 +138,8 @@ int Deck::load(const std::string &path) {
         case B_POWDER: v = numbers(line); powder.insert(powder.end(), v.begin(), v.end()); break;
         case B_VOLHEAT: v = numbers(line); volheat.insert(volheat.end(), v.begin(), v.end()); break;
         case B_SPECIES: v = numbers(line); species.insert(species.end(), v.begin(), v.end()); break;
+        case B_FREESURF: v = numbers(line); freesurf.insert(freesurf.end(), v.begin(), v.end()); break;
+        case B_SECTION: v = numbers(line); section.insert(section.end(), v.begin(), v.end()); break;
         case B_MOVING:   // each line sets both 1-based zone indices, Mesh.cpp:364-369
             v = numbers(line);
             if (v.size() < 2)
@@ -248,6 +249,16 @@ int Deck::load(const std::string &path) {
     if (flags.isevap) {
         if ((st = need(evap, 4, "*BOUNDARY_EVAPORATION"))) return st;
         p.evaplatent = evap[0]; p.evaptemp = evap[1]; p.evapfact = evap[2]; p.molarmass = evap[3];
+    }
+    if (flags.energyfreesurface) {   // DomainManager.cpp:133-144
+        if ((st = need(freesurf, 8, "*TRACK_ENERGY_BASED_FREE_SURFACE"))) return st;
+        p.initlam1 = freesurf[0]; p.initlam2 = freesurf[1]; p.surftens = freesurf[2]; p.maxiterenergy = freesurf[3];
+        p.tolenergy = freesurf[4]; p.maxiterbisect = freesurf[5]; p.tolbisect = freesurf[6]; p.urfree = freesurf[7];
+    }
+    if (flags.outputsection) {   // DomainManager.cpp:189-198
+        if ((st = need(section, 7, "*OUTPUT_SECTION"))) return st;
+        p.gxmin = section[0]; p.gymin = section[1]; p.gzmin = section[2];
+        p.gxmax = section[3]; p.gymax = section[4]; p.gzmax = section[5]; p.toutsect = section[6];
     }
     if (flags.species) {   // DomainManager.cpp:123-130
         if ((st = need(species, 4, "*SOLVE_SPECIES"))) return st;
@@ -372,6 +383,8 @@ void Deck::finalize_tables() {
     p.udtoolpath = flags.udtoolpath;
     p.movingmesh = flags.movingmesh;
     p.species = flags.species;
+    p.energyfreesurface = flags.energyfreesurface;
+    p.outputsection = flags.outputsection;
     p.solidification = flags.solidification;
     p.gaussheatsource = flags.gaussheatsource;
     p.volheatsource = flags.volheatsource;

@@ -87,6 +87,10 @@ class Solver {
     double alen_ = 0.0;
     bool steadylength_ = false;
     bool layer_overflow_ = false;            // a powder layer beyond the grid was requested
+    std::vector<double> hg0_z_;              // z at set-up (the free surface's initial heights)
+    double *fs_out_ = nullptr;               // free-surface search results (device, 6 values)
+    double tracksect_ = 0.0;                 // time since the last extracted section
+    bool section_sci_ = false;               // the section stream has switched to scientific
     double *xold_ = nullptr, *yold_ = nullptr;   // node coordinates before a mesh move
     Box box_{};
 
@@ -96,6 +100,16 @@ class Solver {
     void laser_position();
     int launch_laser_map();
     int move_mesh();
+  public:
+    // energy-based free surface at an output time (monitor appended if path)
+    int free_surface(const char *monitor_path);
+    // PrintManager::outputSection: accumulate dt, write when due or at the end
+    int output_section(const char *base, int finish, int *written);
+    struct FsMonitor {
+        double xmid, fmid, fimin, fimax, vd;
+        int niter, warn;
+    } fs_monitor_{};
+  private:
     int upload_geometry();
     SourceParams source_params();
     int sweep(double *phi, SweepPlan &plan, int ibeg, int iend, int jbeg, int jend, int kbeg, int kend, int guard,
@@ -114,5 +128,7 @@ int write_vts(Solver &s, const char *path, const char *const *names, int nnames)
 int write_pvd(const char *path, int n, const double *times, const char *const *files);
 int append_residuals(Solver &s, const char *path);
 int append_final_geometry(Solver &s, const char *path);
+int append_free_surface_monitor(Solver &s, const char *path);
+int write_section(Solver &s, const char *base, bool &scientific_state);
 
 }  // namespace amcfd

@@ -18,7 +18,7 @@ int set_error(int code, const std::string &msg);
 // ---------------------------------------------------------------------------
 struct Flags {
     int navierstokes = 1, steady = 0, isevap = 0, powderbed = 0, udtoolpath = 0, movingmesh = 0;
-    int species = 0, solidification = 0;
+    int species = 0, solidification = 0, energyfreesurface = 0, outputsection = 0;
     int gaussheatsource = 0, volheatsource = 0, volheatsourcetable = 0;
 };
 
@@ -68,6 +68,9 @@ struct DevFields {
     double *heatin;   // [nj*ni]
     // species (with *SOLVE_SPECIES) and G/R (with *CALCULATE_SOLIDIFICATION_PARAMETERS), else null
     double *conc, *concnot, *concl, *grad, *rate;
+    // energy-based free surface (with *TRACK_ENERGY_BASED_FREE_SURFACE), else null:
+    // surface shift and its previous iterate [nj*ni], node heights and temperatures
+    double *fi, *zzm, *zr, *tempr;
 };
 
 // dimensions and constants every kernel needs (passed by value)
@@ -161,6 +164,12 @@ void launch_species_assemble(hipStream_t st, const Dims &d, const DevGeom &g, De
 void launch_species_end(hipStream_t st, const Dims &d, DevFields f, const amcfd_params &p);
 void launch_solidification(hipStream_t st, const Dims &d, const DevGeom &g, DevFields f, double tsolid,
                            double scanvel);
+
+// energy-based free surface at an output time (surface.hip); vd = added
+// droplet volume; out[6] receives multiplier, residual, min/max shift,
+// bisection count and the bracket-warning flag
+void launch_free_surface(hipStream_t st, const Dims &d, const DevGeom &g, DevFields f, const amcfd_params &p,
+                         double vd, double *out);
 
 // moving mesh: remap the twelve (fourteen with species) solution fields from the node coordinates
 // xold/yold onto g.x/g.y (x pass, then y pass), dirx/diry = sign of the move

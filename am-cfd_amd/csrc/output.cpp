@@ -33,8 +33,7 @@ namespace {
 
 // the names the reference's solution registry holds for this run
 // (CFDSolverManager.cpp:528-560, PrintManager.cpp:283-285,
-// FreeSurfaceManager.cpp:64-65 register tempr/zr, whose writer is not part
-// of this build)
+// FreeSurfaceManager.cpp:64-65)
 bool scalar_output_known(const std::string &n, const amcfd_params &p) {
     static const char *base[] = {"pp", "vis", "rho", "uvel", "vvel", "wvel", "temp", "diff", "hnot", "fracl",
                                  "fraclnot", "pressure", "enthalpy", "sourceinput", "auvel", "avvel", "awvel"};
@@ -43,6 +42,7 @@ bool scalar_output_known(const std::string &n, const amcfd_params &p) {
     if (p.powderbed && (n == "csfrac" || n == "solfrac" || n == "mpPhase" || n == "mpLiqPhase")) return true;
     if (p.species && (n == "avgconcentration" || n == "concentration")) return true;
     if (p.solidification && (n == "rate" || n == "grad")) return true;
+    if (n == "tempr" || n == "zr") return true;   // FreeSurfaceManager.cpp:64-65, registered for every deck
     return false;
 }
 
@@ -131,6 +131,9 @@ int write_vts(Solver &s, const char *path, const char *const *names, int nnames)
     f << "<AppendedData encoding=\"raw\">_";
 
     const HostGrid &g = s.grid();
+    std::vector<double> buf(nn);
+    // with the free surface the point heights are the shifted ones (vtsBinWriter.cpp:163-170)
+    if (p.energyfreesurface && (st = s.get_field("zr", buf.data(), nn))) return st;
     int bytes = (int)(sizeof(double) * npoints * 3);
     f.write((const char *)&bytes, sizeof(int));
     for (int k = 0; k < gridz - 1; ++k)
@@ -138,9 +141,9 @@ int write_vts(Solver &s, const char *path, const char *const *names, int nnames)
             for (int i = 0; i < ni; ++i) {
                 f.write((const char *)&g.x[i], sizeof(double));
                 f.write((const char *)&g.y[j], sizeof(double));
-                f.write((const char *)&g.z[k], sizeof(double));
+                const double *zp = p.energyfreesurface ? &buf[((size_t)k * nj + j) * ni + i] : &g.z[k];
+                f.write((const char *)zp, sizeof(double));
             }
-    std::vector<double> buf(nn);
     for (const std::string &n : out) {
         const double *src;
         if (n == "auvel") src = au.data();
@@ -299,6 +302,89 @@ int append_final_geometry(Solver &s, const char *path) {
     f << "===================================================================\n";
     f.close();
     if (!f) return set_error(AMCFD_E_IO, std::string("write failed: ") + path);
+    return AMCFD_OK;
+}
+
+// the free-surface monitor block, FreeSurfaceManager.cpp:197-206
+int append_free_surface_monitor(Solver &s, const char *path) {
+    amcfd_scalars sc;
+    int st = s.get_scalars(&sc);
+    if (st) return st;
+    const Solver::FsMonitor &m = s.fs_monitor_;
+    std::ofstream f(path, std::ios::out | std::ios::app);
+    if (!f) return set_error(AMCFD_E_IO, std::string("cannot write ") + path);
+    using std::scientific;
+    using std::setprecision;
+    using std::setw;
+    const char *rule = "====================================================================================================\n";
+    f << rule;
+    f << "  time         lambda        res_bisect    min_fi         max_fi        droplet_volume   iter_bisect\n";
+    f << setprecision(4) << scientific << setw(12) << sc.timet << setw(14) << m.xmid << setw(14) << m.fmid
+      << setw(14) << m.fimin << setw(14) << m.fimax << setw(14) << m.vd << setw(9) << m.niter << "\n";
+    f << rule;
+    f.close();
+    if (!f) return set_error(AMCFD_E_IO, std::string("write failed: ") + path);
+    return AMCFD_OK;
+}
+
+// C-style conversion of a double to int as x86-64 performs it (cvttsd2si):
+// out-of-range and NaN values give INT_MIN
+static int x86_int(double v) {
+    if (!(v > -2147483649.0 && v < 2147483648.0)) return std::numeric_limits<int>::min();
+    return (int)v;
+}
+
+// PrintManager::outputSection, PrintManager.cpp:538-624: temperatures in a
+// box of nodes, appended to <base>_ExtractedSection<layer>.txt.  The layer
+// number divides by the powder layer height, which is zero without a powder
+// bed (the reference's file is then ...Section-2147483648.txt).  The stream
+// keeps its scientific format from the first rows on, so the time of every
+// block after the first one with rows is printed in scientific notation.
+int write_section(Solver &s, const char *base, bool &sci) {
+    int ni, nj, nk;
+    s.dims(&ni, &nj, &nk);
+    const int gridz = nk;
+    const amcfd_params &p = s.params();
+    const HostGrid &g = s.grid();
+    amcfd_scalars sc;
+    int st = s.get_scalars(&sc);
+    if (st) return st;
+    const int nkm1 = s.active_nk() - 1;
+    const size_t nn = (size_t)ni * nj * nk;
+    std::vector<double> t(nn);
+    if ((st = s.get_field("temp", t.data(), nn))) return st;
+    auto inx = [&](int i) { return g.x[i] >= p.gxmin && g.x[i] <= p.gxmax; };
+    auto iny = [&](int j) { return g.y[j] >= p.gymin && g.y[j] <= p.gymax; };
+    auto inz = [&](int k) { return g.z[k] >= p.gzmin && g.z[k] <= p.gzmax; };
+    int outx = 0, outy = 0, outz = 0;
+    for (int i = 0; i < ni; ++i) outx += inx(i);
+    for (int j = 0; j < nj; ++j) outy += iny(j);
+    for (int k = 0; k < gridz - 1; ++k) outz += inz(k);
+    // baseheightk is set for powder beds only (PrintManager.cpp:72); zero otherwise
+    const int base_k = p.powderbed ? p.nk_active0 - 2 - p.ncvzlayer : 0;
+    const double tol = 1e-9;
+    const int layer = x86_int((g.z[nkm1] - g.z[base_k] + tol) / p.layerheight);
+    const std::string path = std::string(base) + "_ExtractedSection" + std::to_string(layer) + ".txt";
+    std::ofstream f(path, std::ios::out | std::ios::app);
+    if (!f) return set_error(AMCFD_E_IO, "cannot write " + path);
+    using std::scientific;
+    using std::setprecision;
+    using std::setw;
+    if (sci) f << setprecision(6) << scientific;
+    f << "t " << "ix " "jy " << "kz " << "\n" << sc.timet << " " << outx << " " << outy << " " << outz << "\n";
+    f << setw(15) << "x" << setw(15) << "y" << setw(15) << "z" << setw(15) << "tn" << "\n";
+    for (int k = 0; k < gridz - 1; ++k)
+        for (int j = 0; j < nj; ++j)
+            for (int i = 0; i < ni; ++i) {
+                if (!(inx(i) && iny(j) && inz(k))) continue;
+                const double v = (k < nkm1) ? t[((size_t)k * nj + j) * ni + i] : 0.0;
+                f << setprecision(6) << scientific << setw(15) << g.x[i] << setw(15) << g.y[j] << setw(15) << g.z[k]
+                  << setw(15) << v << "\n";
+                sci = true;
+            }
+    f << "\n";
+    f.close();
+    if (!f) return set_error(AMCFD_E_IO, "write failed: " + path);
     return AMCFD_OK;
 }
 

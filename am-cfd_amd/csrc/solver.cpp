@@ -88,6 +88,7 @@ int Solver::create(const amcfd_params &p, int mode) {
     set_active_nk(p.nk);
 
     hg_.build(p_);
+    hg0_z_ = hg_.z;
     const size_t nn = (size_t)d_.ni * d_.nj * d_.nkf;
     const int n1 = d_.ni + d_.nj + d_.nkf + 8;
     int st;
@@ -127,6 +128,17 @@ int Solver::create(const amcfd_params &p, int mode) {
             if ((st = alloc(q, nn))) return st;
             if ((st = upload(*q, std::vector<double>(nn, p_.initconcentration)))) return st;
         }
+    }
+    if (p_.energyfreesurface) {   // FreeSurfaceManager::initializeFreeSurface, :35-67
+        if ((st = alloc(&f_.fi, (size_t)d_.ni * d_.nj))) return st;
+        if ((st = alloc(&f_.zzm, (size_t)d_.ni * d_.nj))) return st;
+        if ((st = alloc(&f_.zr, nn))) return st;
+        if ((st = alloc(&f_.tempr, nn))) return st;
+        std::vector<double> zr(nn);
+        for (size_t c = 0; c < nn; ++c) zr[c] = hg_.z[c / ((size_t)d_.ni * d_.nj)];
+        // tempr starts as the initial temperature, uniform preheat (CFDSolverManager.cpp:429-447)
+        if ((st = upload(f_.zr, zr)) || (st = upload(f_.tempr, std::vector<double>(nn, p_.temppreheat)))) return st;
+        if ((st = alloc(&fs_out_, 8))) return st;
     }
     if (p_.solidification) {   // zero until the first molten step (CFDSolverManager.cpp:469-473)
         if ((st = alloc(&f_.grad, nn))) return st;
@@ -280,6 +292,44 @@ int Solver::begin_step() {
         if (st) return st;
     }
     if (p_.movingmesh) return move_mesh();   // main.cpp:144-145
+    return AMCFD_OK;
+}
+
+// FreeSurfaceManager::execute (FreeSurfaceManager.cpp:72-220) at an output
+// time: the added droplet volume on the host, the multiplier search and the
+// profile on the device; appends the monitor block when path is given
+int Solver::free_surface(const char *monitor_path) {
+    if (!p_.energyfreesurface) return set_error(AMCFD_E_ARG, "the deck has no *TRACK_ENERGY_BASED_FREE_SURFACE");
+    const double vdp = 2e-9, startdeltn = 0.0;
+    double melttime = p_.delt * (1.0 + startdeltn), vd;
+    if (sc_host_->tpeak <= p_.tsolid) {
+        vd = 0.0;
+        melttime = 0.0;
+    } else {
+        vd = vdp * (timet_ - melttime - startdeltn * p_.delt);
+        if (powerindicator_ < 1) vd = vdp * (p_.lasertime - melttime);
+    }
+    launch_free_surface(st_, d_, g_, f_, p_, vd, fs_out_);
+    HIPCALL(hipGetLastError());
+    double r[6];
+    HIPCALL(hipMemcpyAsync(r, fs_out_, sizeof r, hipMemcpyDeviceToHost, st_));
+    HIPCALL(hipStreamSynchronize(st_));
+    fs_monitor_ = {r[0], r[1], r[2], r[3], vd, (int)r[4], (int)r[5]};
+    if (monitor_path) return append_free_surface_monitor(*this, monitor_path);
+    return AMCFD_OK;
+}
+
+// PrintManager::outputSection (PrintManager.cpp:538-624): called every step
+// after the solve, and once more at the end with finish set
+int Solver::output_section(const char *base, int finish, int *written) {
+    *written = 0;
+    if (!p_.outputsection) return set_error(AMCFD_E_ARG, "the deck has no *OUTPUT_SECTION");
+    tracksect_ += p_.delt;
+    if (tracksect_ < p_.toutsect && !finish) return AMCFD_OK;
+    int st = write_section(*this, base, section_sci_);
+    if (st) return st;
+    tracksect_ = 0.0;
+    *written = 1;
     return AMCFD_OK;
 }
 
@@ -571,7 +621,7 @@ double *Solver::field_ptr(const char *n) {
              {"mpPhase", f_.mpphase}, {"mpLiqPhase", f_.mpliqphase}, {"ap", f_.ap}, {"an", f_.an},
              {"as", f_.as}, {"ae", f_.ae}, {"aw", f_.aw}, {"at", f_.at}, {"ab", f_.ab}, {"su", f_.su},
              {"avgconcentration", f_.conc}, {"avgconcentrationnot", f_.concnot}, {"concentration", f_.concl},
-             {"grad", f_.grad}, {"rate", f_.rate}};
+             {"grad", f_.grad}, {"rate", f_.rate}, {"tempr", f_.tempr}, {"zr", f_.zr}};
     // species and G/R fields exist only with their keywords (null otherwise)
     for (auto &e : t)
         if (std::strcmp(e.name, n) == 0) return e.p;
@@ -579,9 +629,18 @@ double *Solver::field_ptr(const char *n) {
 }
 
 int Solver::get_field(const char *name, double *host, size_t n) {
+    size_t nn = (size_t)d_.ni * d_.nj * d_.nkf;
+    const bool fs_name = !std::strcmp(name, "tempr") || !std::strcmp(name, "zr");
+    if (fs_name && !p_.energyfreesurface) {
+        // registered whatever the deck (FreeSurfaceManager.cpp:35-67) and never
+        // updated without the free surface: the initial temperature and z
+        if (n != nn) return set_error(AMCFD_E_ARG, "field buffer must hold ni*nj*nk values");
+        for (size_t c = 0; c < nn; ++c)
+            host[c] = name[0] == 'z' ? hg0_z_[c / ((size_t)d_.ni * d_.nj)] : p_.temppreheat;
+        return AMCFD_OK;
+    }
     double *p = field_ptr(name);
     if (!p) return set_error(AMCFD_E_NAME, std::string("unknown field ") + name);
-    size_t nn = (size_t)d_.ni * d_.nj * d_.nkf;
     if (n != nn) return set_error(AMCFD_E_ARG, "field buffer must hold ni*nj*nk values");
     HIPCALL(hipMemcpyAsync(host, p, nn * sizeof(double), hipMemcpyDeviceToHost, st_));
     HIPCALL(hipStreamSynchronize(st_));

@@ -123,6 +123,9 @@ typedef struct amcfd_params {
      * bisection limit and tolerance, relaxation); evaluated at output times */
     int    energyfreesurface;
     double initlam1, initlam2, surftens, maxiterenergy, tolenergy, maxiterbisect, tolbisect, urfree;
+    /* *OUTPUT_SECTION: DomainManager.cpp:189-198 (box min, box max, interval) */
+    int    outputsection;
+    double gxmin, gymin, gzmin, gxmax, gymax, gzmax, toutsect;
 } amcfd_params;
 
 /* scalars the reference keeps as members (CFDSolverManager.h:81-83,
@@ -194,7 +197,6 @@ int  amcfd_last_timing(amcfd_solver *s, double *total_ms, double *tdma_ms);
 int  amcfd_set_profiling(amcfd_solver *s, int on);
 int  amcfd_kernel_stats(amcfd_solver *s, const char *kernel, long *launches, double *total_ms);
 
-/* Thread-local description of the last error. */
 /* Output files in the reference's formats, so a run leaves the files the
  * reference executable writes (src/main.cpp:115-241):
  *   write_vts       one StructuredGrid file, vtsBinWriter::visualOut
@@ -216,6 +218,31 @@ int  amcfd_append_final_geometry(amcfd_solver *s, const char *path);
  * "scalar_out" / "vector_out" with an index; NULL when absent. */
 const char *amcfd_deck_string(const amcfd_deck *deck, const char *key, int index);
 
+/* Energy-based free surface at an output time: FreeSurfaceManager::execute
+ * (FreeSurfaceManager.cpp:72-220), called by main.cpp:175-176 and :211-212.
+ * Updates the registry fields "zr" (shifted node heights, then used as the
+ * VTS point heights) and "tempr"; appends the monitor block to monitor_path
+ * when it is not NULL.  Needs *TRACK_ENERGY_BASED_FREE_SURFACE. */
+int  amcfd_free_surface(amcfd_solver *s, const char *monitor_path);
+/* Extracted section: PrintManager::outputSection (PrintManager.cpp:538-624),
+ * called every step after the solve (main.cpp:155-156) and once with finish
+ * set at the end (:214-215).  Accumulates the time step and, when the
+ * *OUTPUT_SECTION interval has passed (or finish), appends the box's nodes
+ * to <base>_ExtractedSection<layer>.txt; *written tells which. */
+int  amcfd_output_section(amcfd_solver *s, const char *base, int finish, int *written);
+
+/* Independent instances on the one GPU at once (a parameter sweep): each
+ * solver advances up to max_steps time steps (to its final time if
+ * max_steps < 0) on its own host thread and stream, so their kernels and
+ * line sweeps overlap on the device; every instance computes exactly what it
+ * computes alone.  steps_done[i] receives the steps instance i took.  On an
+ * error the first failing instance's status is returned.  The number of
+ * streams the device runs side by side is the HIP runtime's hardware-queue
+ * count (GPU_MAX_HW_QUEUES, 4 by default).  No reference counterpart: the
+ * reference runs one deck per process. */
+int  amcfd_run_batch(amcfd_solver *const *s, int n, int max_steps, int *steps_done);
+
+/* Thread-local description of the last error. */
 const char *amcfd_last_error(void);
 const char *amcfd_version(void);
 

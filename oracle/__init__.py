@@ -47,6 +47,10 @@ def load(path=LIB_PATH):
     lib.orc_field.argtypes = [vp, ctypes.c_char_p]
     lib.orc_field.restype = dp
     lib.orc_get_scalars.argtypes = [vp, vp]
+    lib.orc_free_surface.argtypes = [vp]
+    lib.orc_free_surface.restype = ctypes.c_int
+    lib.orc_free_surface_monitor.argtypes = [vp] + [ctypes.POINTER(ctypes.c_double)] * 5 + [
+        ctypes.POINTER(ctypes.c_int)]
     lib.orc_dims.argtypes = [vp, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int),
                              ctypes.POINTER(ctypes.c_int)]
     lib.orc_disc_power.restype = ctypes.c_double
@@ -106,6 +110,16 @@ class Oracle:
         s = amcfd.Scalars()
         load().orc_get_scalars(self._h, ctypes.byref(s))
         return s.as_dict()
+
+    def free_surface(self):
+        """FreeSurfaceManager::execute; returns the monitor values
+        (lambda, residual, min fi, max fi, droplet volume, bisections)."""
+        lib = load()
+        lib.orc_free_surface(self._h)
+        d = [ctypes.c_double() for _ in range(5)]
+        n = ctypes.c_int()
+        lib.orc_free_surface_monitor(self._h, *[ctypes.byref(v) for v in d], ctypes.byref(n))
+        return tuple(v.value for v in d) + (n.value,)
 
     def energy_iteration(self):
         load().orc_energy_iteration(self._h)

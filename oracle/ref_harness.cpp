@@ -1,5 +1,7 @@
 // Test-only driver around the reference solver classes (compiled from the
-// reference sources in place by oracle/Makefile; nothing here ships).
+// reference sources in place by oracle/Makefile; nothing here ships).  The
+// energy-based free surface runs at output times as in main.cpp, so its
+// monitor file (<id>_FreeSurfaceMonitor.txt) lands in the working directory.
 //
 // It reproduces the call order of the reference executable
 //   src/main.cpp:53-74      construct Mesh / managers
@@ -38,6 +40,7 @@
 #include <SolutionVariableManager.h>
 #include <BoundCondManager.h>
 #include <CFDSolverManager.h>
+#include <FreeSurfaceManager.h>
 #undef private
 
 static void write_raw(const std::string &path, const double *p, size_t n) {
@@ -67,6 +70,7 @@ int main(int argc, char **argv) {
     SolutionVariableManager *sv = new SolutionVariableManager();
     BoundCondManager *bc = new BoundCondManager(&meshObj, dm, sv);
     CFDSolverManager *cfd = new CFDSolverManager(&meshObj, dm, sv, bc);
+    FreeSurfaceManager *fs = new FreeSurfaceManager(&meshObj, dm, sv, bc);
 
     // PrintManager::initialization order
     dm->assignInputValues();
@@ -75,6 +79,7 @@ int main(int argc, char **argv) {
     cfd->init_ = false;
     cfd->initializeSolver();
     bc->initializeBoundaries();
+    fs->initializeFreeSurface();
     if (meshObj.powderbed_) {
         dm->nk_ -= (dm->numlayers_ - 1) * dm->ncvzlayer_;
         dm->nkm1_ = dm->nk_ - 1;
@@ -121,11 +126,13 @@ int main(int argc, char **argv) {
             write_raw(buf, ptr[f], nn);
         }
         // species and G/R fields, present only with their keywords
-        const char *xnames[] = {"avgconcentration", "avgconcentrationnot", "concentration", "grad", "rate"};
+        const char *xnames[] = {"avgconcentration", "avgconcentrationnot", "concentration", "grad", "rate",
+                                "zr", "tempr"};
         const double *xptr[] = {cfd->avgconcentration_, cfd->avgconcentrationnot_, cfd->concentration_,
-                                cfd->grad_, cfd->rate_};
-        for (int f = 0; f < 5; ++f) {
-            if (f < 3 ? !meshObj.species_ : !meshObj.solidificationparams_) continue;
+                                cfd->grad_, cfd->rate_, fs->zr_, fs->tempr_};
+        for (int f = 0; f < 7; ++f) {
+            if (f < 3 ? !meshObj.species_ : f < 5 ? !meshObj.solidificationparams_ : !meshObj.energyfreesurface_)
+                continue;
             char buf[512];
             std::snprintf(buf, sizeof buf, "%s/f%05d_%s.bin", outdir.c_str(), step, xnames[f]);
             write_raw(buf, xptr[f], nn);
@@ -134,6 +141,7 @@ int main(int argc, char **argv) {
     if (fields_every > 0) dump_fields(0);
 
     int variableTimeStepState = 0;
+    double outtrack = 0.0;
     dm->timet_ = 0.0;
     double simtime = meshObj.finaltime_;
     int step = 0;
@@ -148,11 +156,19 @@ int main(int argc, char **argv) {
             meshObj.outtime_ = meshObj.variableTimeStepValues[variableTimeStepState][2];
         }
         dm->timet_ += meshObj.delt_;
+        outtrack += meshObj.delt_;
         if (!dm->steady_) bc->updateLaser();
         if (meshObj.movingmesh_) cfd->generateGrid();   // main.cpp:144-145
         cfd->iterativeLoopSolve();
         if (meshObj.solidificationparams_) cfd->getSolidificationParameters();   // main.cpp:151-152
-        if (!dm->steady_) cfd->updateSolutions();
+        if (!dm->steady_) {
+            cfd->updateSolutions();
+            // the free surface runs at output times (main.cpp:167-176)
+            if (outtrack >= meshObj.outtime_) {
+                if (meshObj.energyfreesurface_) fs->execute();
+                outtrack = 0.0;
+            }
+        }
         ++step;
         std::fprintf(sc,
                      "%d,%.17g,%d,%d,%.17g,%.17g,%.17g,%.17g,%.17g,%.17g,%.17g,%.17g,%.17g,%.17g,%.17g,%.17g,"

@@ -31,7 +31,7 @@ The named cases are the workloads the solver is measured and checked on:
 * ``tiny_move_species`` the moving mesh with species and G/R (the remap then
                 carries the concentration fields too).
 * ``tiny_fs``   the tiny_out outputs plus the energy-based free surface (at
-                every output) and an extracted section every 10 us.
+                every output) and an extracted section every 20 us.
 * ``plate120``  120x48x32 cells (2.0x0.6x0.4 mm), 400 steps, P=200 W,
                 r=50 um, 1 m/s: the CPU-runnable single-track case.
 * ``track400``  400x160x80 cells (4.0x1.2x0.6 mm, ~5.1M cells), P=300 W,
@@ -268,7 +268,7 @@ CASES = {
                   xstart=0.15e-3, ystart=0.15e-3, speed=1.0, power=200.0, radius=50e-6,
                   outtime=2.5e-5, scalar_out=("temp", "tempr", "zr", "fracl"),
                   free_surface=(-1.0e6, 1.0e6, 1.8, 20, 1e-6, 30, 1e-16, 0.5),
-                  section=(0.1e-3, 0.0, 0.1e-3, 0.4e-3, 0.15e-3, 0.25e-3, 1e-5))),
+                  section=(0.2e-3, 0.1e-3, 0.15e-3, 0.3e-3, 0.15e-3, 0.25e-3, 2e-5))),
     "plate120": dict(
         mesh=(([0.5e-3, 1.0e-3, 0.5e-3], [20, 80, 20], [-1.2, 1.0, 1.2]),
               ([0.2e-3, 0.2e-3, 0.2e-3], [16, 16, 16], [1.0, 1.0, 1.0]),
@@ -293,3 +293,32 @@ def write_case(name, outdir, **overrides):
     kw.update(overrides)
     write_deck(deck_path, mesh_path, keyword=name, **kw)
     return deck_path
+
+
+def sweep_instance(s):
+    """Instance s of the 64-instance parameter sweep: drawn with numpy's PCG64
+    seeded 20261203 + s, P ~ U(150, 400) W, v ~ U(0.5, 1.5) m/s,
+    r ~ U(40, 80) um, nx ~ U{120..320}, ny ~ U{48..128}, nz ~ U{32..64}
+    uniform 12.5 um cells, dt = 2e-6 s; the track starts a fifth of the way
+    along x on the centre line and runs for min(2 ms, the time to cross
+    three fifths of the plate).  Returns a CASES-style dict."""
+    import numpy as np
+    rng = np.random.default_rng(20261203 + s)
+    power = float(rng.uniform(150.0, 400.0))
+    speed = float(rng.uniform(0.5, 1.5))
+    radius = float(rng.uniform(40e-6, 80e-6))
+    nx, ny, nz = (int(rng.integers(lo, hi + 1)) for lo, hi in ((120, 320), (48, 128), (32, 64)))
+    h = 12.5e-6
+    lx, ly, lz = nx * h, ny * h, nz * h
+    xstart = 0.2 * lx
+    lasertime = min(2e-3, 0.6 * lx / speed)
+    return dict(mesh=(([lx], [nx], [1.0]), ([ly], [ny], [1.0]), ([lz], [nz], [1.0])),
+                deck=dict(finaltime=lasertime, maxit=100, dt=2e-6, lasertime=lasertime, xstart=xstart,
+                          ystart=0.5 * ly, speed=speed, power=power, radius=radius))
+
+
+def write_sweep_instance(s, outdir):
+    """Write instance s of the sweep as <outdir>/sweep<s>.k; return the path."""
+    name = "sweep%02d" % s
+    CASES[name] = sweep_instance(s)
+    return write_case(name, outdir)

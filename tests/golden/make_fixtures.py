@@ -13,6 +13,7 @@ call order, OMP_NUM_THREADS=1) on the decks of decks.py and stores, per case:
   <case>_tool.txt        the toolpath file, when the deck has one
   <case>_grid_steps.txt  per step, the regenerated x and y node coordinates
                          (moving-mesh decks)
+  <case>_FreeSurfaceMonitor.txt  the free-surface monitor (free-surface decks)
 
 Only data is stored; no reference source text is copied.  Run from the repo
 root:  python tests/golden/make_fixtures.py [case ...]
@@ -36,7 +37,7 @@ HARNESS = os.path.join(ROOT, "oracle", "_ref", "ref_harness")
 FIELDS = ["enthalpy", "temp", "fracl", "uvel", "vvel", "wvel", "pressure", "pp", "rho", "diff",
           "vis", "hnot", "fraclnot", "unot", "vnot", "wnot", "sourceinput", "dux", "dvy", "dwz"]
 # written by the harness only when the deck enables species / G/R
-OPTIONAL_FIELDS = ["avgconcentration", "avgconcentrationnot", "concentration", "grad", "rate"]
+OPTIONAL_FIELDS = ["avgconcentration", "avgconcentrationnot", "concentration", "grad", "rate", "zr", "tempr"]
 # case -> (fields_every, max_steps, steps whose full fields are kept)
 PLAN = {
     "tiny": (1, -1, [2, 40]),
@@ -50,6 +51,7 @@ PLAN = {
     "tiny_move_powder": (1, -1, []),
     "tiny_species": (1, -1, []),
     "tiny_move_species": (1, -1, []),
+    "tiny_fs": (1, -1, []),
     "plate120": (50, -1, []),
 }
 
@@ -98,6 +100,9 @@ def run_case(case, workdir=None, reuse=False):
         text = f.read().replace(mesh, "@MESH@").replace(tool, "@TOOL@")
     if os.path.exists(tool):
         shutil.copy(tool, os.path.join(HERE, case + "_tool.txt"))
+    monitor = os.path.join(tmp, case + "_FreeSurfaceMonitor.txt")
+    if os.path.exists(monitor):
+        shutil.copy(monitor, os.path.join(HERE, case + "_FreeSurfaceMonitor.txt"))
     if os.path.exists(os.path.join(tmp, "grid_steps.txt")):
         shutil.copy(os.path.join(tmp, "grid_steps.txt"), os.path.join(HERE, case + "_grid_steps.txt"))
     with open(os.path.join(HERE, case + "_deck.txt"), "w") as f:

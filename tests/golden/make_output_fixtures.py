@@ -7,7 +7,8 @@ in tests/golden/<case>_outputs.json:
   - for every file it wrote: path relative to the run directory, size,
     SHA-256;
   - the full text of the small text files (PVD collection, residual log,
-    final geometry).
+    final geometry, free-surface monitor, extracted sections);
+  - the XML header of every VTS file.
 Only data is stored.  Run from the repo root:
     python tests/golden/make_output_fixtures.py [case]
 """
@@ -38,8 +39,10 @@ def collect(run_dir, skip=()):
                 continue
             data = open(path, "rb").read()
             rec = {"size": len(data), "sha256": hashlib.sha256(data).hexdigest()}
-            if rel.endswith(TEXT_SUFFIXES):
+            if rel.endswith(TEXT_SUFFIXES) and len(data) <= 65536:
                 rec["text"] = data.decode("latin-1")
+            if rel.endswith(".vts"):   # the XML part, up to the raw appended data
+                rec["header"] = data[:data.index(b"<AppendedData encoding=\"raw\">_") + 30].decode("latin-1")
             out[rel] = rec
     return out
 

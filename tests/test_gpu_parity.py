@@ -94,6 +94,40 @@ def test_gpu_matches_oracle_species(case, case_dir, amcfd, oracle_mod):
     assert steps == 30
 
 
+def test_gpu_matches_oracle_free_surface(case_dir, amcfd, oracle_mod, tmp_path):
+    """Energy-based free surface at every output time (FreeSurfaceManager.cpp:
+    72-342).  pow(v, 1.5) in its energy balance is correctly rounded on the
+    device while the C library's misses by one ulp about once in 1300 calls,
+    so the shifted heights and temperatures are held to 1e-12 rather than to
+    the bit; the bisection takes the same steps."""
+    deck = amcfd.Deck(case_dir("tiny_fs"))
+    gpu = amcfd.Solver(deck)
+    orc = oracle_mod.Oracle(deck.params)
+    outtrack, n = 0.0, 0
+    mon = str(tmp_path / "monitor.txt")
+    while not orc.finished():
+        orc.step()
+        gpu.step()
+        s = orc.scalars()
+        outtrack += s["delt"]
+        if outtrack < s["outtime"]:
+            continue
+        outtrack = 0.0
+        print("free surface at t=%.3e" % s["timet"], flush=True)
+        want = orc.free_surface()
+        gpu.free_surface(mon)
+        n += 1
+        for f in ("zr", "tempr", "temp"):
+            e = rel_l2(gpu.field(f), orc.field(f))
+            assert e <= 1e-12, (n, f, e)
+        line = [l for l in open(mon).read().splitlines() if l.startswith(" ") and "time" not in l][-1]
+        got = [float(v) for v in line.split()]
+        assert int(got[6]) == want[5], (n, got, want)
+        for a, b in zip(got[1:6], want[:5]):
+            assert abs(a - b) <= 1e-3 * abs(b) + 1e-30, (n, got, want)   # printed to 4 digits
+    assert n == 4
+
+
 def test_gpu_matches_oracle_surfactant_until_first_log_difference(case_dir, amcfd, oracle_mod):
     """Evaporation plus the Fe-O surfactant model (BoundCondManager.cpp:706-741).
 
@@ -157,3 +191,22 @@ def test_scan_mode_tracks_fp64_on_plate120(case_dir, amcfd):
     for f in ("enthalpy", "temp", "fracl", "uvel", "vvel", "wvel"):
         e = rel_l2(fast.field(f), ref.field(f))
         assert e <= 1e-9, (f, e)
+
+
+def test_batch_runs_are_the_solo_runs(case_dir, amcfd):
+    """amcfd_run_batch: four different decks advanced side by side on the GPU
+    (own host thread and stream each) give, instance by instance, the very
+    bits of running each alone."""
+    cases = ["tiny", "tiny_vap", "tiny_vol", "tiny_tool"]
+    decks_ = [amcfd.Deck(case_dir(c)) for c in cases]
+    batch = [amcfd.Solver(d) for d in decks_]
+    print("batch of %d, 12 steps" % len(batch), flush=True)
+    done = amcfd.run_batch(batch, 12)
+    assert done == [12] * len(batch)
+    for d, b in zip(decks_, batch):
+        solo = amcfd.Solver(d)
+        solo.run(max_steps=12)
+        assert solo.scalars()["itertot"] == b.scalars()["itertot"]
+        for f in ("enthalpy", "temp", "uvel", "vvel", "wvel", "pp"):
+            assert np.array_equal(solo.field(f), b.field(f)), f
+        solo.close()

@@ -12,7 +12,7 @@ import pytest
 from conftest import golden_path
 
 CASES = ["tiny", "tiny_cond", "tiny_evap", "tiny_vap", "tiny_vol", "tiny_tool", "tiny_powder",
-         "tiny_move", "tiny_move_powder", "tiny_species", "tiny_move_species"]
+         "tiny_move", "tiny_move_powder", "tiny_species", "tiny_move_species", "tiny_fs"]
 MOVING = ["tiny_move", "tiny_move_powder", "tiny_move_species"]
 INT_KEYS = ["niter", "itertot", "powerindicator", "istat", "iend", "jstat", "jend", "kstat", "jstart",
             "istart", "nk"]
@@ -51,9 +51,15 @@ def test_oracle_matches_reference_bitwise(case, case_dir, amcfd, oracle_mod):
     rows = list(csv.DictReader(open(golden_path(case + "_scalars.csv"))))
     dig = json.load(open(golden_path(case + "_digests.json")))
     assert list(orc.shape) == dig["shape"]
+    outtrack, monitor = 0.0, []
     for step, row in enumerate(rows, start=1):
         orc.step()
         s = orc.scalars()
+        outtrack += s["delt"]
+        if outtrack >= s["outtime"]:   # the free surface runs at output times (main.cpp:167-176)
+            if deck.params.energyfreesurface:
+                monitor.append((s["timet"],) + orc.free_surface())
+            outtrack = 0.0
         for key in INT_KEYS:
             if key in ("istat", "iend", "jstat", "jend", "kstat") and s["tpeak"] <= deck.params.tsolid:
                 continue
@@ -67,6 +73,12 @@ def test_oracle_matches_reference_bitwise(case, case_dir, amcfd, oracle_mod):
             h = hashlib.sha256(np.ascontiguousarray(a, dtype="<f8").tobytes()).hexdigest()
             assert h == rec["sha256"], (case, step, fld, float(np.linalg.norm(a)), rec["l2"])
     assert orc.finished()
+    if deck.params.energyfreesurface:
+        # the monitor lines the reference printed, FreeSurfaceManager.cpp:197-206
+        want = [l for l in open(golden_path(case + "_FreeSurfaceMonitor.txt")).read().splitlines()
+                if l and l[0] == " " and not l.lstrip().startswith("time")]
+        got = ["%12.4e%14.4e%14.4e%14.4e%14.4e%14.4e%9d" % m for m in monitor]
+        assert got == want
 
 
 def read_grid_steps(case):

@@ -7,6 +7,7 @@ import json
 import os
 import subprocess
 
+import numpy as np
 import pytest
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
@@ -58,3 +59,70 @@ def test_cli_writes_the_reference_files(case_dir, tmp_path):
             assert data.decode("latin-1") == rec["text"], rel
         assert len(data) == rec["size"], rel
         assert hashlib.sha256(data).hexdigest() == rec["sha256"], rel
+
+
+def _vts_arrays(data):
+    """The appended raw arrays of a VTS file: [points, field1, ...] as float64."""
+    start = data.index(b'<AppendedData encoding="raw">_') + 30
+    out, pos = [], start
+    while pos < len(data) and data[pos:pos + 1] != b"\n":
+        n = int(np.frombuffer(data[pos:pos + 4], dtype="<i4")[0])
+        out.append(np.frombuffer(data[pos + 4:pos + 4 + n], dtype="<f8"))
+        pos += 4 + n
+    return out
+
+
+@pytest.mark.gpu
+def test_cli_free_surface_and_sections(case_dir, tmp_path, amcfd, oracle_mod):
+    """The free-surface deck (energy balance at every output, sections every
+    20 us, tempr/zr outputs): the residual log, the PVD collection, the
+    extracted-section file and the free-surface monitor are byte-identical to
+    the reference's; every VTS file has the reference's XML header and size,
+    and its binary arrays (point heights from zr, temp, tempr, fracl) match
+    the oracle to 1e-12 (the device's pow(v, 1.5) is correctly rounded where
+    the C library's occasionally is not, so the bits can differ)."""
+    src = case_dir("tiny_fs")
+    run = tmp_path / "run"
+    run.mkdir()
+    for f in os.listdir(os.path.dirname(src)):
+        if f.startswith("tiny_fs"):
+            os.link(os.path.join(os.path.dirname(src), f), run / f)
+    print("running the CLI on tiny_fs", flush=True)
+    out = subprocess.run([CLI, "tiny_fs.k"], cwd=run, capture_output=True, text=True, timeout=600)
+    assert out.returncode == 0, out.stderr
+    want = json.load(open(os.path.join(ROOT, "tests", "golden", "tiny_fs_outputs.json")))["files"]
+    got = {}
+    for base, _, files in os.walk(run):
+        for f in files:
+            rel = os.path.relpath(os.path.join(base, f), run)
+            if not rel.endswith((".k",)):
+                got[rel] = open(os.path.join(base, f), "rb").read()
+    assert sorted(got) == sorted(want), (sorted(got), sorted(want))
+    for rel, rec in want.items():
+        data = got[rel]
+        assert len(data) == rec["size"], rel
+        if rel.endswith(".vts"):
+            assert data[:len(rec["header"])].decode("latin-1") == rec["header"], rel
+        else:
+            if "text" in rec:
+                assert data.decode("latin-1") == rec["text"], rel
+            assert hashlib.sha256(data).hexdigest() == rec["sha256"], rel
+    # the last VTS file against the oracle run to the end with the same outputs
+    deck = amcfd.Deck(src)
+    orc = oracle_mod.Oracle(deck.params)
+    outtrack = 0.0
+    while not orc.finished():
+        orc.step()
+        s = orc.scalars()
+        outtrack += s["delt"]
+        if outtrack >= s["outtime"]:
+            orc.free_surface()
+            outtrack = 0.0
+    orc.free_surface()   # main.cpp:211-212
+    arrays = _vts_arrays(got["tiny_fs/tiny_fs_5.vts"])
+    nk, nj, ni = orc.shape
+    npts = ni * nj * (nk - 1)
+    pts = arrays[0].reshape(npts, 3)
+    for a, ref in ((pts[:, 2], orc.field("zr").ravel()[:npts]), (arrays[1], orc.field("temp").ravel()[:npts]),
+                   (arrays[2], orc.field("tempr").ravel()[:npts]), (arrays[4], orc.field("fracl").ravel()[:npts])):
+        assert np.linalg.norm(a - ref) <= 1e-12 * np.linalg.norm(ref)
