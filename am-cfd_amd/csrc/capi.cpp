@@ -140,14 +140,17 @@ int amcfd_run_batch(amcfd_solver *const *s, int n, int max_steps, int *steps_don
         }
         steps_done[i] = k;
     };
+    for (int i = 0; i < n; ++i) s[i]->s.set_shared_device(n > 1);
     try {
         std::vector<std::thread> pool;
         pool.reserve(n);
         for (int i = 0; i < n; ++i) pool.emplace_back(run, i);
         for (auto &t : pool) t.join();
     } catch (const std::exception &e) {
+        for (int i = 0; i < n; ++i) s[i]->s.set_shared_device(false);
         return amcfd::set_error(AMCFD_E_ARG, std::string("cannot start batch threads: ") + e.what());
     }
+    for (int i = 0; i < n; ++i) s[i]->s.set_shared_device(false);
     for (int i = 0; i < n; ++i)
         if (rc[i] < 0) return amcfd::set_error(rc[i], "batch instance " + std::to_string(i) + ": " + msg[i]);
     return AMCFD_OK;

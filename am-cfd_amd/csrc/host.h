@@ -55,6 +55,8 @@ class Solver {
     int active_nk() const { return d_.nk; }
     double last_total_ms = -1.0, last_tdma_ms = -1.0;
     void set_profiling(int on) { profile_ = on; }
+    // the solver runs beside others on the device (amcfd_run_batch)
+    void set_shared_device(bool on) { shared_device_ = on; }
     std::map<std::string, KernelTimer> timers;
 
   private:
@@ -76,6 +78,7 @@ class Solver {
     unsigned long long counter_base_ = 0;
     double *pass_buf_ = nullptr;   // four field-sized pass buffers of the line sweeps
     int profile_ = 0;
+    bool shared_device_ = false;
     std::vector<hipEvent_t> ev_pool_;
     std::vector<std::pair<std::string, std::pair<int, int>>> ev_pending_;
     size_t ev_next_ = 0;

@@ -152,10 +152,11 @@ void launch_end_step(hipStream_t st, const Dims &d, DevFields f, const amcfd_par
 void launch_line_prep(hipStream_t st, const Dims &d, DevFields f, double *pb, size_t pstride, const SweepPlan &plan,
                       int ibeg, int iend, int guard);
 // one Gauss-Seidel x-line sweep; pb holds four pass buffers of pstride
-// doubles each (the field's size); scan selects the fast-mode line solve
+// doubles each (the field's size); scan selects the fast-mode line solve;
+// shared selects the one-wave-per-line form (solvers sharing the device)
 void launch_line_sweeps(hipStream_t st, const Dims &d, DevFields f, double *phi, double *pb, size_t pstride,
                         const SweepPlan &plan, int ibeg, int iend, int guard, int scan, unsigned long long *counter,
-                        unsigned long long counter_base, DevScalars *sc, int nblocks);
+                        unsigned long long counter_base, DevScalars *sc, int nblocks, int shared);
 
 // species (species.hip)
 void launch_bc_species(hipStream_t st, const Dims &d, const DevGeom &g, DevFields f, const amcfd_params &p);

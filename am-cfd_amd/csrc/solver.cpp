@@ -473,13 +473,16 @@ int Solver::sweep(double *phi, SweepPlan &plan, int ibeg, int iend, int jbeg, in
     int st = ensure_plan(plan, kbeg, kend, jbeg, jend);
     if (st) return st;
     if (plan.ntasks == 0 || iend < ibeg) return 0;
-    int nblocks = std::min(plan.ntasks, std::max(2 * plan.width, 1));
+    // alone, twice the wavefront width of workgroups keeps the next tasks'
+    // coefficients in flight; beside other solvers the device's workgroup
+    // slots are shared, and one per wavefront task is enough
+    int nblocks = std::min(plan.ntasks, std::max((shared_device_ ? 1 : 2) * plan.width, 1));
     nblocks = std::min(nblocks, 2048);
     launch_line_prep(st_, d_, f_, pass_buf_, (size_t)d_.ni * d_.nj * d_.nkf, plan, ibeg, iend, guard);
     int slot;
     tic(timer, &slot);
     launch_line_sweeps(st_, d_, f_, phi, pass_buf_, (size_t)d_.ni * d_.nj * d_.nkf, plan, ibeg, iend, guard,
-                       mode_ == AMCFD_MODE_SCAN, counter_, counter_base_, sc_, nblocks);
+                       mode_ == AMCFD_MODE_SCAN, counter_, counter_base_, sc_, nblocks, shared_device_);
     HIPCALL(hipGetLastError());
     toc(timer, slot);
     counter_base_ += (unsigned long long)plan.ntasks + (unsigned long long)nblocks;

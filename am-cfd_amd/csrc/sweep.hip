@@ -44,6 +44,7 @@ namespace {
 constexpr int SW_MAXN = 1024;                 // longest x-line handled
 constexpr int SW_M = 20;                      // rows per lane in the recurrence (64 * SW_M >= SW_MAXN)
 constexpr int SW_M_SHORT = 6;                 // rows per lane for lines of at most 384 cells
+constexpr int SW_M_THROUGHPUT = 8;            // rows per lane of the one-wave form (lines <= 512)
 constexpr long long SPIN_LIMIT = 1LL << 20;   // bounded wait (~ a second)
 static_assert(64 * SW_M >= SW_MAXN, "one wave must hold a whole line");
 
@@ -396,19 +397,25 @@ __global__ __launch_bounds__(256) void k_fill_pending(Dims d, double *pb, size_t
     }
 }
 
-// One x-line solve per task; M rows per lane in the recurrence (64 M >= n).
-// DOMAIN marks the instantiation for the whole-domain enthalpy sweep, so it
-// shows up as its own kernel in a profile; the code is the same.  SCAN
-// selects the fast-mode line solve (ScanLine) instead of the reference's
-// recurrence.
+// One x-line solve per task by a workgroup of NT threads; M rows per lane in
+// the recurrence (lines up to LMAX = min(64 M, SW_MAXN) cells).  DOMAIN marks
+// the instantiation for the whole-domain enthalpy sweep, so it shows up as
+// its own kernel in a profile; the code is the same.  SCAN selects the
+// fast-mode line solve (ScanLine) instead of the reference's recurrence.
+// NT = 256 with M = 20 is the latency form: four waves gather the right-hand
+// side, one runs the recurrence with few lane hand-offs, and the register
+// file holds one such workgroup per CU.  NT = 64 with M = 8 is the throughput
+// form for solvers sharing the device: one wave per line, eight per CU.  The
+// recurrence performs the same operations in the same order either way.
 template <int NT, int M, bool DOMAIN, bool SCAN>
-__global__ __launch_bounds__(NT) void k_line_sweeps(Dims d, Coefs cf, double *phi, double *pb, size_t pstride,
+__global__ __launch_bounds__(NT, (NT == 64 ? 2 : 1)) void k_line_sweeps(Dims d, Coefs cf, double *phi, double *pb, size_t pstride,
                                                     const int *__restrict__ tasks, int ntasks, int ibeg, int iend,
                                                     int jbeg, int jend, int kbeg, int kend,
                                                     unsigned long long *counter, unsigned long long base,
                                                     DevScalars *sc) {
-    constexpr int MAXR = SW_MAXN / NT;
-    __shared__ double A[SW_MAXN], B[SW_MAXN], C[SW_MAXN], D[SW_MAXN], E[SW_MAXN];
+    constexpr int LMAX = (64 * M < SW_MAXN) ? 64 * M : SW_MAXN;
+    constexpr int MAXR = (LMAX + NT - 1) / NT;
+    __shared__ double A[LMAX], B[LMAX], C[LMAX], D[LMAX], E[LMAX];
     __shared__ int s_task;
     const int tid = threadIdx.x;
     const int n = iend - ibeg + 1;
@@ -537,8 +544,21 @@ void launch_line_prep(hipStream_t st, const Dims &d, DevFields f, double *pb, si
 template <bool SCAN>
 static void launch_sweeps_t(hipStream_t st, const Dims &d, const Coefs &cf, double *phi, double *pb, size_t pstride,
                             const SweepPlan &plan, int ibeg, int iend, int guard, unsigned long long *counter,
-                            unsigned long long counter_base, DevScalars *sc, int nblocks) {
+                            unsigned long long counter_base, DevScalars *sc, int nblocks, int shared) {
     const int n = iend - ibeg + 1;
+    // beside other solvers: one wave per line, for lines of up to 512 cells
+    // (the scan mode keeps its lane layout, on which its rounding depends)
+    if (shared && !SCAN && n <= 64 * SW_M_THROUGHPUT) {
+        if (!guard)
+            hipLaunchKernelGGL((k_line_sweeps<64, SW_M_THROUGHPUT, true, false>), dim3(nblocks), dim3(64), 0, st, d,
+                               cf, phi, pb, pstride, plan.tasks, plan.ntasks, ibeg, iend, plan.jbeg, plan.jend,
+                               plan.kbeg, plan.kend, counter, counter_base, sc);
+        else
+            hipLaunchKernelGGL((k_line_sweeps<64, SW_M_THROUGHPUT, false, false>), dim3(nblocks), dim3(64), 0, st, d,
+                               cf, phi, pb, pstride, plan.tasks, plan.ntasks, ibeg, iend, plan.jbeg, plan.jend,
+                               plan.kbeg, plan.kend, counter, counter_base, sc);
+        return;
+    }
     // short lines (the melt-pool box) get a lighter instantiation: fewer
     // registers, more waves per SIMD
     if (!guard)
@@ -557,15 +577,15 @@ static void launch_sweeps_t(hipStream_t st, const Dims &d, const Coefs &cf, doub
 
 void launch_line_sweeps(hipStream_t st, const Dims &d, DevFields f, double *phi, double *pb, size_t pstride,
                         const SweepPlan &plan, int ibeg, int iend, int guard, int scan, unsigned long long *counter,
-                        unsigned long long counter_base, DevScalars *sc, int nblocks) {
+                        unsigned long long counter_base, DevScalars *sc, int nblocks, int shared) {
     if (plan.ntasks == 0 || iend < ibeg) return;
     Coefs cf{f.an, f.as, f.ae, f.aw, f.at, f.ab, f.ap, f.su, f.lpr, f.lden, f.ly};
     if (scan)
         launch_sweeps_t<true>(st, d, cf, phi, pb, pstride, plan, ibeg, iend, guard, counter, counter_base, sc,
-                              nblocks);
+                              nblocks, shared);
     else
         launch_sweeps_t<false>(st, d, cf, phi, pb, pstride, plan, ibeg, iend, guard, counter, counter_base, sc,
-                               nblocks);
+                               nblocks, shared);
 }
 
 }  // namespace amcfd

@@ -22,6 +22,12 @@ import sys
 import tempfile
 import time
 
+# the batched leg runs one stream per instance; HIP maps streams onto
+# GPU_MAX_HW_QUEUES hardware queues (4 by default), raised here to 16 before
+# anything initialises HIP
+if int(os.environ.get("GPU_MAX_HW_QUEUES", "4") or 4) < 16:
+    os.environ["GPU_MAX_HW_QUEUES"] = "16"
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 sys.path.insert(0, os.path.join(ROOT, "tests", "golden"))
@@ -90,6 +96,46 @@ def cpu_baseline(deck_params, dom, gpu_iters_per_step, budget_s=20.0):
                       "%.1f iterations/step" % (steps, iters, el, per_iter, gpu_iters_per_step)}
 
 
+def batch_leg(amcfd, decks, tmp, n, warmup, steps):
+    """Instances 0..n-1 of the seeded parameter sweep (uneven grids, 12.5 um
+    cells, P/v/r drawn per instance) in the plain FP64 mode: advanced side by
+    side with amcfd_run_batch, then the same instances one at a time over the
+    same steps.  Both runs compute the same bits; the ratio is the gain from
+    sharing the device."""
+    import numpy as np
+    dks = [amcfd.Deck(decks.write_sweep_instance(s, tmp)) for s in range(n)]
+    cells = [(d.params.ni - 2) * (d.params.nj - 2) * (d.params.nk - 2) for d in dks]
+    work = sum(cells) * steps
+    print("batch leg: %d instances, %d cells" % (n, sum(cells)), file=sys.stderr, flush=True)
+    side = [amcfd.Solver(d) for d in dks]
+    amcfd.run_batch(side, warmup)
+    i0 = sum(s.scalars()["itertot"] for s in side)
+    t0 = time.perf_counter()
+    amcfd.run_batch(side, steps)
+    tb = time.perf_counter() - t0
+    iters = sum(s.scalars()["itertot"] for s in side) - i0
+    print("batch leg: side by side %.3f s" % tb, file=sys.stderr, flush=True)
+    ts, same = 0.0, True
+    for d, a in zip(dks, side):
+        b = amcfd.Solver(d)
+        b.run(max_steps=warmup)
+        t0 = time.perf_counter()
+        b.run(max_steps=steps)
+        ts += time.perf_counter() - t0
+        same &= b.scalars()["itertot"] == a.scalars()["itertot"] and np.array_equal(b.field("temp"), a.field("temp"))
+        b.close()
+    for s in side:
+        s.close()
+    print("batch leg: one at a time %.3f s" % ts, file=sys.stderr, flush=True)
+    return {"value": work / tb, "unit": "cell-steps/s", "instances": n, "interior_cells": sum(cells),
+            "warmup_steps": warmup, "timed_steps": steps, "iters_per_instance_step": iters / (n * steps),
+            "one_at_a_time_value": work / ts, "speedup_vs_one_at_a_time": ts / tb,
+            "identical_to_one_at_a_time": bool(same),
+            "hw_queues": int(os.environ.get("GPU_MAX_HW_QUEUES", "4")),
+            "note": "instances 0..n-1 of the seeded parameter sweep (numpy PCG64 seed 20261203+s), plain FP64 "
+                    "mode, each on its own stream and host thread (amcfd_run_batch)"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -100,6 +146,10 @@ def main():
                     help="fp64: reference program order, bit for bit; scan: parallel line solves")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-fast", action="store_true", help="skip the scan-mode leg")
+    ap.add_argument("--batch", type=int, default=16,
+                    help="instances of the seeded parameter sweep in the batched leg (0 skips it)")
+    ap.add_argument("--batch-warmup", type=int, default=8)
+    ap.add_argument("--batch-steps", type=int, default=3)
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1),
                     help="OpenMP threads of the reference CPU baseline (the host cores used)")
@@ -206,6 +256,9 @@ def main():
             "note": "FP64, x-line solves by parallel scans in the reference's sweep order; drift vs the "
                     "FP64 mode equals a 1-ulp perturbation's (tests/drift_study.py)"}}
         fast.close()
+    if args.mode == "fp64" and args.batch > 0:
+        result.setdefault("fast_modes", {})["batch"] = batch_leg(amcfd, decks, tmp, args.batch, args.batch_warmup,
+                                                                 args.batch_steps)
     result["cpu_baseline"] = None
     if not args.no_cpu:
         print("cpu baseline: reference harness, %d threads" % args.cpu_threads, file=sys.stderr, flush=True)
