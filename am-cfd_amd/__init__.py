@@ -143,7 +143,7 @@ def load_library(path=LIB_PATH):
         "amcfd_deck_string": (cp, [vp, cp, ip]),
         "amcfd_free_surface": (ip, [vp, cp]),
         "amcfd_output_section": (ip, [vp, cp, ip, ctypes.POINTER(ip)]),
-        "amcfd_run_batch": (ip, [ctypes.POINTER(vp), ip, ip, ctypes.POINTER(ip)]),
+        "amcfd_run_batch": (ip, [ctypes.POINTER(vp), ip, ip, ip, ctypes.POINTER(ip)]),
         "amcfd_last_error": (cp, []),
         "amcfd_version": (cp, []),
     }
@@ -296,13 +296,13 @@ class Solver:
             pass
 
 
-def run_batch(solvers, max_steps=-1):
-    """Advance independent solvers concurrently on the one GPU (each on its
-    own host thread and stream); returns the steps each took."""
+def run_batch(solvers, max_steps=-1, workers=0):
+    """Advance independent solvers concurrently on the one GPU (worker host
+    threads, one stream per instance); returns the steps each took."""
     n = len(solvers)
     handles = (ctypes.c_void_p * max(n, 1))(*[s._h.value for s in solvers])
     done = (ctypes.c_int * max(n, 1))()
-    _check(load_library().amcfd_run_batch(handles, n, int(max_steps), done))
+    _check(load_library().amcfd_run_batch(handles, n, int(max_steps), int(workers), done))
     return [done[i] for i in range(n)]
 
 

@@ -5,6 +5,8 @@
 #include <cstring>
 #include <exception>
 #include <new>
+#include <algorithm>
+#include <atomic>
 #include <thread>
 #include <vector>
 
@@ -113,38 +115,55 @@ int amcfd_step(amcfd_solver *s, int *finished) {
     return rs;
 }
 
-int amcfd_run_batch(amcfd_solver *const *s, int n, int max_steps, int *steps_done) {
+int amcfd_run_batch(amcfd_solver *const *s, int n, int max_steps, int workers, int *steps_done) {
     if (n < 0 || (n > 0 && (!s || !steps_done))) return amcfd::set_error(AMCFD_E_ARG, "null argument");
     for (int i = 0; i < n; ++i)
         if (!s[i]) return amcfd::set_error(AMCFD_E_ARG, "null solver in batch");
     int dev = 0;
     if (hipGetDevice(&dev) != hipSuccess) return amcfd::set_error(AMCFD_E_HIP, "hipGetDevice failed");
+    if (workers <= 0) workers = 16;
+    workers = std::min(workers, n);
+    // largest grids first, so that the last instances to start are short
+    std::vector<int> order(n);
+    for (int i = 0; i < n; ++i) order[i] = i;
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+        int ia, ja, ka, ib, jb, kb;
+        s[a]->s.dims(&ia, &ja, &ka);
+        s[b]->s.dims(&ib, &jb, &kb);
+        return (long long)ia * ja * ka > (long long)ib * jb * kb;
+    });
     std::vector<int> rc(n, AMCFD_OK);
     std::vector<std::string> msg(n);
-    auto run = [&](int i) {
+    std::atomic<int> next{0};
+    auto worker = [&]() {
         // the calling thread's device, for the allocations a plan change makes
-        if (hipSetDevice(dev) != hipSuccess) {
-            rc[i] = AMCFD_E_HIP;
-            msg[i] = "hipSetDevice failed";
-            return;
-        }
-        int k = 0, fin = 0;
-        while ((max_steps < 0 || k < max_steps) && !s[i]->s.finished()) {
-            const int st = amcfd_step(s[i], &fin);
-            if (st < 0) {
-                rc[i] = st;
-                msg[i] = amcfd_last_error();
-                break;
+        const bool dev_ok = hipSetDevice(dev) == hipSuccess;
+        for (int q = next++; q < n; q = next++) {
+            const int i = order[q];
+            if (!dev_ok) {
+                rc[i] = AMCFD_E_HIP;
+                msg[i] = "hipSetDevice failed";
+                steps_done[i] = 0;
+                continue;
             }
-            ++k;
+            int k = 0, fin = 0;
+            while ((max_steps < 0 || k < max_steps) && !s[i]->s.finished()) {
+                const int st = amcfd_step(s[i], &fin);
+                if (st < 0) {
+                    rc[i] = st;
+                    msg[i] = amcfd_last_error();
+                    break;
+                }
+                ++k;
+            }
+            steps_done[i] = k;
         }
-        steps_done[i] = k;
     };
-    for (int i = 0; i < n; ++i) s[i]->s.set_shared_device(n > 1);
+    for (int i = 0; i < n; ++i) s[i]->s.set_shared_device(workers > 1);
     try {
         std::vector<std::thread> pool;
-        pool.reserve(n);
-        for (int i = 0; i < n; ++i) pool.emplace_back(run, i);
+        pool.reserve(workers);
+        for (int w = 0; w < workers; ++w) pool.emplace_back(worker);
         for (auto &t : pool) t.join();
     } catch (const std::exception &e) {
         for (int i = 0; i < n; ++i) s[i]->s.set_shared_device(false);

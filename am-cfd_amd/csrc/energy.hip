@@ -316,9 +316,12 @@ __global__ __launch_bounds__(256) void k_props(size_t n, PropConst c, const doub
 // one workgroup serves CS_COLS columns of one quantity: lanes 0..CS_COLS-1 of
 // wave 0 hold the running sums while the other waves stream the next CS_CH
 // rows into an LDS double buffer.  Output: blk[q*ni + i], q = bl, blp, blm,
-// blc.
-constexpr int CS_COLS = 16, CS_CH = 512, CS_NT = 1024;
+// blc.  CS_NT = 1024 threads with 128 KB of LDS is the form for a solver
+// alone; beside other solvers a 256-thread form with 32 KB finds a CU sooner
+// (the sums are the same chains in the same order).
+constexpr int CS_COLS = 16;
 
+template <int CS_NT, int CS_CH>
 __global__ __launch_bounds__(CS_NT) void k_column_sums(Dims d, const double *__restrict__ term_bl,
                                                        const double *__restrict__ ae,
                                                        const double *__restrict__ aw,
@@ -868,9 +871,12 @@ void launch_props(hipStream_t st, const Dims &d, DevFields f, const amcfd_params
                        f.csfrac, f.diff, f.rho, f.vis);
 }
 
-void launch_block_correction(hipStream_t st, const Dims &d, DevFields f, Partials pr, double *phi) {
-    hipLaunchKernelGGL(k_column_sums, dim3((d.nim2 + CS_COLS - 1) / CS_COLS, 4), dim3(CS_NT), 0, st, d, f.sp, f.ae, f.aw,
-                       f.apnot, pr.blk);
+void launch_block_correction(hipStream_t st, const Dims &d, DevFields f, Partials pr, double *phi, int shared) {
+    const dim3 grid((d.nim2 + CS_COLS - 1) / CS_COLS, 4);
+    if (shared)
+        hipLaunchKernelGGL((k_column_sums<256, 128>), grid, dim3(256), 0, st, d, f.sp, f.ae, f.aw, f.apnot, pr.blk);
+    else
+        hipLaunchKernelGGL((k_column_sums<1024, 512>), grid, dim3(1024), 0, st, d, f.sp, f.ae, f.aw, f.apnot, pr.blk);
     hipLaunchKernelGGL(k_block_solve, dim3(1), dim3(256), 0, st, d, pr.blk, pr.delphi);
     hipLaunchKernelGGL(k_add_delphi, energy_grid(d), dim3(64, 4), 0, st, d, pr.delphi, phi);
 }

@@ -130,7 +130,7 @@ void launch_laser_map(hipStream_t st, const Dims &d, const DevGeom &g, DevFields
 void launch_props(hipStream_t st, const Dims &d, DevFields f, const amcfd_params &p);
 // x-direction block correction of phi from the column-sum terms the assemble
 // kernel left in f.sp / f.apnot (enthalpy or concentration)
-void launch_block_correction(hipStream_t st, const Dims &d, DevFields f, Partials pr, double *phi);
+void launch_block_correction(hipStream_t st, const Dims &d, DevFields f, Partials pr, double *phi, int shared);
 void launch_h2t(hipStream_t st, const Dims &d, const DevGeom &g, DevFields f, Partials pr,
                 const amcfd_params &p);
 void launch_melt_pool(hipStream_t st, const Dims &d, const DevGeom &g, DevFields f, Partials pr,

@@ -534,7 +534,7 @@ int Solver::solve_step() {
         launch_energy_assemble(st_, d_, g_, f_, pr_, p_, sp);
         toc("assemble_h", slot);
         launch_props(st_, d_, f_, p_);
-        launch_block_correction(st_, d_, f_, pr_, f_.enthalpy);
+        launch_block_correction(st_, d_, f_, pr_, f_.enthalpy, shared_device_);
         int st = sweep(f_.enthalpy, plan_dom_, 1, d_.nim2, 1, d_.njm2, 1, d_.nkm2, 0, "sweep_h");
         if (st) return st;
         tic("h2t", &slot);
@@ -554,7 +554,7 @@ int Solver::solve_step() {
             tic("assemble_c", &slot);
             launch_species_assemble(st_, d_, g_, f_, pr_, p_);
             toc("assemble_c", slot);
-            launch_block_correction(st_, d_, f_, pr_, f_.conc);
+            launch_block_correction(st_, d_, f_, pr_, f_.conc, shared_device_);
             if ((st = sweep(f_.conc, plan_dom_, 1, d_.nim2, 1, d_.njm2, 1, d_.nkm2, 0, "sweep_c"))) return st;
         }
         launch_finalize(st_, d_, pr_, sc_, p_, flow, p_.species, p_.steady, sc_host_->width);

@@ -96,7 +96,7 @@ def cpu_baseline(deck_params, dom, gpu_iters_per_step, budget_s=20.0):
                       "%.1f iterations/step" % (steps, iters, el, per_iter, gpu_iters_per_step)}
 
 
-def batch_leg(amcfd, decks, tmp, n, warmup, steps):
+def batch_leg(amcfd, decks, tmp, n, warmup, steps, workers):
     """Instances 0..n-1 of the seeded parameter sweep (uneven grids, 12.5 um
     cells, P/v/r drawn per instance) in the plain FP64 mode: advanced side by
     side with amcfd_run_batch, then the same instances one at a time over the
@@ -108,10 +108,10 @@ def batch_leg(amcfd, decks, tmp, n, warmup, steps):
     work = sum(cells) * steps
     print("batch leg: %d instances, %d cells" % (n, sum(cells)), file=sys.stderr, flush=True)
     side = [amcfd.Solver(d) for d in dks]
-    amcfd.run_batch(side, warmup)
+    amcfd.run_batch(side, warmup, workers)
     i0 = sum(s.scalars()["itertot"] for s in side)
     t0 = time.perf_counter()
-    amcfd.run_batch(side, steps)
+    amcfd.run_batch(side, steps, workers)
     tb = time.perf_counter() - t0
     iters = sum(s.scalars()["itertot"] for s in side) - i0
     print("batch leg: side by side %.3f s" % tb, file=sys.stderr, flush=True)
@@ -127,13 +127,14 @@ def batch_leg(amcfd, decks, tmp, n, warmup, steps):
     for s in side:
         s.close()
     print("batch leg: one at a time %.3f s" % ts, file=sys.stderr, flush=True)
-    return {"value": work / tb, "unit": "cell-steps/s", "instances": n, "interior_cells": sum(cells),
+    return {"value": work / tb, "unit": "cell-steps/s", "instances": n, "workers": workers,
+            "interior_cells": sum(cells),
             "warmup_steps": warmup, "timed_steps": steps, "iters_per_instance_step": iters / (n * steps),
             "one_at_a_time_value": work / ts, "speedup_vs_one_at_a_time": ts / tb,
             "identical_to_one_at_a_time": bool(same),
             "hw_queues": int(os.environ.get("GPU_MAX_HW_QUEUES", "4")),
             "note": "instances 0..n-1 of the seeded parameter sweep (numpy PCG64 seed 20261203+s), plain FP64 "
-                    "mode, each on its own stream and host thread (amcfd_run_batch)"}
+                    "mode, each on its own stream, taken largest first by the worker threads (amcfd_run_batch)"}
 
 
 def main():
@@ -146,9 +147,10 @@ def main():
                     help="fp64: reference program order, bit for bit; scan: parallel line solves")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-fast", action="store_true", help="skip the scan-mode leg")
-    ap.add_argument("--batch", type=int, default=16,
+    ap.add_argument("--batch", type=int, default=64,
                     help="instances of the seeded parameter sweep in the batched leg (0 skips it)")
-    ap.add_argument("--batch-warmup", type=int, default=8)
+    ap.add_argument("--batch-workers", type=int, default=16)
+    ap.add_argument("--batch-warmup", type=int, default=5)
     ap.add_argument("--batch-steps", type=int, default=3)
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1),
@@ -258,7 +260,7 @@ def main():
         fast.close()
     if args.mode == "fp64" and args.batch > 0:
         result.setdefault("fast_modes", {})["batch"] = batch_leg(amcfd, decks, tmp, args.batch, args.batch_warmup,
-                                                                 args.batch_steps)
+                                                                 args.batch_steps, args.batch_workers)
     result["cpu_baseline"] = None
     if not args.no_cpu:
         print("cpu baseline: reference harness, %d threads" % args.cpu_threads, file=sys.stderr, flush=True)

@@ -231,16 +231,18 @@ int  amcfd_free_surface(amcfd_solver *s, const char *monitor_path);
  * to <base>_ExtractedSection<layer>.txt; *written tells which. */
 int  amcfd_output_section(amcfd_solver *s, const char *base, int finish, int *written);
 
-/* Independent instances on the one GPU at once (a parameter sweep): each
+/* Independent instances on the one GPU at once (a parameter sweep): every
  * solver advances up to max_steps time steps (to its final time if
- * max_steps < 0) on its own host thread and stream, so their kernels and
- * line sweeps overlap on the device; every instance computes exactly what it
- * computes alone.  steps_done[i] receives the steps instance i took.  On an
- * error the first failing instance's status is returned.  The number of
- * streams the device runs side by side is the HIP runtime's hardware-queue
- * count (GPU_MAX_HW_QUEUES, 4 by default).  No reference counterpart: the
- * reference runs one deck per process. */
-int  amcfd_run_batch(amcfd_solver *const *s, int n, int max_steps, int *steps_done);
+ * max_steps < 0).  `workers` host threads (16 if <= 0, at most n) take the
+ * instances largest grid first, each running one instance at a time on that
+ * instance's own stream, so the instances' kernels and line sweeps overlap
+ * on the device; every instance computes exactly what it computes alone.
+ * steps_done[i] receives the steps instance i took.  On an error the first
+ * failing instance's status is returned.  The device runs as many streams
+ * side by side as the HIP runtime has hardware queues (GPU_MAX_HW_QUEUES, 4
+ * by default).  No reference counterpart: the reference runs one deck per
+ * process. */
+int  amcfd_run_batch(amcfd_solver *const *s, int n, int max_steps, int workers, int *steps_done);
 
 /* Thread-local description of the last error. */
 const char *amcfd_last_error(void);
