@@ -32,6 +32,12 @@ The named cases are the workloads the solver is measured and checked on:
                 carries the concentration fields too).
 * ``tiny_fs``   the tiny_out outputs plus the energy-based free surface (at
                 every output) and an extracted section every 20 us.
+* ``tiny_dtvar`` a *CONTROL_TIMESTEP_VARIABLE table: dt 5, 2.5, then 4 us.
+* ``tiny_heattable`` a *VOLUMETRIC_HEAT_SOURCE_TABLE whose row changes with time.
+* ``tiny_steady`` dt = 100 s: the reference's steady-state mode, two steps of
+                up to 150 iterations with the beam fixed at xstart, conduction
+                only (with the flow on, the reference's steady run turns NaN
+                on this grid).
 * ``plate120``  120x48x32 cells (2.0x0.6x0.4 mm), 400 steps, P=200 W,
                 r=50 um, 1 m/s: the CPU-runnable single-track case.
 * ``track400``  400x160x80 cells (4.0x1.2x0.6 mm, ~5.1M cells), P=300 W,
@@ -78,14 +84,25 @@ def write_deck(path, mesh_path, *, keyword="case", finaltime, maxit=100, tol=1e-
                momentum=(5e-5, -2.6e-4, 1e7, -1.0), material=None, evaporation=None,
                volumetric=None, outtime=None, scalar_out=("temp",), final_geometry=None,
                toolpath=None, powder=None, moving=None, species=None, solidification=False,
-               free_surface=None, section=None, extra=()):
+               free_surface=None, section=None, dt_table=None, heat_table=None, extra=()):
     """toolpath: rows of (t, x, y, z, laser_on, track) written to <deck>_tool.crs;
     powder: the 8 *POWDER_BED_MODE values; moving: the two 1-based zone
     indices of *ACTIVATE_MOVING_MESH; species: the 4 *SOLVE_SPECIES values
     (initial concentration, mass diffusivity, partition coefficient,
     relaxation); solidification: add *CALCULATE_SOLIDIFICATION_PARAMETERS;
     free_surface: the 8 *TRACK_ENERGY_BASED_FREE_SURFACE values; section: the
-    7 *OUTPUT_SECTION values (box min, box max, interval)."""
+    7 *OUTPUT_SECTION values (box min, box max, interval); dt_table: rows of
+    (t_until, dt, outtime) for *CONTROL_TIMESTEP_VARIABLE (<deck>_dt.k);
+    heat_table: rows of (t, P, eta, thickness, radius, factor) for
+    *VOLUMETRIC_HEAT_SOURCE_TABLE (<deck>_heat.k), replacing the fixed source."""
+    stem = os.path.splitext(os.path.abspath(path))[0]
+
+    def table(suffix, rows):
+        aux = stem + suffix
+        with open(aux, "w") as f:
+            for row in rows:
+                f.write(" ".join(_fmt(v) for v in row) + "\n")
+        return aux
     if material is None:
         material = (4000.0, 4000.0, 5e-3, TS, TL, HSMELT, HLFRIZ, ACPA, ACPB, ACPB,
                     0.0, 0.0, 25.0, 0.0, 25.0)
@@ -99,13 +116,18 @@ def write_deck(path, mesh_path, *, keyword="case", finaltime, maxit=100, tol=1e-
           "  %s %d %s" % (_fmt(finaltime), maxit, _fmt(tol))]
     L += ["*CONTROL_TIMESTEP", "$ sim_timestep  output_timestep",
           "  %s %s" % (_fmt(dt), _fmt(finaltime if outtime is None else outtime))]
+    if dt_table is not None:
+        L += ["*CONTROL_TIMESTEP_VARIABLE", table("_dt.k", dt_table)]
     L += ["*SINGLE_TRACK_TOOL", "$ lasertime  xstart  ystart  vel",
           "  %s %s %s %s" % tuple(_fmt(v) for v in (lasertime, xstart, ystart, speed))]
     L += ["*GAUSS_LASER_PARAMETERS", "$ power  absorb  radius  factor",
           "  %s %s %s %s" % tuple(_fmt(v) for v in (power, eta, radius, factor))]
-    L += ["*VOLUMETRIC_HEAT_SOURCE", "$ power apowseta heatthick heatrb factor",
-          "  0.0 0.0 1e-5 5e-5 2.0" if volumetric is None else
-          "  " + " ".join(_fmt(v) for v in volumetric)]
+    if heat_table is not None:
+        L += ["*VOLUMETRIC_HEAT_SOURCE_TABLE", table("_heat.k", heat_table)]
+    else:
+        L += ["*VOLUMETRIC_HEAT_SOURCE", "$ power apowseta heatthick heatrb factor",
+              "  0.0 0.0 1e-5 5e-5 2.0" if volumetric is None else
+              "  " + " ".join(_fmt(v) for v in volumetric)]
     m = material
     L += ["*MATERIAL_PROPERTIES", "$ dens denl viscos tsolid tliquid hsmelt hlfriz",
           "  " + " ".join(_fmt(v) for v in m[0:7]), "$ acpa acpb acpl",
@@ -142,7 +164,7 @@ def write_deck(path, mesh_path, *, keyword="case", finaltime, maxit=100, tol=1e-
     L += list(extra)
     L += ["*MESH_FILE", mesh_path]
     if toolpath is not None:
-        tool_path = os.path.splitext(os.path.abspath(path))[0] + "_tool.crs"
+        tool_path = stem + "_tool.crs"
         with open(tool_path, "w") as f:
             for row in toolpath:
                 f.write(" ".join(_fmt(v) for v in row[:4]) + " %d %d\n" % (row[4], row[5]))
@@ -269,6 +291,27 @@ CASES = {
                   outtime=2.5e-5, scalar_out=("temp", "tempr", "zr", "fracl"),
                   free_surface=(-1.0e6, 1.0e6, 1.8, 20, 1e-6, 30, 1e-16, 0.5),
                   section=(0.2e-3, 0.1e-3, 0.15e-3, 0.3e-3, 0.15e-3, 0.25e-3, 2e-5))),
+    "tiny_dtvar": dict(
+        mesh=(([0.1e-3, 0.3e-3, 0.1e-3], [4, 12, 4], [-1.2, 1.0, 1.2]),
+              ([0.15e-3, 0.15e-3], [6, 6], [1.0, 1.0]),
+              ([0.15e-3, 0.1e-3], [4, 8], [1.0, 1.0])),
+        deck=dict(finaltime=0.12e-3, maxit=60, dt=5e-6, lasertime=0.1e-3,
+                  xstart=0.15e-3, ystart=0.15e-3, speed=1.0, power=200.0, radius=50e-6,
+                  dt_table=[(3e-5, 5e-6, 1e-4), (6e-5, 2.5e-6, 1e-4), (1.0, 4e-6, 1e-4)])),
+    "tiny_heattable": dict(
+        mesh=(([0.1e-3, 0.3e-3, 0.1e-3], [4, 12, 4], [-1.2, 1.0, 1.2]),
+              ([0.15e-3, 0.15e-3], [6, 6], [1.0, 1.0]),
+              ([0.15e-3, 0.1e-3], [4, 8], [1.0, 1.0])),
+        deck=dict(finaltime=0.1e-3 - 1e-9, maxit=60, dt=5e-6, lasertime=0.1e-3,
+                  xstart=0.15e-3, ystart=0.15e-3, speed=1.0, power=120.0, radius=50e-6,
+                  heat_table=[(2e-5, 60.0, 0.3, 5e-5, 5e-5, 2.0), (5e-5, 120.0, 0.35, 6e-5, 4e-5, 2.0),
+                              (1.0, 80.0, 0.3, 4e-5, 5e-5, 2.0)])),
+    "tiny_steady": dict(
+        mesh=(([0.1e-3, 0.3e-3, 0.1e-3], [4, 12, 4], [-1.2, 1.0, 1.2]),
+              ([0.15e-3, 0.15e-3], [6, 6], [1.0, 1.0]),
+              ([0.15e-3, 0.1e-3], [4, 8], [1.0, 1.0])),
+        deck=dict(finaltime=200.0, maxit=150, dt=100.0, lasertime=0.1e-3,
+                  xstart=0.25e-3, ystart=0.15e-3, speed=1.0, power=120.0, radius=50e-6, flow=False)),
     "plate120": dict(
         mesh=(([0.5e-3, 1.0e-3, 0.5e-3], [20, 80, 20], [-1.2, 1.0, 1.2]),
               ([0.2e-3, 0.2e-3, 0.2e-3], [16, 16, 16], [1.0, 1.0, 1.0]),

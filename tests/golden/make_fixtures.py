@@ -10,7 +10,9 @@ call order, OMP_NUM_THREADS=1) on the decks of decks.py and stores, per case:
   <case>_fields.npz      full fields at a few steps (tiny cases only)
   <case>_deck.txt        the deck text (mesh path replaced by @MESH@)
   <case>_mesh.txt        the mesh text
-  <case>_tool.txt        the toolpath file, when the deck has one
+  <case>_tool.txt, _dt.txt, _heat.txt
+                         the toolpath, time-step and heat-source tables, when
+                         the deck names them
   <case>_grid_steps.txt  per step, the regenerated x and y node coordinates
                          (moving-mesh decks)
   <case>_FreeSurfaceMonitor.txt  the free-surface monitor (free-surface decks)
@@ -36,6 +38,8 @@ import decks  # noqa: E402
 HARNESS = os.path.join(ROOT, "oracle", "_ref", "ref_harness")
 FIELDS = ["enthalpy", "temp", "fracl", "uvel", "vvel", "wvel", "pressure", "pp", "rho", "diff",
           "vis", "hnot", "fraclnot", "unot", "vnot", "wnot", "sourceinput", "dux", "dvy", "dwz"]
+# table files a deck may name: suffix, placeholder in <case>_deck.txt, fixture kind
+AUX_FILES = [("_tool.crs", "@TOOL@", "tool"), ("_dt.k", "@DT@", "dt"), ("_heat.k", "@HEAT@", "heat")]
 # written by the harness only when the deck enables species / G/R
 OPTIONAL_FIELDS = ["avgconcentration", "avgconcentrationnot", "concentration", "grad", "rate", "zr", "tempr"]
 # case -> (fields_every, max_steps, steps whose full fields are kept)
@@ -52,6 +56,9 @@ PLAN = {
     "tiny_species": (1, -1, []),
     "tiny_move_species": (1, -1, []),
     "tiny_fs": (1, -1, []),
+    "tiny_dtvar": (1, -1, []),
+    "tiny_heattable": (1, -1, []),
+    "tiny_steady": (1, -1, []),
     "plate120": (50, -1, []),
 }
 
@@ -95,11 +102,13 @@ def run_case(case, workdir=None, reuse=False):
     if kept:
         np.savez_compressed(os.path.join(HERE, case + "_fields.npz"), **kept)
     mesh = os.path.abspath(os.path.join(tmp, case + "_mesh.k"))
-    tool = os.path.abspath(os.path.join(tmp, case + "_tool.crs"))
     with open(deck) as f:
-        text = f.read().replace(mesh, "@MESH@").replace(tool, "@TOOL@")
-    if os.path.exists(tool):
-        shutil.copy(tool, os.path.join(HERE, case + "_tool.txt"))
+        text = f.read().replace(mesh, "@MESH@")
+    for suffix, mark, kind in AUX_FILES:
+        aux = os.path.abspath(os.path.join(tmp, case + suffix))
+        text = text.replace(aux, mark)
+        if os.path.exists(aux):
+            shutil.copy(aux, os.path.join(HERE, case + "_" + kind + ".txt"))
     monitor = os.path.join(tmp, case + "_FreeSurfaceMonitor.txt")
     if os.path.exists(monitor):
         shutil.copy(monitor, os.path.join(HERE, case + "_FreeSurfaceMonitor.txt"))

@@ -12,7 +12,9 @@ import pytest
 from conftest import golden_path
 
 CASES = ["tiny", "tiny_cond", "tiny_evap", "tiny_vap", "tiny_vol", "tiny_tool", "tiny_powder",
-         "tiny_move", "tiny_move_powder", "tiny_species", "tiny_move_species", "tiny_fs"]
+         "tiny_move", "tiny_move_powder", "tiny_species", "tiny_move_species", "tiny_fs", "tiny_dtvar",
+         "tiny_heattable", "tiny_steady"]
+AUX_FILES = [("_tool.crs", "@TOOL@", "tool"), ("_dt.k", "@DT@", "dt"), ("_heat.k", "@HEAT@", "heat")]
 MOVING = ["tiny_move", "tiny_move_powder", "tiny_move_species"]
 INT_KEYS = ["niter", "itertot", "powerindicator", "istat", "iend", "jstat", "jend", "kstat", "jstart",
             "istart", "nk"]
@@ -31,17 +33,19 @@ def _same(a, b):
 def test_deck_text_matches_fixture(case, case_dir):
     path = case_dir(case)
     mesh = os.path.join(os.path.dirname(path), case + "_mesh.k")
-    tool = os.path.join(os.path.dirname(path), case + "_tool.crs")
     with open(path) as f:
-        text = f.read().replace(os.path.abspath(mesh), "@MESH@").replace(os.path.abspath(tool), "@TOOL@")
+        text = f.read().replace(os.path.abspath(mesh), "@MESH@")
+    for suffix, mark, kind in AUX_FILES:
+        aux = os.path.join(os.path.dirname(path), case + suffix)
+        text = text.replace(os.path.abspath(aux), mark)
+        assert os.path.exists(aux) == os.path.exists(golden_path(case + "_" + kind + ".txt")), aux
+        if os.path.exists(aux):
+            with open(aux) as f, open(golden_path(case + "_" + kind + ".txt")) as g:
+                assert f.read() == g.read()
     with open(golden_path(case + "_deck.txt")) as f:
         assert text == f.read()
     with open(mesh) as f, open(golden_path(case + "_mesh.txt")) as g:
         assert f.read() == g.read()
-    assert os.path.exists(tool) == os.path.exists(golden_path(case + "_tool.txt"))
-    if os.path.exists(tool):
-        with open(tool) as f, open(golden_path(case + "_tool.txt")) as g:
-            assert f.read() == g.read()
 
 
 @pytest.mark.parametrize("case", CASES)
