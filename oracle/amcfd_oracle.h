@@ -10,12 +10,20 @@
  *
  * Parity pinning (what checks each function):
  *   every function on the path  - per-step full-field golden fixtures written
- *                                 by the compiled reference (tests/golden/)
+ *                                 by the compiled reference (tests/golden/):
+ *                                 bare plate and powder bed, single track and
+ *                                 toolpath, variable time steps, heat-source
+ *                                 tables, steady mode, moving mesh (with the
+ *                                 per-step grids), species and G/R, the
+ *                                 energy-based free surface (with its monitor)
  *   grid                        - closed form (uniform zones, zone ends)
  *   H->T                        - inverse of h(T) = acpa/2 T^2 + acpb T
  *   power law                   - closed form, zero for |P| >= 10
  *   line TDMA                   - dense solve on small random lines
  *   block correction            - column residual sums vanish after it
+ *   exponential scheme          - parity unpinned: pe/(exp(pe)-1) is 0/0 at
+ *                                 pe = 0, so every deck using it is NaN in the
+ *                                 reference and no fixture can pin it
  */
 #ifndef AMCFD_ORACLE_H
 #define AMCFD_ORACLE_H
