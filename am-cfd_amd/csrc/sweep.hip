@@ -35,6 +35,7 @@
 
 #include <algorithm>
 
+#include "div_pos.h"
 #include "fv.h"
 
 namespace amcfd {
@@ -63,22 +64,6 @@ __device__ __forceinline__ double lane_bcast(double v, int lane) {
     return __hiloint2double(hi, lo);
 }
 
-// Correctly rounded x / b for b > 0, given y = RN(1/b): a first quotient and
-// two exact-remainder corrections (Markstein's theorem: with y the rounded
-// reciprocal and q within an ulp of x/b, RN(q + (x - b q) y) = RN(x / b); the
-// first correction brings x*y within an ulp).  The remainders are formed
-// negated, b q - x, which for b > 0 leaves a zero x with the sign IEEE
-// division gives it, so no select is needed.  Valid when b is positive and
-// finite and x is zero or finite with |x| >= 2^-900 (the remainders are then
-// exact); the caller checks that after the pass.  Five dependent operations
-// against the eleven of a full division.
-__device__ __forceinline__ double div_pos(double x, double b, double y) {
-    const double q0 = x * y;
-    const double e0 = __fma_rn(b, q0, -x);
-    const double q1 = __fma_rn(-e0, y, q0);
-    const double e1 = __fma_rn(b, q1, -x);
-    return __fma_rn(-e1, y, q1);
-}
 
 // Per-line pivots of the reference recurrence.  den[i] = ap - aw*pr[i-1]
 // (plus the melt-pool guard) and pr[i] = ae/den[i] depend only on the
@@ -208,15 +193,16 @@ struct LineRegs {
             }
             q = lane_bcast(q, L);
         }
-        // the fast division's preconditions, for every row at once (bitwise
-        // ops, no branches); a line that misses one is solved again with IEEE
-        // division
+        // the fast division's preconditions and its rounding, for every row
+        // at once (no branches); a line that misses one is solved again with
+        // IEEE division
         int bad = 0;
 #pragma unroll
         for (int r = 0; r < M; ++r) {
             const double ax = fabs(rx[r]);
             bad |= (int)!(rden[r] > 0.0) | (int)!(rden[r] < __builtin_inf()) |
-                   ((int)(ax != 0.0) & ((int)!(ax >= 0x1p-900) | (int)!(ax < __builtin_inf())));
+                   ((int)(ax != 0.0) & ((int)!(ax >= 0x1p-900) | (int)!(ax < __builtin_inf()))) |
+                   (int)!div_pos_exact(rx[r], rden[r], ry[r], rd[r]);
         }
         if (__any(bad)) {
             q = lo;
