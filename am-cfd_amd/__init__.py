@@ -276,7 +276,9 @@ class Solver:
         return t.value, d.value
 
     def set_profiling(self, on=True):
-        _check(load_library().amcfd_set_profiling(self._h, int(bool(on))))
+        """Time the kernel families with HIP events: True/1 every family,
+        2 the whole-domain enthalpy sweep alone, False/0 none."""
+        _check(load_library().amcfd_set_profiling(self._h, 2 if on == 2 else int(bool(on))))
 
     def kernel_stats(self, kernel):
         n, ms = ctypes.c_long(), ctypes.c_double()

@@ -210,6 +210,7 @@ int amcfd_last_timing(amcfd_solver *s, double *total_ms, double *tdma_ms) {
 
 int amcfd_set_profiling(amcfd_solver *s, int on) {
     if (!s) return amcfd::set_error(AMCFD_E_ARG, "null solver");
+    if (on < 0 || on > 2) return amcfd::set_error(AMCFD_E_ARG, "profiling level must be 0, 1 or 2");
     s->s.set_profiling(on);
     return AMCFD_OK;
 }

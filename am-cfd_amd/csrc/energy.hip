@@ -104,6 +104,19 @@ struct AsmConst {
     int pi, scheme;
 };
 
+// Tiles of 64 (i) x 4 (j) columns; each thread marches KA planes up k and
+// carries the (k-1, k, k+1) values of the fields read at k-neighbours in
+// registers, so every input array is read about once.  Workgroups are
+// dispatched to the 8 XCDs round-robin by linear id; the tiles are
+// renumbered so that each XCD takes one contiguous run of them (x, then y,
+// then z), which keeps the j-neighbour rows of a tile in that XCD's L2.
+constexpr int KA = 8;
+
+__device__ __forceinline__ int xcd_tile(int b, int nb) {
+    const int q = nb / 8, r = nb % 8, x = b % 8, idx = b / 8;
+    return (x < r) ? x * (q + 1) + idx : r * (q + 1) + (x - r) * q + idx;
+}
+
 __global__ __launch_bounds__(TX *TY) void k_energy_assemble(
     Dims d, AsmConst q, DevGeom g, const double *__restrict__ uvel, const double *__restrict__ vvel,
     const double *__restrict__ wvel, const double *__restrict__ rho, const double *__restrict__ diff,
@@ -113,9 +126,14 @@ __global__ __launch_bounds__(TX *TY) void k_energy_assemble(
     double *term_blc) {
     __shared__ double scratch[2 * 4];
     const int tx = threadIdx.x, ty = threadIdx.y;
-    const int i = 1 + blockIdx.x * TX + tx;
-    const int j = 1 + blockIdx.y * TY + ty;
-    const bool col_ok = (i <= d.nim2) && (j <= d.njm2);
+    const int gx = gridDim.x, gy = gridDim.y;
+    const int nb = gx * gy * gridDim.z;
+    const int t = xcd_tile(blockIdx.x + gx * (blockIdx.y + gy * blockIdx.z), nb);
+    const int bx = t % gx, by = (t / gx) % gy, bz = t / (gx * gy);
+    const int i = 1 + bx * TX + tx;
+    const int j = 1 + by * TY + ty;
+    const int k0 = 1 + bz * KA;
+    const bool col_ok = (i <= d.nim2) && (j <= d.njm2) && (k0 <= d.nkm2);
     double sumd = 0.0, sumh = 0.0;
     const int sch = q.scheme;
     const double ztop = g.z[d.nkm1];
@@ -124,29 +142,39 @@ __global__ __launch_bounds__(TX *TY) void k_energy_assemble(
         const double dxc = g.xu[i + 1] - g.xu[i], dyc = g.yv[j + 1] - g.yv[j];
         const double aij = g.areaij[j * d.ni + i];
         const double xb = q.beamx - g.x[i], yb = q.beamy - g.y[j];
-        for (int kk = 0; kk < KB; ++kk) {
-            const int k = 1 + blockIdx.z * KB + kk;
-            if (k > d.nkm2) break;
-            const size_t c = cix(d, i, j, k);
+        const int kend = min(k0 + KA - 1, d.nkm2);
+        // values at k-neighbours, carried from plane to plane
+        size_t c = cix(d, i, j, k0);
+        double dB = diff[c - d.sk], dP = diff[c];
+        double rP = rho[c], wb = wvel[c];
+        double flB = fracl[c - d.sk], flP = fracl[c];
+        double hB = h[c - d.sk], hP = h[c];
+        for (int k = k0; k <= kend; ++k, c += d.sk) {
+            const double dT = diff[c + d.sk], rT = rho[c + d.sk], wt = wvel[c + d.sk];
+            const double flT = fracl[c + d.sk], hT = h[c + d.sk];
+            // same-plane neighbours, all inside the ghost layer, read up front
+            const double dN = diff[c + d.sj], dS = diff[c - d.sj], dE = diff[c + 1], dW = diff[c - 1];
+            const double rN = rho[c + d.sj], rE = rho[c + 1];
+            const double vs = vvel[c], vn = vvel[c + d.sj], uw = uvel[c], ue = uvel[c + 1];
+            const double flE = fracl[c + 1], flW = fracl[c - 1], flN = fracl[c + d.sj], flS = fracl[c - d.sj];
+            const double hN = h[c + d.sj], hS = h[c - d.sj], hE = h[c + 1], hW = h[c - 1];
+            const double hnP = hnot[c], flnP = fraclnot[c];
             const double aik = g.areaik[k * d.ni + i], ajk = g.areajk[k * d.nj + j];
-            const double dP = diff[c], rP = rho[c];
             const double vol = dxc * dyc * (g.zw[k + 1] - g.zw[k]);
             double an, as, ae, aw, at, ab;
             // north face
             if (j < d.njm2) {
-                double dN = diff[c + d.sj];
-                double fn = rho[c + d.sj] * vvel[c + d.sj] * aik;
+                double fn = rN * vn * aik;
                 double dif = dP * dN / ((1.0 - fyj) * dN + fyj * dP);
                 an = coef_minus(sch, dif * aik * g.dypsinv[j + 1], fn);
             } else {
-                double fn = rho[c + d.sj] * vvel[c + d.sj] * aik;
+                double fn = rN * vn * aik;
                 an = coef_minus(sch, dP * aik * g.dypsinv[d.njm1], fn);
             }
             // south face
             {
-                double fs = rP * vvel[c] * aik;
+                double fs = rP * vs * aik;
                 if (j > 1) {
-                    double dS = diff[c - d.sj];
                     double dif = dS * dP / ((1.0 - fys) * dP + fys * dS);
                     as = coef_plus(sch, dif * aik * g.dypsinv[j], fs);
                 } else {
@@ -155,19 +183,17 @@ __global__ __launch_bounds__(TX *TY) void k_energy_assemble(
             }
             // east face
             if (i < d.nim2) {
-                double dE = diff[c + 1];
-                double fe = rho[c + 1] * uvel[c + 1] * ajk;
+                double fe = rE * ue * ajk;
                 double dif = dP * dE / ((1.0 - fxi) * dE + fxi * dP);
                 ae = coef_minus(sch, dif * ajk * g.dxpwinv[i + 1], fe);
             } else {
-                double fe = rho[c + 1] * uvel[c + 1] * ajk;
+                double fe = rE * ue * ajk;
                 ae = coef_minus(sch, dP * ajk * g.dxpwinv[d.nim1], fe);
             }
             // west face
             {
-                double fw = rP * uvel[c] * ajk;
+                double fw = rP * uw * ajk;
                 if (i > 1) {
-                    double dW = diff[c - 1];
                     double dif = dW * dP / ((1.0 - fxw) * dP + fxw * dW);
                     aw = coef_plus(sch, dif * ajk * g.dxpwinv[i], fw);
                 } else {
@@ -176,20 +202,18 @@ __global__ __launch_bounds__(TX *TY) void k_energy_assemble(
             }
             // top face
             if (k < d.nkm2) {
-                double dT = diff[c + d.sk];
-                double ft = rho[c + d.sk] * wvel[c + d.sk] * aij;
+                double ft = rT * wt * aij;
                 double fz = g.fracz[k];
                 double dif = dP * dT / ((1.0 - fz) * dT + fz * dP);
                 at = coef_minus(sch, dif * aij * g.dzpbinv[k + 1], ft);
             } else {
-                double ft = rho[c + d.sk] * wvel[c + d.sk] * aij;
+                double ft = rT * wt * aij;
                 at = coef_minus(sch, dP * aij * g.dzpbinv[d.nkm1], ft);
             }
             // bottom face
             {
-                double fb = rP * wvel[c] * aij;
+                double fb = rP * wb * aij;
                 if (k > 1) {
-                    double dB = diff[c - d.sk];
                     double fz = g.fracz[k - 1];
                     double dif = dB * dP / ((1.0 - fz) * dP + fz * dB);
                     ab = coef_plus(sch, dif * aij * g.dzpbinv[k], fb);
@@ -199,7 +223,7 @@ __global__ __launch_bounds__(TX *TY) void k_energy_assemble(
             }
             const double apnot = rP / q.delt * vol;
             double sp = 0.0;
-            double su = apnot * hnot[c];
+            double su = apnot * hnP;
             // volumetric heat source
             double src = 0.0;
             if (ztop - g.z[k] <= q.heatthick) {
@@ -208,24 +232,18 @@ __global__ __launch_bounds__(TX *TY) void k_energy_assemble(
                 su += vol * src;
             }
             // latent heat: transient and advection terms
-            const double flP = fracl[c];
             double v1 = q.hlatnt * rP / q.delt;
             double volht = vol * v1;
-            su -= volht * (flP - fraclnot[c]);
+            su -= volht * (flP - flnP);
             {
-                double uw = uvel[c], ue = uvel[c + 1], vs = vvel[c], vn = vvel[c + d.sj];
-                double wb = wvel[c], wt = wvel[c + d.sk];
-                double flew = ajk * (dmax(uw, 0.0) * fracl[c - 1] - dmax(-uw, 0.0) * flP +
-                                     dmax(-ue, 0.0) * fracl[c + 1] - dmax(ue, 0.0) * flP);
-                double flns = aik * (dmax(vs, 0.0) * fracl[c - d.sj] - dmax(-vs, 0.0) * flP +
-                                     dmax(-vn, 0.0) * fracl[c + d.sj] - dmax(vn, 0.0) * flP);
-                double fltb = aij * (dmax(wb, 0.0) * fracl[c - d.sk] - dmax(-wb, 0.0) * flP +
-                                     dmax(-wt, 0.0) * fracl[c + d.sk] - dmax(wt, 0.0) * flP);
+                double flew = ajk * (dmax(uw, 0.0) * flW - dmax(-uw, 0.0) * flP +
+                                     dmax(-ue, 0.0) * flE - dmax(ue, 0.0) * flP);
+                double flns = aik * (dmax(vs, 0.0) * flS - dmax(-vs, 0.0) * flP +
+                                     dmax(-vn, 0.0) * flN - dmax(vn, 0.0) * flP);
+                double fltb = aij * (dmax(wb, 0.0) * flB - dmax(-wb, 0.0) * flP +
+                                     dmax(-wt, 0.0) * flT - dmax(wt, 0.0) * flP);
                 su += rP * q.hlatnt * (flew + flns + fltb);
             }
-            const double hP = h[c];
-            const double hN = h[c + d.sj], hS = h[c - d.sj], hE = h[c + 1], hW = h[c - 1];
-            const double hT = h[c + d.sk], hB = h[c - d.sk];
             // fixed boundary values folded into the source (bottom, top, south, north, west, east)
             if (k == 1) { su += ab * hB; sp -= ab; ab = 0.0; }
             if (k == d.nkm2) { su += at * hT; sp -= at; at = 0.0; }
@@ -246,14 +264,18 @@ __global__ __launch_bounds__(TX *TY) void k_energy_assemble(
             // x-column in the reference's (k, j) order by k_column_sums)
             term_bl[c] = ap - an - as - at - ab;
             term_blc[c] = ae * hE + aw * hW + an * hN + as * hS + at * hT + ab * hB + su - ap * hP;
+            // the next plane's bottom and centre values
+            dB = dP; dP = dT;
+            rP = rT; wb = wt;
+            flB = flP; flP = flT;
+            hB = hP; hP = hT;
         }
     }
     double v[2] = {sumd, sumh};
     block_sum<2>(v, scratch);   // contains __syncthreads
     if (tx == 0 && ty == 0) {
-        const int b = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
-        part_res[2 * b] = v[0];
-        part_res[2 * b + 1] = v[1];
+        part_res[2 * t] = v[0];
+        part_res[2 * t + 1] = v[1];
     }
 }
 
@@ -675,7 +697,7 @@ __global__ __launch_bounds__(1024) void k_melt_pool(Dims d, int nblk, const doub
 // end-of-iteration scalars: residual folds, heat balance, energy ratio
 // (CFDSolverManager.cpp:1989-2071, BoundCondManager.cpp:846-857)
 // ------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_finalize(int nblk_t, int nblk_b, int nblk_f, int nblk_e,
+__global__ __launch_bounds__(256) void k_finalize(int nblk_t, int nblk_b, int nblk_f, int nblk_e, int nblk_c,
                                                   const double *__restrict__ part_t,
                                                   const double *__restrict__ part_b,
                                                   const double *__restrict__ part_f,
@@ -699,7 +721,7 @@ __global__ __launch_bounds__(256) void k_finalize(int nblk_t, int nblk_b, int nb
         a[10] += part_e[2 * b + 1];
     }
     if (species_done)
-        for (int b = tid; b < nblk_e; b += nt) {
+        for (int b = tid; b < nblk_c; b += nt) {
             a[15] += part_c[2 * b];
             a[16] += part_c[2 * b + 1];
         }
@@ -853,11 +875,15 @@ static dim3 energy_grid(const Dims &d) {
     return dim3((d.nim2 + TX - 1) / TX, (d.njm2 + TY - 1) / TY, (d.nkm2 + KB - 1) / KB);
 }
 
+static dim3 assemble_grid(const Dims &d) {
+    return dim3((d.nim2 + TX - 1) / TX, (d.njm2 + TY - 1) / TY, (d.nkm2 + KA - 1) / KA);
+}
+
 void launch_energy_assemble(hipStream_t st, const Dims &d, const DevGeom &g, DevFields f, Partials pr,
                             const amcfd_params &p, const SourceParams &s) {
     AsmConst q{p.delt, p.hlatnt, p.urfh, s.avolpow, s.apowseta, s.heatthick, s.heatrb, s.avolfact,
                s.beamposx, s.beamposy, s.powerindicator, p.discretization};
-    hipLaunchKernelGGL(k_energy_assemble, energy_grid(d), dim3(TX, TY), 0, st, d, q, g, f.uvel, f.vvel, f.wvel,
+    hipLaunchKernelGGL(k_energy_assemble, assemble_grid(d), dim3(TX, TY), 0, st, d, q, g, f.uvel, f.vvel, f.wvel,
                        f.rho, f.diff, f.hnot, f.fracl, f.fraclnot, f.enthalpy, f.an, f.as, f.ae, f.aw, f.at,
                        f.ab, f.ap, f.su, f.sourceinput, pr.energy, f.sp, f.apnot);
 }
@@ -904,9 +930,9 @@ void launch_finalize(hipStream_t st, const Dims &d, Partials pr, DevScalars *sc,
     dim3 gr = node_grid(d);
     int nblk_t = gr.x * gr.y * gr.z;
     const int nblk_b = (d.njm2 * d.nim2 + d.nkm2 * d.njm2 + d.nkm2 * d.nim2 + 255) / 256;
-    const dim3 e = energy_grid(d);
+    const dim3 e = assemble_grid(d), c = energy_grid(d);   // enthalpy and species assembly partials
     hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, st, nblk_t, nblk_b, pr.nblk_f, (int)(e.x * e.y * e.z),
-                       pr.h2t, pr.bc, pr.flow, pr.energy, pr.species, flow_done, species_done, steady, p.dens,
+                       (int)(c.x * c.y * c.z), pr.h2t, pr.bc, pr.flow, pr.energy, pr.species, flow_done, species_done, steady, p.dens,
                        width, sc);
 }
 
@@ -919,8 +945,10 @@ void launch_end_step(hipStream_t st, const Dims &d, DevFields f, const amcfd_par
 }
 
 int energy_partial_sizes(const Dims &d, int *nblk_e, int *nrow_e, int *nblk_t) {
-    dim3 e = energy_grid(d), t = node_grid(d);
-    *nblk_e = e.x * e.y * e.z;
+    // sized for the larger of the enthalpy (assemble_grid) and species
+    // (energy_grid) assembly partials
+    dim3 e = energy_grid(d), t = node_grid(d), a = assemble_grid(d);
+    *nblk_e = std::max(e.x * e.y * e.z, a.x * a.y * a.z);
     *nrow_e = e.y * e.z;
     *nblk_t = t.x * t.y * t.z;
     return 0;

@@ -402,9 +402,9 @@ SourceParams Solver::source_params() {
 // profiling helpers (events on the solver stream)
 // ---------------------------------------------------------------------------
 void Solver::tic(const char *name, int *slot) {
-    (void)name;
     *slot = -1;
     if (!profile_) return;
+    if (profile_ == 2 && std::strcmp(name, "sweep_h") != 0) return;   // the enthalpy sweep alone
     if (ev_next_ + 2 > ev_pool_.size()) {
         for (int a = 0; a < 64; ++a) {
             hipEvent_t e;

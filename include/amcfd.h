@@ -193,7 +193,9 @@ int  amcfd_dims(amcfd_solver *s, int *ni, int *nj, int *nk);
 int  amcfd_last_timing(amcfd_solver *s, double *total_ms, double *tdma_ms);
 /* Record HIP events around the kernel families ("sweep_h", "sweep_u",
  * "sweep_v", "sweep_w", "sweep_p", "assemble_h", "h2t") and report the
- * accumulated launches and device milliseconds.  No reference counterpart. */
+ * accumulated launches and device milliseconds: on = 1 times every family,
+ * 2 the whole-domain enthalpy sweep ("sweep_h") alone, 0 none.  No reference
+ * counterpart. */
 int  amcfd_set_profiling(amcfd_solver *s, int on);
 int  amcfd_kernel_stats(amcfd_solver *s, const char *kernel, long *launches, double *total_ms);
 

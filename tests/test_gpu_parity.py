@@ -172,6 +172,28 @@ def test_gpu_runs_are_bitwise_reproducible(case_dir, amcfd):
         assert np.array_equal(out[0][f], out[1][f]), f
 
 
+def test_profiling_levels(case_dir, amcfd):
+    """Level 2 times the enthalpy sweep alone, level 1 every family; the
+    timers do not change the result."""
+    path = case_dir("tiny")
+    ref = amcfd.Solver(path)
+    ref.run(max_steps=3)
+    s = amcfd.Solver(path)
+    s.set_profiling(2)
+    s.run(max_steps=2)
+    n_h, ms_h = s.kernel_stats("sweep_h")
+    assert n_h == s.scalars()["itertot"] and ms_h > 0.0
+    assert s.kernel_stats("assemble_h")[0] == 0 and s.kernel_stats("sweep_u")[0] == 0
+    s.set_profiling(1)
+    s.step()
+    assert s.kernel_stats("assemble_h")[0] == s.scalars()["itertot"] - n_h
+    assert s.kernel_stats("sweep_h")[0] == s.scalars()["itertot"]
+    assert np.array_equal(s.field("enthalpy"), ref.field("enthalpy"))
+    assert amcfd.load_library().amcfd_set_profiling(s._h, 3) == -1   # AMCFD_E_ARG
+    s.close()
+    ref.close()
+
+
 def test_set_field_roundtrip(case_dir, amcfd):
     s = amcfd.Solver(case_dir("tiny"))
     h = s.field("enthalpy")
