@@ -25,8 +25,12 @@ import time
 # the batched leg runs one stream per instance; HIP maps streams onto
 # GPU_MAX_HW_QUEUES hardware queues (4 by default), raised here to 32 before
 # anything initialises HIP; with 64 instances, 32 workers on 32 queues move
-# about 15% more cell-iterations per second than 16 on 16
-if int(os.environ.get("GPU_MAX_HW_QUEUES", "4") or 4) < 32:
+# about 15% more cell-iterations per second than 16 on 16.  The batched
+# leg's one-at-a-time comparison runs first, in a child process with the
+# default 4: while any process holds 32 queues, single-stream runs of the
+# small instances take about 1.7x as long (2.0 against 3.3 M cell-steps/s).
+SOLO_CHILD = "--batch-solo-child" in sys.argv
+if not SOLO_CHILD and int(os.environ.get("GPU_MAX_HW_QUEUES", "4") or 4) < 32:
     os.environ["GPU_MAX_HW_QUEUES"] = "32"
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
@@ -97,17 +101,43 @@ def cpu_baseline(deck_params, dom, gpu_iters_per_step, budget_s=20.0):
                       "%.1f iterations/step" % (steps, iters, el, per_iter, gpu_iters_per_step)}
 
 
+def temp_digest(solver):
+    import hashlib
+    return hashlib.sha256(solver.field("temp").tobytes()).hexdigest()
+
+
+def batch_solo(amcfd, decks, tmp, n, warmup, steps):
+    """Instances 0..n-1 one at a time: seconds over the timed steps and, per
+    instance, its total iteration count and a digest of its temperature field."""
+    ts, runs = 0.0, []
+    for s in range(n):
+        b = amcfd.Solver(amcfd.Deck(decks.write_sweep_instance(s, tmp)))
+        b.run(max_steps=warmup)
+        t0 = time.perf_counter()
+        b.run(max_steps=steps)
+        ts += time.perf_counter() - t0
+        runs.append([b.scalars()["itertot"], temp_digest(b)])
+        b.close()
+    return {"seconds": ts, "runs": runs}
+
+
 def batch_leg(amcfd, decks, tmp, n, warmup, steps, workers):
     """Instances 0..n-1 of the seeded parameter sweep (uneven grids, 12.5 um
-    cells, P/v/r drawn per instance) in the plain FP64 mode: advanced side by
-    side with amcfd_run_batch, then the same instances one at a time over the
-    same steps.  Both runs compute the same bits; the ratio is the gain from
-    sharing the device."""
-    import numpy as np
+    cells, P/v/r drawn per instance) in the plain FP64 mode: one at a time in
+    a child process with the default hardware-queue count, then advanced side
+    by side with amcfd_run_batch over the same steps.  Both runs compute the
+    same bits; the ratio is the gain from sharing the device."""
     dks = [amcfd.Deck(decks.write_sweep_instance(s, tmp)) for s in range(n)]
     cells = [(d.params.ni - 2) * (d.params.nj - 2) * (d.params.nk - 2) for d in dks]
     work = sum(cells) * steps
     print("batch leg: %d instances, %d cells" % (n, sum(cells)), file=sys.stderr, flush=True)
+    env = dict(os.environ, GPU_MAX_HW_QUEUES="4")
+    cmd = [sys.executable, os.path.abspath(__file__), "--batch-solo-child", "--batch", str(n),
+           "--batch-warmup", str(warmup), "--batch-steps", str(steps)]
+    out = subprocess.run(cmd, env=env, stdout=subprocess.PIPE, check=True, text=True, timeout=900).stdout
+    solo = json.loads(out.strip().splitlines()[-1])
+    ts = solo["seconds"]
+    print("batch leg: one at a time %.3f s" % ts, file=sys.stderr, flush=True)
     side = [amcfd.Solver(d) for d in dks]
     amcfd.run_batch(side, warmup, workers)
     i0 = sum(s.scalars()["itertot"] for s in side)
@@ -116,29 +146,20 @@ def batch_leg(amcfd, decks, tmp, n, warmup, steps, workers):
     tb = time.perf_counter() - t0
     iters = sum(s.scalars()["itertot"] for s in side) - i0
     print("batch leg: side by side %.3f s" % tb, file=sys.stderr, flush=True)
-    # the side-by-side solvers (and their streams) are released before the
-    # one-at-a-time runs, so those run on an otherwise idle device
-    done = [(s.scalars()["itertot"], s.field("temp")) for s in side]
+    done = [[s.scalars()["itertot"], temp_digest(s)] for s in side]
     for s in side:
         s.close()
-    ts, same = 0.0, True
-    for d, (itot, temp) in zip(dks, done):
-        b = amcfd.Solver(d)
-        b.run(max_steps=warmup)
-        t0 = time.perf_counter()
-        b.run(max_steps=steps)
-        ts += time.perf_counter() - t0
-        same &= b.scalars()["itertot"] == itot and np.array_equal(b.field("temp"), temp)
-        b.close()
-    print("batch leg: one at a time %.3f s" % ts, file=sys.stderr, flush=True)
+    same = solo["runs"] == done
     return {"value": work / tb, "unit": "cell-steps/s", "instances": n, "workers": workers,
             "interior_cells": sum(cells),
             "warmup_steps": warmup, "timed_steps": steps, "iters_per_instance_step": iters / (n * steps),
             "one_at_a_time_value": work / ts, "speedup_vs_one_at_a_time": ts / tb,
             "identical_to_one_at_a_time": bool(same),
-            "hw_queues": int(os.environ.get("GPU_MAX_HW_QUEUES", "4")),
+            "hw_queues": int(os.environ.get("GPU_MAX_HW_QUEUES", "4")), "one_at_a_time_hw_queues": 4,
             "note": "instances 0..n-1 of the seeded parameter sweep (numpy PCG64 seed 20261203+s), plain FP64 "
-                    "mode, each on its own stream, taken largest first by the worker threads (amcfd_run_batch)"}
+                    "mode, each on its own stream, taken largest first by the worker threads (amcfd_run_batch); "
+                    "one at a time beforehand in a child process, same steps, compared by iteration count and a "
+                    "digest of the temperature field"}
 
 
 def main():
@@ -162,7 +183,15 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=2, help="time steps in the CPU baseline sample")
     ap.add_argument("--copy-probe-mb", type=float, default=0.0,
                     help="only run one device copy of this size (counter calibration)")
+    ap.add_argument("--batch-solo-child", action="store_true",
+                    help="internal: run the batched leg's one-at-a-time comparison and print it as JSON")
     args = ap.parse_args()
+    if args.batch_solo_child:
+        import decks
+        amcfd = importlib.import_module("am-cfd_amd")
+        print(json.dumps(batch_solo(amcfd, decks, tempfile.mkdtemp(prefix="amcfd_solo_"), args.batch,
+                                    args.batch_warmup, args.batch_steps)))
+        return
     if args.gpus != 1:
         print("bench.py measures one GPU: the solve is a single instance whose line sweeps are "
               "Gauss-Seidel ordered across the whole grid; multi-GPU runs are not part of this build.",
