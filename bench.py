@@ -122,6 +122,7 @@ def batch_solo(amcfd, decks, tmp, n, warmup, steps):
 
 
 def batch_leg(amcfd, decks, tmp, n, warmup, steps, workers):
+    import numpy as np
     """Instances 0..n-1 of the seeded parameter sweep (uneven grids, 12.5 um
     cells, P/v/r drawn per instance) in the plain FP64 mode: one at a time in
     a child process with the default hardware-queue count, then advanced side
@@ -147,15 +148,53 @@ def batch_leg(amcfd, decks, tmp, n, warmup, steps, workers):
     iters = sum(s.scalars()["itertot"] for s in side) - i0
     print("batch leg: side by side %.3f s" % tb, file=sys.stderr, flush=True)
     done = [[s.scalars()["itertot"], temp_digest(s)] for s in side]
+    ref = [(s.field("temp"), s.scalars()) for s in side]
     for s in side:
         s.close()
     same = solo["runs"] == done
+    # the same instances and steps in the scan mode, side by side, measured
+    # against the FP64 results
+    scan = [amcfd.Solver(d, mode=amcfd.MODE_SCAN) for d in dks]
+    amcfd.run_batch(scan, warmup, workers)
+    t0 = time.perf_counter()
+    amcfd.run_batch(scan, steps, workers)
+    tsc = time.perf_counter() - t0
+    print("batch leg: scan mode side by side %.3f s" % tsc, file=sys.stderr, flush=True)
+    def drift(solvers):
+        l2, dims = 0.0, 0.0
+        for s, (t64, s64) in zip(solvers, ref):
+            t = s.field("temp")
+            l2 = max(l2, float(np.linalg.norm(t - t64) / np.linalg.norm(t64)))
+            for key in ("alen", "depth", "width"):
+                if s64[key] > 0:
+                    dims = max(dims, abs(s.scalars()[key] - s64[key]) / s64[key])
+            s.close()
+        return l2, dims
+    l2, dims = drift(scan)
+    # the floor: FP64 runs whose enthalpy is moved up by one ulp after the
+    # warm-up steps, when the melt pool has formed (a one-ulp change of the
+    # uniform initial field is mostly rounded away before it can grow)
+    ulp = [amcfd.Solver(d) for d in dks]
+    amcfd.run_batch(ulp, warmup, workers)
+    for s in ulp:
+        s.set_field("enthalpy", np.nextafter(s.field("enthalpy"), np.inf))
+    amcfd.run_batch(ulp, steps, workers)
+    l2u, dimsu = drift(ulp)
     return {"value": work / tb, "unit": "cell-steps/s", "instances": n, "workers": workers,
             "interior_cells": sum(cells),
             "warmup_steps": warmup, "timed_steps": steps, "iters_per_instance_step": iters / (n * steps),
             "one_at_a_time_value": work / ts, "speedup_vs_one_at_a_time": ts / tb,
             "identical_to_one_at_a_time": bool(same),
             "hw_queues": int(os.environ.get("GPU_MAX_HW_QUEUES", "4")), "one_at_a_time_hw_queues": 4,
+            "scan_mode": {"value": work / tsc, "speedup_vs_fp64_one_at_a_time": ts / tsc,
+                          "max_temp_rel_l2_vs_fp64": l2, "max_melt_pool_dim_rel_diff_vs_fp64": dims,
+                          "fp64_one_ulp_floor": {"max_temp_rel_l2_vs_fp64": l2u,
+                                                 "max_melt_pool_dim_rel_diff_vs_fp64": dimsu},
+                          "note": "the same instances and steps with the x-line solves by parallel scans, side by "
+                                  "side; temperature and melt-pool length/depth/width against the FP64 side-by-side "
+                                  "results after the timed steps (the scan runs differ from step 1 on), next to the "
+                                  "same measures for FP64 runs whose enthalpy was moved up by one ulp after the "
+                                  "warm-up steps"},
             "note": "instances 0..n-1 of the seeded parameter sweep (numpy PCG64 seed 20261203+s), plain FP64 "
                     "mode, each on its own stream, taken largest first by the worker threads (amcfd_run_batch); "
                     "one at a time beforehand in a child process, same steps, compared by iteration count and a "

@@ -226,18 +226,20 @@ def test_scan_mode_tracks_fp64_on_plate120(case_dir, amcfd):
         assert e <= 1e-9, (f, e)
 
 
-def test_batch_runs_are_the_solo_runs(case_dir, amcfd):
+@pytest.mark.parametrize("mode", ["fp64", "scan"])
+def test_batch_runs_are_the_solo_runs(case_dir, amcfd, mode):
     """amcfd_run_batch: four different decks advanced side by side on the GPU
     (own host thread and stream each) give, instance by instance, the very
-    bits of running each alone."""
+    bits of running each alone, in either mode."""
+    m = amcfd.MODE_SCAN if mode == "scan" else amcfd.MODE_FP64
     cases = ["tiny", "tiny_vap", "tiny_vol", "tiny_tool"]
     decks_ = [amcfd.Deck(case_dir(c)) for c in cases]
-    batch = [amcfd.Solver(d) for d in decks_]
-    print("batch of %d, 12 steps" % len(batch), flush=True)
+    batch = [amcfd.Solver(d, mode=m) for d in decks_]
+    print("batch of %d, 12 steps, %s" % (len(batch), mode), flush=True)
     done = amcfd.run_batch(batch, 12)
     assert done == [12] * len(batch)
     for d, b in zip(decks_, batch):
-        solo = amcfd.Solver(d)
+        solo = amcfd.Solver(d, mode=m)
         solo.run(max_steps=12)
         assert solo.scalars()["itertot"] == b.scalars()["itertot"]
         for f in ("enthalpy", "temp", "uvel", "vvel", "wvel", "pp"):
