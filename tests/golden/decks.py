@@ -200,6 +200,13 @@ CASES = {
               ([0.15e-3, 0.1e-3], [4, 8], [1.0, 1.0])),
         deck=dict(finaltime=0.2e-3 - 1e-9, maxit=100, dt=5e-6, lasertime=0.15e-3,
                   xstart=0.15e-3, ystart=0.15e-3, speed=1.0, power=200.0, radius=50e-6)),
+    # x-lines of 1024 interior cells, the longest the line solvers take
+    "tiny_long": dict(
+        mesh=(([0.1e-3, 2.0e-3, 0.1e-3], [10, 1004, 10], [-1.2, 1.0, 1.2]),
+              ([0.15e-3, 0.15e-3], [3, 3], [1.0, 1.0]),
+              ([0.15e-3, 0.1e-3], [2, 4], [1.0, 1.0])),
+        deck=dict(finaltime=0.03e-3 - 1e-9, maxit=40, dt=5e-6, lasertime=0.02e-3,
+                  xstart=0.15e-3, ystart=0.15e-3, speed=1.0, power=200.0, radius=50e-6)),
     "tiny_cond": dict(
         mesh=(([0.1e-3, 0.3e-3, 0.1e-3], [4, 12, 4], [-1.2, 1.0, 1.2]),
               ([0.15e-3, 0.15e-3], [6, 6], [1.0, 1.0]),

@@ -116,3 +116,17 @@ def test_deck_grammar_details(amcfd, tmp_path, case_dir):
     t2 = good.replace("*SELECT_DISCRETIZATION\npower_law\n", "*FLUID_FLOW_OFF\n")
     p2 = amcfd.Deck(_write(tmp_path, "g2.k", t2)).params
     assert p2.navierstokes == 0 and p2.discretization == 0
+
+
+def test_longest_lines_accepted_one_more_refused(amcfd, case_dir):
+    """x-lines of 1024 interior cells are the line solvers' limit; a deck with
+    1025 is refused with AMCFD_E_CAPACITY before the device is touched."""
+    import ctypes
+    deck = amcfd.Deck(case_dir("tiny_long"))
+    assert deck.params.ni - 2 == 1024
+    p = amcfd.Params.from_buffer_copy(deck.params)
+    p.ni = 1027
+    h = ctypes.c_void_p()
+    rc = amcfd.load_library().amcfd_create(ctypes.byref(p), 0, ctypes.byref(h))
+    assert rc == -7 and not h.value   # AMCFD_E_CAPACITY
+    assert b"1024" in amcfd.load_library().amcfd_last_error()

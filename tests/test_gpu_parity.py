@@ -69,7 +69,7 @@ def compare_run(amcfd, oracle_mod, deck_path, max_steps=None, fields=FIELDS, tol
 
 
 @pytest.mark.parametrize("case", ["tiny", "tiny_cond", "tiny_vap", "tiny_vol", "tiny_dtvar", "tiny_heattable",
-                                  "tiny_steady"])
+                                  "tiny_steady", "tiny_long"])
 def test_gpu_matches_oracle_tiny(case, case_dir, amcfd, oracle_mod):
     steps, worst = compare_run(amcfd, oracle_mod, case_dir(case))
     print(case, steps, {k: "%.1e" % v for k, v in worst.items()})
