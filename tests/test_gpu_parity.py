@@ -245,3 +245,16 @@ def test_batch_runs_are_the_solo_runs(case_dir, amcfd, mode):
         for f in ("enthalpy", "temp", "uvel", "vvel", "wvel", "pp"):
             assert np.array_equal(solo.field(f), b.field(f)), f
         solo.close()
+
+
+@pytest.mark.parametrize("inst", [0, 2])
+def test_gpu_matches_oracle_sweep_instance(amcfd, oracle_mod, tmp_path, inst):
+    """Two instances of the bench's seeded parameter sweep (uneven grid sizes,
+    12.5 um cells, their own power, speed and radius) against the oracle, bit
+    for bit, over their first three steps (the third runs to maxit with a
+    molten pool)."""
+    import decks
+    path = decks.write_sweep_instance(inst, str(tmp_path))
+    steps, worst = compare_run(amcfd, oracle_mod, path, max_steps=3)
+    print("sweep instance", inst, steps, {k: "%.1e" % v for k, v in worst.items()})
+    assert steps == 3
