@@ -354,11 +354,12 @@ __device__ __forceinline__ bool pending(double v) {
     return __builtin_bit_cast(unsigned long long, v) == SW_PENDING;
 }
 // Re-read v from p until it has been written; false if the bounded wait
-// expired (or another wait of this launch already gave up).
+// expired (or another wait of this launch already gave up).  Each read waits
+// out its own memory round trip, so the loop needs no pause of its own (a
+// pause of 64 or 128 cycles between reads made the sweeps slower).
 __device__ bool await_value(const double *p, double &v, const int *error) {
     long long spins = 0;
     while (pending(v)) {
-        __builtin_amdgcn_s_sleep(1);
         v = ld_line(p);
         if (++spins > SPIN_LIMIT) return false;
         if ((spins & 255) == 0 && __hip_atomic_load(error, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0)

@@ -41,7 +41,7 @@ METRIC = "cell-steps/s on 1 GPU, mixed+warm-start vs plain FP64, ≤1% melt-pool
 # HBM bytes per launch of the enthalpy sweep on the 400x160x80 track, from two
 # rocprofv3 --pmc runs (FETCH_SIZE x 2.00, WRITE_SIZE x 1.00, factors from a
 # plain 512 MB device copy): profiles/k_line_sweeps_track400_hbm_traffic.md
-SWEEP_TRAFFIC_BYTES = {"track400": 2.127e9 + 0.218e9}
+SWEEP_TRAFFIC_BYTES = {"track400": 2.129e9 + 0.218e9}
 HBM_PEAK_GBS = 8000.0
 CASE = "track400"
 
