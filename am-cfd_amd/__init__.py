@@ -26,6 +26,7 @@ MAX_ZONES = 32
 OK, I_MAXIT = 0, 1
 MODE_FP64 = 0
 MODE_SCAN = 1
+MODE_MIXED = 2
 
 FIELD_NAMES = ("enthalpy", "temp", "fracl", "uvel", "vvel", "wvel", "pressure", "pp", "rho",
                "diff", "vis", "hnot", "fraclnot", "unot", "vnot", "wnot", "sourceinput", "dux",
@@ -38,7 +39,8 @@ EXPORTS = ("amcfd_deck_load", "amcfd_deck_params", "amcfd_deck_free", "amcfd_cre
            "amcfd_get_scalars", "amcfd_dims", "amcfd_last_timing", "amcfd_set_profiling",
            "amcfd_kernel_stats", "amcfd_write_vts", "amcfd_write_pvd", "amcfd_append_residuals",
            "amcfd_append_final_geometry", "amcfd_deck_string", "amcfd_free_surface",
-           "amcfd_output_section", "amcfd_run_batch", "amcfd_last_error", "amcfd_version")
+           "amcfd_output_section", "amcfd_run_batch", "amcfd_set_mixed_schedule", "amcfd_test_expire_waits",
+           "amcfd_last_error", "amcfd_version")
 
 
 class Params(ctypes.Structure):
@@ -144,6 +146,8 @@ def load_library(path=LIB_PATH):
         "amcfd_free_surface": (ip, [vp, cp]),
         "amcfd_output_section": (ip, [vp, cp, ip, ctypes.POINTER(ip)]),
         "amcfd_run_batch": (ip, [ctypes.POINTER(vp), ip, ip, ip, ctypes.POINTER(ip)]),
+        "amcfd_set_mixed_schedule": (ip, [vp, cp, ip]),
+        "amcfd_test_expire_waits": (ip, [vp]),
         "amcfd_last_error": (cp, []),
         "amcfd_version": (cp, []),
     }
@@ -285,6 +289,15 @@ class Solver:
         _check(load_library().amcfd_kernel_stats(self._h, kernel.encode(), ctypes.byref(n),
                                                  ctypes.byref(ms)))
         return n.value, ms.value
+
+    def set_mixed_schedule(self, schedule, box_passes=4):
+        """Mixed mode: order of the enthalpy line passes ('x', 'y', 'z') and
+        red-black x passes per melt-pool box variable."""
+        _check(load_library().amcfd_set_mixed_schedule(self._h, schedule.encode(), int(box_passes)))
+
+    def test_expire_waits(self):
+        """Raise the line sweeps' device error flag (test hook)."""
+        _check(load_library().amcfd_test_expire_waits(self._h))
 
     def close(self):
         if self._h:

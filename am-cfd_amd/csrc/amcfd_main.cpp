@@ -39,7 +39,10 @@ int main(int argc, char **argv) {
     }
     int mode = AMCFD_MODE_FP64;
     for (int a = 2; a + 1 < argc; a += 2)
-        if (!std::strcmp(argv[a], "--mode")) mode = !std::strcmp(argv[a + 1], "scan") ? AMCFD_MODE_SCAN : AMCFD_MODE_FP64;
+        if (!std::strcmp(argv[a], "--mode"))
+            mode = !std::strcmp(argv[a + 1], "scan")    ? AMCFD_MODE_SCAN
+                   : !std::strcmp(argv[a + 1], "mixed") ? AMCFD_MODE_MIXED
+                                                        : AMCFD_MODE_FP64;
 
     amcfd_deck *deck = nullptr;
     if (amcfd_deck_load(argv[1], &deck)) return fail("deck");

@@ -119,6 +119,12 @@ int amcfd_run_batch(amcfd_solver *const *s, int n, int max_steps, int workers, i
     if (n < 0 || (n > 0 && (!s || !steps_done))) return amcfd::set_error(AMCFD_E_ARG, "null argument");
     for (int i = 0; i < n; ++i)
         if (!s[i]) return amcfd::set_error(AMCFD_E_ARG, "null solver in batch");
+    {   // two workers must never step one solver (and its stream) at once
+        std::vector<const amcfd_solver *> seen(s, s + n);
+        std::sort(seen.begin(), seen.end());
+        if (std::adjacent_find(seen.begin(), seen.end()) != seen.end())
+            return amcfd::set_error(AMCFD_E_ARG, "the same solver appears twice in the batch");
+    }
     int dev = 0;
     if (hipGetDevice(&dev) != hipSuccess) return amcfd::set_error(AMCFD_E_HIP, "hipGetDevice failed");
     if (workers <= 0) workers = 16;
@@ -160,16 +166,21 @@ int amcfd_run_batch(amcfd_solver *const *s, int n, int max_steps, int workers, i
         }
     };
     for (int i = 0; i < n; ++i) s[i]->s.set_shared_device(workers > 1);
+    // the pool outlives the try block, so threads already started are joined
+    // before an error returns (a joinable std::thread must not be destroyed)
+    std::vector<std::thread> pool;
+    std::string start_error;
     try {
-        std::vector<std::thread> pool;
         pool.reserve(workers);
         for (int w = 0; w < workers; ++w) pool.emplace_back(worker);
-        for (auto &t : pool) t.join();
     } catch (const std::exception &e) {
-        for (int i = 0; i < n; ++i) s[i]->s.set_shared_device(false);
-        return amcfd::set_error(AMCFD_E_ARG, std::string("cannot start batch threads: ") + e.what());
+        start_error = e.what();
+        next = n;   // no further instance is taken up
     }
+    for (auto &t : pool) t.join();
     for (int i = 0; i < n; ++i) s[i]->s.set_shared_device(false);
+    if (!start_error.empty())
+        return amcfd::set_error(AMCFD_E_ARG, "cannot start batch threads: " + start_error);
     for (int i = 0; i < n; ++i)
         if (rc[i] < 0) return amcfd::set_error(rc[i], "batch instance " + std::to_string(i) + ": " + msg[i]);
     return AMCFD_OK;
@@ -271,8 +282,19 @@ const char *amcfd_deck_string(const amcfd_deck *deck, const char *key, int index
     return v->c_str();
 }
 
+int amcfd_set_mixed_schedule(amcfd_solver *s, const char *schedule, int box_passes) {
+    if (!s || !schedule) return amcfd::set_error(AMCFD_E_ARG, "null argument");
+    if (s->s.mode() != AMCFD_MODE_MIXED) return amcfd::set_error(AMCFD_E_ARG, "the solver is not in the mixed mode");
+    GUARD(s->s.set_mixed_schedule(schedule, box_passes));
+}
+
+int amcfd_test_expire_waits(amcfd_solver *s) {
+    if (!s) return amcfd::set_error(AMCFD_E_ARG, "null solver");
+    GUARD(s->s.raise_sweep_error());
+}
+
 const char *amcfd_last_error(void) { return amcfd::g_last_error.c_str(); }
 
-const char *amcfd_version(void) { return "amcfd-mi355x 0.1 (gfx950, FP64 reference-order mode)"; }
+const char *amcfd_version(void) { return "amcfd-mi355x 0.2 (gfx950; FP64 reference-order, scan and mixed modes)"; }
 
 }  // extern "C"

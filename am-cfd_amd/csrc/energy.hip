@@ -13,6 +13,7 @@ namespace amcfd {
 namespace {
 
 constexpr int TX = 64, TY = 4, KB = 4;
+static_assert(TY == 4, "the mixed-mode column partials give one quantity to each of the four thread rows");
 
 __device__ __forceinline__ size_t cix(const Dims &d, int i, int j, int k) {
     return (size_t)k * d.sk + (size_t)j * d.sj + (size_t)i;
@@ -117,14 +118,30 @@ __device__ __forceinline__ int xcd_tile(int b, int nb) {
     return (x < r) ? x * (q + 1) + idx : r * (q + 1) + (x - r) * q + idx;
 }
 
+// Output arrays of the assembly.  FP64 mode: the nine coefficient arrays and
+// the per-cell block-correction terms.  Mixed mode (MX): the seven matrix
+// coefficients rounded to FP32, su in FP64, and per-tile column partials of
+// the block-correction sums, part[q][tile][i] (q = bl, blp, blm, blc; tile =
+// the (j, k) tile index), formed from the FP32-rounded coefficients so that
+// the correction matches the system the sweeps solve.
+struct AsmOut {
+    double *an, *as, *ae, *aw, *at, *ab, *ap, *su, *src, *part_res, *term_bl, *term_blc;
+    float *fan, *fas, *fae, *faw, *fat, *fab, *fap;
+    double *blkpart;
+};
+
+template <bool MX>
 __global__ __launch_bounds__(TX *TY) void k_energy_assemble(
     Dims d, AsmConst q, DevGeom g, const double *__restrict__ uvel, const double *__restrict__ vvel,
     const double *__restrict__ wvel, const double *__restrict__ rho, const double *__restrict__ diff,
     const double *__restrict__ hnot, const double *__restrict__ fracl, const double *__restrict__ fraclnot,
-    const double *__restrict__ h, double *an_o, double *as_o, double *ae_o, double *aw_o, double *at_o,
-    double *ab_o, double *ap_o, double *su_o, double *src_o, double *part_res, double *term_bl,
-    double *term_blc) {
+    const double *__restrict__ h, AsmOut o, const int *__restrict__ skip) {
     __shared__ double scratch[2 * 4];
+    __shared__ double s_blk[MX ? 4 * TY * TX : 1];
+    if (MX && skip && *skip) return;
+    double *an_o = o.an, *as_o = o.as, *ae_o = o.ae, *aw_o = o.aw, *at_o = o.at, *ab_o = o.ab, *ap_o = o.ap;
+    double *su_o = o.su, *src_o = o.src, *part_res = o.part_res, *term_bl = o.term_bl, *term_blc = o.term_blc;
+    double cbl = 0.0, cblp = 0.0, cblm = 0.0, cblc = 0.0;   // mixed mode: this column's partial sums
     const int tx = threadIdx.x, ty = threadIdx.y;
     const int gx = gridDim.x, gy = gridDim.y;
     const int nb = gx * gy * gridDim.z;
@@ -254,21 +271,53 @@ __global__ __launch_bounds__(TX *TY) void k_energy_assemble(
             double ap = an + as + ae + aw + at + ab + apnot - sp;
             ap /= q.urfh;
             su += (1.0 - q.urfh) * ap * hP;
-            an_o[c] = an; as_o[c] = as; ae_o[c] = ae; aw_o[c] = aw; at_o[c] = at; ab_o[c] = ab;
-            ap_o[c] = ap; su_o[c] = su; src_o[c] = src;
-            // residual (calculateResiduals(5))
-            double r = (an * hN + as * hS + ae * hE + aw * hW + at * hT + ab * hB + su) / ap - hP;
-            sumd += fabs(r);
-            sumh += fabs(hP);
-            // per-cell terms of the block-correction column sums (summed per
-            // x-column in the reference's (k, j) order by k_column_sums)
-            term_bl[c] = ap - an - as - at - ab;
-            term_blc[c] = ae * hE + aw * hW + an * hN + as * hS + at * hT + ab * hB + su - ap * hP;
+            if constexpr (!MX) {
+                an_o[c] = an; as_o[c] = as; ae_o[c] = ae; aw_o[c] = aw; at_o[c] = at; ab_o[c] = ab;
+                ap_o[c] = ap; su_o[c] = su; src_o[c] = src;
+                // residual (calculateResiduals(5))
+                double r = (an * hN + as * hS + ae * hE + aw * hW + at * hT + ab * hB + su) / ap - hP;
+                sumd += fabs(r);
+                sumh += fabs(hP);
+                // per-cell terms of the block-correction column sums (summed per
+                // x-column in the reference's (k, j) order by k_column_sums)
+                term_bl[c] = ap - an - as - at - ab;
+                term_blc[c] = ae * hE + aw * hW + an * hN + as * hS + at * hT + ab * hB + su - ap * hP;
+            } else {
+                const float fan = (float)an, fas = (float)as, fae = (float)ae, faw = (float)aw;
+                const float fat = (float)at, fab = (float)ab, fap = (float)ap;
+                o.fan[c] = fan; o.fas[c] = fas; o.fae[c] = fae; o.faw[c] = faw; o.fat[c] = fat; o.fab[c] = fab;
+                o.fap[c] = fap;
+                su_o[c] = su;
+                src_o[c] = src;
+                const double ran = fan, ras = fas, rae = fae, raw = faw, rat = fat, rab = fab, rap = fap;
+                const double num = rae * hE + raw * hW + ran * hN + ras * hS + rat * hT + rab * hB + su;
+                sumd += fabs(num / rap - hP);
+                sumh += fabs(hP);
+                cbl += rap - ran - ras - rat - rab;
+                cblp += rae;
+                cblm += raw;
+                cblc += num - rap * hP;
+            }
             // the next plane's bottom and centre values
             dB = dP; dP = dT;
             rP = rT; wb = wt;
             flB = flP; flP = flT;
             hB = hP; hP = hT;
+        }
+    }
+    if constexpr (MX) {
+        // column partials of this tile: the four j-rows of each i, in order
+        s_blk[(0 * TY + ty) * TX + tx] = cbl;
+        s_blk[(1 * TY + ty) * TX + tx] = cblp;
+        s_blk[(2 * TY + ty) * TX + tx] = cblm;
+        s_blk[(3 * TY + ty) * TX + tx] = cblc;
+        __syncthreads();
+        if (i <= d.nim2) {
+            const int ntile = gy * gridDim.z, tile = by + gy * bz;
+            const int qq = ty;   // one quantity per j-row of threads
+            double a = 0.0;
+            for (int r = 0; r < TY; ++r) a += s_blk[(qq * TY + r) * TX + tx];
+            o.blkpart[((size_t)qq * ntile + tile) * d.ni + i] = a;
         }
     }
     double v[2] = {sumd, sumh};
@@ -883,9 +932,25 @@ void launch_energy_assemble(hipStream_t st, const Dims &d, const DevGeom &g, Dev
                             const amcfd_params &p, const SourceParams &s) {
     AsmConst q{p.delt, p.hlatnt, p.urfh, s.avolpow, s.apowseta, s.heatthick, s.heatrb, s.avolfact,
                s.beamposx, s.beamposy, s.powerindicator, p.discretization};
-    hipLaunchKernelGGL(k_energy_assemble, assemble_grid(d), dim3(TX, TY), 0, st, d, q, g, f.uvel, f.vvel, f.wvel,
-                       f.rho, f.diff, f.hnot, f.fracl, f.fraclnot, f.enthalpy, f.an, f.as, f.ae, f.aw, f.at,
-                       f.ab, f.ap, f.su, f.sourceinput, pr.energy, f.sp, f.apnot);
+    AsmOut o{f.an, f.as, f.ae, f.aw, f.at, f.ab, f.ap, f.su, f.sourceinput, pr.energy, f.sp, f.apnot,
+             nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+    hipLaunchKernelGGL(k_energy_assemble<false>, assemble_grid(d), dim3(TX, TY), 0, st, d, q, g, f.uvel, f.vvel,
+                       f.wvel, f.rho, f.diff, f.hnot, f.fracl, f.fraclnot, f.enthalpy, o, nullptr);
+}
+
+void launch_energy_assemble_mixed(hipStream_t st, const Dims &d, const DevGeom &g, DevFields f, Partials pr,
+                                  const MxFields &m, const amcfd_params &p, const SourceParams &s, const int *skip) {
+    AsmConst q{p.delt, p.hlatnt, p.urfh, s.avolpow, s.apowseta, s.heatthick, s.heatrb, s.avolfact,
+               s.beamposx, s.beamposy, s.powerindicator, p.discretization};
+    AsmOut o{nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, m.su, f.sourceinput, pr.energy,
+             nullptr, nullptr, m.an, m.as, m.ae, m.aw, m.at, m.ab, m.ap, m.blkpart};
+    hipLaunchKernelGGL(k_energy_assemble<true>, assemble_grid(d), dim3(TX, TY), 0, st, d, q, g, f.uvel, f.vvel,
+                       f.wvel, f.rho, f.diff, f.hnot, f.fracl, f.fraclnot, f.enthalpy, o, skip);
+}
+
+int mixed_tile_count(const Dims &d) {
+    const dim3 a = assemble_grid(d);
+    return (int)(a.y * a.z);
 }
 
 void launch_props(hipStream_t st, const Dims &d, DevFields f, const amcfd_params &p) {

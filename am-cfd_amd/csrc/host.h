@@ -57,6 +57,13 @@ class Solver {
     void set_profiling(int on) { profile_ = on; }
     // the solver runs beside others on the device (amcfd_run_batch)
     void set_shared_device(bool on) { shared_device_ = on; }
+    int mode() const { return mode_; }
+    // mixed mode: order of the enthalpy line passes ('x' red-black x-lines,
+    // 'y' y-lines, 'z' z-lines; an even number of y/z) and red-black x
+    // passes per melt-pool box variable
+    int set_mixed_schedule(const char *sched, int box_passes);
+    // raise the sweeps' device error flag (test hook for the bounded waits)
+    int raise_sweep_error();
     std::map<std::string, KernelTimer> timers;
 
   private:
@@ -77,6 +84,9 @@ class Solver {
     unsigned long long *counter_ = nullptr;
     unsigned long long counter_base_ = 0;
     double *pass_buf_ = nullptr;   // four field-sized pass buffers of the line sweeps
+    MxFields mx_{};                // mixed mode: FP32 coefficients, partials, second buffer
+    std::string mx_sched_ = "xzxy";   // mixed mode: order of the enthalpy line passes
+    int mx_box_passes_ = 4;          // mixed mode: red-black x passes per box variable
     int profile_ = 0;
     bool shared_device_ = false;
     std::vector<hipEvent_t> ev_pool_;
@@ -120,6 +130,10 @@ class Solver {
     int ensure_plan(SweepPlan &plan, int kbeg, int kend, int jbeg, int jend);
     int read_scalars();
     int flow_iteration();
+    int solve_step_mixed();
+    int flow_iteration_mixed();
+    int energy_sweeps_mixed();
+    int allocf(float **ptr, size_t n);
     void tic(const char *name, int *slot);
     void toc(const char *name, int slot);
     void collect_timers();

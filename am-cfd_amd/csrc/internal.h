@@ -111,6 +111,17 @@ struct Partials {
     int nblk_e, nrow_e, nblk_t, nblk_f, nblk_b, nblk_l;
 };
 
+// mixed-precision mode: FP32 coefficients of the enthalpy equation, the
+// block-correction partials and the second enthalpy buffer of the y/z passes
+struct MxFields {
+    float *ap, *an, *as, *ae, *aw, *at, *ab;
+    double *su;        // FP64 source (the energy equation's f.su)
+    double *blkpart;   // [4][ntile][ni] per-tile column partials
+    double *blkfold;   // [4][split][ni]
+    double *delphi;    // [ni]
+    double *halt;      // second enthalpy buffer (y/z passes read one, write the other)
+};
+
 // line-sweep schedule (x-lines in Gauss-Seidel program order)
 struct SweepPlan {
     int *tasks = nullptr;          // device: packed (pass, k, j) in wavefront order
@@ -157,6 +168,25 @@ void launch_line_prep(hipStream_t st, const Dims &d, DevFields f, double *pb, si
 void launch_line_sweeps(hipStream_t st, const Dims &d, DevFields f, double *phi, double *pb, size_t pstride,
                         const SweepPlan &plan, int ibeg, int iend, int guard, int scan, unsigned long long *counter,
                         unsigned long long counter_base, DevScalars *sc, int nblocks, int shared);
+
+// mixed-precision mode (mixed.hip, energy.hip)
+void launch_energy_assemble_mixed(hipStream_t st, const Dims &d, const DevGeom &g, DevFields f, Partials pr,
+                                  const MxFields &m, const amcfd_params &p, const SourceParams &sp, const int *skip);
+int mixed_tile_count(const Dims &d);
+int mixed_max_line();
+int mixed_max_yz();
+size_t mixed_yz_lds(int len);   // LDS bytes of one y/z line workgroup
+int mixed_blk_split();
+void launch_mixed_block_correction(hipStream_t st, const Dims &d, const MxFields &m, double *phi, const int *skip);
+// x-lines of one colour ((j + k) % 2 == colour), in place
+void launch_mixed_xline(hipStream_t st, const Dims &d, const MxFields &m, double *phi, int ibeg, int iend, int jbeg,
+                        int jend, int kbeg, int kend, int colour, const int *skip);
+// y-lines (dir 1) or z-lines (dir 2), hin -> hout
+void launch_mixed_yzline(hipStream_t st, const Dims &d, const MxFields &m, const double *hin, double *hout, int dir,
+                         int ibeg, int iend, int jbeg, int jend, int kbeg, int kend, const int *skip);
+// melt-pool box x-lines of one colour (FP64 coefficients, reference guard)
+void launch_box_xline(hipStream_t st, const Dims &d, DevFields f, double *phi, int ibeg, int iend, int jbeg, int jend,
+                      int kbeg, int kend, int colour, const int *skip);
 
 // species (species.hip)
 void launch_bc_species(hipStream_t st, const Dims &d, const DevGeom &g, DevFields f, const amcfd_params &p);

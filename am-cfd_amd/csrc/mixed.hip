@@ -1,0 +1,420 @@
+// Mixed-precision mode (AMCFD_MODE_MIXED): line sweeps in parallel order,
+// FP32 line solves on an FP64 residual.
+//
+// The reference's SIMPLE iteration smooths each linearised equation with a
+// block correction along x and four Gauss-Seidel x-line passes
+// (CFDSolverManager.cpp:2270-2362, 2459-2526 on the whole domain,
+// 2367-2454 on the melt-pool box).  This mode keeps that structure and the
+// reference's coefficients, but
+//   - the coefficient arrays of the enthalpy equation are stored in FP32
+//     (half the bytes of the dominant kernels);
+//   - every pass is a correction in delta form: the residual
+//       r = su + sum_nb a_nb phi_nb - ap phi
+//     is formed in FP64 from the FP64 state, the line system
+//       ap dphi_i - aw dphi_{i-1} - ae dphi_{i+1} = r_i
+//     is solved in FP32 and phi += dphi is applied in FP64.  Rounding in the
+//     FP32 solve only perturbs the correction, which later passes and
+//     iterations remove (iterative refinement): the fixed point is that of
+//     the FP64 residual;
+//   - the passes run in parallel orders instead of the sequential j/k loop
+//     nest: x-lines in red-black (zebra) order over (j, k), one wave per
+//     line with the line solved by wave-wide scans (any length up to
+//     MX_MAXN); y- and z-lines one lane per line (Jacobi between lines,
+//     reading one buffer and writing another), which alternate the implicit
+//     direction.
+// Block-correction column sums come from per-tile partials that the
+// assembly kernel writes; they and the 1-D correction along x are folded in
+// FP64 in a fixed order, so a run is reproducible bit for bit.
+
+#include "fv.h"
+
+namespace amcfd {
+
+namespace {
+
+constexpr int MX_MAXN = 4096;   // longest x-line of the wave solver (64 lanes x 64 rows)
+constexpr int YZ_MAXL = 300;    // longest y- or z-line of the lane solver (LDS: 64 lanes x 8 B x 300)
+constexpr int BF_SPLIT = 16;    // first-level splits of the column-sum fold
+
+__device__ __forceinline__ size_t cix(const Dims &d, int i, int j, int k) {
+    return (size_t)k * d.sk + (size_t)j * d.sj + (size_t)i;
+}
+
+// exact power-of-two rescale (projective quantities only)
+template <typename T>
+__device__ __forceinline__ T pow2_inv(T v) {
+    const int e = ilogb(fabs(v));
+    return (v == T(0) || e > 1000 || e < -1000) ? T(1) : ldexp(T(1), -e);
+}
+
+// ------------------------------------------------------------------------
+// Wave-wide tridiagonal solve of one line,
+//   -w_p x_{p-1} + dg_p x_p - e_p x_{p+1} = r_p,   p = 0..n-1,
+// with x outside the line taken as 0 (w_0 and e_{n-1} are not read).  Lane
+// l holds rows l*M .. l*M+M-1.  Thomas' forward pivots c_p = e_p/den_p,
+// den_p = dg_p - w_p c_{p-1}, and g_p = (r_p + w_p g_{p-1})/den_p follow
+// from the projective recurrence
+//   (N, D, H)_p = [[0, e, 0], [-w, dg, 0], [0, r, w]]_p (N, D, H)_{p-1}
+// with c = N/D, g = H/D and (N, D, H)_{-1} = (0, 1, 0): each lane composes
+// its rows' matrices, the lanes combine them with log2(64) shuffle steps,
+// and each lane replays its rows from the state entering it (plain Thomas
+// steps).  Back substitution x_p = g_p + c_p x_{p+1} is a suffix scan of
+// affine maps.  Matrix products are rescaled by powers of two (exact; the
+// recurrence is projective), so FP32 neither overflows nor underflows.
+// ------------------------------------------------------------------------
+template <typename T>
+struct Mob {   // [[a, b, 0], [c, d, 0], [f, g, h]]
+    T a, b, c, d, f, g, h;
+    __device__ __forceinline__ static Mob ident() { return Mob{T(1), T(0), T(0), T(1), T(0), T(0), T(1)}; }
+    // this = L * this
+    __device__ __forceinline__ void lmul(const Mob &L) {
+        const T na = L.a * a + L.b * c, nb = L.a * b + L.b * d;
+        const T nc = L.c * a + L.d * c, nd = L.c * b + L.d * d;
+        const T nf = L.f * a + L.g * c + L.h * f, ng = L.f * b + L.g * d + L.h * g;
+        const T nh = L.h * h;
+        a = na; b = nb; c = nc; d = nd; f = nf; g = ng; h = nh;
+        const T s = pow2_inv(fabs(d) > fabs(b) ? d : b);
+        a *= s; b *= s; c *= s; d *= s; f *= s; g *= s; h *= s;
+    }
+    // this = R-row step (the row matrix on the left of this)
+    __device__ __forceinline__ void row(T w, T dg, T e, T r) {
+        const T na = e * c, nb = e * d;
+        const T nc = -w * a + dg * c, nd = -w * b + dg * d;
+        const T nf = r * c + w * f, ng = r * d + w * g;
+        const T nh = w * h;
+        a = na; b = nb; c = nc; d = nd; f = nf; g = ng; h = nh;
+        const T s = pow2_inv(fabs(d) > fabs(b) ? d : b);
+        a *= s; b *= s; c *= s; d *= s; f *= s; g *= s; h *= s;
+    }
+};
+
+template <typename T>
+__device__ __forceinline__ T shfl_up_t(T v, int off) { return __shfl_up(v, off, 64); }
+template <typename T>
+__device__ __forceinline__ T shfl_down_t(T v, int off) { return __shfl_down(v, off, 64); }
+
+template <typename T, int M>
+__device__ void wave_tridiag(int n, const T (&w)[M], const T (&dg)[M], const T (&e)[M], T (&r)[M], bool guard) {
+    const int lane = threadIdx.x & 63;
+    Mob<T> S = Mob<T>::ident();
+#pragma unroll
+    for (int q = 0; q < M; ++q)
+        if (lane * M + q < n) S.row(w[q], dg[q], e[q], r[q]);
+    // inclusive prefix over lanes: P_l = S_l * ... * S_0
+    Mob<T> P = S;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        Mob<T> L;
+        L.a = shfl_up_t(P.a, off); L.b = shfl_up_t(P.b, off); L.c = shfl_up_t(P.c, off);
+        L.d = shfl_up_t(P.d, off); L.f = shfl_up_t(P.f, off); L.g = shfl_up_t(P.g, off);
+        L.h = shfl_up_t(P.h, off);
+        if (lane >= off) {   // P = P * L  (the earlier lanes act first)
+            Mob<T> t = L;
+            t.lmul(P);
+            P = t;
+        }
+    }
+    // state entering this lane: column 2 of the exclusive prefix
+    T N = shfl_up_t(P.b, 1), D = shfl_up_t(P.d, 1), H = shfl_up_t(P.g, 1);
+    T cp = T(0), gp = T(0);
+    if (lane > 0) {
+        cp = N / D;
+        gp = H / D;
+    }
+    T c[M];
+#pragma unroll
+    for (int q = 0; q < M; ++q) {
+        T den = dg[q] - w[q] * cp;
+        if (guard && fabs(den) <= T(1e-12)) den += T(AMCFD_SMALL);
+        cp = e[q] / den;
+        gp = (r[q] + w[q] * gp) / den;
+        c[q] = (lane * M + q < n) ? cp : T(0);
+        r[q] = (lane * M + q < n) ? gp : T(0);
+    }
+    // back substitution: x_p = r_p + c_p x_{p+1}, suffix scan over lanes
+    T A = T(1), B = T(0);
+#pragma unroll
+    for (int q = M - 1; q >= 0; --q) {
+        B = c[q] * B + r[q];
+        A = c[q] * A;
+    }
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const T Ap = shfl_down_t(A, off), Bp = shfl_down_t(B, off);
+        if (lane + off < 64) {
+            B = A * Bp + B;
+            A = A * Ap;
+        }
+    }
+    const T Bn = shfl_down_t(B, 1);
+    T x = (lane == 63) ? T(0) : Bn;
+#pragma unroll
+    for (int q = M - 1; q >= 0; --q) {
+        x = c[q] * x + r[q];
+        r[q] = x;
+    }
+}
+
+template <typename TC>
+struct MxCoefs {
+    const TC *ap, *an, *as, *ae, *aw, *at, *ab;
+    const double *su;
+};
+
+// ------------------------------------------------------------------------
+// x-lines of one colour: (j + k) % 2 == colour over [jbeg, jend] x [kbeg,
+// kend], rows i in [ibeg, iend]; ends ibeg-1 and iend+1 are fixed.  One
+// wave per line; global loads are row-contiguous (lane-cyclic), the solve
+// is lane-blocked, LDS transposes between the two.  Lines of one colour
+// read only lines of the other, so the pass is race-free and in place.
+// ------------------------------------------------------------------------
+template <typename TC, int M>
+__global__ __launch_bounds__(64) void k_xline_mx(Dims d, MxCoefs<TC> cf, double *phi, int ibeg, int iend, int jbeg,
+                                                  int jend, int kbeg, int kend, int colour, int guard,
+                                                  const int *__restrict__ skip) {
+    __shared__ float sw[64 * M], sd[64 * M], se[64 * M], sr[64 * M];
+    if (skip && *skip) return;
+    const int lane = threadIdx.x;
+    const int n = iend - ibeg + 1, nj = jend - jbeg + 1, nk = kend - kbeg + 1;
+    const int cm = (nj + 1) / 2;
+    const long long nlines = (long long)nk * cm;
+    for (long long u = blockIdx.x; u < nlines; u += gridDim.x) {
+        const int kk = (int)(u / cm), jj = (int)(u % cm);
+        const int k = kbeg + kk;
+        const int j = jbeg + ((colour - k - jbeg) & 1) + 2 * jj;
+        if (j > jend) continue;   // uniform over the wave
+        const size_t row = (size_t)k * d.sk + (size_t)j * d.sj + ibeg;
+        double hp[M];
+#pragma unroll
+        for (int q = 0; q < M; ++q) {
+            const int ii = lane + 64 * q;
+            hp[q] = 0.0;
+            if (ii < n) {
+                const size_t c = row + ii;
+                const double h = phi[c];
+                const double ap = (double)cf.ap[c], aw = (double)cf.aw[c], ae = (double)cf.ae[c];
+                const double r = cf.su[c] + aw * phi[c - 1] + ae * phi[c + 1] + (double)cf.an[c] * phi[c + d.sj] +
+                                 (double)cf.as[c] * phi[c - d.sj] + (double)cf.at[c] * phi[c + d.sk] +
+                                 (double)cf.ab[c] * phi[c - d.sk] - ap * h;
+                hp[q] = h;
+                sw[ii] = (float)aw;
+                sd[ii] = (float)ap;
+                se[ii] = (float)ae;
+                sr[ii] = (float)r;
+            }
+        }
+        __syncthreads();
+        float w[M], dg[M], e[M], r[M];
+#pragma unroll
+        for (int q = 0; q < M; ++q) {
+            const int p = lane * M + q;
+            const bool in = p < n;
+            w[q] = in ? sw[p] : 0.f;
+            dg[q] = in ? sd[p] : 1.f;
+            e[q] = in ? se[p] : 0.f;
+            r[q] = in ? sr[p] : 0.f;
+        }
+        wave_tridiag<float, M>(n, w, dg, e, r, guard != 0);
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < M; ++q) {
+            const int p = lane * M + q;
+            if (p < n) sr[p] = r[q];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < M; ++q) {
+            const int ii = lane + 64 * q;
+            if (ii < n) phi[row + ii] = hp[q] + (double)sr[ii];
+        }
+        __syncthreads();
+    }
+}
+
+// ------------------------------------------------------------------------
+// y-lines (DIR 1, along j) or z-lines (DIR 2, along k), one lane per line,
+// lanes along i.  Reads hin everywhere and writes hout on the lines' cells
+// (Jacobi between lines: no line reads another's update).  The forward
+// Thomas values of every row go to LDS, then back substitution writes
+// hout = hin + dphi.  blockIdx.y is the coordinate the lines do not run
+// along (k for y-lines, j for z-lines).
+// ------------------------------------------------------------------------
+template <typename TC, int DIR>
+__global__ __launch_bounds__(64) void k_yzline_mx(Dims d, MxCoefs<TC> cf, const double *__restrict__ hin,
+                                                   double *__restrict__ hout, int ibeg, int iend, int jbeg, int jend,
+                                                   int kbeg, int kend, const int *__restrict__ skip) {
+    extern __shared__ float sbuf[];   // [L][64] c, then [L][64] g
+    if (skip && *skip) return;
+    const int lane = threadIdx.x;
+    const int i = ibeg + blockIdx.x * 64 + lane;
+    const bool act = i <= iend;
+    const int L = DIR == 1 ? jend - jbeg + 1 : kend - kbeg + 1;
+    const size_t stride = DIR == 1 ? d.sj : d.sk;
+    const size_t base = DIR == 1 ? cix(d, i, jbeg, kbeg + blockIdx.y) : cix(d, i, jbeg + blockIdx.y, kbeg);
+    float *sc = sbuf, *sg = sbuf + (size_t)L * 64;
+    float cp = 0.f, gp = 0.f;
+    if (act) {
+        for (int s = 0; s < L; ++s) {
+            const size_t c = base + (size_t)s * stride;
+            const double h = hin[c];
+            const double ap = (double)cf.ap[c];
+            const double an = (double)cf.an[c], as = (double)cf.as[c], ae = (double)cf.ae[c], aw = (double)cf.aw[c];
+            const double at = (double)cf.at[c], ab = (double)cf.ab[c];
+            const double r = cf.su[c] + aw * hin[c - 1] + ae * hin[c + 1] + an * hin[c + d.sj] + as * hin[c - d.sj] +
+                             at * hin[c + d.sk] + ab * hin[c - d.sk] - ap * h;
+            const float sub = DIR == 1 ? (float)as : (float)ab;
+            const float sup = DIR == 1 ? (float)an : (float)at;
+            const float den = (float)ap - sub * cp;
+            cp = sup / den;
+            gp = ((float)r + sub * gp) / den;
+            sc[s * 64 + lane] = cp;
+            sg[s * 64 + lane] = gp;
+        }
+        float x = 0.f;
+        for (int s = L - 1; s >= 0; --s) {
+            x = sg[s * 64 + lane] + sc[s * 64 + lane] * x;
+            const size_t c = base + (size_t)s * stride;
+            hout[c] = hin[c] + (double)x;
+        }
+    }
+}
+
+// ------------------------------------------------------------------------
+// block correction along x from the assembly's per-tile column partials
+// part[q][t][i] (q = bl, blp, blm, blc; t = (j, k) tile): a fixed-order fold
+// in two levels, then the 1-D system
+//   bl_i d_i - blm_i d_{i-1} - blp_i d_{i+1} = blc_i
+// (CFDSolverManager.cpp:2334-2349) solved by one wave, then phi += d_i.
+// ------------------------------------------------------------------------
+__global__ __launch_bounds__(64) void k_blk_fold(int ni, int ntile, const double *__restrict__ part, double *out) {
+    const int i = blockIdx.x * 64 + threadIdx.x, q = blockIdx.y, s = blockIdx.z;
+    if (i >= ni) return;
+    const double *p = part + (size_t)q * ntile * ni + i;
+    double a = 0.0;
+    for (int t = s; t < ntile; t += BF_SPLIT) a += p[(size_t)t * ni];
+    out[((size_t)q * BF_SPLIT + s) * ni + i] = a;
+}
+
+template <int M>
+__global__ __launch_bounds__(64) void k_blk_solve_mx(int nim2, int ni, const double *__restrict__ f1, double *delphi,
+                                                      const int *__restrict__ skip) {
+    if (skip && *skip) return;
+    const int lane = threadIdx.x;
+    const int n = nim2;
+    double w[M], dg[M], e[M], r[M];
+#pragma unroll
+    for (int q = 0; q < M; ++q) {
+        const int p = lane * M + q, i = p + 1;
+        double v[4] = {0.0, 0.0, 0.0, 0.0};
+        if (p < n)
+            for (int qq = 0; qq < 4; ++qq)
+                for (int s = 0; s < BF_SPLIT; ++s) v[qq] += f1[((size_t)qq * BF_SPLIT + s) * ni + i];
+        dg[q] = p < n ? v[0] : 1.0;
+        e[q] = v[1];
+        w[q] = v[2];
+        r[q] = v[3];
+    }
+    wave_tridiag<double, M>(n, w, dg, e, r, false);
+#pragma unroll
+    for (int q = 0; q < M; ++q) {
+        const int p = lane * M + q;
+        if (p < n) delphi[p + 1] = r[q];
+    }
+}
+
+__global__ __launch_bounds__(256) void k_add_delphi_mx(Dims d, const double *__restrict__ delphi, double *phi,
+                                                      const int *__restrict__ skip) {
+    if (skip && *skip) return;
+    const int i = 1 + blockIdx.x * 64 + threadIdx.x;
+    const int j = 1 + blockIdx.y * 4 + threadIdx.y;
+    if (i > d.nim2 || j > d.njm2) return;
+    const double dp = delphi[i];
+    for (int k = 1 + blockIdx.z * 4; k < 1 + (blockIdx.z + 1) * 4 && k <= d.nkm2; ++k) phi[cix(d, i, j, k)] += dp;
+}
+
+template <typename TC>
+void launch_xline_t(hipStream_t st, const Dims &d, const MxCoefs<TC> &cf, double *phi, int ibeg, int iend, int jbeg,
+                    int jend, int kbeg, int kend, int colour, int guard, const int *skip) {
+    const int n = iend - ibeg + 1;
+    const long long nlines = (long long)(kend - kbeg + 1) * ((jend - jbeg + 2) / 2);
+    const int nb = (int)std::min<long long>(nlines, 16384);
+#define XL(MM) hipLaunchKernelGGL((k_xline_mx<TC, MM>), dim3(nb), dim3(64), 0, st, d, cf, phi, ibeg, iend, jbeg, jend, \
+                                  kbeg, kend, colour, guard, skip)
+    if (n <= 128) XL(2);
+    else if (n <= 256) XL(4);
+    else if (n <= 512) XL(8);
+    else if (n <= 1024) XL(16);
+    else if (n <= 2048) XL(32);
+    else XL(64);
+#undef XL
+}
+
+template <typename TC>
+void launch_yzline_t(hipStream_t st, const Dims &d, const MxCoefs<TC> &cf, const double *hin, double *hout, int dir,
+                     int ibeg, int iend, int jbeg, int jend, int kbeg, int kend, const int *skip) {
+    const int L = dir == 1 ? jend - jbeg + 1 : kend - kbeg + 1;
+    const int other = dir == 1 ? kend - kbeg + 1 : jend - jbeg + 1;
+    const dim3 grid((iend - ibeg + 64) / 64, other);
+    const size_t lds = (size_t)L * 64 * 2 * sizeof(float);
+    // more dynamic LDS than the default per-launch limit is allowed once per
+    // kernel (the line length was checked against the device's limit)
+    static bool attr_set = false;
+    if (!attr_set) {
+        (void)hipFuncSetAttribute((const void *)k_yzline_mx<TC, 1>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  (int)((size_t)YZ_MAXL * 64 * 2 * sizeof(float)));
+        (void)hipFuncSetAttribute((const void *)k_yzline_mx<TC, 2>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  (int)((size_t)YZ_MAXL * 64 * 2 * sizeof(float)));
+        attr_set = true;
+    }
+    if (dir == 1)
+        hipLaunchKernelGGL((k_yzline_mx<TC, 1>), grid, dim3(64), lds, st, d, cf, hin, hout, ibeg, iend, jbeg, jend,
+                           kbeg, kend, skip);
+    else
+        hipLaunchKernelGGL((k_yzline_mx<TC, 2>), grid, dim3(64), lds, st, d, cf, hin, hout, ibeg, iend, jbeg, jend,
+                           kbeg, kend, skip);
+}
+
+}  // namespace
+
+int mixed_max_line() { return MX_MAXN; }
+int mixed_max_yz() { return YZ_MAXL; }
+size_t mixed_yz_lds(int len) { return (size_t)len * 64 * 2 * sizeof(float); }
+int mixed_blk_split() { return BF_SPLIT; }
+
+void launch_mixed_block_correction(hipStream_t st, const Dims &d, const MxFields &m, double *phi, const int *skip) {
+    hipLaunchKernelGGL(k_blk_fold, dim3((d.ni + 63) / 64, 4, BF_SPLIT), dim3(64), 0, st, d.ni, mixed_tile_count(d),
+                       m.blkpart, m.blkfold);
+    const int n = d.nim2;
+#define BS(MM) hipLaunchKernelGGL((k_blk_solve_mx<MM>), dim3(1), dim3(64), 0, st, d.nim2, d.ni, m.blkfold, m.delphi, skip)
+    if (n <= 128) BS(2);
+    else if (n <= 256) BS(4);
+    else if (n <= 512) BS(8);
+    else if (n <= 1024) BS(16);
+    else if (n <= 2048) BS(32);
+    else BS(64);
+#undef BS
+    hipLaunchKernelGGL(k_add_delphi_mx, dim3((d.nim2 + 63) / 64, (d.njm2 + 3) / 4, (d.nkm2 + 3) / 4), dim3(64, 4), 0,
+                       st, d, m.delphi, phi, skip);
+}
+
+void launch_mixed_xline(hipStream_t st, const Dims &d, const MxFields &m, double *phi, int ibeg, int iend, int jbeg,
+                        int jend, int kbeg, int kend, int colour, const int *skip) {
+    MxCoefs<float> cf{m.ap, m.an, m.as, m.ae, m.aw, m.at, m.ab, m.su};
+    launch_xline_t<float>(st, d, cf, phi, ibeg, iend, jbeg, jend, kbeg, kend, colour, 0, skip);
+}
+
+void launch_mixed_yzline(hipStream_t st, const Dims &d, const MxFields &m, const double *hin, double *hout, int dir,
+                         int ibeg, int iend, int jbeg, int jend, int kbeg, int kend, const int *skip) {
+    MxCoefs<float> cf{m.ap, m.an, m.as, m.ae, m.aw, m.at, m.ab, m.su};
+    launch_yzline_t<float>(st, d, cf, hin, hout, dir, ibeg, iend, jbeg, jend, kbeg, kend, skip);
+}
+
+// melt-pool box lines (FP64 coefficients of the momentum / pressure
+// equations, FP32 solve on the FP64 residual, guard as the reference's)
+void launch_box_xline(hipStream_t st, const Dims &d, DevFields f, double *phi, int ibeg, int iend, int jbeg, int jend,
+                      int kbeg, int kend, int colour, const int *skip) {
+    MxCoefs<double> cf{f.ap, f.an, f.as, f.ae, f.aw, f.at, f.ab, f.su};
+    launch_xline_t<double>(st, d, cf, phi, ibeg, iend, jbeg, jend, kbeg, kend, colour, 1, skip);
+}
+
+}  // namespace amcfd

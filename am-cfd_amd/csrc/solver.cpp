@@ -42,6 +42,31 @@ int Solver::alloc(double **ptr, size_t n) {
     return AMCFD_OK;
 }
 
+int Solver::allocf(float **ptr, size_t n) {
+    void *p = nullptr;
+    HIPCALL(hipMalloc(&p, std::max<size_t>(n, 1) * sizeof(float)));
+    HIPCALL(hipMemsetAsync(p, 0, std::max<size_t>(n, 1) * sizeof(float), st_));
+    allocs_.push_back(p);
+    *ptr = (float *)p;
+    return AMCFD_OK;
+}
+
+int Solver::set_mixed_schedule(const char *sched, int box_passes) {
+    if (!sched || !*sched) return set_error(AMCFD_E_ARG, "empty pass schedule");
+    int yz = 0;
+    for (const char *c = sched; *c; ++c) {
+        if (*c != 'x' && *c != 'y' && *c != 'z') return set_error(AMCFD_E_ARG, "passes are 'x', 'y' or 'z'");
+        yz += *c != 'x';
+    }
+    // the y/z passes alternate between the field and its second buffer; an
+    // even count ends in the field itself
+    if (yz % 2) return set_error(AMCFD_E_ARG, "the schedule needs an even number of y/z passes");
+    if (box_passes < 1 || box_passes > 16) return set_error(AMCFD_E_ARG, "box passes must be 1..16");
+    mx_sched_ = sched;
+    mx_box_passes_ = box_passes;
+    return AMCFD_OK;
+}
+
 int Solver::upload(double *dst, const std::vector<double> &src) {
     HIPCALL(hipMemcpyAsync(dst, src.data(), src.size() * sizeof(double), hipMemcpyHostToDevice, st_));
     return AMCFD_OK;
@@ -54,7 +79,9 @@ void Solver::set_active_nk(int nk) {
 }
 
 int Solver::create(const amcfd_params &p, int mode) {
-    if (mode != AMCFD_MODE_FP64 && mode != AMCFD_MODE_SCAN) return set_error(AMCFD_E_ARG, "unknown solver mode");
+    if (mode != AMCFD_MODE_FP64 && mode != AMCFD_MODE_SCAN && mode != AMCFD_MODE_MIXED)
+        return set_error(AMCFD_E_ARG, "unknown solver mode");
+    const bool mixed = mode == AMCFD_MODE_MIXED;
     mode_ = mode;
     p_ = p;
     if (p.n_toolpath > 0) tool_.assign(p.toolpath, p.toolpath + 6 * p.n_toolpath);
@@ -64,14 +91,28 @@ int Solver::create(const amcfd_params &p, int mode) {
     p_.dt_table = dtt_.empty() ? nullptr : dtt_.data();
     p_.heat_table = heatt_.empty() ? nullptr : heatt_.data();
     if (p.ni < 4 || p.nj < 4 || p.nk < 4) return set_error(AMCFD_E_ARG, "grid too small");
-    if (p.ni - 2 > sweep_max_line())
+    if (!mixed && p.ni - 2 > sweep_max_line())
         return set_error(AMCFD_E_CAPACITY, "x-lines longer than " + std::to_string(sweep_max_line()) + " cells");
+    if (mixed && p.ni - 2 > mixed_max_line())
+        return set_error(AMCFD_E_CAPACITY, "x-lines longer than " + std::to_string(mixed_max_line()) +
+                                               " cells in the mixed mode");
+    if (mixed && (p.nj - 2 > mixed_max_yz() || p.nk - 2 > mixed_max_yz()))
+        return set_error(AMCFD_E_CAPACITY, "y- or z-lines longer than " + std::to_string(mixed_max_yz()) +
+                                               " cells in the mixed mode");
+    if (mixed && p.species)
+        return set_error(AMCFD_E_UNSUPPORTED, "species transport runs in the FP64 and scan modes only");
     if (p.nj >= 16384 || p.nk >= 16384) return set_error(AMCFD_E_CAPACITY, "more than 16383 nodes in y or z");
     if (p.maxit < 1) return set_error(AMCFD_E_ARG, "maxit must be >= 1");
     if (p.acpa == 0.0) return set_error(AMCFD_E_ARG, "acpa must be nonzero (solid H->T divides by it)");
 
     int dev = 0;
     HIPCALL(hipGetDevice(&dev));
+    if (mixed) {
+        int lds_max = 0;
+        HIPCALL(hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+        if (mixed_yz_lds(std::max(p.nj - 2, p.nk - 2)) > (size_t)lds_max)
+            return set_error(AMCFD_E_CAPACITY, "y- or z-lines too long for the device's LDS in the mixed mode");
+    }
     HIPCALL(hipStreamCreateWithFlags(&st_, hipStreamNonBlocking));
     HIPCALL(hipEventCreate(&step_ev_[0]));
     HIPCALL(hipEventCreate(&step_ev_[1]));
@@ -146,6 +187,7 @@ int Solver::create(const amcfd_params &p, int mode) {
     }
     InitialState is;
     initial_fields(p_, hg_, is);
+    if (mx_.halt && (st = upload(mx_.halt, is.enthalpy))) return st;   // finite ghost values
     if ((st = upload(f_.enthalpy, is.enthalpy)) || (st = upload(f_.hnot, is.hnot)) ||
         (st = upload(f_.temp, is.temp)) || (st = upload(f_.rho, is.rho)) || (st = upload(f_.vis, is.vis)) ||
         (st = upload(f_.diff, is.diff)) || (st = upload(f_.csfrac, is.csfrac)))
@@ -175,8 +217,22 @@ int Solver::create(const amcfd_params &p, int mode) {
     allocs_.push_back(v);
     counter_ = (unsigned long long *)v;
     HIPCALL(hipMemsetAsync(counter_, 0, sizeof(unsigned long long), st_));
-    // four pass buffers for the line sweeps
-    if ((st = alloc(&pass_buf_, 4 * nn))) return st;
+    if (!mixed) {   // four pass buffers for the Gauss-Seidel line sweeps
+        if ((st = alloc(&pass_buf_, 4 * nn))) return st;
+    } else {
+        float **c32[] = {&mx_.ap, &mx_.an, &mx_.as, &mx_.ae, &mx_.aw, &mx_.at, &mx_.ab};
+        for (auto *q : c32)
+            if ((st = allocf(q, nn))) return st;
+        mx_.su = f_.su;
+        Dims full = d_;   // partials sized for the full z extent
+        full.nk = d_.nkf;
+        full.nkm1 = d_.nkf - 1;
+        full.nkm2 = d_.nkf - 2;
+        if ((st = alloc(&mx_.blkpart, (size_t)4 * mixed_tile_count(full) * d_.ni))) return st;
+        if ((st = alloc(&mx_.blkfold, (size_t)4 * mixed_blk_split() * d_.ni))) return st;
+        if ((st = alloc(&mx_.delphi, (size_t)d_.ni + 2))) return st;
+        if ((st = alloc(&mx_.halt, nn))) return st;
+    }
 
     scanvel_ = p_.scanvel;
     set_active_nk(p_.nk_active0);
@@ -489,6 +545,13 @@ int Solver::sweep(double *phi, SweepPlan &plan, int ibeg, int iend, int jbeg, in
     return 0;
 }
 
+int Solver::raise_sweep_error() {
+    const int one = 1;
+    HIPCALL(hipMemcpyAsync(&sc_->error, &one, sizeof one, hipMemcpyHostToDevice, st_));
+    HIPCALL(hipStreamSynchronize(st_));
+    return AMCFD_OK;
+}
+
 int Solver::read_scalars() {
     HIPCALL(hipMemcpyAsync(sc_host_, sc_, sizeof(DevScalars), hipMemcpyDeviceToHost, st_));
     HIPCALL(hipStreamSynchronize(st_));
@@ -519,6 +582,7 @@ int Solver::flow_iteration() {
 }
 
 int Solver::solve_step() {
+    if (mode_ == AMCFD_MODE_MIXED) return solve_step_mixed();
     HIPCALL(hipEventRecord(step_ev_[0], st_));
     double tdma_before = timers.count("sweep_h") ? timers["sweep_h"].total_ms : 0.0;
     const double alenold = alen_;
@@ -584,6 +648,114 @@ int Solver::solve_step() {
     HIPCALL(hipEventElapsedTime(&ms, step_ev_[0], step_ev_[1]));
     last_total_ms = ms;
     last_tdma_ms = profile_ ? (timers["sweep_h"].total_ms - tdma_before) : -1.0;
+    if (!std::isfinite(sc_host_->resorh))
+        return set_error(AMCFD_E_NAN, "the enthalpy residual is not finite after step " + std::to_string(step_));
+    return rc;
+}
+
+// ---------------------------------------------------------------------------
+// mixed-precision mode: the same SIMPLE loop (CFDSolverManager.cpp:595-722)
+// with FP32 enthalpy coefficients and delta-form line passes in parallel
+// order (mixed.hip); properties, H->T, the melt pool, the heat balance, the
+// residual folds and the convergence test are the FP64 mode's kernels
+// ---------------------------------------------------------------------------
+int Solver::energy_sweeps_mixed() {
+    double *cur = f_.enthalpy, *other = mx_.halt;
+    for (char c : mx_sched_) {
+        if (c == 'x') {
+            launch_mixed_xline(st_, d_, mx_, cur, 1, d_.nim2, 1, d_.njm2, 1, d_.nkm2, 0, nullptr);
+            launch_mixed_xline(st_, d_, mx_, cur, 1, d_.nim2, 1, d_.njm2, 1, d_.nkm2, 1, nullptr);
+        } else {
+            launch_mixed_yzline(st_, d_, mx_, cur, other, c == 'y' ? 1 : 2, 1, d_.nim2, 1, d_.njm2, 1, d_.nkm2,
+                                nullptr);
+            std::swap(cur, other);
+        }
+    }
+    HIPCALL(hipGetLastError());
+    return 0;
+}
+
+int Solver::flow_iteration_mixed() {
+    box_ = Box{sc_host_->istat, sc_host_->iend, sc_host_->jstat, sc_host_->jend, sc_host_->kstat};
+    pr_.nblk_f = flow_partial_blocks(box_, d_);
+    launch_clean_velocities(st_, d_, f_, box_, p_.tsolid);
+    double *phis[5] = {nullptr, f_.uvel, f_.vvel, f_.wvel, f_.pp};
+    const int ib = box_.istatp1(), ie = box_.iendm1();
+    for (int ivar = 1; ivar <= 4; ++ivar) {
+        launch_bc_flow(st_, d_, g_, f_, box_, p_, ivar);
+        if (ivar < 4) launch_momentum(st_, d_, g_, f_, pr_, box_, p_, scanvel_, ivar);
+        else launch_pressure(st_, d_, g_, f_, pr_, box_, p_);
+        if (ie >= ib && box_.jend >= box_.jstat && d_.nkm2 >= box_.kstat)
+            for (int pass = 0; pass < mx_box_passes_; ++pass)
+                for (int colour = 0; colour < 2; ++colour)
+                    launch_box_xline(st_, d_, f_, phis[ivar], ib, ie, box_.jstat, box_.jend, box_.kstat, d_.nkm2,
+                                     colour, nullptr);
+        if (ivar == 4) launch_correct(st_, d_, f_, box_, p_);
+    }
+    HIPCALL(hipGetLastError());
+    return 0;
+}
+
+int Solver::solve_step_mixed() {
+    HIPCALL(hipEventRecord(step_ev_[0], st_));
+    const double alenold = alen_;
+    niter_ = 0;
+    int rc = AMCFD_I_MAXIT;
+    while (niter_ < p_.maxit) {
+        ++niter_;
+        ++itertot_;
+        SourceParams sp = source_params();
+        int slot;
+        launch_bc_enthalpy(st_, d_, g_, f_, pr_, p_);
+        tic("assemble_h", &slot);
+        launch_energy_assemble_mixed(st_, d_, g_, f_, pr_, mx_, p_, sp, nullptr);
+        toc("assemble_h", slot);
+        launch_props(st_, d_, f_, p_);
+        launch_mixed_block_correction(st_, d_, mx_, f_.enthalpy, nullptr);
+        tic("sweep_h", &slot);
+        int st = energy_sweeps_mixed();
+        toc("sweep_h", slot);
+        if (st) return st;
+        tic("h2t", &slot);
+        launch_h2t(st_, d_, g_, f_, pr_, p_);
+        toc("h2t", slot);
+        launch_melt_pool(st_, d_, g_, f_, pr_, p_, istart_, jstart_, sc_);
+        HIPCALL(hipGetLastError());
+        if ((st = read_scalars())) return st;
+        alen_ = sc_host_->alen;
+        int flow = 0;
+        if (sc_host_->tpeak > p_.tsolid && p_.navierstokes) {
+            if ((st = flow_iteration_mixed())) return st;
+            flow = 1;
+        }
+        launch_finalize(st_, d_, pr_, sc_, p_, flow, 0, p_.steady, sc_host_->width);
+        HIPCALL(hipGetLastError());
+        if ((st = read_scalars())) return st;
+        if (profile_) collect_timers();
+        const DevScalars &s = *sc_host_;
+        double res[4] = {s.resorm, s.resoru, s.resorv, s.resorw};
+        double amaxres = res[0];
+        for (int r = 1; r < 4; ++r)
+            if (amaxres < res[r]) amaxres = res[r];
+        const double ratio = s.ratio, tol = p_.nonlintol;
+        bool done;
+        if (p_.steady) done = ratio <= 1.001 && ratio >= 0.999 && amaxres < tol;
+        else if (powerindicator_ == 1) done = ratio <= 1.01 && ratio >= 0.99 && amaxres < tol;
+        else done = s.resorh < 1e-6 && amaxres < tol;
+        if (done) {
+            rc = AMCFD_OK;
+            break;
+        }
+    }
+    if (rc == AMCFD_I_MAXIT && std::fabs(alen_ - alenold) < 1.0e-6) steadylength_ = true;
+    HIPCALL(hipEventRecord(step_ev_[1], st_));
+    HIPCALL(hipEventSynchronize(step_ev_[1]));
+    float ms = 0.f;
+    HIPCALL(hipEventElapsedTime(&ms, step_ev_[0], step_ev_[1]));
+    last_total_ms = ms;
+    last_tdma_ms = -1.0;
+    if (!std::isfinite(sc_host_->resorh))
+        return set_error(AMCFD_E_NAN, "the enthalpy residual is not finite after step " + std::to_string(step_));
     return rc;
 }
 

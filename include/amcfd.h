@@ -57,6 +57,15 @@ extern "C" {
                                       differs from the reference's, so results
                                       agree to the iteration's sensitivity,
                                       not bit for bit                         */
+#define AMCFD_MODE_MIXED       2   /* enthalpy coefficients stored in FP32;
+                                      every line pass a delta-form correction
+                                      (FP64 residual, FP32 line solve, FP64
+                                      update) in parallel order: x-lines
+                                      red-black, y- and z-lines alternating;
+                                      H->T, heat balance, mass imbalance and
+                                      the convergence test stay FP64.  Agrees
+                                      with the FP64 mode to the tolerance of
+                                      the converged iteration, not bit for bit */
 
 /*
  * Every value the path reads from the reference's Mesh and DomainManager
@@ -245,6 +254,20 @@ int  amcfd_output_section(amcfd_solver *s, const char *base, int finish, int *wr
  * by default).  No reference counterpart: the reference runs one deck per
  * process. */
 int  amcfd_run_batch(amcfd_solver *const *s, int n, int max_steps, int workers, int *steps_done);
+
+/* Mixed mode only: the order of the enthalpy line passes of one SIMPLE
+ * iteration ('x' = x-lines red then black, in place; 'y' / 'z' = y- or
+ * z-lines, one lane per line, between the field and a second buffer; an
+ * even number of 'y'/'z'; default "xzxy") and the red-black x passes per
+ * melt-pool box variable (default 4).  The reference sweeps x-lines only,
+ * four Gauss-Seidel passes (CFDSolverManager.cpp:2415-2453, 2492-2525). */
+int  amcfd_set_mixed_schedule(amcfd_solver *s, const char *schedule, int box_passes);
+
+/* Test hook, no reference counterpart: raise the device error flag that
+ * the Gauss-Seidel line sweeps' bounded waits watch, so that every wait of
+ * the next sweep gives up at once, every kernel still ends, and the solve
+ * returns AMCFD_E_HIP.  The flag stays raised for the solver's life. */
+int  amcfd_test_expire_waits(amcfd_solver *s);
 
 /* Thread-local description of the last error. */
 const char *amcfd_last_error(void);

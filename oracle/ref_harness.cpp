@@ -137,6 +137,14 @@ int main(int argc, char **argv) {
             std::snprintf(buf, sizeof buf, "%s/f%05d_%s.bin", outdir.c_str(), step, xnames[f]);
             write_raw(buf, xptr[f], nn);
         }
+        // powder history fields, written on powder-bed decks
+        const char *pnames[] = {"csfrac", "solfrac", "mpPhase", "mpLiqPhase"};
+        const double *pptr[] = {cfd->csfrac_, cfd->solfrac_, cfd->mpPhase, cfd->mpLiqPhase};
+        for (int f = 0; meshObj.powderbed_ && f < 4; ++f) {
+            char buf[512];
+            std::snprintf(buf, sizeof buf, "%s/f%05d_%s.bin", outdir.c_str(), step, pnames[f]);
+            write_raw(buf, pptr[f], nn);
+        }
     };
     if (fields_every > 0) dump_fields(0);
 

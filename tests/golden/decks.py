@@ -42,6 +42,14 @@ The named cases are the workloads the solver is measured and checked on:
                 r=50 um, 1 m/s: the CPU-runnable single-track case.
 * ``track400``  400x160x80 cells (4.0x1.2x0.6 mm, ~5.1M cells), P=300 W,
                 r=60 um, 0.8 m/s, dt=2e-6 s: the single-GPU throughput case.
+* ``multilayer`` two 40 um powder layers (conductivity 0.3x bulk) of four
+                tracks each, 80 um hatch, alternating +x/-x, 3.0x1.0x0.8 mm
+                on 480x160x126 cells (~9.7M), P=250 W, 1 m/s, dt=2e-6 s; the
+                laser stays on across the hatch jumps inside a layer, so only
+                the layer change (laser off, then on) adds the second layer.
+* ``multilayer_small`` the same toolpath pattern (two layers, four tracks,
+                80 um hatch, alternating scan) on a 60x48x24-cell plate that
+                the unmodified reference runs (dt=5e-6 s, 100 um tracks).
 """
 import os
 
@@ -193,6 +201,47 @@ TOOL_MOVING_LAYERS = [(0.0, 0.25e-3, 0.15e-3, ZTOP, 1, 1), (60e-6, 0.31e-3, 0.15
                       (135e-6, 0.31e-3, 0.13e-3, ZTOP + 2e-5, 1, 2),
                       (200e-6, 0.31e-3, 0.13e-3, ZTOP + 2e-5, 0, 2)]
 
+
+def multilayer_toolpath(x0, x1, y0, hatch, ntracks, z0, layer, speed, hatch_time, change_time,
+                        layers=2, t0=0.0):
+    """Rows (t, x, y, z, laser_on, track) of a multi-track, multi-layer scan:
+    per layer ntracks tracks between x0 and x1 (alternating +x/-x), joined by
+    hatch jumps of `hatch` in +y taken with the laser on; between layers the
+    laser is off while the head returns to (x0, y0) one layer higher.  Each
+    laser off -> on transition adds one powder layer (BoundCondManager.cpp:
+    265-273), so only the layer changes do."""
+    rows = []
+    t = t0
+    track = 1
+    for lay in range(layers):
+        z = z0 + lay * layer
+        x, y = x0, y0
+        if lay == 0:
+            rows.append((t, x, y, z, 1, track))
+        else:
+            t += change_time
+            rows.append((t, x, y, z, 0, track - 1))
+        for n in range(ntracks):
+            if n > 0:
+                y += hatch
+                t += hatch_time
+                rows.append((t, x, y, z, 1, track))
+            x = x1 if x == x0 else x0
+            t += abs(x1 - x0) / speed
+            rows.append((t, x, y, z, 1, track))
+            track += 1
+    rows.append((t + hatch_time, rows[-1][1], rows[-1][2], rows[-1][3], 0, track - 1))
+    return rows
+
+
+# configs: two 40 um layers, 4 tracks each, 80 um hatch, 1 m/s
+TOOL_MULTILAYER = multilayer_toolpath(0.5e-3, 2.5e-3, 0.38e-3, 80e-6, 4, 0.72e-3, 40e-6, 1.0, 80e-6, 0.5e-3)
+TOOL_MULTILAYER_SMALL = multilayer_toolpath(0.3e-3, 0.4e-3, 0.18e-3, 80e-6, 4, 0.16e-3, 40e-6, 1.0,
+                                            20e-6, 25e-6)
+# powder: 2 layers, ncv per layer, 40 um, density, cpa, cpb, conductivity a, b (0.3 x 25 W/mK)
+POWDER_MULTILAYER = (2, 6, 40e-6, 4000.0, 1.0e-6, 700.0, 0.0, 7.5)
+POWDER_MULTILAYER_SMALL = (2, 4, 40e-6, 4000.0, 1.0e-6, 700.0, 0.0, 7.5)
+
 CASES = {
     "tiny": dict(
         mesh=(([0.1e-3, 0.3e-3, 0.1e-3], [4, 12, 4], [-1.2, 1.0, 1.2]),
@@ -329,6 +378,20 @@ CASES = {
         mesh=(([4.0e-3], [400], [1.0]), ([1.2e-3], [160], [1.0]), ([0.6e-3], [80], [1.0])),
         deck=dict(finaltime=3.999e-3, maxit=100, dt=2e-6, lasertime=3.0e-3,
                   xstart=0.4e-3, ystart=0.6e-3, speed=0.8, power=300.0, radius=60e-6)),
+    # z: a 0.64 mm base zone and a 0.08 mm top zone that the powder layers
+    # extend by 2 x 40 um (6 cells each): 126 cells, 0.8 mm
+    "multilayer": dict(
+        mesh=(([3.0e-3], [480], [1.0]), ([1.0e-3], [160], [1.0]),
+              ([0.64e-3, 0.08e-3], [102, 12], [1.0, 1.0])),
+        deck=dict(finaltime=TOOL_MULTILAYER[-1][0], maxit=100, dt=2e-6, lasertime=TOOL_MULTILAYER[-1][0],
+                  xstart=0.5e-3, ystart=0.38e-3, speed=1.0, power=250.0, radius=50e-6,
+                  powder=POWDER_MULTILAYER, toolpath=TOOL_MULTILAYER)),
+    "multilayer_small": dict(
+        mesh=(([0.75e-3], [60], [1.0]), ([0.6e-3], [48], [1.0]),
+              ([0.12e-3, 0.04e-3], [12, 4], [1.0, 1.0])),
+        deck=dict(finaltime=TOOL_MULTILAYER_SMALL[-1][0] - 1e-9, maxit=40, dt=5e-6,
+                  lasertime=TOOL_MULTILAYER_SMALL[-1][0], xstart=0.3e-3, ystart=0.18e-3, speed=1.0,
+                  power=250.0, radius=50e-6, powder=POWDER_MULTILAYER_SMALL, toolpath=TOOL_MULTILAYER_SMALL)),
 }
 
 

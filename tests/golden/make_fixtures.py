@@ -40,8 +40,9 @@ FIELDS = ["enthalpy", "temp", "fracl", "uvel", "vvel", "wvel", "pressure", "pp",
           "vis", "hnot", "fraclnot", "unot", "vnot", "wnot", "sourceinput", "dux", "dvy", "dwz"]
 # table files a deck may name: suffix, placeholder in <case>_deck.txt, fixture kind
 AUX_FILES = [("_tool.crs", "@TOOL@", "tool"), ("_dt.k", "@DT@", "dt"), ("_heat.k", "@HEAT@", "heat")]
-# written by the harness only when the deck enables species / G/R
-OPTIONAL_FIELDS = ["avgconcentration", "avgconcentrationnot", "concentration", "grad", "rate", "zr", "tempr"]
+# written by the harness only when the deck enables species / G/R / free surface / powder bed
+OPTIONAL_FIELDS = ["avgconcentration", "avgconcentrationnot", "concentration", "grad", "rate", "zr", "tempr",
+                   "csfrac", "solfrac", "mpPhase", "mpLiqPhase"]
 # case -> (fields_every, max_steps, steps whose full fields are kept)
 PLAN = {
     "tiny": (1, -1, [2, 40]),
@@ -60,6 +61,7 @@ PLAN = {
     "tiny_heattable": (1, -1, []),
     "tiny_steady": (1, -1, []),
     "plate120": (50, -1, []),
+    "multilayer_small": (5, -1, []),
 }
 
 

@@ -1,0 +1,91 @@
+"""Mixed-precision mode (AMCFD_MODE_MIXED) on the GPU: FP32 enthalpy
+coefficients, delta-form line passes (FP64 residual, FP32 solve) in
+parallel order.  It is not bit-compatible with the reference by design; the
+tests hold it to the FP64 mode (itself bitwise equal to the oracle) within
+stated bounds, check reproducibility and the line lengths it adds."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def rel_l2(a, b):
+    nb = np.linalg.norm(b)
+    return float(np.linalg.norm(a - b) / nb) if nb else float(np.linalg.norm(a))
+
+
+def run_pair(amcfd, path, steps, schedule=None):
+    ref = amcfd.Solver(path, mode=amcfd.MODE_FP64)
+    mx = amcfd.Solver(path, mode=amcfd.MODE_MIXED)
+    if schedule:
+        mx.set_mixed_schedule(schedule)
+    rows = []
+    for step in range(1, steps + 1):
+        print("mixed vs fp64 %s step %d" % (path, step), flush=True)
+        ref.step()
+        mx.step()
+        sr, sm = ref.scalars(), mx.scalars()
+        rows.append((step, sr["niter"], sm["niter"], rel_l2(mx.field("temp"), ref.field("temp")), sr, sm))
+    return ref, mx, rows
+
+
+def test_mixed_mode_is_reproducible(case_dir, amcfd):
+    path = case_dir("tiny")
+    out = []
+    for _ in range(2):
+        s = amcfd.Solver(path, mode=amcfd.MODE_MIXED)
+        s.run(max_steps=5)
+        out.append({f: s.field(f) for f in ("enthalpy", "uvel", "pp")})
+        s.close()
+    for f in out[0]:
+        assert np.array_equal(out[0][f], out[1][f]), f
+
+
+def test_mixed_mode_conduction_tracks_fp64(case_dir, amcfd):
+    """Conduction only (no flow): the mixed mode's temperature stays close to
+    the FP64 mode's over the whole run."""
+    ref, mx, rows = run_pair(amcfd, case_dir("tiny_cond"), 20)
+    for step, n64, nmx, eT, sr, sm in rows:
+        print("step %d niter fp64 %d mixed %d  T rel-L2 %.2e" % (step, n64, nmx, eT))
+    assert max(r[3] for r in rows) < 1e-2
+    assert np.isfinite(mx.field("enthalpy")).all()
+
+
+def test_mixed_mode_molten_runs(case_dir, amcfd):
+    """Molten single track (flow on, maxit-bound): finite, and the pool
+    tracks the FP64 run's to within the iteration's own sensitivity."""
+    ref, mx, rows = run_pair(amcfd, case_dir("tiny"), 12)
+    for step, n64, nmx, eT, sr, sm in rows:
+        print("step %d niter %d/%d T %.2e tpeak %.1f/%.1f depth %.3e/%.3e" % (
+            step, n64, nmx, eT, sr["tpeak"], sm["tpeak"], sr["depth"], sm["depth"]))
+    assert np.isfinite(mx.field("uvel")).all() and np.isfinite(mx.field("enthalpy")).all()
+    assert max(r[3] for r in rows) < 5e-2
+
+
+def test_mixed_mode_lines_beyond_the_fp64_cap(amcfd, tmp_path):
+    """x-lines of 2000 interior cells: refused by the FP64 sweeps (1024 cap),
+    solved by the mixed mode's wave solver; conduction only, compared with
+    the oracle's FP64 run."""
+    import decks
+    import oracle
+    oracle.load()
+    decks.CASES["tiny_xlong"] = dict(
+        mesh=(([0.1e-3, 4.0e-3, 0.1e-3], [10, 1980, 10], [-1.2, 1.0, 1.2]),
+              ([0.15e-3, 0.15e-3], [3, 3], [1.0, 1.0]),
+              ([0.15e-3, 0.1e-3], [2, 4], [1.0, 1.0])),
+        deck=dict(finaltime=0.02e-3 - 1e-9, maxit=40, dt=5e-6, lasertime=0.02e-3,
+                  xstart=0.15e-3, ystart=0.15e-3, speed=1.0, power=200.0, radius=50e-6, flow=False))
+    path = decks.write_case("tiny_xlong", str(tmp_path))
+    deck = amcfd.Deck(path)
+    assert deck.params.ni - 2 == 2000
+    with pytest.raises(amcfd.AmcfdError):
+        amcfd.Solver(deck, mode=amcfd.MODE_FP64)
+    mx = amcfd.Solver(deck, mode=amcfd.MODE_MIXED)
+    orc = oracle.Oracle(deck.params)
+    for step in range(4):
+        orc.step()
+        mx.step()
+        e = rel_l2(mx.field("temp"), orc.field("temp"))
+        print("tiny_xlong step %d T rel-L2 vs oracle %.2e niter %d/%d" % (
+            step + 1, e, mx.scalars()["niter"], orc.scalars()["niter"]))
+        assert e < 1e-3

@@ -258,3 +258,101 @@ def test_gpu_matches_oracle_sweep_instance(amcfd, oracle_mod, tmp_path, inst):
     steps, worst = compare_run(amcfd, oracle_mod, path, max_steps=3)
     print("sweep instance", inst, steps, {k: "%.1e" % v for k, v in worst.items()})
     assert steps == 3
+
+
+def test_gpu_matches_oracle_multilayer_small(case_dir, amcfd, oracle_mod):
+    """Two 40 um powder layers of four alternating tracks each with an 80 um
+    hatch (laser on across the hatch jumps, off between the layers), on the
+    reduced plate the unmodified reference runs: every field and the powder
+    history fields after every one of the 193 steps, the second layer's
+    activation (nk 22 -> 26 at step 98) included."""
+    steps, worst = compare_run(amcfd, oracle_mod, case_dir("multilayer_small"), fields=POWDER_FIELDS,
+                               geometry=("z", "dzpbinv", "fracz"))
+    print("multilayer_small", steps, {k: "%.1e" % v for k, v in worst.items()})
+    assert steps == 193
+
+
+def test_gpu_matches_oracle_track400_first_steps(case_dir, amcfd, oracle_mod):
+    """The 400x160x80 single track (5.12M cells, the throughput workload) in
+    the plain FP64 mode against the oracle over its first two steps, bit for
+    bit (the second runs to maxit)."""
+    steps, worst = compare_run(amcfd, oracle_mod, case_dir("track400"), max_steps=2,
+                               fields=("enthalpy", "temp", "fracl", "uvel", "vvel", "wvel", "pp", "pressure"))
+    print("track400", steps, {k: "%.1e" % v for k, v in worst.items()})
+    assert steps == 2
+
+
+def multilayer_layer_change_toolpath():
+    """The full-size multi-layer deck's layer change, brought forward: two
+    steps on the last track of the first layer, two with the laser off while
+    the head moves up one layer, then the first track of the second layer
+    (the laser's off -> on transition adds the second powder layer)."""
+    import decks
+    z1 = decks.TOOL_MULTILAYER[0][3]
+    z2 = z1 + decks.POWDER_MULTILAYER[2]
+    return [(0.0, 0.5e-3, 0.62e-3, z1, 1, 4), (4e-6, 0.504e-3, 0.62e-3, z1, 1, 4),
+            (8e-6, 0.5e-3, 0.38e-3, z2, 0, 4), (2.008e-3, 2.5e-3, 0.38e-3, z2, 1, 5)]
+
+
+def test_multilayer_full_size_properties(case_dir, amcfd):
+    """The ~9.7M-cell multi-layer deck (480x160x126 cells) in the plain FP64
+    mode, too large for the oracle: over its layer change the fields stay
+    finite, the active nk grows by one layer exactly when the laser comes back
+    on, the heat balance sits in the band the convergence test uses once the
+    step converges, and two runs give the same bits."""
+    path = case_dir("multilayer", toolpath=multilayer_layer_change_toolpath(), finaltime=12e-6 - 1e-9,
+                    lasertime=12e-6)
+    runs = []
+    for rep in range(2):
+        s = amcfd.Solver(path)
+        nks, pis = [], []
+        fin = False
+        while not fin:
+            print("multilayer run %d step %d" % (rep, len(nks) + 1), flush=True)
+            fin = s.step()
+            sc = s.scalars()
+            nks.append(sc["nk"])
+            pis.append(sc["powerindicator"])
+            for f in ("enthalpy", "temp", "uvel", "vvel", "wvel", "pressure"):
+                assert np.isfinite(s.field(f)).all(), (rep, len(nks), f)
+            assert sc["tpeak"] > 300.0
+            if sc["niter"] < s.params.maxit and sc["powerindicator"] == 1:
+                assert 0.99 <= sc["ratio"] <= 1.01, (len(nks), sc["ratio"])
+        runs.append({f: s.field(f) for f in ("enthalpy", "uvel", "pp", "csfrac", "solfrac", "mpPhase")})
+        s.close()
+    nk0 = nks[0]
+    assert pis == [1, 1, 0, 0, 1, 1], pis
+    assert nks == [nk0] * 4 + [nk0 + 6] * 2, nks
+    for f in runs[0]:
+        assert np.array_equal(runs[0][f], runs[1][f], equal_nan=True), f
+
+
+def test_non_finite_state_is_reported(case_dir, amcfd):
+    """A NaN in the enthalpy makes the step return AMCFD_E_NAN (the reference
+    carries NaN on silently, CFDSolverManager.cpp:612-717)."""
+    s = amcfd.Solver(case_dir("tiny"))
+    s.step()
+    h = s.field("enthalpy")
+    h[5, 5, 5] = np.nan
+    s.set_field("enthalpy", h)
+    with pytest.raises(amcfd.AmcfdError) as e:
+        s.step()
+    assert e.value.code == -6
+    s.close()
+
+
+def test_expired_waits_end_the_step(case_dir, amcfd):
+    """With the sweeps' error flag raised from the host, every bounded wait of
+    the Gauss-Seidel line sweep gives up at once, the kernels end, and the
+    step returns AMCFD_E_HIP instead of hanging."""
+    s = amcfd.Solver(case_dir("tiny"))
+    s.step()
+    s.test_expire_waits()
+    with pytest.raises(amcfd.AmcfdError) as e:
+        s.step()
+    assert e.value.code == -5
+    s.close()
+    ok = amcfd.Solver(case_dir("tiny"))   # the device is still usable
+    ok.run(max_steps=2)
+    assert np.isfinite(ok.field("enthalpy")).all()
+    ok.close()

@@ -13,7 +13,7 @@ from conftest import golden_path
 
 CASES = ["tiny", "tiny_cond", "tiny_evap", "tiny_vap", "tiny_vol", "tiny_tool", "tiny_powder",
          "tiny_move", "tiny_move_powder", "tiny_species", "tiny_move_species", "tiny_fs", "tiny_dtvar",
-         "tiny_heattable", "tiny_steady"]
+         "tiny_heattable", "tiny_steady", "multilayer_small"]
 AUX_FILES = [("_tool.crs", "@TOOL@", "tool"), ("_dt.k", "@DT@", "dt"), ("_heat.k", "@HEAT@", "heat")]
 MOVING = ["tiny_move", "tiny_move_powder", "tiny_move_species"]
 INT_KEYS = ["niter", "itertot", "powerindicator", "istat", "iend", "jstat", "jend", "kstat", "jstart",
