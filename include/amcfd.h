@@ -255,6 +255,20 @@ int  amcfd_output_section(amcfd_solver *s, const char *base, int finish, int *wr
  * process. */
 int  amcfd_run_batch(amcfd_solver *const *s, int n, int max_steps, int workers, int *steps_done);
 
+/* The reference's x-line TDMA over many independent lines held in device
+ * memory: the recurrence of CFDSolverManager::solveEntireDomain
+ * (CFDSolverManager.cpp:2500-2522; guard = 0) or, with the |den| <= 1e-12
+ * guard, CFDSolverManager::solveLiquidDomain (:2421-2450), with the
+ * reference's roundings.  Line l occupies entries [l*n, (l+1)*n) of every
+ * array; phi[l*n] and phi[l*n+n-1] are the fixed end values, rows 1..n-2 are
+ * solved from aw, ap, ae and the right-hand side d (su plus the neighbour
+ * lines' terms).  Device pointers; stream is a hipStream_t (NULL: the null
+ * stream); returns once the work is queued.  The pr/qr scratch (2*n*nlines
+ * doubles) is a per-device workspace that the library allocates on first use
+ * and grows; calls that overlap on one device share it and race. */
+int  amcfd_tdma_lines(const double *aw, const double *ap, const double *ae, const double *d, double *phi,
+                      int n, int nlines, int guard, void *stream);
+
 /* Mixed mode only: the order of the enthalpy line passes of one SIMPLE
  * iteration ('x' = x-lines red then black, in place; 'y' / 'z' = y- or
  * z-lines, one lane per line, between the field and a second buffer; an
