@@ -112,6 +112,9 @@ struct AsmConst {
 // renumbered so that each XCD takes one contiguous run of them (x, then y,
 // then z), which keeps the j-neighbour rows of a tile in that XCD's L2.
 constexpr int KA = 8;
+#ifndef MX_ASM_WAVES
+#define MX_ASM_WAVES 2
+#endif
 
 __device__ __forceinline__ int xcd_tile(int b, int nb) {
     const int q = nb / 8, r = nb % 8, x = b % 8, idx = b / 8;
@@ -131,7 +134,7 @@ struct AsmOut {
 };
 
 template <bool MX>
-__global__ __launch_bounds__(TX *TY) void k_energy_assemble(
+__device__ __forceinline__ void energy_assemble_body(
     Dims d, AsmConst q, DevGeom g, const double *__restrict__ uvel, const double *__restrict__ vvel,
     const double *__restrict__ wvel, const double *__restrict__ rho, const double *__restrict__ diff,
     const double *__restrict__ hnot, const double *__restrict__ fracl, const double *__restrict__ fraclnot,
@@ -326,6 +329,25 @@ __global__ __launch_bounds__(TX *TY) void k_energy_assemble(
         part_res[2 * t] = v[0];
         part_res[2 * t + 1] = v[1];
     }
+}
+
+__global__ __launch_bounds__(TX *TY) void k_energy_assemble(
+    Dims d, AsmConst q, DevGeom g, const double *__restrict__ uvel, const double *__restrict__ vvel,
+    const double *__restrict__ wvel, const double *__restrict__ rho, const double *__restrict__ diff,
+    const double *__restrict__ hnot, const double *__restrict__ fracl, const double *__restrict__ fraclnot,
+    const double *__restrict__ h, AsmOut o) {
+    energy_assemble_body<false>(d, q, g, uvel, vvel, wvel, rho, diff, hnot, fracl, fraclnot, h, o, nullptr);
+}
+
+// the mixed-mode form is memory-bound with little else to hide latency, so
+// it trades registers for more waves per SIMD
+__global__ __launch_bounds__(TX *TY) __attribute__((amdgpu_waves_per_eu(MX_ASM_WAVES, 8))) void
+k_energy_assemble_mx(Dims d, AsmConst q, DevGeom g, const double *__restrict__ uvel, const double *__restrict__ vvel,
+                     const double *__restrict__ wvel, const double *__restrict__ rho,
+                     const double *__restrict__ diff, const double *__restrict__ hnot,
+                     const double *__restrict__ fracl, const double *__restrict__ fraclnot,
+                     const double *__restrict__ h, AsmOut o, const int *__restrict__ skip) {
+    energy_assemble_body<true>(d, q, g, uvel, vvel, wvel, rho, diff, hnot, fracl, fraclnot, h, o, skip);
 }
 
 // ------------------------------------------------------------------------
@@ -934,8 +956,8 @@ void launch_energy_assemble(hipStream_t st, const Dims &d, const DevGeom &g, Dev
                s.beamposx, s.beamposy, s.powerindicator, p.discretization};
     AsmOut o{f.an, f.as, f.ae, f.aw, f.at, f.ab, f.ap, f.su, f.sourceinput, pr.energy, f.sp, f.apnot,
              nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
-    hipLaunchKernelGGL(k_energy_assemble<false>, assemble_grid(d), dim3(TX, TY), 0, st, d, q, g, f.uvel, f.vvel,
-                       f.wvel, f.rho, f.diff, f.hnot, f.fracl, f.fraclnot, f.enthalpy, o, nullptr);
+    hipLaunchKernelGGL(k_energy_assemble, assemble_grid(d), dim3(TX, TY), 0, st, d, q, g, f.uvel, f.vvel, f.wvel,
+                       f.rho, f.diff, f.hnot, f.fracl, f.fraclnot, f.enthalpy, o);
 }
 
 void launch_energy_assemble_mixed(hipStream_t st, const Dims &d, const DevGeom &g, DevFields f, Partials pr,
@@ -944,7 +966,7 @@ void launch_energy_assemble_mixed(hipStream_t st, const Dims &d, const DevGeom &
                s.beamposx, s.beamposy, s.powerindicator, p.discretization};
     AsmOut o{nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, m.su, f.sourceinput, pr.energy,
              nullptr, nullptr, m.an, m.as, m.ae, m.aw, m.at, m.ab, m.ap, m.blkpart};
-    hipLaunchKernelGGL(k_energy_assemble<true>, assemble_grid(d), dim3(TX, TY), 0, st, d, q, g, f.uvel, f.vvel,
+    hipLaunchKernelGGL(k_energy_assemble_mx, assemble_grid(d), dim3(TX, TY), 0, st, d, q, g, f.uvel, f.vvel,
                        f.wvel, f.rho, f.diff, f.hnot, f.fracl, f.fraclnot, f.enthalpy, o, skip);
 }
 

@@ -28,6 +28,10 @@
 
 #include "fv.h"
 
+// multiply-adds may fuse here: this mode does not reproduce the reference's
+// roundings, and its FP64 residuals are the refinement's accuracy anyway
+#pragma clang fp contract(fast)
+
 namespace amcfd {
 
 namespace {
@@ -178,7 +182,14 @@ __global__ __launch_bounds__(64) void k_xline_mx(Dims d, MxCoefs<TC> cf, double 
     const int n = iend - ibeg + 1, nj = jend - jbeg + 1, nk = kend - kbeg + 1;
     const int cm = (nj + 1) / 2;
     const long long nlines = (long long)nk * cm;
-    for (long long u = blockIdx.x; u < nlines; u += gridDim.x) {
+    // workgroups go to the 8 XCDs round-robin by id; give each XCD one
+    // contiguous run of lines, so that the lines sharing a neighbour line
+    // (j +- 2 and k +- 2 of the same colour) read it through one L2
+    const int G = gridDim.x, q8 = G / 8, r8 = G % 8, x8 = blockIdx.x % 8, i8 = blockIdx.x / 8;
+    const int t = (x8 < r8) ? x8 * (q8 + 1) + i8 : r8 * (q8 + 1) + (x8 - r8) * q8 + i8;
+    const long long chunk = (nlines + G - 1) / G;
+    const long long u0 = (long long)t * chunk, u1 = min(u0 + chunk, nlines);
+    for (long long u = u0; u < u1; ++u) {
         const int kk = (int)(u / cm), jj = (int)(u % cm);
         const int k = kbeg + kk;
         const int j = jbeg + ((colour - k - jbeg) & 1) + 2 * jj;
@@ -237,8 +248,30 @@ __global__ __launch_bounds__(64) void k_xline_mx(Dims d, MxCoefs<TC> cf, double 
 // (Jacobi between lines: no line reads another's update).  The forward
 // Thomas values of every row go to LDS, then back substitution writes
 // hout = hin + dphi.  blockIdx.y is the coordinate the lines do not run
-// along (k for y-lines, j for z-lines).
+// along (k for y-lines, j for z-lines).  Few lines per device means few
+// waves, so the next row's loads are in flight while the lane works on the
+// current one, and the line's own neighbours come from the rows held.
 // ------------------------------------------------------------------------
+template <typename TC>
+struct YzRow {
+    float ap, an, as, ae, aw, at, ab;
+    double su, h, hw, he, hx1, hx2;   // own, i-1, i+1, the two cross-line neighbours
+};
+
+template <typename TC, int DIR>
+__device__ __forceinline__ void yz_load(const Dims &d, const MxCoefs<TC> &cf, const double *__restrict__ hin,
+                                        size_t c, YzRow<TC> &r) {
+    r.ap = (float)cf.ap[c]; r.an = (float)cf.an[c]; r.as = (float)cf.as[c]; r.ae = (float)cf.ae[c];
+    r.aw = (float)cf.aw[c]; r.at = (float)cf.at[c]; r.ab = (float)cf.ab[c];
+    r.su = cf.su[c];
+    r.h = hin[c];
+    r.hw = hin[c - 1];
+    r.he = hin[c + 1];
+    const size_t x = DIR == 1 ? d.sk : d.sj;   // the direction across the line
+    r.hx1 = hin[c + x];
+    r.hx2 = hin[c - x];
+}
+
 template <typename TC, int DIR>
 __global__ __launch_bounds__(64) void k_yzline_mx(Dims d, MxCoefs<TC> cf, const double *__restrict__ hin,
                                                    double *__restrict__ hout, int ibeg, int iend, int jbeg, int jend,
@@ -246,36 +279,41 @@ __global__ __launch_bounds__(64) void k_yzline_mx(Dims d, MxCoefs<TC> cf, const 
     extern __shared__ float sbuf[];   // [L][64] c, then [L][64] g
     if (skip && *skip) return;
     const int lane = threadIdx.x;
-    const int i = ibeg + blockIdx.x * 64 + lane;
-    const bool act = i <= iend;
+    const int i = min(ibeg + (int)blockIdx.x * 64 + lane, iend);   // idle lanes repeat the last line, unstored
+    const bool act = ibeg + (int)blockIdx.x * 64 + lane <= iend;
     const int L = DIR == 1 ? jend - jbeg + 1 : kend - kbeg + 1;
     const size_t stride = DIR == 1 ? d.sj : d.sk;
     const size_t base = DIR == 1 ? cix(d, i, jbeg, kbeg + blockIdx.y) : cix(d, i, jbeg + blockIdx.y, kbeg);
     float *sc = sbuf, *sg = sbuf + (size_t)L * 64;
     float cp = 0.f, gp = 0.f;
-    if (act) {
-        for (int s = 0; s < L; ++s) {
-            const size_t c = base + (size_t)s * stride;
-            const double h = hin[c];
-            const double ap = (double)cf.ap[c];
-            const double an = (double)cf.an[c], as = (double)cf.as[c], ae = (double)cf.ae[c], aw = (double)cf.aw[c];
-            const double at = (double)cf.at[c], ab = (double)cf.ab[c];
-            const double r = cf.su[c] + aw * hin[c - 1] + ae * hin[c + 1] + an * hin[c + d.sj] + as * hin[c - d.sj] +
-                             at * hin[c + d.sk] + ab * hin[c - d.sk] - ap * h;
-            const float sub = DIR == 1 ? (float)as : (float)ab;
-            const float sup = DIR == 1 ? (float)an : (float)at;
-            const float den = (float)ap - sub * cp;
-            cp = sup / den;
-            gp = ((float)r + sub * gp) / den;
-            sc[s * 64 + lane] = cp;
-            sg[s * 64 + lane] = gp;
-        }
-        float x = 0.f;
-        for (int s = L - 1; s >= 0; --s) {
-            x = sg[s * 64 + lane] + sc[s * 64 + lane] * x;
-            const size_t c = base + (size_t)s * stride;
-            hout[c] = hin[c] + (double)x;
-        }
+    YzRow<TC> cur, nxt;
+    yz_load<TC, DIR>(d, cf, hin, base, cur);
+    double hprev = hin[base - stride];   // the fixed end below the line
+    for (int s = 0; s < L; ++s) {
+        const bool more = s + 1 < L;
+        if (more) yz_load<TC, DIR>(d, cf, hin, base + (size_t)(s + 1) * stride, nxt);
+        // the next row's own value, or the fixed end above the line
+        const double hnext = more ? nxt.h : hin[base + (size_t)L * stride];
+        double hN, hS, hT, hB;
+        if (DIR == 1) { hN = hnext; hS = hprev; hT = cur.hx1; hB = cur.hx2; }
+        else { hT = hnext; hB = hprev; hN = cur.hx1; hS = cur.hx2; }
+        const double res = cur.su + (double)cur.aw * cur.hw + (double)cur.ae * cur.he + (double)cur.an * hN +
+                           (double)cur.as * hS + (double)cur.at * hT + (double)cur.ab * hB - (double)cur.ap * cur.h;
+        const float sub = DIR == 1 ? cur.as : cur.ab;
+        const float sup = DIR == 1 ? cur.an : cur.at;
+        const float den = cur.ap - sub * cp;
+        cp = sup / den;
+        gp = ((float)res + sub * gp) / den;
+        sc[s * 64 + lane] = cp;
+        sg[s * 64 + lane] = gp;
+        hprev = cur.h;
+        if (more) cur = nxt;
+    }
+    float x = 0.f;
+    for (int s = L - 1; s >= 0; --s) {
+        x = sg[s * 64 + lane] + sc[s * 64 + lane] * x;
+        const size_t c = base + (size_t)s * stride;
+        if (act) hout[c] = hin[c] + (double)x;
     }
 }
 
@@ -295,24 +333,35 @@ __global__ __launch_bounds__(64) void k_blk_fold(int ni, int ntile, const double
     out[((size_t)q * BF_SPLIT + s) * ni + i] = a;
 }
 
+constexpr int BS_NT = 1024;
+constexpr int BS_MAXNI = 4100;   // ni of the longest x-line, rounded up
+
 template <int M>
-__global__ __launch_bounds__(64) void k_blk_solve_mx(int nim2, int ni, const double *__restrict__ f1, double *delphi,
-                                                      const int *__restrict__ skip) {
+__global__ __launch_bounds__(BS_NT) void k_blk_solve_mx(int nim2, int ni, const double *__restrict__ f1,
+                                                        double *delphi, const int *__restrict__ skip) {
+    __shared__ double sq[4][BS_MAXNI];
     if (skip && *skip) return;
+    // second fold level, all threads: sq[q][i] = sum over the splits
+    for (int e = threadIdx.x; e < 4 * ni; e += BS_NT) {
+        const int q = e / ni, i = e % ni;
+        double a = 0.0;
+#pragma unroll
+        for (int sp = 0; sp < BF_SPLIT; ++sp) a += f1[((size_t)q * BF_SPLIT + sp) * ni + i];
+        sq[q][i] = a;
+    }
+    __syncthreads();
+    if (threadIdx.x >= 64) return;
     const int lane = threadIdx.x;
     const int n = nim2;
     double w[M], dg[M], e[M], r[M];
 #pragma unroll
     for (int q = 0; q < M; ++q) {
         const int p = lane * M + q, i = p + 1;
-        double v[4] = {0.0, 0.0, 0.0, 0.0};
-        if (p < n)
-            for (int qq = 0; qq < 4; ++qq)
-                for (int s = 0; s < BF_SPLIT; ++s) v[qq] += f1[((size_t)qq * BF_SPLIT + s) * ni + i];
-        dg[q] = p < n ? v[0] : 1.0;
-        e[q] = v[1];
-        w[q] = v[2];
-        r[q] = v[3];
+        const bool in = p < n;
+        dg[q] = in ? sq[0][i] : 1.0;
+        e[q] = in ? sq[1][i] : 0.0;
+        w[q] = in ? sq[2][i] : 0.0;
+        r[q] = in ? sq[3][i] : 0.0;
     }
     wave_tridiag<double, M>(n, w, dg, e, r, false);
 #pragma unroll
@@ -385,7 +434,7 @@ void launch_mixed_block_correction(hipStream_t st, const Dims &d, const MxFields
     hipLaunchKernelGGL(k_blk_fold, dim3((d.ni + 63) / 64, 4, BF_SPLIT), dim3(64), 0, st, d.ni, mixed_tile_count(d),
                        m.blkpart, m.blkfold);
     const int n = d.nim2;
-#define BS(MM) hipLaunchKernelGGL((k_blk_solve_mx<MM>), dim3(1), dim3(64), 0, st, d.nim2, d.ni, m.blkfold, m.delphi, skip)
+#define BS(MM) hipLaunchKernelGGL((k_blk_solve_mx<MM>), dim3(1), dim3(BS_NT), 0, st, d.nim2, d.ni, m.blkfold, m.delphi, skip)
     if (n <= 128) BS(2);
     else if (n <= 256) BS(4);
     else if (n <= 512) BS(8);
