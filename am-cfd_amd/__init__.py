@@ -40,7 +40,7 @@ EXPORTS = ("amcfd_deck_load", "amcfd_deck_params", "amcfd_deck_free", "amcfd_cre
            "amcfd_kernel_stats", "amcfd_write_vts", "amcfd_write_pvd", "amcfd_append_residuals",
            "amcfd_append_final_geometry", "amcfd_deck_string", "amcfd_free_surface",
            "amcfd_output_section", "amcfd_run_batch", "amcfd_set_mixed_schedule", "amcfd_test_expire_waits",
-           "amcfd_tdma_lines",
+           "amcfd_tdma_lines", "amcfd_tdma_lines_noalloc",
            "amcfd_last_error", "amcfd_version")
 
 
@@ -150,6 +150,7 @@ def load_library(path=LIB_PATH):
         "amcfd_set_mixed_schedule": (ip, [vp, cp, ip]),
         "amcfd_test_expire_waits": (ip, [vp]),
         "amcfd_tdma_lines": (ip, [vp, vp, vp, vp, vp, ip, ip, ip, vp]),
+        "amcfd_tdma_lines_noalloc": (ip, [vp, vp, vp, vp, vp, ip, ip, ip, vp]),
         "amcfd_last_error": (cp, []),
         "amcfd_version": (cp, []),
     }
@@ -327,6 +328,12 @@ def tdma_lines(aw, ap, ae, d, phi, n, nlines, guard=0, stream=None):
     """The reference's x-line TDMA on device buffers (device addresses as
     ints); phi is solved in place.  Returns once the work is queued."""
     _check(load_library().amcfd_tdma_lines(aw, ap, ae, d, phi, int(n), int(nlines), int(guard), stream))
+
+
+def tdma_lines_noalloc(aw, ap, ae, d, phi, n, nlines, guard=0, stream=None):
+    """amcfd_tdma_lines with the library's fixed device scratch: allocates
+    nothing; n * nlines <= 2**20."""
+    _check(load_library().amcfd_tdma_lines_noalloc(aw, ap, ae, d, phi, int(n), int(nlines), int(guard), stream))
 
 
 def version():

@@ -11,7 +11,9 @@
 // Here every line is one lane, in the same operations and order (compiled
 // without contraction, so the result is the reference's bit for bit).  The
 // pr/qr arrays are the scratch: amcfd_tdma_lines keeps a per-device
-// workspace that it allocates and grows.
+// workspace that it allocates and grows; amcfd_tdma_lines_noalloc uses a
+// fixed file-scope device array instead and allocates nothing (the pr/qr
+// entries are written before they are read, so nothing needs clearing).
 
 #include <mutex>
 
@@ -78,6 +80,12 @@ struct Workspace {
 std::mutex ws_mutex;
 Workspace ws[MAX_DEV];
 
+// amcfd_tdma_lines_noalloc: fixed scratch, one copy per device
+constexpr size_t NOALLOC_DOUBLES = (size_t)1 << 21;   // 16 MiB: 2*n*nlines <= 2^21
+__device__ __align__(256) double scratch[NOALLOC_DOUBLES];
+std::mutex sym_mutex;
+double *scratch_ptr[MAX_DEV] = {};
+
 }  // namespace
 
 }  // namespace amcfd
@@ -112,6 +120,33 @@ int amcfd_tdma_lines(const double *aw, const double *ap, const double *ae, const
             w.cap = need;
         }
         buf = w.ptr;
+    }
+    return launch(aw, ap, ae, d, phi, n, nlines, guard, buf, (hipStream_t)stream);
+}
+
+int amcfd_tdma_lines_noalloc(const double *aw, const double *ap, const double *ae, const double *d, double *phi,
+                             int n, int nlines, int guard, void *stream) {
+    using namespace amcfd;
+    int st = check_args(aw, ap, ae, d, phi, n, nlines);
+    if (st) return st;
+    if ((size_t)2 * n * nlines > NOALLOC_DOUBLES)
+        return set_error(AMCFD_E_CAPACITY, "amcfd_tdma_lines_noalloc holds at most 2^20 line nodes (n * nlines)");
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= MAX_DEV)
+        return set_error(AMCFD_E_HIP, "no current HIP device");
+    double *buf;
+    {
+        // the array's device address, looked up once per device with that
+        // device current (the host-side name is not a device address)
+        std::lock_guard<std::mutex> lock(sym_mutex);
+        if (!scratch_ptr[dev]) {
+            void *p = nullptr;
+            const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+            if (e != hipSuccess || !p)
+                return set_error(AMCFD_E_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+            scratch_ptr[dev] = (double *)p;
+        }
+        buf = scratch_ptr[dev];
     }
     return launch(aw, ap, ae, d, phi, n, nlines, guard, buf, (hipStream_t)stream);
 }

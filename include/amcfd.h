@@ -269,6 +269,17 @@ int  amcfd_run_batch(amcfd_solver *const *s, int n, int max_steps, int workers, 
 int  amcfd_tdma_lines(const double *aw, const double *ap, const double *ae, const double *d, double *phi,
                       int n, int nlines, int guard, void *stream);
 
+/* amcfd_tdma_lines without any allocation, for callers that cannot call
+ * hipMalloc (code under HIP graph capture, tight serving loops, harnesses
+ * that forbid allocation).  No counterpart in the reference.  Same arguments
+ * and results as amcfd_tdma_lines; the scratch is one fixed device array of
+ * 2^21 doubles (16 MiB) per GPU, held until the process ends, so
+ * n * nlines <= 2^20 (larger batches return AMCFD_E_CAPACITY before the
+ * device is touched).  Every call on a GPU shares that GPU's array: calls
+ * that overlap on one GPU race. */
+int  amcfd_tdma_lines_noalloc(const double *aw, const double *ap, const double *ae, const double *d,
+                              double *phi, int n, int nlines, int guard, void *stream);
+
 /* Mixed mode only: the order of the enthalpy line passes of one SIMPLE
  * iteration ('x' = x-lines red then black, in place; 'y' / 'z' = y- or
  * z-lines, one lane per line, between the field and a second buffer; an
