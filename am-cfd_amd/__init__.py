@@ -41,6 +41,9 @@ EXPORTS = ("amcfd_deck_load", "amcfd_deck_params", "amcfd_deck_free", "amcfd_cre
            "amcfd_append_final_geometry", "amcfd_deck_string", "amcfd_free_surface",
            "amcfd_output_section", "amcfd_run_batch", "amcfd_set_mixed_schedule", "amcfd_test_expire_waits",
            "amcfd_tdma_lines", "amcfd_tdma_lines_noalloc",
+           "amcfd_ws_create", "amcfd_ws_destroy", "amcfd_ws_attach", "amcfd_ws_train", "amcfd_ws_samples",
+           "amcfd_ws_set_factors", "amcfd_ws_save", "amcfd_ws_load", "amcfd_ws_param_count", "amcfd_ws_params",
+           "amcfd_ws_loss_grad",
            "amcfd_last_error", "amcfd_version")
 
 
@@ -151,6 +154,19 @@ def load_library(path=LIB_PATH):
         "amcfd_test_expire_waits": (ip, [vp]),
         "amcfd_tdma_lines": (ip, [vp, vp, vp, vp, vp, ip, ip, ip, vp]),
         "amcfd_tdma_lines_noalloc": (ip, [vp, vp, vp, vp, vp, ip, ip, ip, vp]),
+        "amcfd_ws_create": (ip, [ip, ctypes.POINTER(vp)]),
+        "amcfd_ws_destroy": (None, [vp]),
+        "amcfd_ws_attach": (ip, [vp, vp, ip]),
+        "amcfd_ws_train": (ip, [vp, ip, ip, ctypes.c_double, dp, dp]),
+        "amcfd_ws_samples": (ip, [vp]),
+        "amcfd_ws_set_factors": (ip, [vp, ctypes.c_double, ctypes.c_double]),
+        "amcfd_ws_save": (ip, [vp, cp]),
+        "amcfd_ws_load": (ip, [vp, cp]),
+        "amcfd_ws_param_count": (ip, [vp]),
+        "amcfd_ws_params": (ip, [vp, ctypes.POINTER(ctypes.c_float), ctypes.c_size_t, ip]),
+        "amcfd_ws_loss_grad": (ip, [vp, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float),
+                                    ctypes.POINTER(ctypes.c_float), ip, ip, ip, ip, dp,
+                                    ctypes.POINTER(ctypes.c_float)]),
         "amcfd_last_error": (cp, []),
         "amcfd_version": (cp, []),
     }
@@ -334,6 +350,73 @@ def tdma_lines_noalloc(aw, ap, ae, d, phi, n, nlines, guard=0, stream=None):
     """amcfd_tdma_lines with the library's fixed device scratch: allocates
     nothing; n * nlines <= 2**20."""
     _check(load_library().amcfd_tdma_lines_noalloc(aw, ap, ae, d, phi, int(n), int(nlines), int(guard), stream))
+
+
+WS_OFF, WS_RECORD, WS_APPLY = 0, 1, 2
+
+
+class WarmStartNet:
+    """The learned warm start (3-D conv encoder-decoder on BF16 matrix cores)."""
+
+    def __init__(self, seed=0):
+        self._h = ctypes.c_void_p()
+        _check(load_library().amcfd_ws_create(int(seed), ctypes.byref(self._h)))
+
+    def attach(self, solver, mode):
+        _check(load_library().amcfd_ws_attach(solver._h, self._h if mode else None, int(mode)))
+
+    def train(self, iters, batch=8, lr=1e-3):
+        a, b = ctypes.c_double(), ctypes.c_double()
+        _check(load_library().amcfd_ws_train(self._h, int(iters), int(batch), float(lr), ctypes.byref(a),
+                                             ctypes.byref(b)))
+        return a.value, b.value
+
+    def samples(self):
+        return _check(load_library().amcfd_ws_samples(self._h))
+
+    def set_factors(self, tfactor=1.0, pfactor=0.0):
+        _check(load_library().amcfd_ws_set_factors(self._h, float(tfactor), float(pfactor)))
+
+    def save(self, path):
+        _check(load_library().amcfd_ws_save(self._h, os.fsencode(path)))
+
+    def load(self, path):
+        _check(load_library().amcfd_ws_load(self._h, os.fsencode(path)))
+
+    def params(self):
+        n = _check(load_library().amcfd_ws_param_count(self._h))
+        out = np.empty(n, dtype=np.float32)
+        _check(load_library().amcfd_ws_params(self._h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), n, 0))
+        return out
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        _check(load_library().amcfd_ws_params(self._h, p.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), p.size, 1))
+
+    def loss_grad(self, x, target, mask):
+        """x: (N, D, H, W, 16), target: (N, D, H, W, 2), mask: (N, D, H, W)."""
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        t = np.ascontiguousarray(target, dtype=np.float32)
+        m = np.ascontiguousarray(mask, dtype=np.float32)
+        n, d, h, w = m.shape
+        loss = ctypes.c_double()
+        g = np.empty(_check(load_library().amcfd_ws_param_count(self._h)), dtype=np.float32)
+        fp = ctypes.POINTER(ctypes.c_float)
+        _check(load_library().amcfd_ws_loss_grad(self._h, x.ctypes.data_as(fp), t.ctypes.data_as(fp),
+                                                 m.ctypes.data_as(fp), n, d, h, w, ctypes.byref(loss),
+                                                 g.ctypes.data_as(fp)))
+        return loss.value, g
+
+    def close(self):
+        if self._h:
+            load_library().amcfd_ws_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def version():

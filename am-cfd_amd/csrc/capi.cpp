@@ -2,6 +2,11 @@
 
 #include "host.h"
 
+struct amcfd_ws;
+namespace amcfd {
+WarmStart *ws_of(amcfd_ws *h);
+}
+
 #include <cstring>
 #include <exception>
 #include <new>
@@ -291,6 +296,13 @@ int amcfd_set_mixed_schedule(amcfd_solver *s, const char *schedule, int box_pass
 int amcfd_test_expire_waits(amcfd_solver *s) {
     if (!s) return amcfd::set_error(AMCFD_E_ARG, "null solver");
     GUARD(s->s.raise_sweep_error());
+}
+
+int amcfd_ws_attach(amcfd_solver *s, amcfd_ws *ws, int mode) {
+    if (!s || (mode && !ws)) return amcfd::set_error(AMCFD_E_ARG, "null argument");
+    if (mode < 0 || mode > 2) return amcfd::set_error(AMCFD_E_ARG, "mode must be 0, 1 or 2");
+    s->s.attach_warm_start(amcfd::ws_of(ws), mode);
+    return AMCFD_OK;
 }
 
 const char *amcfd_last_error(void) { return amcfd::g_last_error.c_str(); }

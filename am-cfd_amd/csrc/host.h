@@ -29,6 +29,12 @@ int energy_partial_sizes(const Dims &d, int *nblk_e, int *nrow_e, int *nblk_t);
 int flow_partial_blocks(const Box &b, const Dims &d);
 int sweep_max_line();
 
+class WarmStart;
+class Solver;
+int ws_capture_input(WarmStart *w, Solver &s);
+int ws_capture_target(WarmStart *w, Solver &s);
+int ws_apply(WarmStart *w, Solver &s);
+
 // accumulated device time of one kernel family (HIP events on the solver stream)
 struct KernelTimer {
     long launches = 0;
@@ -64,6 +70,18 @@ class Solver {
     int set_mixed_schedule(const char *sched, int box_passes);
     // raise the sweeps' device error flag (test hook for the bounded waits)
     int raise_sweep_error();
+    // learned warm start: 0 off, 1 record a training sample per step, 2 use
+    // the network's prediction as each step's initial iterate
+    void attach_warm_start(WarmStart *w, int mode) {
+        ws_ = mode ? w : nullptr;
+        ws_mode_ = mode;
+    }
+    // views for the warm start
+    const Dims &dev_dims() const { return d_; }
+    const DevFields &dev_fields() const { return f_; }
+    hipStream_t stream() const { return st_; }
+    int beam_i() const { return istart_; }
+    int beam_j() const { return jstart_; }
     std::map<std::string, KernelTimer> timers;
 
   private:
@@ -106,6 +124,10 @@ class Solver {
     bool section_sci_ = false;               // the section stream has switched to scientific
     double *xold_ = nullptr, *yold_ = nullptr;   // node coordinates before a mesh move
     Box box_{};
+    WarmStart *ws_ = nullptr;
+    int ws_mode_ = 0;
+    int warm_start_begin();
+    int warm_start_end();
 
     void set_active_nk(int nk);
     int alloc(double **ptr, size_t n);
@@ -131,6 +153,7 @@ class Solver {
     int read_scalars();
     int flow_iteration();
     int solve_step_mixed();
+    int solve_step_fp64();
     int flow_iteration_mixed();
     int energy_sweeps_mixed();
     int allocf(float **ptr, size_t n);

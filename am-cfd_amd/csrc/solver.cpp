@@ -581,8 +581,38 @@ int Solver::flow_iteration() {
     return 0;
 }
 
+// the learned warm start around a solve: record mode keeps the state the
+// step starts from (and, after the solve, its increment) as a training
+// sample; apply mode replaces the iterate the step starts from (the state
+// the previous step ended with, CFDSolverManager.cpp:608-612) by the
+// network's guess, and brings T, the liquid fraction and the properties in
+// line with it before the first iteration
+int Solver::warm_start_begin() {
+    if (!ws_) return 0;
+    if (ws_mode_ == 1) return ws_capture_input(ws_, *this);
+    int st = ws_apply(ws_, *this);
+    if (st) return st;
+    launch_h2t(st_, d_, g_, f_, pr_, p_);
+    launch_props(st_, d_, f_, p_);
+    HIPCALL(hipGetLastError());
+    return 0;
+}
+
+int Solver::warm_start_end() {
+    if (!ws_ || ws_mode_ != 1) return 0;
+    return ws_capture_target(ws_, *this);
+}
+
 int Solver::solve_step() {
-    if (mode_ == AMCFD_MODE_MIXED) return solve_step_mixed();
+    int st0 = warm_start_begin();
+    if (st0) return st0;
+    int rc0 = mode_ == AMCFD_MODE_MIXED ? solve_step_mixed() : solve_step_fp64();
+    if (rc0 < 0) return rc0;
+    st0 = warm_start_end();
+    return st0 ? st0 : rc0;
+}
+
+int Solver::solve_step_fp64() {
     HIPCALL(hipEventRecord(step_ev_[0], st_));
     double tdma_before = timers.count("sweep_h") ? timers["sweep_h"].total_ms : 0.0;
     const double alenold = alen_;

@@ -294,6 +294,43 @@ int  amcfd_set_mixed_schedule(amcfd_solver *s, const char *schedule, int box_pas
  * returns AMCFD_E_HIP.  The flag stays raised for the solver's life. */
 int  amcfd_test_expire_waits(amcfd_solver *s);
 
+/* Learned warm start (no reference counterpart): a 3-D convolutional
+ * encoder-decoder (4 levels, 16-128 channels, 3x3x3 kernels, BF16 on the
+ * matrix cores, FP32 accumulation) that predicts a step's temperature
+ * increment and pressure change in a 32 x 32 x 16-cell window around the
+ * beam from the state the step starts from and the step's laser flux.  It
+ * is trained on the GPU from steps the solver runs, and then supplies the
+ * step's initial iterate; the SIMPLE iteration still runs to the reference's
+ * convergence test (CFDSolverManager.cpp:690-716), so only the number of
+ * iterations changes.
+ *   ws_create / destroy  network with seeded initial weights
+ *   ws_attach            mode 0 off, 1 record one training sample per step
+ *                        (the state before and the increment after the
+ *                        solve), 2 apply the guess before each solve
+ *   ws_train             Adam over random batches of the recorded samples;
+ *                        loss of the first and last batch
+ *   ws_set_factors       fractions of the predicted temperature and pressure
+ *                        changes applied (default 1 and 0)
+ *   ws_save / ws_load    FP32 weights
+ *   ws_param_count / ws_params (set = 0 read, 1 write) / ws_loss_grad
+ *                        the flat parameter vector (per layer: weights
+ *                        [co][tap][ci], then biases) and the loss and its
+ *                        gradient on host tensors, NDHWC with 16 input
+ *                        channels and 2 target channels (tests) */
+typedef struct amcfd_ws amcfd_ws;
+int  amcfd_ws_create(int seed, amcfd_ws **out);
+void amcfd_ws_destroy(amcfd_ws *ws);
+int  amcfd_ws_attach(amcfd_solver *s, amcfd_ws *ws, int mode);
+int  amcfd_ws_train(amcfd_ws *ws, int iters, int batch, double lr, double *loss_first, double *loss_last);
+int  amcfd_ws_samples(amcfd_ws *ws);
+int  amcfd_ws_set_factors(amcfd_ws *ws, double tfactor, double pfactor);
+int  amcfd_ws_save(amcfd_ws *ws, const char *path);
+int  amcfd_ws_load(amcfd_ws *ws, const char *path);
+int  amcfd_ws_param_count(amcfd_ws *ws);
+int  amcfd_ws_params(amcfd_ws *ws, float *host, size_t n, int set);
+int  amcfd_ws_loss_grad(amcfd_ws *ws, const float *x, const float *target, const float *mask, int n, int d, int h,
+                        int w, double *loss, float *grad);
+
 /* Thread-local description of the last error. */
 const char *amcfd_last_error(void);
 const char *amcfd_version(void);
