@@ -39,7 +39,7 @@ EXPORTS = ("amcfd_deck_load", "amcfd_deck_params", "amcfd_deck_free", "amcfd_cre
            "amcfd_get_scalars", "amcfd_dims", "amcfd_last_timing", "amcfd_set_profiling",
            "amcfd_kernel_stats", "amcfd_write_vts", "amcfd_write_pvd", "amcfd_append_residuals",
            "amcfd_append_final_geometry", "amcfd_deck_string", "amcfd_free_surface",
-           "amcfd_output_section", "amcfd_run_batch", "amcfd_set_mixed_schedule", "amcfd_test_expire_waits",
+           "amcfd_output_section", "amcfd_run_batch", "amcfd_set_mixed_schedule", "amcfd_set_graph", "amcfd_test_expire_waits",
            "amcfd_tdma_lines", "amcfd_tdma_lines_noalloc",
            "amcfd_ws_create", "amcfd_ws_destroy", "amcfd_ws_attach", "amcfd_ws_train", "amcfd_ws_samples",
            "amcfd_ws_set_factors", "amcfd_ws_save", "amcfd_ws_load", "amcfd_ws_param_count", "amcfd_ws_params",
@@ -152,6 +152,7 @@ def load_library(path=LIB_PATH):
         "amcfd_run_batch": (ip, [ctypes.POINTER(vp), ip, ip, ip, ctypes.POINTER(ip)]),
         "amcfd_set_mixed_schedule": (ip, [vp, cp, ip]),
         "amcfd_test_expire_waits": (ip, [vp]),
+        "amcfd_set_graph": (ip, [vp, ip, ip, ip]),
         "amcfd_tdma_lines": (ip, [vp, vp, vp, vp, vp, ip, ip, ip, vp]),
         "amcfd_tdma_lines_noalloc": (ip, [vp, vp, vp, vp, vp, ip, ip, ip, vp]),
         "amcfd_ws_create": (ip, [ip, ctypes.POINTER(vp)]),
@@ -313,6 +314,10 @@ class Solver:
         """Mixed mode: order of the enthalpy line passes ('x', 'y', 'z') and
         red-black x passes per melt-pool box variable."""
         _check(load_library().amcfd_set_mixed_schedule(self._h, schedule.encode(), int(box_passes)))
+
+    def set_graph(self, on=True, block=8, box_blocks=256):
+        """Mixed mode: captured-graph SIMPLE loop (device-side control) or host loop."""
+        _check(load_library().amcfd_set_graph(self._h, int(bool(on)), int(block), int(box_blocks)))
 
     def test_expire_waits(self):
         """Raise the line sweeps' device error flag (test hook)."""

@@ -293,6 +293,12 @@ int amcfd_set_mixed_schedule(amcfd_solver *s, const char *schedule, int box_pass
     GUARD(s->s.set_mixed_schedule(schedule, box_passes));
 }
 
+int amcfd_set_graph(amcfd_solver *s, int on, int block, int box_blocks) {
+    if (!s) return amcfd::set_error(AMCFD_E_ARG, "null solver");
+    if (s->s.mode() != AMCFD_MODE_MIXED) return amcfd::set_error(AMCFD_E_ARG, "the solver is not in the mixed mode");
+    GUARD(s->s.set_graph(on, block, box_blocks));
+}
+
 int amcfd_test_expire_waits(amcfd_solver *s) {
     if (!s) return amcfd::set_error(AMCFD_E_ARG, "null solver");
     GUARD(s->s.raise_sweep_error());

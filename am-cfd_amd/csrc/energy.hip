@@ -36,8 +36,9 @@ __global__ __launch_bounds__(256) void k_bc_enthalpy(Dims d, BcConst k, const do
                                                      const double *__restrict__ heatin,
                                                      const double *__restrict__ temp,
                                                      const double *__restrict__ diff, double *h,
-                                                     double *partial) {
+                                                     double *partial, const int *skip) {
     __shared__ double scratch[8];
+    if (skip && *skip) return;
     const int ntop = d.njm2 * d.nim2, new_ = d.nkm2 * d.njm2, nns = d.nkm2 * d.nim2;
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     double loss = 0.0;
@@ -346,7 +347,13 @@ k_energy_assemble_mx(Dims d, AsmConst q, DevGeom g, const double *__restrict__ u
                      const double *__restrict__ wvel, const double *__restrict__ rho,
                      const double *__restrict__ diff, const double *__restrict__ hnot,
                      const double *__restrict__ fracl, const double *__restrict__ fraclnot,
-                     const double *__restrict__ h, AsmOut o, const int *__restrict__ skip) {
+                     const double *__restrict__ h, AsmOut o, const int *__restrict__ skip,
+                     const StepDev *__restrict__ step) {
+    if (step) {   // a captured loop: the step's source parameters from device memory
+        const SourceParams sp = step->sp;
+        q.avolpow = sp.avolpow; q.apowseta = sp.apowseta; q.heatthick = sp.heatthick; q.heatrb = sp.heatrb;
+        q.avolfact = sp.avolfact; q.beamx = sp.beamposx; q.beamy = sp.beamposy; q.pi = sp.powerindicator;
+    }
     energy_assemble_body<true>(d, q, g, uvel, vvel, wvel, rho, diff, hnot, fracl, fraclnot, h, o, skip);
 }
 
@@ -362,7 +369,8 @@ struct PropConst {
 __global__ __launch_bounds__(256) void k_props(size_t n, PropConst c, const double *__restrict__ temp,
                                                const double *__restrict__ fracl,
                                                const double *__restrict__ csfrac, double *diff, double *rho,
-                                               double *vis) {
+                                               double *vis, const int *skip) {
+    if (skip && *skip) return;
     size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= n) return;
     double T = temp[t];
@@ -533,8 +541,9 @@ __global__ __launch_bounds__(TX *TY) void k_h2t(Dims d, H2tConst q, DevGeom g, c
                                                  const double *__restrict__ rho,
                                                  const double *__restrict__ diff,
                                                  const double *__restrict__ src, double *temp, double *fracl,
-                                                 double *partial) {
+                                                 double *partial, const int *skip) {
     __shared__ double scratch[10 * 4];
+    if (skip && *skip) return;
     const int tx = threadIdx.x, ty = threadIdx.y;
     const int i = blockIdx.x * TX + tx;
     const int j = blockIdx.y * TY + ty;
@@ -627,8 +636,14 @@ __global__ __launch_bounds__(1024) void k_melt_pool(Dims d, int nblk, const doub
                                                     const double *__restrict__ temp,
                                                     const double *__restrict__ x, const double *__restrict__ y,
                                                     const double *__restrict__ z, double ts, int istart,
-                                                    int jstart, DevScalars *sc) {
+                                                    int jstart, DevScalars *sc, int ctl, int navierstokes,
+                                                    const StepDev *step) {
     __shared__ int si[1];
+    if (ctl && sc->done) return;
+    if (step) {   // the beam's nearest node of this step (captured loops)
+        istart = step->istart;
+        jstart = step->jstart;
+    }
     __shared__ double sd[32];
     __shared__ double tp_s;
     const int tid = threadIdx.x, nt = blockDim.x;
@@ -643,6 +658,7 @@ __global__ __launch_bounds__(1024) void k_melt_pool(Dims d, int nblk, const doub
         for (int w = 1; w < nt / 64; ++w) mm = dmax(mm, sd[w]);
         tp_s = mm;
         sc->tpeak = mm;
+        if (ctl) sc->noflow = (mm > ts && navierstokes) ? 0 : 1;   // CFDSolverManager.cpp:660
     }
     __syncthreads();
     const double tpeak = tp_s;
@@ -774,7 +790,12 @@ __global__ __launch_bounds__(256) void k_finalize(int nblk_t, int nblk_b, int nb
                                                   const double *__restrict__ part_f,
                                                   const double *__restrict__ part_e,
                                                   const double *__restrict__ part_c, int flow_done, int species_done,
-                                                  int steady, double dens, double width, DevScalars *sc) {
+                                                  int steady, double dens, double width, DevScalars *sc, int ctl) {
+    if (ctl) {   // the loop's state lives on the device
+        if (sc->done) return;
+        flow_done = !sc->noflow;
+        width = sc->width;
+    }
     // every quantity here only feeds the convergence test, so the partials
     // are folded by a parallel tree (fixed order: still reproducible)
     constexpr int NQ = 17;   // 8 heat-balance, toploss, 2 energy residual, 4 flow, 2 species (+ umax)
@@ -888,13 +909,13 @@ __global__ __launch_bounds__(256) void k_end_step(size_t n, double tsolid, doubl
 // launchers
 // ---------------------------------------------------------------------------
 void launch_bc_enthalpy(hipStream_t st, const Dims &d, const DevGeom &g, DevFields f, Partials pr,
-                        const amcfd_params &p) {
+                        const amcfd_params &p, const int *skip) {
     BcConst k{p.sigm, p.emiss, p.htci, p.htcj, p.tempamb, p.pressamb, p.gasconst, p.evaplatent,
               p.evaptemp, p.evapfact, p.molarmass, p.isevap};
     int n = d.njm2 * d.nim2 + d.nkm2 * d.njm2 + d.nkm2 * d.nim2;
     int nb = (n + 255) / 256;
     hipLaunchKernelGGL(k_bc_enthalpy, dim3(nb), dim3(256), 0, st, d, k, g.dxpwinv, g.dypsinv, g.dzpbinv,
-                       g.areaij, f.heatin, f.temp, f.diff, f.enthalpy, pr.bc);
+                       g.areaij, f.heatin, f.temp, f.diff, f.enthalpy, pr.bc, skip);
 }
 
 // Gaussian surface flux map, BoundCondManager.cpp:128-159: one thread per
@@ -961,13 +982,14 @@ void launch_energy_assemble(hipStream_t st, const Dims &d, const DevGeom &g, Dev
 }
 
 void launch_energy_assemble_mixed(hipStream_t st, const Dims &d, const DevGeom &g, DevFields f, Partials pr,
-                                  const MxFields &m, const amcfd_params &p, const SourceParams &s, const int *skip) {
+                                  const MxFields &m, const amcfd_params &p, const SourceParams &s, const int *skip,
+                                  const StepDev *step) {
     AsmConst q{p.delt, p.hlatnt, p.urfh, s.avolpow, s.apowseta, s.heatthick, s.heatrb, s.avolfact,
                s.beamposx, s.beamposy, s.powerindicator, p.discretization};
     AsmOut o{nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, m.su, f.sourceinput, pr.energy,
              nullptr, nullptr, m.an, m.as, m.ae, m.aw, m.at, m.ab, m.ap, m.blkpart};
     hipLaunchKernelGGL(k_energy_assemble_mx, assemble_grid(d), dim3(TX, TY), 0, st, d, q, g, f.uvel, f.vvel,
-                       f.wvel, f.rho, f.diff, f.hnot, f.fracl, f.fraclnot, f.enthalpy, o, skip);
+                       f.wvel, f.rho, f.diff, f.hnot, f.fracl, f.fraclnot, f.enthalpy, o, skip, step);
 }
 
 int mixed_tile_count(const Dims &d) {
@@ -975,13 +997,13 @@ int mixed_tile_count(const Dims &d) {
     return (int)(a.y * a.z);
 }
 
-void launch_props(hipStream_t st, const Dims &d, DevFields f, const amcfd_params &p) {
+void launch_props(hipStream_t st, const Dims &d, DevFields f, const amcfd_params &p, const int *skip) {
     PropConst c{p.tliquid, p.tsolid, p.viscos, p.dens, p.denl, p.acpa, p.acpb, p.acpl, p.thconsa,
                 p.thconsb, p.thconsc, p.thconla, p.thconlb, p.powderden, p.powdercpa, p.powdercpb,
                 p.powderthcona, p.powderthconb, p.powderbed};
     size_t n = (size_t)d.ni * d.nj * d.nk;
     hipLaunchKernelGGL(k_props, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, n, c, f.temp, f.fracl,
-                       f.csfrac, f.diff, f.rho, f.vis);
+                       f.csfrac, f.diff, f.rho, f.vis, skip);
 }
 
 void launch_block_correction(hipStream_t st, const Dims &d, DevFields f, Partials pr, double *phi, int shared) {
@@ -998,29 +1020,68 @@ static dim3 node_grid(const Dims &d) {
     return dim3((d.ni + TX - 1) / TX, (d.nj + TY - 1) / TY, (d.nk + KB - 1) / KB);
 }
 
-void launch_h2t(hipStream_t st, const Dims &d, const DevGeom &g, DevFields f, Partials pr, const amcfd_params &p) {
+void launch_h2t(hipStream_t st, const Dims &d, const DevGeom &g, DevFields f, Partials pr, const amcfd_params &p,
+                const int *skip) {
     H2tConst q{p.hlcal, p.hsmelt, p.acpa, p.acpb, p.acpl, p.tliquid, p.tsolid, p.hlatnt, p.delt, p.steady};
     hipLaunchKernelGGL(k_h2t, node_grid(d), dim3(TX, TY), 0, st, d, q, g, f.enthalpy, f.hnot, f.fraclnot, f.rho,
-                       f.diff, f.sourceinput, f.temp, f.fracl, pr.h2t);
+                       f.diff, f.sourceinput, f.temp, f.fracl, pr.h2t, skip);
 }
 
 void launch_melt_pool(hipStream_t st, const Dims &d, const DevGeom &g, DevFields f, Partials pr,
-                      const amcfd_params &p, int istart, int jstart, DevScalars *sc) {
+                      const amcfd_params &p, int istart, int jstart, DevScalars *sc, bool ctl,
+                      const StepDev *step) {
     dim3 gr = node_grid(d);
     int nblk = gr.x * gr.y * gr.z;
     hipLaunchKernelGGL(k_melt_pool, dim3(1), dim3(1024), 0, st, d, nblk, pr.h2t, f.temp, g.x, g.y, g.z, p.tsolid,
-                       istart, jstart, sc);
+                       istart, jstart, sc, (int)ctl, p.navierstokes, step);
 }
 
 void launch_finalize(hipStream_t st, const Dims &d, Partials pr, DevScalars *sc, const amcfd_params &p,
-                     int flow_done, int species_done, int steady, double width) {
+                     int flow_done, int species_done, int steady, double width, bool ctl) {
     dim3 gr = node_grid(d);
     int nblk_t = gr.x * gr.y * gr.z;
     const int nblk_b = (d.njm2 * d.nim2 + d.nkm2 * d.njm2 + d.nkm2 * d.nim2 + 255) / 256;
     const dim3 e = assemble_grid(d), c = energy_grid(d);   // enthalpy and species assembly partials
     hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, st, nblk_t, nblk_b, pr.nblk_f, (int)(e.x * e.y * e.z),
                        (int)(c.x * c.y * c.z), pr.h2t, pr.bc, pr.flow, pr.energy, pr.species, flow_done, species_done, steady, p.dens,
-                       width, sc);
+                       width, sc, (int)ctl);
+}
+
+// the convergence test of CFDSolverManager.cpp:690-716 on the device: the
+// first largest of resorm, resoru, resorv, resorw; the heat-balance band with
+// the laser on (or in steady mode), resorh < 1e-6 with it off
+__global__ void k_converge(DevScalars *sc, const StepDev *step, int steady, double tol) {
+    if (sc->done) return;
+    sc->niter += 1;
+    double res[4] = {sc->resorm, sc->resoru, sc->resorv, sc->resorw};
+    double amaxres = res[0];
+    for (int r = 1; r < 4; ++r)
+        if (amaxres < res[r]) amaxres = res[r];
+    const double ratio = sc->ratio;
+    bool ok;
+    if (steady) ok = ratio <= 1.001 && ratio >= 0.999 && amaxres < tol;
+    else if (step->powerindicator == 1) ok = ratio <= 1.01 && ratio >= 0.99 && amaxres < tol;
+    else ok = sc->resorh < 1e-6 && amaxres < tol;
+    if (ok || sc->niter >= step->maxit) {
+        sc->converged = ok ? 1 : 0;
+        sc->done = 1;
+        sc->noflow = 1;
+    }
+}
+
+__global__ void k_loop_reset(DevScalars *sc) {
+    sc->niter = 0;
+    sc->done = 0;
+    sc->converged = 0;
+    sc->noflow = 1;
+}
+
+void launch_converge(hipStream_t st, DevScalars *sc, const StepDev *step, const amcfd_params &p) {
+    hipLaunchKernelGGL(k_converge, dim3(1), dim3(1), 0, st, sc, step, p.steady, p.nonlintol);
+}
+
+void launch_loop_reset(hipStream_t st, DevScalars *sc) {
+    hipLaunchKernelGGL(k_loop_reset, dim3(1), dim3(1), 0, st, sc);
 }
 
 void launch_end_step(hipStream_t st, const Dims &d, DevFields f, const amcfd_params &p, double track,

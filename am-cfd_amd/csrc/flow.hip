@@ -19,18 +19,47 @@ __device__ __forceinline__ size_t cix(const Dims &d, int i, int j, int k) {
     return (size_t)k * d.sk + (size_t)j * d.sj + (size_t)i;
 }
 
+// The box a kernel works on: the launch value, or (with dbox) the one the
+// melt-pool kernel left in device memory (istat, iend, jstat, jend, kstat),
+// for launches whose grid was fixed before the box was known (captured
+// graphs).  Its tiles are TX (i) x TY (j) x 1 (k); workgroup b takes tiles
+// b, b + G, ...  With G equal to the tile count (the FP64 mode's launches)
+// workgroup b takes exactly tile b.
+struct BoxTiles {
+    BoxDev b;
+    int gx, gy, n;
+};
+__device__ __forceinline__ BoxTiles box_tiles(const Dims &d, BoxDev b, const int *dbox) {
+    if (dbox) b = BoxDev{dbox[0], dbox[1], dbox[2], dbox[3], dbox[4]};
+    const int ni_b = b.iend - b.istat - 1, nj_b = b.jend - b.jstat + 1, nk_b = d.nkm1 - b.kstat;
+    BoxTiles t;
+    t.b = b;
+    t.gx = (ni_b + TX - 1) / TX;
+    t.gy = (nj_b + TY - 1) / TY;
+    t.n = (ni_b < 1 || nj_b < 1 || nk_b < 1) ? 0 : t.gx * t.gy * nk_b;
+    return t;
+}
+__device__ __forceinline__ int block_lin() { return blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z); }
+__device__ __forceinline__ int grid_size() { return gridDim.x * gridDim.y * gridDim.z; }
+
 // zero face velocities next to solid cells, CFDSolverManager.cpp:2577-2620
-__global__ __launch_bounds__(TX *TY) void k_clean(Dims d, BoxDev b, double ts, const double *__restrict__ temp,
-                                                   double *u, double *v, double *w) {
-    const int i = b.istat + 1 + blockIdx.x * TX + threadIdx.x;
-    const int j = b.jstat + blockIdx.y * TY + threadIdx.y;
-    const int k = b.kstat + blockIdx.z;
-    if (i > b.iend - 1 || j > b.jend) return;
-    const size_t c = cix(d, i, j, k);
-    const double tP = temp[c];
-    if (dmin(tP, temp[c + 1]) <= ts) u[c + 1] = 0.0;
-    if (dmin(tP, temp[c + d.sj]) <= ts) v[c + d.sj] = 0.0;
-    if (dmin(tP, temp[c + d.sk]) <= ts) w[c + d.sk] = 0.0;
+__global__ __launch_bounds__(TX *TY) void k_clean(Dims d, BoxDev b0, double ts, const double *__restrict__ temp,
+                                                   double *u, double *v, double *w, const int *dbox,
+                                                   const int *skip) {
+    if (skip && *skip) return;
+    const BoxTiles bt = box_tiles(d, b0, dbox);
+    const BoxDev b = bt.b;
+    for (int t = block_lin(); t < bt.n; t += grid_size()) {
+        const int i = b.istat + 1 + (t % bt.gx) * TX + threadIdx.x;
+        const int j = b.jstat + ((t / bt.gx) % bt.gy) * TY + threadIdx.y;
+        const int k = b.kstat + t / (bt.gx * bt.gy);
+        if (i > b.iend - 1 || j > b.jend) continue;
+        const size_t c = cix(d, i, j, k);
+        const double tP = temp[c];
+        if (dmin(tP, temp[c + 1]) <= ts) u[c + 1] = 0.0;
+        if (dmin(tP, temp[c + d.sj]) <= ts) v[c + d.sj] = 0.0;
+        if (dmin(tP, temp[c + d.sk]) <= ts) w[c + d.sk] = 0.0;
+    }
 }
 
 // Fe-O surfactant surface-tension gradient, BoundCondManager.cpp:706-741
@@ -45,7 +74,7 @@ __device__ double surface_tension_gradient(double t, double a0, double dgdtp) {
 
 // Marangoni top ghost (ivar 1: u, 2: v) and zeroed box sides (ivar 1..4),
 // BoundCondManager.cpp:472-564
-__global__ __launch_bounds__(256) void k_bc_flow(Dims d, BoxDev b, int ivar, double a_oxygen, double dgdtp,
+__global__ __launch_bounds__(256) void k_bc_flow(Dims d, BoxDev b0, int ivar, double a_oxygen, double dgdtp,
                                                  const double *__restrict__ dxpwinv,
                                                  const double *__restrict__ dypsinv,
                                                  const double *__restrict__ dzpbinv,
@@ -53,10 +82,15 @@ __global__ __launch_bounds__(256) void k_bc_flow(Dims d, BoxDev b, int ivar, dou
                                                  const double *__restrict__ fracy,
                                                  const double *__restrict__ temp,
                                                  const double *__restrict__ fracl,
-                                                 const double *__restrict__ vis, double *phi) {
+                                                 const double *__restrict__ vis, double *phi, const int *dbox,
+                                                 const int *skip) {
+    if (skip && *skip) return;
+    const BoxDev b = dbox ? BoxDev{dbox[0], dbox[1], dbox[2], dbox[3], dbox[4]} : b0;
     const int ni_b = b.iend - b.istat - 1, nj_b = b.jend - b.jstat + 1, nk_b = d.nkm1 - b.kstat;
+    if (ni_b < 1 || nj_b < 1) return;
     const int ntop = (ivar <= 2) ? ni_b * nj_b : 0;
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int nall = ntop + (nk_b > 0 ? nk_b * nj_b : 0);
+    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < nall; t += gridDim.x * blockDim.x) {
     if (t < ntop) {
         const int j = b.jstat + t / ni_b, i = b.istat + 1 + t % ni_b;
         const size_t top = cix(d, i, j, d.nkm1), below = top - d.sk;
@@ -82,6 +116,7 @@ __global__ __launch_bounds__(256) void k_bc_flow(Dims d, BoxDev b, int ivar, dou
         phi[cix(d, b.istat, j, k)] = 0.0;
         phi[cix(d, b.iend, j, k)] = 0.0;
     }
+    }
 }
 
 struct MomConst {
@@ -96,7 +131,7 @@ struct MomConst {
 //   calculateResiduals(1/2/3)  CFDSolverManager.cpp:1960-2046
 // IVAR selects the staggering: 1 = u (x faces), 2 = v (y faces), 3 = w (z faces).
 template <int IVAR>
-__global__ __launch_bounds__(TX *TY) void k_momentum(Dims d, BoxDev b, MomConst q, DevGeom g,
+__global__ __launch_bounds__(TX *TY) void k_momentum(Dims d, BoxDev b0, MomConst q, DevGeom g,
                                                      const double *__restrict__ U, const double *__restrict__ Vv,
                                                      const double *__restrict__ W, const double *__restrict__ V,
                                                      const double *__restrict__ rho,
@@ -105,13 +140,18 @@ __global__ __launch_bounds__(TX *TY) void k_momentum(Dims d, BoxDev b, MomConst 
                                                      const double *__restrict__ fracl,
                                                      const double *__restrict__ temp, double *an_o, double *as_o,
                                                      double *ae_o, double *aw_o, double *at_o, double *ab_o,
-                                                     double *ap_o, double *su_o, double *dco, double *partial) {
+                                                     double *ap_o, double *su_o, double *dco, double *partial,
+                                                     const int *dbox, const int *skip) {
     __shared__ double scratch[2 * 4];
-    const int i = b.istat + 1 + blockIdx.x * TX + threadIdx.x;
-    const int j = b.jstat + blockIdx.y * TY + threadIdx.y;
-    const int k = b.kstat + blockIdx.z;
+    if (skip && *skip) return;
+    const BoxTiles bt = box_tiles(d, b0, dbox);
+    const BoxDev b = bt.b;
     const int sch = q.scheme;
     double resid = 0.0, umax = 0.0;
+    for (int tt = block_lin(); tt < bt.n; tt += grid_size()) {
+    const int i = b.istat + 1 + (tt % bt.gx) * TX + threadIdx.x;
+    const int j = b.jstat + ((tt / bt.gx) % bt.gy) * TY + threadIdx.y;
+    const int k = b.kstat + tt / (bt.gx * bt.gy);
     if (i <= b.iend - 1 && j <= b.jend) {
         const size_t c = cix(d, i, j, k);
         const size_t sj = d.sj, sk = d.sk;
@@ -189,8 +229,9 @@ __global__ __launch_bounds__(TX *TY) void k_momentum(Dims d, BoxDev b, MomConst 
             ap_o[c] = ap; su_o[c] = su; dco[c] = dd;
             double res = an * phi[c + sj] + as * phi[c - sj] + ae * phi[c + 1] + aw * phi[c - 1] + at * phi[c + sk] +
                          ab * phi[c - sk] + su - ap * phi[c];
-            resid = fabs(res);
-            if (k == d.nkm2) umax = fabs(U[c + sk]);
+            resid += fabs(res);
+            // first tile: plain assignment, as with one tile per workgroup
+            if (k == d.nkm2) umax = (tt == block_lin()) ? fabs(U[c + sk]) : dmax(umax, fabs(U[c + sk]));
         } else if (IVAR == 2) {
             const double fy = g.fracy[j - 1];
             double vn = (Vv[c] + Vv[c + sj]) * 0.5, vs = (Vv[c] + Vv[c - sj]) * 0.5;
@@ -259,7 +300,7 @@ __global__ __launch_bounds__(TX *TY) void k_momentum(Dims d, BoxDev b, MomConst 
             ap_o[c] = ap; su_o[c] = su; dco[c] = dd;
             double res = an * phi[c + sj] + as * phi[c - sj] + ae * phi[c + 1] + aw * phi[c - 1] + at * phi[c + sk] +
                          ab * phi[c - sk] + su - ap * phi[c];
-            resid = fabs(res);
+            resid += fabs(res);
         } else {
             const double fz = g.fracz[k - 1];
             double vn = Vv[c + sj] * (1.0 - fz) + Vv[c - sk + sj] * fz;
@@ -330,8 +371,9 @@ __global__ __launch_bounds__(TX *TY) void k_momentum(Dims d, BoxDev b, MomConst 
             ap_o[c] = ap; su_o[c] = su; dco[c] = dd;
             double res = an * phi[c + sj] + as * phi[c - sj] + ae * phi[c + 1] + aw * phi[c - 1] + at * phi[c + sk] +
                          ab * phi[c - sk] + su - ap * phi[c];
-            resid = fabs(res);
+            resid += fabs(res);
         }
+    }
     }
     // block partials: [sum |resid|, max |u_top|]
     const int lane = threadIdx.x + threadIdx.y * TX;
@@ -354,19 +396,24 @@ __global__ __launch_bounds__(TX *TY) void k_momentum(Dims d, BoxDev b, MomConst 
 
 // pressure-correction coefficients, mass imbalance, solid pinning and the
 // continuity normaliser: CFDSolverManager.cpp:1112-1156, 1645-1676, 2047-2071
-__global__ __launch_bounds__(TX *TY) void k_pressure(Dims d, BoxDev b, double tsolid, DevGeom g,
+__global__ __launch_bounds__(TX *TY) void k_pressure(Dims d, BoxDev b0, double tsolid, DevGeom g,
                                                      const double *__restrict__ U, const double *__restrict__ Vv,
                                                      const double *__restrict__ W, const double *__restrict__ rho,
                                                      const double *__restrict__ dux, const double *__restrict__ dvy,
                                                      const double *__restrict__ dwz,
                                                      const double *__restrict__ temp, double *an_o, double *as_o,
                                                      double *ae_o, double *aw_o, double *at_o, double *ab_o,
-                                                     double *ap_o, double *su_o, double *partial) {
+                                                     double *ap_o, double *su_o, double *partial, const int *dbox,
+                                                     const int *skip) {
     __shared__ double scratch[2 * 4];
-    const int i = b.istat + 1 + blockIdx.x * TX + threadIdx.x;
-    const int j = b.jstat + blockIdx.y * TY + threadIdx.y;
-    const int k = b.kstat + blockIdx.z;
+    if (skip && *skip) return;
+    const BoxTiles bt = box_tiles(d, b0, dbox);
+    const BoxDev b = bt.b;
     double dsum = 0.0, msum = 0.0;
+    for (int tt = block_lin(); tt < bt.n; tt += grid_size()) {
+    const int i = b.istat + 1 + (tt % bt.gx) * TX + threadIdx.x;
+    const int j = b.jstat + ((tt / bt.gx) % bt.gy) * TY + threadIdx.y;
+    const int k = b.kstat + tt / (bt.gx * bt.gy);
     if (i <= b.iend - 1 && j <= b.jend) {
         const size_t c = cix(d, i, j, k);
         const size_t sj = d.sj, sk = d.sk;
@@ -380,7 +427,7 @@ __global__ __launch_bounds__(TX *TY) void k_pressure(Dims d, BoxDev b, double ts
         double ft = W[c + sk] * aij * r, fb = W[c] * aij * r;
         double delf = fn - fs + fe - fw + ft - fb;
         double su = -delf;
-        msum = fabs(delf);
+        msum += fabs(delf);
         double ap = an + as + ae + aw + at + ab - 0.0;
         if (temp[c] <= tsolid) {
             su = 0.0; an = 0.0; as = 0.0; ae = 0.0; aw = 0.0; at = 0.0; ab = 0.0; ap = AMCFD_LARGE;
@@ -389,7 +436,8 @@ __global__ __launch_bounds__(TX *TY) void k_pressure(Dims d, BoxDev b, double ts
         ap_o[c] = ap; su_o[c] = su;
         double dtp = (fabs(U[c]) + fabs(U[c + 1])) * ajk + (fabs(Vv[c]) + fabs(Vv[c + sj])) * aik +
                      (fabs(W[c]) + fabs(W[c + sk])) * aij;
-        dsum = 0.5 * fabs(dtp);
+        dsum += 0.5 * fabs(dtp);
+    }
     }
     const int lane = threadIdx.x + threadIdx.y * TX;
     dsum = wave_sum(dsum);
@@ -410,35 +458,45 @@ __global__ __launch_bounds__(TX *TY) void k_pressure(Dims d, BoxDev b, double ts
 }
 
 // velocity correction from p', CFDSolverManager.cpp:2719-2742
-__global__ __launch_bounds__(TX *TY) void k_correct_vel(Dims d, BoxDev b, double ts, const double *__restrict__ temp,
+__global__ __launch_bounds__(TX *TY) void k_correct_vel(Dims d, BoxDev b0, double ts, const double *__restrict__ temp,
                                                         const double *__restrict__ pp,
                                                         const double *__restrict__ dux,
                                                         const double *__restrict__ dvy,
                                                         const double *__restrict__ dwz, double *u, double *v,
-                                                        double *w) {
-    const int i = b.istat + 1 + blockIdx.x * TX + threadIdx.x;
-    const int j = b.jstat + blockIdx.y * TY + threadIdx.y;
-    const int k = b.kstat + blockIdx.z;
-    if (i > b.iend - 1 || j > b.jend) return;
-    const size_t c = cix(d, i, j, k);
-    const double tP = temp[c];
-    if (dmin(tP, temp[c - 1]) > ts) u[c] += dux[c] * (pp[c - 1] - pp[c]);
-    if (dmin(tP, temp[c - d.sj]) > ts) v[c] += dvy[c] * (pp[c - d.sj] - pp[c]);
-    if (dmin(tP, temp[c - d.sk]) > ts) w[c] += dwz[c] * (pp[c - d.sk] - pp[c]);
+                                                        double *w, const int *dbox, const int *skip) {
+    if (skip && *skip) return;
+    const BoxTiles bt = box_tiles(d, b0, dbox);
+    const BoxDev b = bt.b;
+    for (int tt = block_lin(); tt < bt.n; tt += grid_size()) {
+        const int i = b.istat + 1 + (tt % bt.gx) * TX + threadIdx.x;
+        const int j = b.jstat + ((tt / bt.gx) % bt.gy) * TY + threadIdx.y;
+        const int k = b.kstat + tt / (bt.gx * bt.gy);
+        if (i > b.iend - 1 || j > b.jend) continue;
+        const size_t c = cix(d, i, j, k);
+        const double tP = temp[c];
+        if (dmin(tP, temp[c - 1]) > ts) u[c] += dux[c] * (pp[c - 1] - pp[c]);
+        if (dmin(tP, temp[c - d.sj]) > ts) v[c] += dvy[c] * (pp[c - d.sj] - pp[c]);
+        if (dmin(tP, temp[c - d.sk]) > ts) w[c] += dwz[c] * (pp[c - d.sk] - pp[c]);
+    }
 }
 
 // pressure update, CFDSolverManager.cpp:2744-2757
-__global__ __launch_bounds__(TX *TY) void k_correct_p(Dims d, BoxDev b, double ts, double urfp,
+__global__ __launch_bounds__(TX *TY) void k_correct_p(Dims d, BoxDev b0, double ts, double urfp,
                                                       const double *__restrict__ temp, double *pp,
-                                                      double *pressure) {
-    const int i = b.istat + 1 + blockIdx.x * TX + threadIdx.x;
-    const int j = b.jstat + blockIdx.y * TY + threadIdx.y;
-    const int k = b.kstat + blockIdx.z;
-    if (i > b.iend - 1 || j > b.jend) return;
-    const size_t c = cix(d, i, j, k);
-    if (temp[c] > ts) {
-        pressure[c] += urfp * pp[c];
-        pp[c] = 0.0;
+                                                      double *pressure, const int *dbox, const int *skip) {
+    if (skip && *skip) return;
+    const BoxTiles bt = box_tiles(d, b0, dbox);
+    const BoxDev b = bt.b;
+    for (int tt = block_lin(); tt < bt.n; tt += grid_size()) {
+        const int i = b.istat + 1 + (tt % bt.gx) * TX + threadIdx.x;
+        const int j = b.jstat + ((tt / bt.gx) % bt.gy) * TY + threadIdx.y;
+        const int k = b.kstat + tt / (bt.gx * bt.gy);
+        if (i > b.iend - 1 || j > b.jend) continue;
+        const size_t c = cix(d, i, j, k);
+        if (temp[c] > ts) {
+            pressure[c] += urfp * pp[c];
+            pp[c] = 0.0;
+        }
     }
 }
 
@@ -457,25 +515,40 @@ int flow_partial_blocks(const Box &b, const Dims &d) {
     return g.x * g.y * g.z;
 }
 
-void launch_clean_velocities(hipStream_t st, const Dims &d, DevFields f, const Box &b, double tsolid) {
-    dim3 g = box_grid(b, d);
+// grid of a box launch: the box's own tiles (FP64 mode), or a fixed number
+// of workgroups that stride over the device box (graphs)
+static dim3 bl_grid(const Box &b, const Dims &d, const BoxLaunch &bl) {
+    if (bl.dbox) return dim3(bl.nblocks, 1, 1);
+    return box_grid(b, d);
+}
+
+void launch_clean_velocities(hipStream_t st, const Dims &d, DevFields f, const Box &b, double tsolid,
+                             const BoxLaunch &bl) {
+    dim3 g = bl_grid(b, d, bl);
     if (!g.x) return;
-    hipLaunchKernelGGL(k_clean, g, dim3(TX, TY), 0, st, d, to_dev(b), tsolid, f.temp, f.uvel, f.vvel, f.wvel);
+    hipLaunchKernelGGL(k_clean, g, dim3(TX, TY), 0, st, d, to_dev(b), tsolid, f.temp, f.uvel, f.vvel, f.wvel, bl.dbox,
+                       bl.skip);
 }
 
 void launch_bc_flow(hipStream_t st, const Dims &d, const DevGeom &g, DevFields f, const Box &b,
-                    const amcfd_params &p, int ivar) {
-    int ni_b = b.iend - b.istat - 1, nj_b = b.jend - b.jstat + 1, nk_b = d.nkm1 - b.kstat;
-    int n = ((ivar <= 2) ? ni_b * nj_b : 0) + nk_b * nj_b;
-    if (n <= 0) return;
+                    const amcfd_params &p, int ivar, const BoxLaunch &bl) {
     double *phi = ivar == 1 ? f.uvel : ivar == 2 ? f.vvel : ivar == 3 ? f.wvel : f.pp;
-    hipLaunchKernelGGL(k_bc_flow, dim3((n + 255) / 256), dim3(256), 0, st, d, to_dev(b), ivar, p.a_oxygen, p.dgdtp,
-                       g.dxpwinv, g.dypsinv, g.dzpbinv, g.fracx, g.fracy, f.temp, f.fracl, f.vis, phi);
+    int nb;
+    if (bl.dbox) {
+        nb = bl.nblocks;
+    } else {
+        int ni_b = b.iend - b.istat - 1, nj_b = b.jend - b.jstat + 1, nk_b = d.nkm1 - b.kstat;
+        int n = ((ivar <= 2) ? ni_b * nj_b : 0) + nk_b * nj_b;
+        if (n <= 0) return;
+        nb = (n + 255) / 256;
+    }
+    hipLaunchKernelGGL(k_bc_flow, dim3(nb), dim3(256), 0, st, d, to_dev(b), ivar, p.a_oxygen, p.dgdtp, g.dxpwinv,
+                       g.dypsinv, g.dzpbinv, g.fracx, g.fracy, f.temp, f.fracl, f.vis, phi, bl.dbox, bl.skip);
 }
 
 void launch_momentum(hipStream_t st, const Dims &d, const DevGeom &g, DevFields f, Partials pr, const Box &b,
-                     const amcfd_params &p, double scanvel, int ivar) {
-    dim3 gr = box_grid(b, d);
+                     const amcfd_params &p, double scanvel, int ivar, const BoxLaunch &bl) {
+    dim3 gr = bl_grid(b, d, bl);
     if (!gr.x) return;
     double urf = ivar == 1 ? p.urfu : ivar == 2 ? p.urfv : p.urfw;
     MomConst q{p.delt, p.tsolid, urf, p.darcyKo, p.grav, p.beta, scanvel, p.discretization, p.steady};
@@ -483,33 +556,35 @@ void launch_momentum(hipStream_t st, const Dims &d, const DevGeom &g, DevFields 
     if (ivar == 1)
         hipLaunchKernelGGL(k_momentum<1>, gr, dim3(TX, TY), 0, st, d, to_dev(b), q, g, f.uvel, f.vvel, f.wvel, f.vis,
                            f.rho, f.pressure, f.unot, f.fracl, f.temp, f.an, f.as, f.ae, f.aw, f.at, f.ab, f.ap, f.su,
-                           f.dux, part);
+                           f.dux, part, bl.dbox, bl.skip);
     else if (ivar == 2)
         hipLaunchKernelGGL(k_momentum<2>, gr, dim3(TX, TY), 0, st, d, to_dev(b), q, g, f.uvel, f.vvel, f.wvel, f.vis,
                            f.rho, f.pressure, f.vnot, f.fracl, f.temp, f.an, f.as, f.ae, f.aw, f.at, f.ab, f.ap, f.su,
-                           f.dvy, part);
+                           f.dvy, part, bl.dbox, bl.skip);
     else
         hipLaunchKernelGGL(k_momentum<3>, gr, dim3(TX, TY), 0, st, d, to_dev(b), q, g, f.uvel, f.vvel, f.wvel, f.vis,
                            f.rho, f.pressure, f.wnot, f.fracl, f.temp, f.an, f.as, f.ae, f.aw, f.at, f.ab, f.ap, f.su,
-                           f.dwz, part);
+                           f.dwz, part, bl.dbox, bl.skip);
 }
 
 void launch_pressure(hipStream_t st, const Dims &d, const DevGeom &g, DevFields f, Partials pr, const Box &b,
-                     const amcfd_params &p) {
-    dim3 gr = box_grid(b, d);
+                     const amcfd_params &p, const BoxLaunch &bl) {
+    dim3 gr = bl_grid(b, d, bl);
     if (!gr.x) return;
     double *part = pr.flow + (size_t)3 * pr.nblk_f * 3;
     hipLaunchKernelGGL(k_pressure, gr, dim3(TX, TY), 0, st, d, to_dev(b), p.tsolid, g, f.uvel, f.vvel, f.wvel, f.rho,
-                       f.dux, f.dvy, f.dwz, f.temp, f.an, f.as, f.ae, f.aw, f.at, f.ab, f.ap, f.su, part);
+                       f.dux, f.dvy, f.dwz, f.temp, f.an, f.as, f.ae, f.aw, f.at, f.ab, f.ap, f.su, part, bl.dbox,
+                       bl.skip);
 }
 
-void launch_correct(hipStream_t st, const Dims &d, DevFields f, const Box &b, const amcfd_params &p) {
-    dim3 gr = box_grid(b, d);
+void launch_correct(hipStream_t st, const Dims &d, DevFields f, const Box &b, const amcfd_params &p,
+                    const BoxLaunch &bl) {
+    dim3 gr = bl_grid(b, d, bl);
     if (!gr.x) return;
     hipLaunchKernelGGL(k_correct_vel, gr, dim3(TX, TY), 0, st, d, to_dev(b), p.tsolid, f.temp, f.pp, f.dux, f.dvy,
-                       f.dwz, f.uvel, f.vvel, f.wvel);
+                       f.dwz, f.uvel, f.vvel, f.wvel, bl.dbox, bl.skip);
     hipLaunchKernelGGL(k_correct_p, gr, dim3(TX, TY), 0, st, d, to_dev(b), p.tsolid, p.urfp, f.temp, f.pp,
-                       f.pressure);
+                       f.pressure, bl.dbox, bl.skip);
 }
 
 }  // namespace amcfd

@@ -68,6 +68,10 @@ class Solver {
     // 'y' y-lines, 'z' z-lines; an even number of y/z) and red-black x
     // passes per melt-pool box variable
     int set_mixed_schedule(const char *sched, int box_passes);
+    // mixed mode: run the SIMPLE loop from a captured graph of `block`
+    // iterations (on, the default) or from host launches (off); box_blocks
+    // workgroups for the melt-pool-box kernels
+    int set_graph(int on, int block, int box_blocks);
     // raise the sweeps' device error flag (test hook for the bounded waits)
     int raise_sweep_error();
     // learned warm start: 0 off, 1 record a training sample per step, 2 use
@@ -103,7 +107,23 @@ class Solver {
     unsigned long long counter_base_ = 0;
     double *pass_buf_ = nullptr;   // four field-sized pass buffers of the line sweeps
     MxFields mx_{};                // mixed mode: FP32 coefficients, partials, second buffer
-    std::string mx_sched_ = "xzxy";   // mixed mode: order of the enthalpy line passes
+    // mixed mode, captured SIMPLE loop: a graph of graph_block_ iterations,
+    // relaunched until the device reports the step done
+    bool use_graph_ = true;
+    int graph_block_ = 8;
+    int box_blocks_ = 256;           // workgroups of the box kernels (fixed grids)
+    static constexpr int max_box_blocks = 4096;   // flow partial slots reserved for them
+    hipGraph_t graph_ = nullptr;
+    hipGraphExec_t gexec_ = nullptr;
+    struct GraphKey {
+        int nk = -1, blocks = 0, box_blocks = 0;
+        double delt = 0.0, scanvel = 0.0;
+        std::string sched;
+        int box_passes = 0;
+    } gkey_;
+    StepDev *stepd_ = nullptr;       // device
+    StepDev *steph_ = nullptr;       // pinned staging
+    std::string mx_sched_ = "xxxx";   // mixed mode: order of the enthalpy line passes
     int mx_box_passes_ = 4;          // mixed mode: red-black x passes per box variable
     int profile_ = 0;
     bool shared_device_ = false;
@@ -153,9 +173,14 @@ class Solver {
     int read_scalars();
     int flow_iteration();
     int solve_step_mixed();
+    int solve_step_mixed_graph();
     int solve_step_fp64();
-    int flow_iteration_mixed();
-    int energy_sweeps_mixed();
+    // one mixed-mode SIMPLE iteration; ctl: every decision on the device
+    // (flags in DevScalars, the step's inputs in stepd_), for capture
+    int enqueue_mixed_iteration(bool ctl);
+    int capture_mixed_graph();
+    void release_graph();
+    int energy_sweeps_mixed(const int *skip);
     int allocf(float **ptr, size_t n);
     void tic(const char *name, int *slot);
     void toc(const char *name, int slot);

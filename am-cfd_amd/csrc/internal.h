@@ -47,8 +47,13 @@ struct DevScalars {
     double fluxwest, fluxeast, fluxbottom, fluxtop, fluxsouth, fluxnorth;
     int istat, iend, jstat, jend, kstat, imin, imax, jmin, jmax, kmin;
     int error;      // nonzero: a bounded wait in the line-sweep kernel expired
+    // device-side control of the SIMPLE loop (captured graphs): the step's
+    // iteration count, set once converged or at maxit, whether the last test
+    // passed, and the flow kernels' skip flag (done, or nothing molten)
+    int niter, done, converged, noflow;
     int pad;
 };
+
 
 // geometry on the device (1-D and 2-D arrays of the reference grid)
 struct DevGeom {
@@ -93,6 +98,12 @@ struct SourceParams {
     int powerindicator;
 };
 
+// per-step inputs of a captured SIMPLE loop, written before its launches
+struct StepDev {
+    SourceParams sp;
+    int powerindicator, maxit, istart, jstart;
+};
+
 // Gaussian surface flux of one laser position (BoundCondManager.cpp:128-131)
 struct LaserParams {
     double beamx, beamy, alasfact, rb2, varlas, peakhin, pind;
@@ -133,29 +144,47 @@ struct SweepPlan {
 // kernel launchers (defined in the .hip files)
 // ---------------------------------------------------------------------------
 void launch_bc_enthalpy(hipStream_t st, const Dims &d, const DevGeom &g, DevFields f, Partials pr,
-                        const amcfd_params &p);
+                        const amcfd_params &p, const int *skip = nullptr);
 void launch_energy_assemble(hipStream_t st, const Dims &d, const DevGeom &g, DevFields f, Partials pr,
                             const amcfd_params &p, const SourceParams &sp);
 void launch_laser_map(hipStream_t st, const Dims &d, const DevGeom &g, DevFields f, Partials pr,
                       const LaserParams &q, DevScalars *sc);
-void launch_props(hipStream_t st, const Dims &d, DevFields f, const amcfd_params &p);
+void launch_props(hipStream_t st, const Dims &d, DevFields f, const amcfd_params &p, const int *skip = nullptr);
 // x-direction block correction of phi from the column-sum terms the assemble
 // kernel left in f.sp / f.apnot (enthalpy or concentration)
 void launch_block_correction(hipStream_t st, const Dims &d, DevFields f, Partials pr, double *phi, int shared);
 void launch_h2t(hipStream_t st, const Dims &d, const DevGeom &g, DevFields f, Partials pr,
-                const amcfd_params &p);
+                const amcfd_params &p, const int *skip = nullptr);
+// with device control (ctl), also sets sc->noflow for the flow kernels
 void launch_melt_pool(hipStream_t st, const Dims &d, const DevGeom &g, DevFields f, Partials pr,
-                      const amcfd_params &p, int istart, int jstart, DevScalars *sc);
-void launch_clean_velocities(hipStream_t st, const Dims &d, DevFields f, const Box &b, double tsolid);
+                      const amcfd_params &p, int istart, int jstart, DevScalars *sc, bool ctl = false,
+                      const StepDev *step = nullptr);
+// how a melt-pool-box kernel is launched: with dbox null, over the host box
+// b with one workgroup per tile (the FP64 mode); otherwise over the box the
+// melt-pool kernel left at dbox (istat, iend, jstat, jend, kstat), with
+// nblocks workgroups striding over its tiles, skipped while *skip is set
+struct BoxLaunch {
+    const int *dbox = nullptr;
+    const int *skip = nullptr;
+    int nblocks = 0;
+};
+void launch_clean_velocities(hipStream_t st, const Dims &d, DevFields f, const Box &b, double tsolid,
+                             const BoxLaunch &bl = BoxLaunch());
 void launch_bc_flow(hipStream_t st, const Dims &d, const DevGeom &g, DevFields f, const Box &b,
-                    const amcfd_params &p, int ivar);
+                    const amcfd_params &p, int ivar, const BoxLaunch &bl = BoxLaunch());
 void launch_momentum(hipStream_t st, const Dims &d, const DevGeom &g, DevFields f, Partials pr,
-                     const Box &b, const amcfd_params &p, double scanvel, int ivar);
+                     const Box &b, const amcfd_params &p, double scanvel, int ivar, const BoxLaunch &bl = BoxLaunch());
 void launch_pressure(hipStream_t st, const Dims &d, const DevGeom &g, DevFields f, Partials pr,
-                     const Box &b, const amcfd_params &p);
-void launch_correct(hipStream_t st, const Dims &d, DevFields f, const Box &b, const amcfd_params &p);
+                     const Box &b, const amcfd_params &p, const BoxLaunch &bl = BoxLaunch());
+void launch_correct(hipStream_t st, const Dims &d, DevFields f, const Box &b, const amcfd_params &p,
+                    const BoxLaunch &bl = BoxLaunch());
 void launch_finalize(hipStream_t st, const Dims &d, Partials pr, DevScalars *sc, const amcfd_params &p,
-                     int flow_done, int species_done, int steady, double width);
+                     int flow_done, int species_done, int steady, double width, bool ctl = false);
+// device-side convergence test of CFDSolverManager.cpp:690-716 after a
+// finalize: counts the iteration, sets done / converged
+void launch_converge(hipStream_t st, DevScalars *sc, const StepDev *step, const amcfd_params &p);
+// resets the device loop control before a step's captured iterations
+void launch_loop_reset(hipStream_t st, DevScalars *sc);
 void launch_end_step(hipStream_t st, const Dims &d, DevFields f, const amcfd_params &p, double track,
                      double trackid);
 // pivots of every line of a sweep and the pass buffers' "pending" fill (must
@@ -171,7 +200,8 @@ void launch_line_sweeps(hipStream_t st, const Dims &d, DevFields f, double *phi,
 
 // mixed-precision mode (mixed.hip, energy.hip)
 void launch_energy_assemble_mixed(hipStream_t st, const Dims &d, const DevGeom &g, DevFields f, Partials pr,
-                                  const MxFields &m, const amcfd_params &p, const SourceParams &sp, const int *skip);
+                                  const MxFields &m, const amcfd_params &p, const SourceParams &sp, const int *skip,
+                                  const StepDev *step = nullptr);
 int mixed_tile_count(const Dims &d);
 int mixed_max_line();
 int mixed_max_yz();
@@ -186,7 +216,7 @@ void launch_mixed_yzline(hipStream_t st, const Dims &d, const MxFields &m, const
                          int ibeg, int iend, int jbeg, int jend, int kbeg, int kend, const int *skip);
 // melt-pool box x-lines of one colour (FP64 coefficients, reference guard)
 void launch_box_xline(hipStream_t st, const Dims &d, DevFields f, double *phi, int ibeg, int iend, int jbeg, int jend,
-                      int kbeg, int kend, int colour, const int *skip);
+                      int kbeg, int kend, int colour, const int *skip, const int *dbox = nullptr, int nblocks = 0);
 
 // species (species.hip)
 void launch_bc_species(hipStream_t st, const Dims &d, const DevGeom &g, DevFields f, const amcfd_params &p);

@@ -175,9 +175,17 @@ struct MxCoefs {
 template <typename TC, int M>
 __global__ __launch_bounds__(64) void k_xline_mx(Dims d, MxCoefs<TC> cf, double *phi, int ibeg, int iend, int jbeg,
                                                   int jend, int kbeg, int kend, int colour, int guard,
-                                                  const int *__restrict__ skip) {
+                                                  const int *__restrict__ skip, const int *__restrict__ dbox) {
     __shared__ float sw[64 * M], sd[64 * M], se[64 * M], sr[64 * M];
     if (skip && *skip) return;
+    if (dbox) {   // the melt-pool box the melt-pool kernel left on the device
+        ibeg = dbox[0] + 1;
+        iend = dbox[1] - 1;
+        jbeg = dbox[2];
+        jend = dbox[3];
+        kbeg = dbox[4];
+        if (iend - ibeg + 1 > 64 * M || iend < ibeg || jend < jbeg || kend < kbeg) return;
+    }
     const int lane = threadIdx.x;
     const int n = iend - ibeg + 1, nj = jend - jbeg + 1, nk = kend - kbeg + 1;
     const int cm = (nj + 1) / 2;
@@ -324,7 +332,9 @@ __global__ __launch_bounds__(64) void k_yzline_mx(Dims d, MxCoefs<TC> cf, const 
 //   bl_i d_i - blm_i d_{i-1} - blp_i d_{i+1} = blc_i
 // (CFDSolverManager.cpp:2334-2349) solved by one wave, then phi += d_i.
 // ------------------------------------------------------------------------
-__global__ __launch_bounds__(64) void k_blk_fold(int ni, int ntile, const double *__restrict__ part, double *out) {
+__global__ __launch_bounds__(64) void k_blk_fold(int ni, int ntile, const double *__restrict__ part, double *out,
+                                                  const int *__restrict__ skip) {
+    if (skip && *skip) return;
     const int i = blockIdx.x * 64 + threadIdx.x, q = blockIdx.y, s = blockIdx.z;
     if (i >= ni) return;
     const double *p = part + (size_t)q * ntile * ni + i;
@@ -383,12 +393,15 @@ __global__ __launch_bounds__(256) void k_add_delphi_mx(Dims d, const double *__r
 
 template <typename TC>
 void launch_xline_t(hipStream_t st, const Dims &d, const MxCoefs<TC> &cf, double *phi, int ibeg, int iend, int jbeg,
-                    int jend, int kbeg, int kend, int colour, int guard, const int *skip) {
+                    int jend, int kbeg, int kend, int colour, int guard, const int *skip, const int *dbox = nullptr,
+                    int nblocks = 0) {
+    // with dbox the line range is read on the device: the host range is an
+    // upper bound (the whole interior) and fixes the kernel's row capacity
     const int n = iend - ibeg + 1;
     const long long nlines = (long long)(kend - kbeg + 1) * ((jend - jbeg + 2) / 2);
-    const int nb = (int)std::min<long long>(nlines, 16384);
+    const int nb = dbox ? nblocks : (int)std::min<long long>(nlines, 16384);
 #define XL(MM) hipLaunchKernelGGL((k_xline_mx<TC, MM>), dim3(nb), dim3(64), 0, st, d, cf, phi, ibeg, iend, jbeg, jend, \
-                                  kbeg, kend, colour, guard, skip)
+                                  kbeg, kend, colour, guard, skip, dbox)
     if (n <= 128) XL(2);
     else if (n <= 256) XL(4);
     else if (n <= 512) XL(8);
@@ -432,7 +445,7 @@ int mixed_blk_split() { return BF_SPLIT; }
 
 void launch_mixed_block_correction(hipStream_t st, const Dims &d, const MxFields &m, double *phi, const int *skip) {
     hipLaunchKernelGGL(k_blk_fold, dim3((d.ni + 63) / 64, 4, BF_SPLIT), dim3(64), 0, st, d.ni, mixed_tile_count(d),
-                       m.blkpart, m.blkfold);
+                       m.blkpart, m.blkfold, skip);
     const int n = d.nim2;
 #define BS(MM) hipLaunchKernelGGL((k_blk_solve_mx<MM>), dim3(1), dim3(BS_NT), 0, st, d.nim2, d.ni, m.blkfold, m.delphi, skip)
     if (n <= 128) BS(2);
@@ -461,9 +474,9 @@ void launch_mixed_yzline(hipStream_t st, const Dims &d, const MxFields &m, const
 // melt-pool box lines (FP64 coefficients of the momentum / pressure
 // equations, FP32 solve on the FP64 residual, guard as the reference's)
 void launch_box_xline(hipStream_t st, const Dims &d, DevFields f, double *phi, int ibeg, int iend, int jbeg, int jend,
-                      int kbeg, int kend, int colour, const int *skip) {
+                      int kbeg, int kend, int colour, const int *skip, const int *dbox, int nblocks) {
     MxCoefs<double> cf{f.ap, f.an, f.as, f.ae, f.aw, f.at, f.ab, f.su};
-    launch_xline_t<double>(st, d, cf, phi, ibeg, iend, jbeg, jend, kbeg, kend, colour, 1, skip);
+    launch_xline_t<double>(st, d, cf, phi, ibeg, iend, jbeg, jend, kbeg, kend, colour, 1, skip, dbox, nblocks);
 }
 
 }  // namespace amcfd

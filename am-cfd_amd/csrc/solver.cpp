@@ -22,9 +22,19 @@ namespace amcfd {
         if (e_ != hipSuccess) return set_error(AMCFD_E_HIP, std::string(#x) + ": " + hipGetErrorString(e_)); \
     } while (0)
 
+void Solver::release_graph() {
+    if (gexec_) (void)hipGraphExecDestroy(gexec_);
+    if (graph_) (void)hipGraphDestroy(graph_);
+    gexec_ = nullptr;
+    graph_ = nullptr;
+    gkey_ = GraphKey();
+}
+
 Solver::~Solver() {
     // teardown: errors here have no caller to go to
     if (st_) (void)hipStreamSynchronize(st_);
+    release_graph();
+    if (steph_) (void)hipHostFree(steph_);
     for (void *p : allocs_) (void)hipFree(p);
     if (sc_host_) (void)hipHostFree(sc_host_);
     for (auto &e : ev_pool_) (void)hipEventDestroy(e);
@@ -48,6 +58,17 @@ int Solver::allocf(float **ptr, size_t n) {
     HIPCALL(hipMemsetAsync(p, 0, std::max<size_t>(n, 1) * sizeof(float), st_));
     allocs_.push_back(p);
     *ptr = (float *)p;
+    return AMCFD_OK;
+}
+
+int Solver::set_graph(int on, int block, int box_blocks) {
+    if (block < 1 || block > 64) return set_error(AMCFD_E_ARG, "graph block must be 1..64 iterations");
+    if (box_blocks < 1 || box_blocks > max_box_blocks)
+        return set_error(AMCFD_E_ARG, "box workgroups must be 1.." + std::to_string(max_box_blocks));
+    use_graph_ = on != 0;
+    graph_block_ = block;
+    box_blocks_ = box_blocks;
+    release_graph();
     return AMCFD_OK;
 }
 
@@ -195,7 +216,10 @@ int Solver::create(const amcfd_params &p, int mode) {
     // partial-sum buffers sized for the whole interior
     energy_partial_sizes(d_, &pr_.nblk_e, &pr_.nrow_e, &pr_.nblk_t);
     Box whole{1, d_.nim2, 1, d_.njm2, 2};
-    int nf = flow_partial_blocks(whole, d_) + 64;
+    // the flow partials hold one slot per workgroup of the box kernels: the
+    // box's own tiles (at most the whole interior's), or the fixed
+    // workgroup count of the mixed mode's launches
+    int nf = std::max(flow_partial_blocks(whole, d_) + 64, mixed ? max_box_blocks : 0);
     if ((st = alloc(&pr_.energy, (size_t)2 * pr_.nblk_e))) return st;
     if ((st = alloc(&pr_.species, (size_t)2 * pr_.nblk_e))) return st;
     // block correction: 4 column sums, each ni long
@@ -232,6 +256,12 @@ int Solver::create(const amcfd_params &p, int mode) {
         if ((st = alloc(&mx_.blkfold, (size_t)4 * mixed_blk_split() * d_.ni))) return st;
         if ((st = alloc(&mx_.delphi, (size_t)d_.ni + 2))) return st;
         if ((st = alloc(&mx_.halt, nn))) return st;
+        void *v = nullptr;
+        HIPCALL(hipMalloc(&v, sizeof(StepDev)));
+        allocs_.push_back(v);
+        stepd_ = (StepDev *)v;
+        HIPCALL(hipHostMalloc((void **)&steph_, sizeof(StepDev), hipHostMallocDefault));
+        std::memset(steph_, 0, sizeof(StepDev));
     }
 
     scanvel_ = p_.scanvel;
@@ -689,15 +719,14 @@ int Solver::solve_step_fp64() {
 // order (mixed.hip); properties, H->T, the melt pool, the heat balance, the
 // residual folds and the convergence test are the FP64 mode's kernels
 // ---------------------------------------------------------------------------
-int Solver::energy_sweeps_mixed() {
+int Solver::energy_sweeps_mixed(const int *skip) {
     double *cur = f_.enthalpy, *other = mx_.halt;
     for (char c : mx_sched_) {
         if (c == 'x') {
-            launch_mixed_xline(st_, d_, mx_, cur, 1, d_.nim2, 1, d_.njm2, 1, d_.nkm2, 0, nullptr);
-            launch_mixed_xline(st_, d_, mx_, cur, 1, d_.nim2, 1, d_.njm2, 1, d_.nkm2, 1, nullptr);
+            launch_mixed_xline(st_, d_, mx_, cur, 1, d_.nim2, 1, d_.njm2, 1, d_.nkm2, 0, skip);
+            launch_mixed_xline(st_, d_, mx_, cur, 1, d_.nim2, 1, d_.njm2, 1, d_.nkm2, 1, skip);
         } else {
-            launch_mixed_yzline(st_, d_, mx_, cur, other, c == 'y' ? 1 : 2, 1, d_.nim2, 1, d_.njm2, 1, d_.nkm2,
-                                nullptr);
+            launch_mixed_yzline(st_, d_, mx_, cur, other, c == 'y' ? 1 : 2, 1, d_.nim2, 1, d_.njm2, 1, d_.nkm2, skip);
             std::swap(cur, other);
         }
     }
@@ -705,28 +734,72 @@ int Solver::energy_sweeps_mixed() {
     return 0;
 }
 
-int Solver::flow_iteration_mixed() {
-    box_ = Box{sc_host_->istat, sc_host_->iend, sc_host_->jstat, sc_host_->jend, sc_host_->kstat};
-    pr_.nblk_f = flow_partial_blocks(box_, d_);
-    launch_clean_velocities(st_, d_, f_, box_, p_.tsolid);
-    double *phis[5] = {nullptr, f_.uvel, f_.vvel, f_.wvel, f_.pp};
-    const int ib = box_.istatp1(), ie = box_.iendm1();
-    for (int ivar = 1; ivar <= 4; ++ivar) {
-        launch_bc_flow(st_, d_, g_, f_, box_, p_, ivar);
-        if (ivar < 4) launch_momentum(st_, d_, g_, f_, pr_, box_, p_, scanvel_, ivar);
-        else launch_pressure(st_, d_, g_, f_, pr_, box_, p_);
-        if (ie >= ib && box_.jend >= box_.jstat && d_.nkm2 >= box_.kstat)
+// One SIMPLE iteration of the mixed mode.  The melt-pool-box kernels always
+// take the box from device memory (the melt-pool kernel writes it) with a
+// fixed number of workgroups, so the host-driven and the captured loop run
+// the very same launches.  Without ctl the host reads the scalars after the
+// melt pool (flow or not) and after the finalize (convergence), as the FP64
+// mode does; with ctl the kernels skip themselves on the device flags and
+// k_converge takes the decision.
+int Solver::enqueue_mixed_iteration(bool ctl) {
+    const int *done = ctl ? &sc_->done : nullptr;
+    int slot;
+    if (!ctl) {
+        SourceParams sp = source_params();
+        *steph_ = StepDev{sp, powerindicator_, (int)p_.maxit, istart_, jstart_};
+        HIPCALL(hipMemcpyAsync(stepd_, steph_, sizeof(StepDev), hipMemcpyHostToDevice, st_));
+    }
+    launch_bc_enthalpy(st_, d_, g_, f_, pr_, p_, done);
+    tic("assemble_h", &slot);
+    launch_energy_assemble_mixed(st_, d_, g_, f_, pr_, mx_, p_, SourceParams{}, done, stepd_);
+    toc("assemble_h", slot);
+    launch_props(st_, d_, f_, p_, done);
+    launch_mixed_block_correction(st_, d_, mx_, f_.enthalpy, done);
+    tic("sweep_h", &slot);
+    int st = energy_sweeps_mixed(done);
+    toc("sweep_h", slot);
+    if (st) return st;
+    tic("h2t", &slot);
+    launch_h2t(st_, d_, g_, f_, pr_, p_, done);
+    toc("h2t", slot);
+    launch_melt_pool(st_, d_, g_, f_, pr_, p_, istart_, jstart_, sc_, ctl, stepd_);
+    HIPCALL(hipGetLastError());
+    int flow = 1;
+    if (!ctl) {
+        if ((st = read_scalars())) return st;
+        alen_ = sc_host_->alen;
+        flow = sc_host_->tpeak > p_.tsolid && p_.navierstokes;
+    }
+    pr_.nblk_f = box_blocks_;
+    if (flow) {
+        BoxLaunch bl;
+        bl.dbox = &sc_->istat;
+        bl.skip = ctl ? &sc_->noflow : nullptr;
+        bl.nblocks = box_blocks_;
+        launch_clean_velocities(st_, d_, f_, box_, p_.tsolid, bl);
+        double *phis[5] = {nullptr, f_.uvel, f_.vvel, f_.wvel, f_.pp};
+        for (int ivar = 1; ivar <= 4; ++ivar) {
+            launch_bc_flow(st_, d_, g_, f_, box_, p_, ivar, bl);
+            if (ivar < 4) launch_momentum(st_, d_, g_, f_, pr_, box_, p_, scanvel_, ivar, bl);
+            else launch_pressure(st_, d_, g_, f_, pr_, box_, p_, bl);
             for (int pass = 0; pass < mx_box_passes_; ++pass)
                 for (int colour = 0; colour < 2; ++colour)
-                    launch_box_xline(st_, d_, f_, phis[ivar], ib, ie, box_.jstat, box_.jend, box_.kstat, d_.nkm2,
-                                     colour, nullptr);
-        if (ivar == 4) launch_correct(st_, d_, f_, box_, p_);
+                    launch_box_xline(st_, d_, f_, phis[ivar], 1, d_.nim2, 1, d_.njm2, 1, d_.nkm2, colour, bl.skip,
+                                     bl.dbox, box_blocks_);
+            if (ivar == 4) launch_correct(st_, d_, f_, box_, p_, bl);
+        }
     }
+    launch_finalize(st_, d_, pr_, sc_, p_, flow, 0, p_.steady, ctl ? 0.0 : sc_host_->width, ctl);
+    if (ctl) launch_converge(st_, sc_, stepd_, p_);
     HIPCALL(hipGetLastError());
     return 0;
 }
 
 int Solver::solve_step_mixed() {
+    // captured loop where the step's inputs are fixed for the whole step (a
+    // heat-source table may advance its row between iterations, which the
+    // host loop does as the reference)
+    if (use_graph_ && heatt_.empty()) return solve_step_mixed_graph();
     HIPCALL(hipEventRecord(step_ev_[0], st_));
     const double alenold = alen_;
     niter_ = 0;
@@ -734,32 +807,8 @@ int Solver::solve_step_mixed() {
     while (niter_ < p_.maxit) {
         ++niter_;
         ++itertot_;
-        SourceParams sp = source_params();
-        int slot;
-        launch_bc_enthalpy(st_, d_, g_, f_, pr_, p_);
-        tic("assemble_h", &slot);
-        launch_energy_assemble_mixed(st_, d_, g_, f_, pr_, mx_, p_, sp, nullptr);
-        toc("assemble_h", slot);
-        launch_props(st_, d_, f_, p_);
-        launch_mixed_block_correction(st_, d_, mx_, f_.enthalpy, nullptr);
-        tic("sweep_h", &slot);
-        int st = energy_sweeps_mixed();
-        toc("sweep_h", slot);
+        int st = enqueue_mixed_iteration(false);
         if (st) return st;
-        tic("h2t", &slot);
-        launch_h2t(st_, d_, g_, f_, pr_, p_);
-        toc("h2t", slot);
-        launch_melt_pool(st_, d_, g_, f_, pr_, p_, istart_, jstart_, sc_);
-        HIPCALL(hipGetLastError());
-        if ((st = read_scalars())) return st;
-        alen_ = sc_host_->alen;
-        int flow = 0;
-        if (sc_host_->tpeak > p_.tsolid && p_.navierstokes) {
-            if ((st = flow_iteration_mixed())) return st;
-            flow = 1;
-        }
-        launch_finalize(st_, d_, pr_, sc_, p_, flow, 0, p_.steady, sc_host_->width);
-        HIPCALL(hipGetLastError());
         if ((st = read_scalars())) return st;
         if (profile_) collect_timers();
         const DevScalars &s = *sc_host_;
@@ -777,6 +826,71 @@ int Solver::solve_step_mixed() {
             break;
         }
     }
+    if (rc == AMCFD_I_MAXIT && std::fabs(alen_ - alenold) < 1.0e-6) steadylength_ = true;
+    HIPCALL(hipEventRecord(step_ev_[1], st_));
+    HIPCALL(hipEventSynchronize(step_ev_[1]));
+    float ms = 0.f;
+    HIPCALL(hipEventElapsedTime(&ms, step_ev_[0], step_ev_[1]));
+    last_total_ms = ms;
+    last_tdma_ms = -1.0;
+    if (!std::isfinite(sc_host_->resorh))
+        return set_error(AMCFD_E_NAN, "the enthalpy residual is not finite after step " + std::to_string(step_));
+    return rc;
+}
+
+int Solver::capture_mixed_graph() {
+    release_graph();
+    const int prof = profile_;
+    profile_ = 0;   // no events inside a capture
+    HIPCALL(hipStreamBeginCapture(st_, hipStreamCaptureModeThreadLocal));
+    int st = 0;
+    for (int it = 0; it < graph_block_ && !st; ++it) st = enqueue_mixed_iteration(true);
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(st_, &g);
+    profile_ = prof;
+    if (st) {
+        if (g) (void)hipGraphDestroy(g);
+        return st;
+    }
+    if (e != hipSuccess) return set_error(AMCFD_E_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+    graph_ = g;
+    HIPCALL(hipGraphInstantiate(&gexec_, graph_, nullptr, nullptr, 0));
+    gkey_.nk = d_.nk;
+    gkey_.blocks = graph_block_;
+    gkey_.box_blocks = box_blocks_;
+    gkey_.delt = p_.delt;
+    gkey_.scanvel = scanvel_;
+    gkey_.sched = mx_sched_;
+    gkey_.box_passes = mx_box_passes_;
+    return 0;
+}
+
+int Solver::solve_step_mixed_graph() {
+    HIPCALL(hipEventRecord(step_ev_[0], st_));
+    const double alenold = alen_;
+    // every launch argument the graph holds by value: rebuild when one moved
+    if (!gexec_ || gkey_.nk != d_.nk || gkey_.blocks != graph_block_ || gkey_.box_blocks != box_blocks_ ||
+        gkey_.delt != p_.delt || gkey_.scanvel != scanvel_ || gkey_.sched != mx_sched_ ||
+        gkey_.box_passes != mx_box_passes_) {
+        int st = capture_mixed_graph();
+        if (st) return st;
+    }
+    SourceParams sp = source_params();
+    *steph_ = StepDev{sp, powerindicator_, (int)p_.maxit, istart_, jstart_};
+    HIPCALL(hipMemcpyAsync(stepd_, steph_, sizeof(StepDev), hipMemcpyHostToDevice, st_));
+    launch_loop_reset(st_, sc_);
+    HIPCALL(hipGetLastError());
+    const int nlaunch = ((int)p_.maxit + graph_block_ - 1) / graph_block_;
+    for (int l = 0; l < nlaunch; ++l) {
+        HIPCALL(hipGraphLaunch(gexec_, st_));
+        int st = read_scalars();
+        if (st) return st;
+        if (sc_host_->done) break;
+    }
+    niter_ = sc_host_->niter;
+    itertot_ += niter_;
+    alen_ = sc_host_->alen;
+    const int rc = sc_host_->converged ? AMCFD_OK : AMCFD_I_MAXIT;
     if (rc == AMCFD_I_MAXIT && std::fabs(alen_ - alenold) < 1.0e-6) steadylength_ = true;
     HIPCALL(hipEventRecord(step_ev_[1], st_));
     HIPCALL(hipEventSynchronize(step_ev_[1]));

@@ -283,10 +283,23 @@ int  amcfd_tdma_lines_noalloc(const double *aw, const double *ap, const double *
 /* Mixed mode only: the order of the enthalpy line passes of one SIMPLE
  * iteration ('x' = x-lines red then black, in place; 'y' / 'z' = y- or
  * z-lines, one lane per line, between the field and a second buffer; an
- * even number of 'y'/'z'; default "xzxy") and the red-black x passes per
+ * even number of 'y'/'z'; default "xxxx") and the red-black x passes per
  * melt-pool box variable (default 4).  The reference sweeps x-lines only,
  * four Gauss-Seidel passes (CFDSolverManager.cpp:2415-2453, 2492-2525). */
 int  amcfd_set_mixed_schedule(amcfd_solver *s, const char *schedule, int box_passes);
+
+/* Mixed mode only: run each step's SIMPLE loop from a captured HIP graph of
+ * `block` iterations (on = 1, the default) or from host launches (0).  In
+ * the graph every kernel reads the loop's state from device memory: the
+ * melt-pool box and the flow/no-flow decision (CFDSolverManager.cpp:657-
+ * 675), and the convergence test (:690-716), which a device kernel takes
+ * after each iteration and which turns the rest of the block into no-ops;
+ * the host relaunches the graph until the step is done or maxit, reading
+ * the scalars once per block.  box_blocks workgroups stride over the box
+ * (default 256, at most 4096).  Both ways run the same launches, so the results agree bit
+ * for bit.  A step with a heat-source table row change always runs from the
+ * host. */
+int  amcfd_set_graph(amcfd_solver *s, int on, int block, int box_blocks);
 
 /* Test hook, no reference counterpart: raise the device error flag that
  * the Gauss-Seidel line sweeps' bounded waits watch, so that every wait of
