@@ -39,9 +39,9 @@ EXPORTS = ("amcfd_deck_load", "amcfd_deck_params", "amcfd_deck_free", "amcfd_cre
            "amcfd_get_scalars", "amcfd_dims", "amcfd_last_timing", "amcfd_set_profiling",
            "amcfd_kernel_stats", "amcfd_write_vts", "amcfd_write_pvd", "amcfd_append_residuals",
            "amcfd_append_final_geometry", "amcfd_deck_string", "amcfd_free_surface",
-           "amcfd_output_section", "amcfd_run_batch", "amcfd_set_mixed_schedule", "amcfd_set_graph", "amcfd_test_expire_waits",
+           "amcfd_output_section", "amcfd_run_batch", "amcfd_set_mixed_schedule", "amcfd_set_graph", "amcfd_copy_state", "amcfd_test_expire_waits",
            "amcfd_tdma_lines", "amcfd_tdma_lines_noalloc",
-           "amcfd_ws_create", "amcfd_ws_destroy", "amcfd_ws_attach", "amcfd_ws_train", "amcfd_ws_samples",
+           "amcfd_ws_create", "amcfd_ws_destroy", "amcfd_ws_attach", "amcfd_ws_counts", "amcfd_ws_train", "amcfd_ws_samples",
            "amcfd_ws_set_factors", "amcfd_ws_save", "amcfd_ws_load", "amcfd_ws_param_count", "amcfd_ws_params",
            "amcfd_ws_loss_grad",
            "amcfd_last_error", "amcfd_version")
@@ -153,11 +153,13 @@ def load_library(path=LIB_PATH):
         "amcfd_set_mixed_schedule": (ip, [vp, cp, ip]),
         "amcfd_test_expire_waits": (ip, [vp]),
         "amcfd_set_graph": (ip, [vp, ip, ip, ip]),
+        "amcfd_copy_state": (ip, [vp, vp]),
         "amcfd_tdma_lines": (ip, [vp, vp, vp, vp, vp, ip, ip, ip, vp]),
         "amcfd_tdma_lines_noalloc": (ip, [vp, vp, vp, vp, vp, ip, ip, ip, vp]),
         "amcfd_ws_create": (ip, [ip, ctypes.POINTER(vp)]),
         "amcfd_ws_destroy": (None, [vp]),
         "amcfd_ws_attach": (ip, [vp, vp, ip]),
+        "amcfd_ws_counts": (ip, [vp, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]),
         "amcfd_ws_train": (ip, [vp, ip, ip, ctypes.c_double, dp, dp]),
         "amcfd_ws_samples": (ip, [vp]),
         "amcfd_ws_set_factors": (ip, [vp, ctypes.c_double, ctypes.c_double]),
@@ -319,6 +321,17 @@ class Solver:
         """Mixed mode: captured-graph SIMPLE loop (device-side control) or host loop."""
         _check(load_library().amcfd_set_graph(self._h, int(bool(on)), int(block), int(box_blocks)))
 
+    def warm_start_counts(self):
+        """(steps started from the warm start's guess, steps its guard kept
+        on the plain start)."""
+        a, g = ctypes.c_int(), ctypes.c_int()
+        _check(load_library().amcfd_ws_counts(self._h, ctypes.byref(a), ctypes.byref(g)))
+        return a.value, g.value
+
+    def copy_state_from(self, other):
+        """Continue from another solver's exact state (same deck)."""
+        _check(load_library().amcfd_copy_state(self._h, other._h))
+
     def test_expire_waits(self):
         """Raise the line sweeps' device error flag (test hook)."""
         _check(load_library().amcfd_test_expire_waits(self._h))
@@ -357,7 +370,9 @@ def tdma_lines_noalloc(aw, ap, ae, d, phi, n, nlines, guard=0, stream=None):
     _check(load_library().amcfd_tdma_lines_noalloc(aw, ap, ae, d, phi, int(n), int(nlines), int(guard), stream))
 
 
-WS_OFF, WS_RECORD, WS_APPLY = 0, 1, 2
+# warm start modes: off, record samples, start a step from the guess when the
+# step before converged, start every step from the guess
+WS_OFF, WS_RECORD, WS_APPLY, WS_APPLY_ALWAYS = 0, 1, 2, 3
 
 
 class WarmStartNet:

@@ -299,6 +299,12 @@ int amcfd_set_graph(amcfd_solver *s, int on, int block, int box_blocks) {
     GUARD(s->s.set_graph(on, block, box_blocks));
 }
 
+int amcfd_copy_state(amcfd_solver *dst, const amcfd_solver *src) {
+    if (!dst || !src) return amcfd::set_error(AMCFD_E_ARG, "null solver");
+    if (dst == src) return AMCFD_OK;
+    GUARD(dst->s.copy_state_from(src->s));
+}
+
 int amcfd_test_expire_waits(amcfd_solver *s) {
     if (!s) return amcfd::set_error(AMCFD_E_ARG, "null solver");
     GUARD(s->s.raise_sweep_error());
@@ -306,8 +312,14 @@ int amcfd_test_expire_waits(amcfd_solver *s) {
 
 int amcfd_ws_attach(amcfd_solver *s, amcfd_ws *ws, int mode) {
     if (!s || (mode && !ws)) return amcfd::set_error(AMCFD_E_ARG, "null argument");
-    if (mode < 0 || mode > 2) return amcfd::set_error(AMCFD_E_ARG, "mode must be 0, 1 or 2");
+    if (mode < 0 || mode > 3) return amcfd::set_error(AMCFD_E_ARG, "mode must be 0, 1, 2 or 3");
     s->s.attach_warm_start(amcfd::ws_of(ws), mode);
+    return AMCFD_OK;
+}
+
+int amcfd_ws_counts(const amcfd_solver *s, int *applied, int *guarded) {
+    if (!s || !applied || !guarded) return amcfd::set_error(AMCFD_E_ARG, "null argument");
+    s->s.warm_start_counts(applied, guarded);
     return AMCFD_OK;
 }
 

@@ -34,6 +34,8 @@ class Solver;
 int ws_capture_input(WarmStart *w, Solver &s);
 int ws_capture_target(WarmStart *w, Solver &s);
 int ws_apply(WarmStart *w, Solver &s);
+void ws_attach(WarmStart *w, Solver *s);
+void ws_detach(WarmStart *w, Solver *s);
 
 // accumulated device time of one kernel family (HIP events on the solver stream)
 struct KernelTimer {
@@ -74,11 +76,29 @@ class Solver {
     int set_graph(int on, int block, int box_blocks);
     // raise the sweeps' device error flag (test hook for the bounded waits)
     int raise_sweep_error();
+    // take over another solver's state (same deck): every solution field on
+    // the device, the grid, the laser / time-loop state and the scalars
+    int copy_state_from(const Solver &o);
     // learned warm start: 0 off, 1 record a training sample per step, 2 use
-    // the network's prediction as each step's initial iterate
+    // the network's prediction as the initial iterate of each step whose
+    // predecessor converged, 3 on every step
+    // the network keeps per-solver recording state: attaching registers this
+    // solver with it, detaching (or destroying the solver) releases that
     void attach_warm_start(WarmStart *w, int mode) {
-        ws_ = mode ? w : nullptr;
+        WarmStart *nw = mode ? w : nullptr;
+        if (ws_ && ws_ != nw) ws_detach(ws_, this);
+        if (nw && nw != ws_) ws_attach(nw, this);
+        ws_ = nw;
         ws_mode_ = mode;
+    }
+    void warm_start_counts(int *applied, int *guarded) const {
+        *applied = ws_applied_;
+        *guarded = ws_guarded_;
+    }
+    // called by a network that is destroyed while this solver is attached
+    void forget_warm_start() {
+        ws_ = nullptr;
+        ws_mode_ = 0;
     }
     // views for the warm start
     const Dims &dev_dims() const { return d_; }
@@ -146,6 +166,8 @@ class Solver {
     Box box_{};
     WarmStart *ws_ = nullptr;
     int ws_mode_ = 0;
+    int last_solve_rc_ = AMCFD_I_MAXIT;   // the previous solve's status (none yet: not converged)
+    int ws_applied_ = 0, ws_guarded_ = 0;  // steps started from the guess / kept on the plain start
     int warm_start_begin();
     int warm_start_end();
 

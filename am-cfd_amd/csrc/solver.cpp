@@ -33,6 +33,7 @@ void Solver::release_graph() {
 Solver::~Solver() {
     // teardown: errors here have no caller to go to
     if (st_) (void)hipStreamSynchronize(st_);
+    if (ws_) ws_detach(ws_, this);
     release_graph();
     if (steph_) (void)hipHostFree(steph_);
     for (void *p : allocs_) (void)hipFree(p);
@@ -575,6 +576,67 @@ int Solver::sweep(double *phi, SweepPlan &plan, int ibeg, int iend, int jbeg, in
     return 0;
 }
 
+int Solver::copy_state_from(const Solver &o) {
+    if (o.d_.ni != d_.ni || o.d_.nj != d_.nj || o.d_.nkf != d_.nkf || o.p_.species != p_.species ||
+        o.p_.energyfreesurface != p_.energyfreesurface || o.p_.solidification != p_.solidification)
+        return set_error(AMCFD_E_ARG, "the solvers do not run the same grid and keywords");
+    const size_t nn = (size_t)d_.ni * d_.nj * d_.nkf;
+    // the state a step starts from; coefficient arrays are rebuilt by every
+    // iteration
+    double *const dst[] = {f_.enthalpy, f_.temp, f_.fracl, f_.uvel, f_.vvel, f_.wvel, f_.pressure, f_.pp,
+                           f_.rho, f_.diff, f_.vis, f_.hnot, f_.fraclnot, f_.unot, f_.vnot, f_.wnot,
+                           f_.sourceinput, f_.dux, f_.dvy, f_.dwz, f_.csfrac, f_.solfrac, f_.mpphase,
+                           f_.mpliqphase, f_.conc, f_.concnot, f_.concl, f_.grad, f_.rate, f_.zr, f_.tempr};
+    const double *const src[] = {o.f_.enthalpy, o.f_.temp, o.f_.fracl, o.f_.uvel, o.f_.vvel, o.f_.wvel,
+                                 o.f_.pressure, o.f_.pp, o.f_.rho, o.f_.diff, o.f_.vis, o.f_.hnot, o.f_.fraclnot,
+                                 o.f_.unot, o.f_.vnot, o.f_.wnot, o.f_.sourceinput, o.f_.dux, o.f_.dvy, o.f_.dwz,
+                                 o.f_.csfrac, o.f_.solfrac, o.f_.mpphase, o.f_.mpliqphase, o.f_.conc,
+                                 o.f_.concnot, o.f_.concl, o.f_.grad, o.f_.rate, o.f_.zr, o.f_.tempr};
+    HIPCALL(hipStreamSynchronize(o.st_));
+    for (size_t a = 0; a < sizeof(dst) / sizeof(dst[0]); ++a)
+        if (dst[a] && src[a]) HIPCALL(hipMemcpyAsync(dst[a], src[a], nn * sizeof(double), hipMemcpyDeviceToDevice, st_));
+    if (mx_.halt) HIPCALL(hipMemcpyAsync(mx_.halt, o.f_.enthalpy, nn * sizeof(double), hipMemcpyDeviceToDevice, st_));
+    HIPCALL(hipMemcpyAsync(f_.heatin, o.f_.heatin, (size_t)d_.ni * d_.nj * sizeof(double), hipMemcpyDeviceToDevice, st_));
+    if (f_.fi) {
+        HIPCALL(hipMemcpyAsync(f_.fi, o.f_.fi, (size_t)d_.ni * d_.nj * sizeof(double), hipMemcpyDeviceToDevice, st_));
+        HIPCALL(hipMemcpyAsync(f_.zzm, o.f_.zzm, (size_t)d_.ni * d_.nj * sizeof(double), hipMemcpyDeviceToDevice, st_));
+    }
+    HIPCALL(hipMemcpyAsync(sc_, o.sc_, sizeof(DevScalars), hipMemcpyDeviceToDevice, st_));
+    // grid (moving mesh, powder layers) and host state
+    p_.delt = o.p_.delt;
+    p_.outtime = o.p_.outtime;
+    std::memcpy(p_.xzone, o.p_.xzone, sizeof p_.xzone);
+    std::memcpy(p_.yzone, o.p_.yzone, sizeof p_.yzone);
+    hg_ = o.hg_;
+    set_active_nk(o.d_.nk);
+    int st = upload_geometry();
+    if (st) return st;
+    timet_ = o.timet_;
+    scanvel_ = o.scanvel_;
+    beamx_ = o.beamx_; beamy_ = o.beamy_; beamz_ = o.beamz_;
+    oldx_ = o.oldx_; oldy_ = o.oldy_; oldz_ = o.oldz_;
+    powerindicator_ = o.powerindicator_;
+    laser_init_ = o.laser_init_;
+    trackindx_ = o.trackindx_;
+    istart_ = o.istart_;
+    jstart_ = o.jstart_;
+    heat_it_ = o.heat_it_;
+    dt_state_ = o.dt_state_;
+    step_ = o.step_;
+    niter_ = o.niter_;
+    itertot_ = o.itertot_;
+    last_solve_rc_ = o.last_solve_rc_;
+    alen_ = o.alen_;
+    steadylength_ = o.steadylength_;
+    tracksect_ = o.tracksect_;
+    section_sci_ = o.section_sci_;
+    box_ = o.box_;
+    HIPCALL(hipStreamSynchronize(st_));
+    *sc_host_ = *o.sc_host_;
+    release_graph();
+    return AMCFD_OK;
+}
+
 int Solver::raise_sweep_error() {
     const int one = 1;
     HIPCALL(hipMemcpyAsync(&sc_->error, &one, sizeof one, hipMemcpyHostToDevice, st_));
@@ -617,9 +679,21 @@ int Solver::flow_iteration() {
 // the previous step ended with, CFDSolverManager.cpp:608-612) by the
 // network's guess, and brings T, the liquid fraction and the properties in
 // line with it before the first iteration
+//
+// The guard of mode 2: a solve that converges ends on the reference's test
+// (CFDSolverManager.cpp:690-716), whatever it started from; one that runs to
+// maxit ends where maxit iterations from its start lead, so there the guess
+// would change the answer.  Mode 2 therefore starts a step from the guess
+// only when the step before it converged, and keeps the plain start on runs
+// whose steps end at maxit.
 int Solver::warm_start_begin() {
     if (!ws_) return 0;
     if (ws_mode_ == 1) return ws_capture_input(ws_, *this);
+    if (ws_mode_ == 2 && last_solve_rc_ != AMCFD_OK) {
+        ++ws_guarded_;
+        return 0;
+    }
+    ++ws_applied_;
     int st = ws_apply(ws_, *this);
     if (st) return st;
     launch_h2t(st_, d_, g_, f_, pr_, p_);
@@ -638,6 +712,7 @@ int Solver::solve_step() {
     if (st0) return st0;
     int rc0 = mode_ == AMCFD_MODE_MIXED ? solve_step_mixed() : solve_step_fp64();
     if (rc0 < 0) return rc0;
+    last_solve_rc_ = rc0;
     st0 = warm_start_end();
     return st0 ? st0 : rc0;
 }

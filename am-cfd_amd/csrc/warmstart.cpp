@@ -22,7 +22,9 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <random>
+#include <unordered_map>
 
 namespace amcfd {
 
@@ -58,11 +60,17 @@ class WarmStart {
     int loss_grad(const float *x, const float *tgt, const float *mask, int N, int D, int H, int W, double *loss,
                   float *grad);
     int train(int iters, int batch, double lr, double *loss_first, double *loss_last);
-    int samples() const { return nsamp_; }
-    // solver hooks (on the solver's stream)
+    int samples() {
+        std::lock_guard<std::mutex> lock(mu_);
+        return nsamp_;
+    }
+    // solver hooks (on the solver's stream); solvers on other host threads
+    // may record and apply at the same time
     int capture_input(Solver &s);
     int capture_target(Solver &s);
     int apply(Solver &s);
+    void attach(Solver *s);
+    void detach(Solver *s);
     double tfactor = 1.0, pfactor = 0.0;
 
   private:
@@ -78,18 +86,28 @@ class WarmStart {
     unsigned short *x_ = nullptr, *a_[8] = {}, *c_[3] = {}, *dz_ = nullptr;
     float *out_ = nullptr, *dout_ = nullptr, *da_[8] = {}, *dc_[3] = {}, *tgt_ = nullptr, *mask_ = nullptr;
     float *acc_ = nullptr;
-    // recorded samples (window-sized)
+    // recorded samples (window-sized ring of cap_ slots; filled_[slot] once
+    // a sample's target is written)
     int cap_ = 0, nsamp_ = 0, next_ = 0;
     unsigned short *sx_ = nullptr;
     float *stgt_ = nullptr, *smask_ = nullptr;
-    double *told_ = nullptr, *pold_ = nullptr;   // the pending sample's start-of-step T and p (full fields)
-    bool pending_ = false;
-    nn::Window pend_win_{};
+    std::vector<char> filled_;
+    // per attached solver: the pending sample's start-of-step T and p (full
+    // fields), its window and its slot (-1: none pending)
+    struct Rec {
+        double *told = nullptr, *pold = nullptr;
+        size_t n = 0;
+        nn::Window win{};
+        int slot = -1;
+    };
+    std::unordered_map<Solver *, Rec> recs_;
+    // guards the ring, recs_ and the network's buffers (one forward/train at
+    // a time)
+    std::mutex mu_;
     int step_t_ = 0;
     hipStream_t st_ = nullptr;
     std::vector<void *> allocs_, act_allocs_;
     std::mt19937 rng_;
-    size_t told_n_ = 0;
 
     template <typename T>
     int dalloc(T **p, size_t n, std::vector<void *> &list) {
@@ -109,6 +127,14 @@ class WarmStart {
 };
 
 WarmStart::~WarmStart() {
+    // solvers still attached forget this network (their recording buffers
+    // are released with it)
+    for (auto &kv : recs_) {
+        (void)hipStreamSynchronize(kv.first->stream());
+        kv.first->forget_warm_start();
+        (void)hipFree(kv.second.told);
+        (void)hipFree(kv.second.pold);
+    }
     if (st_) (void)hipStreamSynchronize(st_);
     for (void *p : act_allocs_) (void)hipFree(p);
     for (void *p : allocs_) (void)hipFree(p);
@@ -310,6 +336,7 @@ int WarmStart::backward(hipStream_t st, float inv_count) {
 
 int WarmStart::loss_grad(const float *x, const float *tgt, const float *mask, int N, int D, int H, int W,
                          double *loss, float *grad) {
+    std::lock_guard<std::mutex> lock(mu_);
     int st = ensure_batch(N, D, H, W);
     if (st) return st;
     const size_t P0 = (size_t)N * D * H * W;
@@ -343,17 +370,22 @@ int WarmStart::loss_grad(const float *x, const float *tgt, const float *mask, in
 }
 
 int WarmStart::train(int iters, int batch, double lr, double *loss_first, double *loss_last) {
+    std::lock_guard<std::mutex> lock(mu_);
     if (nsamp_ < 1) return set_error(AMCFD_E_ARG, "no recorded samples to train on");
     if (iters < 1 || batch < 1) return set_error(AMCFD_E_ARG, "iters and batch must be positive");
     int st = ensure_batch(batch, WD, WH, WW);
     if (st) return st;
     const size_t P = (size_t)WD * WH * WW;
-    std::uniform_int_distribution<int> pick(0, nsamp_ - 1);
+    // the filled slots (samples still pending on some solver are skipped)
+    std::vector<int> slots;
+    for (int i = 0; i < cap_; ++i)
+        if (filled_[i]) slots.push_back(i);
+    std::uniform_int_distribution<int> pick(0, (int)slots.size() - 1);
     const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
     double first = -1.0, last = -1.0;
     for (int it = 0; it < iters; ++it) {
         for (int b = 0; b < batch; ++b) {
-            const int s = pick(rng_);
+            const int s = slots[pick(rng_)];
             WSCALL(hipMemcpyAsync(x_ + b * P * 16, sx_ + (size_t)s * P * 16, P * 16 * sizeof(unsigned short),
                                   hipMemcpyDeviceToDevice, st_));
             WSCALL(hipMemcpyAsync(tgt_ + b * P * nn::NN_OUT, stgt_ + (size_t)s * P * nn::NN_OUT,
@@ -393,8 +425,11 @@ nn::FeatScale WarmStart::scale(const Solver &s) const {
     f.tinv = 1e-3;
     f.vinv = 0.1;
     f.qinv = peak > 0 ? 1.0 / peak : 0.0;
+    // targets of order one: a molten step changes T near the beam by tens
+    // of K (rms; up to ~1e3 K at the spot) and the pressure by ~1e3 Pa (up
+    // to ~1e5 Pa), so that neither channel swamps the other in the loss
     f.dtscale = 100.0;
-    f.dpscale = 100.0;
+    f.dpscale = 1e4;
     f.dtinv = 1.0 / f.dtscale;
     f.dpinv = 1.0 / f.dpscale;
     return f;
@@ -415,47 +450,94 @@ nn::Window WarmStart::window(const Solver &s) const {
     return w;
 }
 
+void WarmStart::attach(Solver *s) {
+    std::lock_guard<std::mutex> lock(mu_);
+    recs_.emplace(s, Rec{});
+}
+
+void WarmStart::detach(Solver *s) {
+    std::lock_guard<std::mutex> lock(mu_);
+    auto it = recs_.find(s);
+    if (it == recs_.end()) return;
+    (void)hipStreamSynchronize(s->stream());
+    (void)hipFree(it->second.told);
+    (void)hipFree(it->second.pold);
+    recs_.erase(it);
+}
+
 int WarmStart::capture_input(Solver &s) {
     const size_t P = (size_t)WD * WH * WW;
-    if (!sx_) {
-        cap_ = 2048;
-        int st;
-        if ((st = dalloc(&sx_, (size_t)cap_ * P * 16, allocs_)) ||
-            (st = dalloc(&stgt_, (size_t)cap_ * P * nn::NN_OUT, allocs_)) ||
-            (st = dalloc(&smask_, (size_t)cap_ * P, allocs_)))
-            return st;
-    }
     const size_t nn_ = (size_t)s.dev_dims().ni * s.dev_dims().nj * s.dev_dims().nkf;
-    if (told_n_ != nn_) {
-        int st;
-        if ((st = dalloc(&told_, nn_, allocs_)) || (st = dalloc(&pold_, nn_, allocs_))) return st;
-        told_n_ = nn_;
+    Rec r;
+    {
+        std::lock_guard<std::mutex> lock(mu_);
+        if (!sx_) {
+            cap_ = 2048;
+            int st;
+            if ((st = dalloc(&sx_, (size_t)cap_ * P * 16, allocs_)) ||
+                (st = dalloc(&stgt_, (size_t)cap_ * P * nn::NN_OUT, allocs_)) ||
+                (st = dalloc(&smask_, (size_t)cap_ * P, allocs_)))
+                return st;
+            WSCALL(hipStreamSynchronize(st_));   // the ring's clears are done before solvers write into it
+            filled_.assign(cap_, 0);
+        }
+        auto it = recs_.find(&s);
+        if (it == recs_.end()) return set_error(AMCFD_E_ARG, "solver is not attached to this warm start");
+        Rec &m = it->second;
+        if (m.n != nn_) {
+            // this solver's own buffers: nothing of its is queued on them now
+            WSCALL(hipStreamSynchronize(s.stream()));
+            (void)hipFree(m.told);
+            (void)hipFree(m.pold);
+            m.told = m.pold = nullptr;
+            m.n = 0;
+            WSCALL(hipMalloc(&m.told, nn_ * sizeof(double)));
+            WSCALL(hipMalloc(&m.pold, nn_ * sizeof(double)));
+            m.n = nn_;
+        }
+        m.slot = next_;   // a sample begun and never finished leaves its slot unfilled
+        next_ = (next_ + 1) % cap_;
+        if (filled_[m.slot]) {   // the ring wrapped onto an old sample
+            filled_[m.slot] = 0;
+            --nsamp_;
+        }
+        m.win = window(s);
+        r = m;
     }
     hipStream_t st = s.stream();
-    pend_win_ = window(s);
-    nn::features(st, s.dev_dims(), pend_win_, scale(s), s.dev_fields(), sx_ + (size_t)next_ * P * 16,
-                 smask_ + (size_t)next_ * P);
-    WSCALL(hipMemcpyAsync(told_, s.dev_fields().temp, nn_ * sizeof(double), hipMemcpyDeviceToDevice, st));
-    WSCALL(hipMemcpyAsync(pold_, s.dev_fields().pressure, nn_ * sizeof(double), hipMemcpyDeviceToDevice, st));
+    nn::features(st, s.dev_dims(), r.win, scale(s), s.dev_fields(), sx_ + (size_t)r.slot * P * 16,
+                 smask_ + (size_t)r.slot * P);
+    WSCALL(hipMemcpyAsync(r.told, s.dev_fields().temp, nn_ * sizeof(double), hipMemcpyDeviceToDevice, st));
+    WSCALL(hipMemcpyAsync(r.pold, s.dev_fields().pressure, nn_ * sizeof(double), hipMemcpyDeviceToDevice, st));
     WSCALL(hipGetLastError());
-    pending_ = true;
     return 0;
 }
 
 int WarmStart::capture_target(Solver &s) {
-    if (!pending_) return 0;
     const size_t P = (size_t)WD * WH * WW;
-    nn::target(s.stream(), s.dev_dims(), pend_win_, scale(s), s.dev_fields().temp, told_, s.dev_fields().pressure,
-               pold_, stgt_ + (size_t)next_ * P * nn::NN_OUT);
+    Rec r;
+    {
+        std::lock_guard<std::mutex> lock(mu_);
+        auto it = recs_.find(&s);
+        if (it == recs_.end() || it->second.slot < 0) return 0;
+        r = it->second;
+    }
+    nn::target(s.stream(), s.dev_dims(), r.win, scale(s), s.dev_fields().temp, r.told, s.dev_fields().pressure,
+               r.pold, stgt_ + (size_t)r.slot * P * nn::NN_OUT);
     WSCALL(hipGetLastError());
     WSCALL(hipStreamSynchronize(s.stream()));
-    pending_ = false;
-    next_ = (next_ + 1) % cap_;
-    nsamp_ = std::min(nsamp_ + 1, cap_);
+    std::lock_guard<std::mutex> lock(mu_);
+    auto it = recs_.find(&s);
+    if (it != recs_.end() && it->second.slot == r.slot) {
+        it->second.slot = -1;
+        filled_[r.slot] = 1;
+        nsamp_ = std::min(nsamp_ + 1, cap_);
+    }
     return 0;
 }
 
 int WarmStart::apply(Solver &s) {
+    std::lock_guard<std::mutex> lock(mu_);
     int st = ensure_batch(1, WD, WH, WW);
     if (st) return st;
     hipStream_t sst = s.stream();
@@ -548,5 +630,7 @@ namespace amcfd {
 int ws_capture_input(WarmStart *w, Solver &s) { return w->capture_input(s); }
 int ws_capture_target(WarmStart *w, Solver &s) { return w->capture_target(s); }
 int ws_apply(WarmStart *w, Solver &s) { return w->apply(s); }
+void ws_attach(WarmStart *w, Solver *s) { w->attach(s); }
+void ws_detach(WarmStart *w, Solver *s) { w->detach(s); }
 WarmStart *ws_of(amcfd_ws *h) { return h ? &h->w : nullptr; }
 }  // namespace amcfd

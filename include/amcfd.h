@@ -301,6 +301,13 @@ int  amcfd_set_mixed_schedule(amcfd_solver *s, const char *schedule, int box_pas
  * host. */
 int  amcfd_set_graph(amcfd_solver *s, int on, int block, int box_blocks);
 
+/* Copy the whole state of src into dst (same deck, any modes): every
+ * solution field, the grid, the laser and time-loop state and the scalars,
+ * so that dst continues exactly where src stands, e.g. to run the same
+ * state on in another mode.  No reference counterpart (the reference has
+ * one solver per process). */
+int  amcfd_copy_state(amcfd_solver *dst, const amcfd_solver *src);
+
 /* Test hook, no reference counterpart: raise the device error flag that
  * the Gauss-Seidel line sweeps' bounded waits watch, so that every wait of
  * the next sweep gives up at once, every kernel still ends, and the solve
@@ -319,7 +326,17 @@ int  amcfd_test_expire_waits(amcfd_solver *s);
  *   ws_create / destroy  network with seeded initial weights
  *   ws_attach            mode 0 off, 1 record one training sample per step
  *                        (the state before and the increment after the
- *                        solve), 2 apply the guess before each solve
+ *                        solve), 2 start a step from the guess when the
+ *                        step before it converged (a step that ends at
+ *                        maxit depends on its start; runs whose steps end
+ *                        there keep the plain start), 3 start every step
+ *                        from the guess; one
+ *                        network may serve many solvers, also solvers that
+ *                        step on other host threads (amcfd_run_batch): each
+ *                        keeps its own pending sample, and the network's
+ *                        forward passes run one at a time.  Destroying the
+ *                        network detaches its solvers; ws_train and ws_load
+ *                        must not overlap steps of attached solvers
  *   ws_train             Adam over random batches of the recorded samples;
  *                        loss of the first and last batch
  *   ws_set_factors       fractions of the predicted temperature and pressure
@@ -334,6 +351,9 @@ typedef struct amcfd_ws amcfd_ws;
 int  amcfd_ws_create(int seed, amcfd_ws **out);
 void amcfd_ws_destroy(amcfd_ws *ws);
 int  amcfd_ws_attach(amcfd_solver *s, amcfd_ws *ws, int mode);
+/* steps of this solver started from the guess, and steps mode 2 kept on the
+ * plain start because the step before did not converge */
+int  amcfd_ws_counts(const amcfd_solver *s, int *applied, int *guarded);
 int  amcfd_ws_train(amcfd_ws *ws, int iters, int batch, double lr, double *loss_first, double *loss_last);
 int  amcfd_ws_samples(amcfd_ws *ws);
 int  amcfd_ws_set_factors(amcfd_ws *ws, double tfactor, double pfactor);

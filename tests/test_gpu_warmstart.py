@@ -154,3 +154,40 @@ def test_warm_start_shortens_converging_steps(case_dir, amcfd):
         assert it1[on == 0].sum() < 0.9 * it0[on == 0].sum()
         assert e < 3e-3
     net.close()
+
+
+def test_guard_keeps_maxit_bound_runs_on_the_plain_start(case_dir, amcfd):
+    """Mode 2 starts a step from the guess only after a converged step: on
+    the molten single track cut to 10 iterations a step, whose steps end at
+    maxit once the pool has formed, no step starts from it and the run is
+    bit for bit the plain one; mode 3 starts every step from it.  On the
+    no-melt deck, whose steps converge, mode 2 uses the guess on every step
+    but the first."""
+    path = case_dir("tiny", maxit=10)
+    net = amcfd.WarmStartNet(seed=5)
+    plain = amcfd.Solver(path, mode=amcfd.MODE_MIXED)
+    guarded = amcfd.Solver(path, mode=amcfd.MODE_MIXED)
+    always = amcfd.Solver(path, mode=amcfd.MODE_MIXED)
+    for s in (plain, guarded, always):   # the first step converges, before the pool forms
+        s.run(max_steps=3)
+    net.attach(guarded, amcfd.WS_APPLY)
+    net.attach(always, amcfd.WS_APPLY_ALWAYS)
+    prev = [plain.scalars()["niter"]]
+    for step in range(3, 9):
+        print("guard, molten track, step %d" % (step + 1), flush=True)
+        for s in (plain, guarded, always):
+            s.step()
+        prev.append(plain.scalars()["niter"])
+    print("iterations per step", prev)
+    assert all(n == 10 for n in prev), prev   # every step from the third on ends at maxit
+    assert guarded.warm_start_counts() == (0, 6)
+    assert always.warm_start_counts() == (6, 0)
+    for f in ("enthalpy", "uvel", "vvel", "wvel", "pp"):
+        assert np.array_equal(plain.field(f), guarded.field(f)), f
+    assert not np.array_equal(plain.field("enthalpy"), always.field("enthalpy"))
+    net.set_factors(tfactor=0.0)   # untrained: apply a zero change, so that every step still converges
+    s = amcfd.Solver(no_melt_deck(case_dir, 30.0, 1.0), mode=amcfd.MODE_MIXED)
+    net.attach(s, amcfd.WS_APPLY)
+    n = s.run()
+    assert s.warm_start_counts() == (n - 1, 1)
+    net.close()
