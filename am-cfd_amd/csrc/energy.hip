@@ -104,6 +104,7 @@ struct AsmConst {
     double delt, hlatnt, urfh;
     double avolpow, apowseta, heatthick, heatrb, avolfact, beamx, beamy;
     int pi, scheme;
+    int vol;   // mixed mode: the deck has a volumetric source (else the source field stays zero)
 };
 
 // Tiles of 64 (i) x 4 (j) columns; each thread marches KA planes up k and
@@ -134,18 +135,15 @@ struct AsmOut {
     double *blkpart;
 };
 
-template <bool MX>
+template <int SCH>
 __device__ __forceinline__ void energy_assemble_body(
     Dims d, AsmConst q, DevGeom g, const double *__restrict__ uvel, const double *__restrict__ vvel,
     const double *__restrict__ wvel, const double *__restrict__ rho, const double *__restrict__ diff,
     const double *__restrict__ hnot, const double *__restrict__ fracl, const double *__restrict__ fraclnot,
-    const double *__restrict__ h, AsmOut o, const int *__restrict__ skip) {
+    const double *__restrict__ h, AsmOut o) {
     __shared__ double scratch[2 * 4];
-    __shared__ double s_blk[MX ? 4 * TY * TX : 1];
-    if (MX && skip && *skip) return;
     double *an_o = o.an, *as_o = o.as, *ae_o = o.ae, *aw_o = o.aw, *at_o = o.at, *ab_o = o.ab, *ap_o = o.ap;
     double *su_o = o.su, *src_o = o.src, *part_res = o.part_res, *term_bl = o.term_bl, *term_blc = o.term_blc;
-    double cbl = 0.0, cblp = 0.0, cblm = 0.0, cblc = 0.0;   // mixed mode: this column's partial sums
     const int tx = threadIdx.x, ty = threadIdx.y;
     const int gx = gridDim.x, gy = gridDim.y;
     const int nb = gx * gy * gridDim.z;
@@ -156,7 +154,9 @@ __device__ __forceinline__ void energy_assemble_body(
     const int k0 = 1 + bz * KA;
     const bool col_ok = (i <= d.nim2) && (j <= d.njm2) && (k0 <= d.nkm2);
     double sumd = 0.0, sumh = 0.0;
-    const int sch = q.scheme;
+    // the scheme is a template constant: the other schemes' weight functions
+    // (the exponential's exp) cost no registers
+    constexpr int sch = SCH;
     const double ztop = g.z[d.nkm1];
     if (col_ok) {
         const double fxi = g.fracx[i], fxw = g.fracx[i - 1], fyj = g.fracy[j], fys = g.fracy[j - 1];
@@ -275,53 +275,21 @@ __device__ __forceinline__ void energy_assemble_body(
             double ap = an + as + ae + aw + at + ab + apnot - sp;
             ap /= q.urfh;
             su += (1.0 - q.urfh) * ap * hP;
-            if constexpr (!MX) {
-                an_o[c] = an; as_o[c] = as; ae_o[c] = ae; aw_o[c] = aw; at_o[c] = at; ab_o[c] = ab;
-                ap_o[c] = ap; su_o[c] = su; src_o[c] = src;
-                // residual (calculateResiduals(5))
-                double r = (an * hN + as * hS + ae * hE + aw * hW + at * hT + ab * hB + su) / ap - hP;
-                sumd += fabs(r);
-                sumh += fabs(hP);
-                // per-cell terms of the block-correction column sums (summed per
-                // x-column in the reference's (k, j) order by k_column_sums)
-                term_bl[c] = ap - an - as - at - ab;
-                term_blc[c] = ae * hE + aw * hW + an * hN + as * hS + at * hT + ab * hB + su - ap * hP;
-            } else {
-                const float fan = (float)an, fas = (float)as, fae = (float)ae, faw = (float)aw;
-                const float fat = (float)at, fab = (float)ab, fap = (float)ap;
-                o.fan[c] = fan; o.fas[c] = fas; o.fae[c] = fae; o.faw[c] = faw; o.fat[c] = fat; o.fab[c] = fab;
-                o.fap[c] = fap;
-                su_o[c] = su;
-                src_o[c] = src;
-                const double ran = fan, ras = fas, rae = fae, raw = faw, rat = fat, rab = fab, rap = fap;
-                const double num = rae * hE + raw * hW + ran * hN + ras * hS + rat * hT + rab * hB + su;
-                sumd += fabs(num / rap - hP);
-                sumh += fabs(hP);
-                cbl += rap - ran - ras - rat - rab;
-                cblp += rae;
-                cblm += raw;
-                cblc += num - rap * hP;
-            }
+            an_o[c] = an; as_o[c] = as; ae_o[c] = ae; aw_o[c] = aw; at_o[c] = at; ab_o[c] = ab;
+            ap_o[c] = ap; su_o[c] = su; src_o[c] = src;
+            // residual (calculateResiduals(5))
+            double r = (an * hN + as * hS + ae * hE + aw * hW + at * hT + ab * hB + su) / ap - hP;
+            sumd += fabs(r);
+            sumh += fabs(hP);
+            // per-cell terms of the block-correction column sums (summed per
+            // x-column in the reference's (k, j) order by k_column_sums)
+            term_bl[c] = ap - an - as - at - ab;
+            term_blc[c] = ae * hE + aw * hW + an * hN + as * hS + at * hT + ab * hB + su - ap * hP;
             // the next plane's bottom and centre values
             dB = dP; dP = dT;
             rP = rT; wb = wt;
             flB = flP; flP = flT;
             hB = hP; hP = hT;
-        }
-    }
-    if constexpr (MX) {
-        // column partials of this tile: the four j-rows of each i, in order
-        s_blk[(0 * TY + ty) * TX + tx] = cbl;
-        s_blk[(1 * TY + ty) * TX + tx] = cblp;
-        s_blk[(2 * TY + ty) * TX + tx] = cblm;
-        s_blk[(3 * TY + ty) * TX + tx] = cblc;
-        __syncthreads();
-        if (i <= d.nim2) {
-            const int ntile = gy * gridDim.z, tile = by + gy * bz;
-            const int qq = ty;   // one quantity per j-row of threads
-            double a = 0.0;
-            for (int r = 0; r < TY; ++r) a += s_blk[(qq * TY + r) * TX + tx];
-            o.blkpart[((size_t)qq * ntile + tile) * d.ni + i] = a;
         }
     }
     double v[2] = {sumd, sumh};
@@ -332,29 +300,200 @@ __device__ __forceinline__ void energy_assemble_body(
     }
 }
 
+template <int SCH>
 __global__ __launch_bounds__(TX *TY) void k_energy_assemble(
     Dims d, AsmConst q, DevGeom g, const double *__restrict__ uvel, const double *__restrict__ vvel,
     const double *__restrict__ wvel, const double *__restrict__ rho, const double *__restrict__ diff,
     const double *__restrict__ hnot, const double *__restrict__ fracl, const double *__restrict__ fraclnot,
     const double *__restrict__ h, AsmOut o) {
-    energy_assemble_body<false>(d, q, g, uvel, vvel, wvel, rho, diff, hnot, fracl, fraclnot, h, o, nullptr);
+    energy_assemble_body<SCH>(d, q, g, uvel, vvel, wvel, rho, diff, hnot, fracl, fraclnot, h, o);
 }
 
-// the mixed-mode form is memory-bound with little else to hide latency, so
-// it trades registers for more waves per SIMD
-__global__ __launch_bounds__(TX *TY) __attribute__((amdgpu_waves_per_eu(MX_ASM_WAVES, 8))) void
+// Mixed mode: the same discretisation with the face coefficients formed in
+// FP32 (the harmonic-mean diffusivities, the face fluxes, the weighting
+// A(|P|), the neighbour coefficients, ap with its relaxation) and stored in
+// FP32; the source su (transient, latent heat, boundary folds, relaxation
+// term), the residual and the block-correction partials stay FP64 and use
+// the FP32 coefficients as stored, so that they describe the very system the
+// line passes solve.  FP32 halves the registers of the coefficient work,
+// which raises the waves per SIMD that hide the loads' latency; its divides
+// take the hardware reciprocal (1 ulp), as the coefficients are FP32 values
+// anyway.  Tiles of 32 (i) x 8 (j) columns waste fewer lanes than 64-wide
+// ones on x-lines that are not multiples of 64.
+constexpr int MTX = 32, MTY = 8;
+__device__ __forceinline__ float fmax_first(float a, float b) { return (a < b) ? b : a; }
+
+template <int SCH>
+__device__ __forceinline__ float weight_f32(float pe) {
+    if constexpr (SCH == AMCFD_DISC_POWERLAW) {
+        const float b = 1.0f - 0.1f * pe;
+        const float b2 = b * b;
+        return fmax_first(0.0f, b2 * b2 * b);
+    } else if constexpr (SCH == AMCFD_DISC_EXPONENTIAL) {
+        return pe / (expf(pe) - 1.0f);
+    } else {
+        return 1.0f;
+    }
+}
+template <int SCH>
+__device__ __forceinline__ float coef_plus_f32(float dif, float f) {
+    return dif * weight_f32<SCH>(fabsf(f) * __builtin_amdgcn_rcpf(dif)) + fmax_first(0.0f, f);
+}
+template <int SCH>
+__device__ __forceinline__ float coef_minus_f32(float dif, float f) {
+    return dif * weight_f32<SCH>(fabsf(f) * __builtin_amdgcn_rcpf(dif)) + fmax_first(0.0f, -f);
+}
+__device__ __forceinline__ float hmean_f32(float p, float q, float wq, float wp) {
+    return p * q * __builtin_amdgcn_rcpf(wq * q + wp * p);
+}
+
+template <int SCH>
+__global__ __launch_bounds__(MTX *MTY) __attribute__((amdgpu_waves_per_eu(MX_ASM_WAVES, 8))) void
 k_energy_assemble_mx(Dims d, AsmConst q, DevGeom g, const double *__restrict__ uvel, const double *__restrict__ vvel,
                      const double *__restrict__ wvel, const double *__restrict__ rho,
                      const double *__restrict__ diff, const double *__restrict__ hnot,
                      const double *__restrict__ fracl, const double *__restrict__ fraclnot,
                      const double *__restrict__ h, AsmOut o, const int *__restrict__ skip,
                      const StepDev *__restrict__ step) {
+    if (skip && *skip) return;
     if (step) {   // a captured loop: the step's source parameters from device memory
         const SourceParams sp = step->sp;
         q.avolpow = sp.avolpow; q.apowseta = sp.apowseta; q.heatthick = sp.heatthick; q.heatrb = sp.heatrb;
         q.avolfact = sp.avolfact; q.beamx = sp.beamposx; q.beamy = sp.beamposy; q.pi = sp.powerindicator;
     }
-    energy_assemble_body<true>(d, q, g, uvel, vvel, wvel, rho, diff, hnot, fracl, fraclnot, h, o, skip);
+    __shared__ double scratch[2 * 4];
+    __shared__ double s_blk[4 * MTY * MTX];
+    double cbl = 0.0, cblp = 0.0, cblm = 0.0, cblc = 0.0;   // this column's block-correction partials
+    const int tx = threadIdx.x, ty = threadIdx.y;
+    const int gx = gridDim.x, gy = gridDim.y;
+    const int nb = gx * gy * gridDim.z;
+    const int t = xcd_tile(blockIdx.x + gx * (blockIdx.y + gy * blockIdx.z), nb);
+    const int bx = t % gx, by = (t / gx) % gy, bz = t / (gx * gy);
+    const int i = 1 + bx * MTX + tx;
+    const int j = 1 + by * MTY + ty;
+    const int k0 = 1 + bz * KA;
+    const bool col_ok = (i <= d.nim2) && (j <= d.njm2) && (k0 <= d.nkm2);
+    double sumd = 0.0, sumh = 0.0;
+    if (col_ok) {
+        const double ztop = g.z[d.nkm1];
+        const float fxi = (float)g.fracx[i], fxw = (float)g.fracx[i - 1];
+        const float fyj = (float)g.fracy[j], fys = (float)g.fracy[j - 1];
+        const float ryn = (float)g.dypsinv[j < d.njm2 ? j + 1 : d.njm1], rys = (float)g.dypsinv[j > 1 ? j : 1];
+        const float rxe = (float)g.dxpwinv[i < d.nim2 ? i + 1 : d.nim1], rxw = (float)g.dxpwinv[i > 1 ? i : 1];
+        const double dxy = (g.xu[i + 1] - g.xu[i]) * (g.yv[j + 1] - g.yv[j]);
+        const double aij = g.areaij[j * d.ni + i];
+        const float faij = (float)aij;
+        // volumetric source: its horizontal Gaussian is the column's own
+        double src_col = 0.0;
+        if (q.heatthick > 0.0) {
+            const double xb = q.beamx - g.x[i], yb = q.beamy - g.y[j];
+            src_col = q.avolpow * q.avolfact / M_PI / (q.heatrb * q.heatrb) / q.heatthick * q.apowseta *
+                      libm_exp(-q.avolfact / (q.heatrb * q.heatrb) * ((xb * xb) + (yb * yb))) * (double)q.pi;
+        }
+        const double rdelt = 1.0 / q.delt;
+        const float rurfh = 1.0f / (float)q.urfh;
+        const int kend = min(k0 + KA - 1, d.nkm2);
+        size_t c = cix(d, i, j, k0);
+        float dB = (float)diff[c - d.sk], dP = (float)diff[c];
+        double rP = rho[c];
+        float wb = (float)wvel[c];
+        float flB = (float)fracl[c - d.sk];
+        double flP = fracl[c];
+        double hB = h[c - d.sk], hP = h[c];
+        for (int k = k0; k <= kend; ++k, c += d.sk) {
+            const float dT = (float)diff[c + d.sk], wt = (float)wvel[c + d.sk];
+            const double rT = rho[c + d.sk], flT = fracl[c + d.sk], hT = h[c + d.sk];
+            const float dN = (float)diff[c + d.sj], dS = (float)diff[c - d.sj];
+            const float dE = (float)diff[c + 1], dW = (float)diff[c - 1];
+            const float rN = (float)rho[c + d.sj], rE = (float)rho[c + 1], rPf = (float)rP, rTf = (float)rT;
+            const float vs = (float)vvel[c], vn = (float)vvel[c + d.sj], uw = (float)uvel[c], ue = (float)uvel[c + 1];
+            const float flE = (float)fracl[c + 1], flW = (float)fracl[c - 1];
+            const float flN = (float)fracl[c + d.sj], flS = (float)fracl[c - d.sj];
+            const double hN = h[c + d.sj], hS = h[c - d.sj], hE = h[c + 1], hW = h[c - 1];
+            const double hnP = hnot[c], flnP = fraclnot[c];
+            const float aik = (float)g.areaik[k * d.ni + i], ajk = (float)g.areajk[k * d.nj + j];
+            const double vol = dxy * (g.zw[k + 1] - g.zw[k]);
+            float an, as, ae, aw, at, ab;
+            an = coef_minus_f32<SCH>((j < d.njm2 ? hmean_f32(dP, dN, 1.0f - fyj, fyj) : dP) * aik * ryn, rN * vn * aik);
+            as = coef_plus_f32<SCH>((j > 1 ? hmean_f32(dS, dP, 1.0f - fys, fys) : dP) * aik * rys, rPf * vs * aik);
+            ae = coef_minus_f32<SCH>((i < d.nim2 ? hmean_f32(dP, dE, 1.0f - fxi, fxi) : dP) * ajk * rxe, rE * ue * ajk);
+            aw = coef_plus_f32<SCH>((i > 1 ? hmean_f32(dW, dP, 1.0f - fxw, fxw) : dP) * ajk * rxw, rPf * uw * ajk);
+            {
+                const float fzt = (float)g.fracz[k], rzt = (float)g.dzpbinv[k < d.nkm2 ? k + 1 : d.nkm1];
+                at = coef_minus_f32<SCH>((k < d.nkm2 ? hmean_f32(dP, dT, 1.0f - fzt, fzt) : dP) * faij * rzt,
+                                         rTf * wt * faij);
+                const float fzb = (float)g.fracz[k - 1], rzb = (float)g.dzpbinv[k > 1 ? k : 1];
+                ab = coef_plus_f32<SCH>((k > 1 ? hmean_f32(dB, dP, 1.0f - fzb, fzb) : dP) * faij * rzb,
+                                        rPf * wb * faij);
+            }
+            const double apnot = rP * rdelt * vol;
+            float sp = 0.0f;
+            double su = apnot * hnP;
+            double src = 0.0;
+            if (ztop - g.z[k] <= q.heatthick) {
+                src = src_col;
+                su += vol * src;
+            }
+            // latent heat: transient (FP64) and advection (FP32) terms
+            su -= vol * (q.hlatnt * rP * rdelt) * (flP - flnP);
+            {
+                const float flPf = (float)flP;
+                const float flew = ajk * (fmax_first(uw, 0.0f) * flW - fmax_first(-uw, 0.0f) * flPf +
+                                          fmax_first(-ue, 0.0f) * flE - fmax_first(ue, 0.0f) * flPf);
+                const float flns = aik * (fmax_first(vs, 0.0f) * flS - fmax_first(-vs, 0.0f) * flPf +
+                                          fmax_first(-vn, 0.0f) * flN - fmax_first(vn, 0.0f) * flPf);
+                const float fltb = faij * (fmax_first(wb, 0.0f) * flB - fmax_first(-wb, 0.0f) * flPf +
+                                           fmax_first(-wt, 0.0f) * (float)flT - fmax_first(wt, 0.0f) * flPf);
+                su += rP * q.hlatnt * (double)(flew + flns + fltb);
+            }
+            // fixed boundary values folded into the source
+            if (k == 1) { su += (double)ab * hB; sp -= ab; ab = 0.0f; }
+            if (k == d.nkm2) { su += (double)at * hT; sp -= at; at = 0.0f; }
+            if (j == 1) { su += (double)as * hS; sp -= as; as = 0.0f; }
+            if (j == d.njm2) { su += (double)an * hN; sp -= an; an = 0.0f; }
+            if (i == 1) { su += (double)aw * hW; sp -= aw; aw = 0.0f; }
+            if (i == d.nim2) { su += (double)ae * hE; sp -= ae; ae = 0.0f; }
+            const float ap = (an + as + ae + aw + at + ab + (float)apnot - sp) * rurfh;
+            su += (1.0 - q.urfh) * (double)ap * hP;
+            o.fan[c] = an; o.fas[c] = as; o.fae[c] = ae; o.faw[c] = aw; o.fat[c] = at; o.fab[c] = ab;
+            o.fap[c] = ap;
+            o.su[c] = su;
+            if (q.vol) o.src[c] = src;
+            const double rap = ap;
+            const double num = (double)ae * hE + (double)aw * hW + (double)an * hN + (double)as * hS +
+                               (double)at * hT + (double)ab * hB + su;
+            sumd += fabs(num / rap - hP);
+            sumh += fabs(hP);
+            cbl += rap - (double)an - (double)as - (double)at - (double)ab;
+            cblp += (double)ae;
+            cblm += (double)aw;
+            cblc += num - rap * hP;
+            // the next plane's bottom and centre values
+            dB = dP; dP = dT;
+            rP = rT; wb = wt;
+            flB = (float)flP; flP = flT;
+            hB = hP; hP = hT;
+        }
+    }
+    // column partials of this tile: the j-rows of each i, in order
+    s_blk[(0 * MTY + ty) * MTX + tx] = cbl;
+    s_blk[(1 * MTY + ty) * MTX + tx] = cblp;
+    s_blk[(2 * MTY + ty) * MTX + tx] = cblm;
+    s_blk[(3 * MTY + ty) * MTX + tx] = cblc;
+    __syncthreads();
+    if (ty < 4 && i <= d.nim2) {
+        const int ntile = gy * gridDim.z, tile = by + gy * bz;
+        const int qq = ty;   // one quantity per j-row of the first four
+        double a = 0.0;
+        for (int r = 0; r < MTY; ++r) a += s_blk[(qq * MTY + r) * MTX + tx];
+        o.blkpart[((size_t)qq * ntile + tile) * d.ni + i] = a;
+    }
+    double v[2] = {sumd, sumh};
+    block_sum<2>(v, scratch);   // contains __syncthreads
+    if (tx == 0 && ty == 0) {
+        o.part_res[2 * t] = v[0];
+        o.part_res[2 * t + 1] = v[1];
+    }
 }
 
 // ------------------------------------------------------------------------
@@ -576,7 +715,7 @@ __global__ __launch_bounds__(TX *TY) void k_h2t(Dims d, H2tConst q, DevGeom g, c
                 double vol = (g.xu[i + 1] - g.xu[i]) * (g.yv[j + 1] - g.yv[j]) * (g.zw[k + 1] - g.zw[k]);
                 double dh = hv - hnot[c] + (fl - fraclnot[c]) * q.hlatnt;
                 acc[0] += vol * rho[c] * dh / q.delt;
-                acc[1] += src[c] * vol;
+                if (src) acc[1] += src[c] * vol;   // null: no volumetric source, the field is zero
             }
             if (i == 1) acc[2] += g.areajk[k * d.nj + j] * (diff[c] * (h[c - 1] - hv) * g.dxpwinv[1]);
             if (i == d.nim2) acc[3] += g.areajk[k * d.nj + j] * (diff[c] * (h[c + 1] - hv) * g.dxpwinv[d.nim1]);
@@ -971,29 +1110,41 @@ static dim3 assemble_grid(const Dims &d) {
     return dim3((d.nim2 + TX - 1) / TX, (d.njm2 + TY - 1) / TY, (d.nkm2 + KA - 1) / KA);
 }
 
+static dim3 assemble_grid_mx(const Dims &d) {
+    return dim3((d.nim2 + MTX - 1) / MTX, (d.njm2 + MTY - 1) / MTY, (d.nkm2 + KA - 1) / KA);
+}
+
 void launch_energy_assemble(hipStream_t st, const Dims &d, const DevGeom &g, DevFields f, Partials pr,
                             const amcfd_params &p, const SourceParams &s) {
     AsmConst q{p.delt, p.hlatnt, p.urfh, s.avolpow, s.apowseta, s.heatthick, s.heatrb, s.avolfact,
                s.beamposx, s.beamposy, s.powerindicator, p.discretization};
     AsmOut o{f.an, f.as, f.ae, f.aw, f.at, f.ab, f.ap, f.su, f.sourceinput, pr.energy, f.sp, f.apnot,
              nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
-    hipLaunchKernelGGL(k_energy_assemble, assemble_grid(d), dim3(TX, TY), 0, st, d, q, g, f.uvel, f.vvel, f.wvel,
-                       f.rho, f.diff, f.hnot, f.fracl, f.fraclnot, f.enthalpy, o);
+    auto kern = p.discretization == AMCFD_DISC_POWERLAW     ? k_energy_assemble<AMCFD_DISC_POWERLAW>
+                : p.discretization == AMCFD_DISC_EXPONENTIAL ? k_energy_assemble<AMCFD_DISC_EXPONENTIAL>
+                                                             : k_energy_assemble<AMCFD_DISC_UPWIND>;
+    hipLaunchKernelGGL(kern, assemble_grid(d), dim3(TX, TY), 0, st, d, q, g, f.uvel, f.vvel, f.wvel, f.rho, f.diff,
+                       f.hnot, f.fracl, f.fraclnot, f.enthalpy, o);
 }
 
 void launch_energy_assemble_mixed(hipStream_t st, const Dims &d, const DevGeom &g, DevFields f, Partials pr,
                                   const MxFields &m, const amcfd_params &p, const SourceParams &s, const int *skip,
                                   const StepDev *step) {
     AsmConst q{p.delt, p.hlatnt, p.urfh, s.avolpow, s.apowseta, s.heatthick, s.heatrb, s.avolfact,
-               s.beamposx, s.beamposy, s.powerindicator, p.discretization};
+               s.beamposx, s.beamposy, s.powerindicator, p.discretization, mixed_has_volumetric_source(p)};
     AsmOut o{nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, m.su, f.sourceinput, pr.energy,
              nullptr, nullptr, m.an, m.as, m.ae, m.aw, m.at, m.ab, m.ap, m.blkpart};
-    hipLaunchKernelGGL(k_energy_assemble_mx, assemble_grid(d), dim3(TX, TY), 0, st, d, q, g, f.uvel, f.vvel,
-                       f.wvel, f.rho, f.diff, f.hnot, f.fracl, f.fraclnot, f.enthalpy, o, skip, step);
+    auto kern = p.discretization == AMCFD_DISC_POWERLAW     ? k_energy_assemble_mx<AMCFD_DISC_POWERLAW>
+                : p.discretization == AMCFD_DISC_EXPONENTIAL ? k_energy_assemble_mx<AMCFD_DISC_EXPONENTIAL>
+                                                             : k_energy_assemble_mx<AMCFD_DISC_UPWIND>;
+    hipLaunchKernelGGL(kern, assemble_grid_mx(d), dim3(MTX, MTY), 0, st, d, q, g, f.uvel, f.vvel, f.wvel, f.rho,
+                       f.diff, f.hnot, f.fracl, f.fraclnot, f.enthalpy, o, skip, step);
 }
 
+int mixed_has_volumetric_source(const amcfd_params &p) { return p.volheatsource || p.n_heat_table > 0; }
+
 int mixed_tile_count(const Dims &d) {
-    const dim3 a = assemble_grid(d);
+    const dim3 a = assemble_grid_mx(d);
     return (int)(a.y * a.z);
 }
 
@@ -1021,10 +1172,10 @@ static dim3 node_grid(const Dims &d) {
 }
 
 void launch_h2t(hipStream_t st, const Dims &d, const DevGeom &g, DevFields f, Partials pr, const amcfd_params &p,
-                const int *skip) {
+                const int *skip, bool src_zero) {
     H2tConst q{p.hlcal, p.hsmelt, p.acpa, p.acpb, p.acpl, p.tliquid, p.tsolid, p.hlatnt, p.delt, p.steady};
     hipLaunchKernelGGL(k_h2t, node_grid(d), dim3(TX, TY), 0, st, d, q, g, f.enthalpy, f.hnot, f.fraclnot, f.rho,
-                       f.diff, f.sourceinput, f.temp, f.fracl, pr.h2t, skip);
+                       f.diff, src_zero ? nullptr : f.sourceinput, f.temp, f.fracl, pr.h2t, skip);
 }
 
 void launch_melt_pool(hipStream_t st, const Dims &d, const DevGeom &g, DevFields f, Partials pr,

@@ -153,8 +153,10 @@ void launch_props(hipStream_t st, const Dims &d, DevFields f, const amcfd_params
 // x-direction block correction of phi from the column-sum terms the assemble
 // kernel left in f.sp / f.apnot (enthalpy or concentration)
 void launch_block_correction(hipStream_t st, const Dims &d, DevFields f, Partials pr, double *phi, int shared);
+// src_zero: the deck has no volumetric source, so the source field is zero
+// and is not read (mixed mode)
 void launch_h2t(hipStream_t st, const Dims &d, const DevGeom &g, DevFields f, Partials pr,
-                const amcfd_params &p, const int *skip = nullptr);
+                const amcfd_params &p, const int *skip = nullptr, bool src_zero = false);
 // with device control (ctl), also sets sc->noflow for the flow kernels
 void launch_melt_pool(hipStream_t st, const Dims &d, const DevGeom &g, DevFields f, Partials pr,
                       const amcfd_params &p, int istart, int jstart, DevScalars *sc, bool ctl = false,
@@ -203,6 +205,8 @@ void launch_energy_assemble_mixed(hipStream_t st, const Dims &d, const DevGeom &
                                   const MxFields &m, const amcfd_params &p, const SourceParams &sp, const int *skip,
                                   const StepDev *step = nullptr);
 int mixed_tile_count(const Dims &d);
+// the mixed assembly writes the volumetric-source field only for decks that have one
+int mixed_has_volumetric_source(const amcfd_params &p);
 int mixed_max_line();
 int mixed_max_yz();
 size_t mixed_yz_lds(int len);   // LDS bytes of one y/z line workgroup

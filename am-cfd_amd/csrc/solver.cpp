@@ -835,7 +835,7 @@ int Solver::enqueue_mixed_iteration(bool ctl) {
     toc("sweep_h", slot);
     if (st) return st;
     tic("h2t", &slot);
-    launch_h2t(st_, d_, g_, f_, pr_, p_, done);
+    launch_h2t(st_, d_, g_, f_, pr_, p_, done, !mixed_has_volumetric_source(p_));
     toc("h2t", slot);
     launch_melt_pool(st_, d_, g_, f_, pr_, p_, istart_, jstart_, sc_, ctl, stepd_);
     HIPCALL(hipGetLastError());

@@ -316,6 +316,39 @@ def ablation_leg(amcfd, decks, tmp):
     return out
 
 
+def profile_leg(args):
+    """Steps of the workload for a profiler: a calibration copy, then the
+    solver's own kernels (host-driven loop, so each launch is a dispatch)."""
+    import torch
+    import decks
+    amcfd = importlib.import_module("am-cfd_amd")
+    tmp = tempfile.mkdtemp(prefix="amcfd_prof_")
+    deck = amcfd.Deck(decks.write_case(args.case, tmp))
+    mode = amcfd.MODE_MIXED if args.profile_leg == "mixed" else amcfd.MODE_FP64
+    s = amcfd.Solver(deck, mode=mode)
+    if mode == amcfd.MODE_MIXED:
+        s.set_graph(False)
+    for w in range(args.warmup):
+        s.step()
+        log("profile leg: warm-up step %d" % (w + 1))
+    src = torch.ones(1 << 28, dtype=torch.float32, device="cuda:0")   # 1 GiB
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    del src, dst
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        s.step()
+        log("profile leg: step %d" % (k + 1))
+    el = time.perf_counter() - t0
+    sc = s.scalars()
+    print(json.dumps({"profile_leg": args.profile_leg, "case": args.case, "steps": args.steps,
+                      "ms_per_step": 1e3 * el / args.steps, "itertot": sc["itertot"],
+                      "calibration": "3 x torch copy_ of 2^28 float32 (1 GiB read + 1 GiB write each)"}))
+    s.close()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -335,6 +368,10 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1),
                     help="OpenMP threads of the reference CPU baseline (the host cores used)")
     ap.add_argument("--cpu-steps", type=int, default=6, help="time steps in the CPU baseline sample")
+    ap.add_argument("--profile-leg", choices=("mixed", "fp64"), default=None,
+                    help="only run --warmup and --steps steps of the workload in this mode with the host-driven "
+                         "loop (every kernel its own dispatch), after a device copy of 1 GiB (read 1 GiB, write "
+                         "1 GiB) that calibrates memory counters; for rocprofv3")
     ap.add_argument("--batch-child", action="store_true",
                     help="internal: the batch leg's side-by-side runs and the warm start's training, as JSON")
     ap.add_argument("--ws-out", default=None, help="internal: where the batch child saves the trained network")
@@ -345,6 +382,9 @@ def main():
         print(json.dumps(batch_side_by_side(amcfd, decks, tempfile.mkdtemp(prefix="amcfd_side_"), args.batch,
                                             args.batch_warmup, args.batch_steps, args.batch_workers,
                                             args.record_steps, args.train_iters, args.ws_out)))
+        return
+    if args.profile_leg:
+        profile_leg(args)
         return
     if args.gpus != 1:
         print("bench.py measures one GPU: every GPU box of this build has one, and the solve's line sweeps "

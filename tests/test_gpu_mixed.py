@@ -41,13 +41,55 @@ def test_mixed_mode_is_reproducible(case_dir, amcfd):
         assert np.array_equal(out[0][f], out[1][f]), f
 
 
+def test_mixed_mode_converging_steps_match_fp64(case_dir, amcfd):
+    """Steps that converge: tiny_cond at powers that melt nothing, to 150
+    iterations, so that every step of both modes ends on the reference's own
+    tests.  The temperature stays within 1e-4 (relative L2) of the FP64
+    mode's over the whole run (measured: 3.4e-5 and 4.8e-5), and the
+    iteration counts agree on nearly every step."""
+    for power in (30.0, 45.0):
+        path = case_dir("tiny_cond", power=power, speed=1.0, maxit=150, finaltime=0.2e-3 - 1e-9,
+                        lasertime=0.08e-3)
+        ref = amcfd.Solver(path, mode=amcfd.MODE_FP64)
+        mx = amcfd.Solver(path, mode=amcfd.MODE_MIXED)
+        worst, same, n = 0.0, 0, 0
+        fin = False
+        while not fin:
+            fin = ref.step()
+            mx.step()
+            worst = max(worst, rel_l2(mx.field("temp"), ref.field("temp")))
+            n += 1
+            same += ref.scalars()["niter"] == mx.scalars()["niter"]
+            assert ref.scalars()["niter"] < 150 and mx.scalars()["niter"] < 150
+        print("P=%.0f W: %d steps, max T rel-L2 %.2e, equal iteration counts on %d" % (power, n, worst, same))
+        assert worst <= 1e-4
+        assert same >= 0.9 * n
+
+
 def test_mixed_mode_conduction_tracks_fp64(case_dir, amcfd):
-    """Conduction only (no flow): the mixed mode's temperature stays close to
-    the FP64 mode's over the whole run."""
-    ref, mx, rows = run_pair(amcfd, case_dir("tiny_cond"), 20)
-    for step, n64, nmx, eT, sr, sm in rows:
-        print("step %d niter fp64 %d mixed %d  T rel-L2 %.2e" % (step, n64, nmx, eT))
-    assert max(r[3] for r in rows) < 1e-2
+    """Conduction only, maxit 40: from step 6 on most steps end at maxit
+    without converging, and there the answer depends on the iterative
+    method itself.  The yardstick is the FP64 mode's own iteration error, the
+    distance between its runs at maxit 40 and 80 (measured 1.9e-2 by step 20);
+    the mixed mode's distance from the FP64 run stays within 1.5 times it
+    (measured 1.2e-2), and within 3e-4 on the first five steps, which
+    converge (measured 2.1e-4: this deck melts, without flow)."""
+    p40 = amcfd.Deck(case_dir("tiny_cond"))   # parsed before the next deck overwrites the file
+    p80 = amcfd.Deck(case_dir("tiny_cond", maxit=80))
+    ref = amcfd.Solver(p40, mode=amcfd.MODE_FP64)
+    mx = amcfd.Solver(p40, mode=amcfd.MODE_MIXED)
+    longer = amcfd.Solver(p80, mode=amcfd.MODE_FP64)
+    e_mx, e_it = [], []
+    for step in range(1, 21):
+        print("conduction step %d" % step, flush=True)
+        for s in (ref, mx, longer):
+            s.step()
+        e_mx.append(rel_l2(mx.field("temp"), ref.field("temp")))
+        e_it.append(rel_l2(longer.field("temp"), ref.field("temp")))
+        print("step %d niter %d/%d/%d  mixed %.2e  FP64 maxit 80 vs 40 %.2e" % (
+            step, ref.scalars()["niter"], mx.scalars()["niter"], longer.scalars()["niter"], e_mx[-1], e_it[-1]))
+    assert max(e_mx[:5]) < 3e-4
+    assert max(e_mx) <= 1.5 * max(e_it)
     assert np.isfinite(mx.field("enthalpy")).all()
 
 
