@@ -1,6 +1,6 @@
 """bench.py's command-line contract on the host (no GPU needed): more than one
-GPU is refused with a message, and the child-process mode used for the
-one-at-a-time comparison is recognised before anything starts HIP."""
+GPU is refused with a message, and the child-process mode that runs the
+side-by-side leg is recognised before anything starts HIP."""
 import os
 import subprocess
 import sys
@@ -16,14 +16,15 @@ def test_more_than_one_gpu_is_refused():
     assert r.stdout == ""
 
 
-def test_queue_count_left_alone_for_the_solo_child():
-    """The bench raises GPU_MAX_HW_QUEUES to 32 for its side-by-side leg, but
-    not in the child that runs the instances one at a time."""
+def test_queue_count_raised_only_in_the_side_by_side_child():
+    """The side-by-side leg runs in a child process with GPU_MAX_HW_QUEUES
+    raised to 32; the parent, which runs the single-stream legs, keeps the
+    default 4."""
     code = ("import os, sys; sys.argv = ['bench.py'] + sys.argv[1:]; "
             "os.environ['GPU_MAX_HW_QUEUES'] = '4'; "
             "sys.path.insert(0, %r); import bench; print(os.environ['GPU_MAX_HW_QUEUES'])" % ROOT)
     parent = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=120, cwd=ROOT)
-    child = subprocess.run([sys.executable, "-c", code, "--batch-solo-child"], capture_output=True, text=True,
+    child = subprocess.run([sys.executable, "-c", code, "--batch-child"], capture_output=True, text=True,
                            timeout=120, cwd=ROOT)
-    assert parent.stdout.strip() == "32", parent.stderr
-    assert child.stdout.strip() == "4", child.stderr
+    assert parent.stdout.strip() == "4", parent.stderr
+    assert child.stdout.strip() == "32", child.stderr
