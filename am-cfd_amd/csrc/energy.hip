@@ -318,9 +318,11 @@ __global__ __launch_bounds__(TX *TY) void k_energy_assemble(
 // line passes solve.  FP32 halves the registers of the coefficient work,
 // which raises the waves per SIMD that hide the loads' latency; its divides
 // take the hardware reciprocal (1 ulp), as the coefficients are FP32 values
-// anyway.  Tiles of 32 (i) x 8 (j) columns waste fewer lanes than 64-wide
-// ones on x-lines that are not multiples of 64.
-constexpr int MTX = 32, MTY = 8;
+// anyway.  Tiles of 64 (i) x 4 (j) columns, as the FP64 form's: narrower
+// tiles idle fewer lanes on x-lines that are not multiples of 64 but write
+// more partial cache lines, and measure slower.
+constexpr int MTX = 64, MTY = 4;
+static_assert(MTY >= 4, "the block-correction fold takes one quantity per j-row of the first four");
 __device__ __forceinline__ float fmax_first(float a, float b) { return (a < b) ? b : a; }
 
 template <int SCH>

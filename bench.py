@@ -68,10 +68,10 @@ def mixed_half_pass_bytes(dom):
 
 def mixed_assembly_bytes(dom):
     """Algorithmic HBM bytes of the mixed mode's enthalpy assembly
-    (k_energy_assemble_mx): reads u, v, w, rho, diff, hnot, fracl, fraclnot, h
-    (9 FP64 fields), writes 7 FP32 coefficients, FP64 su and source:
-    9 x 8 + 7 x 4 + 2 x 8 = 116 B per cell."""
-    return dom * (9 * 8 + 7 * 4 + 2 * 8)
+    (k_energy_assemble_mx) on a deck without a volumetric source: reads u, v,
+    w, rho, diff, hnot, fracl, fraclnot, h (9 FP64 fields), writes 7 FP32
+    coefficients and FP64 su: 9 x 8 + 7 x 4 + 8 = 108 B per cell."""
+    return dom * (9 * 8 + 7 * 4 + 8)
 
 
 def profile_traffic(name):
@@ -493,19 +493,21 @@ def main():
     half_ms = legs["mixed"]["sweep_h_avg_ms"] / 8.0   # 4 red-black passes = 8 launches per event pair
     asm_ms = legs["mixed"]["assemble_h_avg_ms"]
     xb, ab = mixed_half_pass_bytes(dom), mixed_assembly_bytes(dom)
-    cand = [("k_xline_mx<float, 8> (enthalpy x-lines, one red or black half pass)", xb, half_ms, 8),
-            ("k_energy_assemble_mx (enthalpy coefficients, FP32 storage)", ab, asm_ms, 1)]
-    cand.sort(key=lambda c: -c[2] * c[3])   # largest device time per iteration first
-    name, algo, ms, per_iter = cand[0]
+    scheme = {0: "0", 1: "1", 2: "2"}[p.discretization]
+    cand = [("k_xline_mx<float,8>", "k_xline_mx<float, 8> (enthalpy x-lines, one red or black half pass)", xb,
+             half_ms, 8),
+            ("k_energy_assemble_mx<%s>" % scheme, "k_energy_assemble_mx (enthalpy coefficients, FP32)", ab, asm_ms, 1)]
+    cand.sort(key=lambda c: -c[3] * c[4])   # largest device time per iteration first
+    key, name, algo, ms, per_iter = cand[0]
     achieved = algo / (ms * 1e-3) / 1e9
-    traffic = profile_traffic(name.split(" ")[0])
+    traffic = profile_traffic(key)
     roof = {"bound": "hbm", "kernel": name, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS, "traffic": traffic[0] if traffic else None,
             "traffic_source": ("profiles/" + traffic[1]) if traffic else None,
             "avg_launch_ms": ms, "launches_per_iteration": per_iter, "algorithmic_bytes_per_launch": algo,
             "timing": "HIP events on the solver stream, host-driven iteration after the timed region"}
     other = []
-    for nm, al, t, k in cand[1:]:
+    for _, nm, al, t, k in cand[1:]:
         other.append({"kernel": nm, "avg_launch_ms": t, "achieved_gbs": al / (t * 1e-3) / 1e9,
                       "frac": al / (t * 1e-3) / 1e9 / HBM_PEAK_GBS, "launches_per_iteration": k,
                       "algorithmic_bytes_per_launch": al})
