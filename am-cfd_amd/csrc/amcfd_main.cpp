@@ -1,4 +1,4 @@
-// amcfd <deck.k> [--mode fp64|scan] : run a whole simulation through the C ABI.
+// amcfd <deck.k> [--mode fp64|scan|mixed] : run a whole simulation through the C ABI.
 //
 // Same invocation and the same output files as the reference executable
 // (src/main.cpp:25-241): one argument, the keyword deck (--version / -v print
