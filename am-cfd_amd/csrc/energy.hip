@@ -1143,7 +1143,16 @@ void launch_energy_assemble_mixed(hipStream_t st, const Dims &d, const DevGeom &
                        f.diff, f.hnot, f.fracl, f.fraclnot, f.enthalpy, o, skip, step);
 }
 
-int mixed_has_volumetric_source(const amcfd_params &p) { return p.volheatsource || p.n_heat_table > 0; }
+// A fixed volumetric source of 0 W (finite factors, positive radius and
+// thickness) adds exactly zero, as no source does; heat-source tables may
+// change the source during the run.
+int mixed_has_volumetric_source(const amcfd_params &p) {
+    if (p.n_heat_table > 0 || p.volheatsourcetable) return 1;
+    if (!p.volheatsource) return 0;
+    const bool zero = p.avolpow == 0.0 && p.heatrb > 0.0 && p.heatthick > 0.0 && std::isfinite(p.avolfact) &&
+                      std::isfinite(p.apowseta);
+    return zero ? 0 : 1;
+}
 
 int mixed_tile_count(const Dims &d) {
     const dim3 a = assemble_grid_mx(d);
