@@ -299,6 +299,13 @@ int amcfd_set_graph(amcfd_solver *s, int on, int block, int box_blocks) {
     GUARD(s->s.set_graph(on, block, box_blocks));
 }
 
+int amcfd_get_graph(const amcfd_solver *s, int *on, int *block, int *box_blocks) {
+    if (!s || !on || !block || !box_blocks) return amcfd::set_error(AMCFD_E_ARG, "null argument");
+    if (s->s.mode() != AMCFD_MODE_MIXED) return amcfd::set_error(AMCFD_E_ARG, "the solver is not in the mixed mode");
+    s->s.graph_settings(on, block, box_blocks);
+    return AMCFD_OK;
+}
+
 int amcfd_copy_state(amcfd_solver *dst, const amcfd_solver *src) {
     if (!dst || !src) return amcfd::set_error(AMCFD_E_ARG, "null solver");
     if (dst == src) return AMCFD_OK;

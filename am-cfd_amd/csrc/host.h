@@ -35,6 +35,9 @@ int ws_capture_input(WarmStart *w, Solver &s);
 int ws_capture_target(WarmStart *w, Solver &s);
 int ws_apply(WarmStart *w, Solver &s);
 void ws_attach(WarmStart *w, Solver *s);
+// the tuning table's choice for a grid of `cells` interior cells (tuning.cpp);
+// leaves the values alone when no table or no row applies
+void tuning_lookup(long long cells, int *graph_block, int *box_blocks);
 void ws_detach(WarmStart *w, Solver *s);
 
 // accumulated device time of one kernel family (HIP events on the solver stream)
@@ -66,6 +69,11 @@ class Solver {
     // the solver runs beside others on the device (amcfd_run_batch)
     void set_shared_device(bool on) { shared_device_ = on; }
     int mode() const { return mode_; }
+    void graph_settings(int *on, int *block, int *box_blocks) const {
+        *on = use_graph_;
+        *block = graph_block_;
+        *box_blocks = box_blocks_;
+    }
     // mixed mode: order of the enthalpy line passes ('x' red-black x-lines,
     // 'y' y-lines, 'z' z-lines; an even number of y/z) and red-black x
     // passes per melt-pool box variable

@@ -84,6 +84,17 @@ int Solver::set_mixed_schedule(const char *sched, int box_passes) {
     // even count ends in the field itself
     if (yz % 2) return set_error(AMCFD_E_ARG, "the schedule needs an even number of y/z passes");
     if (box_passes < 1 || box_passes > 16) return set_error(AMCFD_E_ARG, "box passes must be 1..16");
+    if (yz) {   // the y/z passes keep a whole line's forward values in LDS
+        const int L = std::max(d_.njm2, d_.nkf - 2);
+        if (L > mixed_max_yz())
+            return set_error(AMCFD_E_CAPACITY, "y/z passes take lines of at most " + std::to_string(mixed_max_yz()) +
+                                                   " cells");
+        int dev = 0, lds_max = 0;
+        HIPCALL(hipGetDevice(&dev));
+        HIPCALL(hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+        if (mixed_yz_lds(L) > (size_t)lds_max)
+            return set_error(AMCFD_E_CAPACITY, "y- or z-lines too long for the device's LDS");
+    }
     mx_sched_ = sched;
     mx_box_passes_ = box_passes;
     return AMCFD_OK;
@@ -118,9 +129,6 @@ int Solver::create(const amcfd_params &p, int mode) {
     if (mixed && p.ni - 2 > mixed_max_line())
         return set_error(AMCFD_E_CAPACITY, "x-lines longer than " + std::to_string(mixed_max_line()) +
                                                " cells in the mixed mode");
-    if (mixed && (p.nj - 2 > mixed_max_yz() || p.nk - 2 > mixed_max_yz()))
-        return set_error(AMCFD_E_CAPACITY, "y- or z-lines longer than " + std::to_string(mixed_max_yz()) +
-                                               " cells in the mixed mode");
     if (mixed && p.species)
         return set_error(AMCFD_E_UNSUPPORTED, "species transport runs in the FP64 and scan modes only");
     if (p.nj >= 16384 || p.nk >= 16384) return set_error(AMCFD_E_CAPACITY, "more than 16383 nodes in y or z");
@@ -129,11 +137,9 @@ int Solver::create(const amcfd_params &p, int mode) {
 
     int dev = 0;
     HIPCALL(hipGetDevice(&dev));
-    if (mixed) {
-        int lds_max = 0;
-        HIPCALL(hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
-        if (mixed_yz_lds(std::max(p.nj - 2, p.nk - 2)) > (size_t)lds_max)
-            return set_error(AMCFD_E_CAPACITY, "y- or z-lines too long for the device's LDS in the mixed mode");
+    if (mixed) {   // the captured loop's launch shapes for this size, from the tuning table if there is one
+        tuning_lookup((long long)(p.ni - 2) * (p.nj - 2) * (p.nk - 2), &graph_block_, &box_blocks_);
+        box_blocks_ = std::min(box_blocks_, max_box_blocks);
     }
     HIPCALL(hipStreamCreateWithFlags(&st_, hipStreamNonBlocking));
     HIPCALL(hipEventCreate(&step_ev_[0]));

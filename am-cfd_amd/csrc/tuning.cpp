@@ -1,0 +1,126 @@
+// The mixed mode's launch-shape table, written by am-cfd_amd/autotune.py and
+// read when a mixed-mode solver is created.
+//
+// The captured SIMPLE loop has two shapes that change its speed and not its
+// results: the iterations per graph launch (the scalars are read once per
+// launch) and the workgroups that stride over the melt-pool box.  Their best
+// values depend on the grid size, so the tuner measures each candidate on
+// representative grids and writes one row per size class:
+//     max_cells  graph_block  box_blocks
+// ('#' starts a comment).  A solver takes the first row whose max_cells is at
+// least its interior cell count; with no row (or no table) it keeps the
+// built-in 8 and 256.  The table is the file named by $AMCFD_TUNING, else
+// tuning.txt beside libamcfd.so.
+
+#include <dlfcn.h>
+
+#include <cctype>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "host.h"
+
+namespace amcfd {
+
+namespace {
+
+struct TuneRow {
+    long long max_cells;
+    int graph_block, box_blocks;
+};
+
+// rows in file order; a malformed line is an error (the whole table is refused)
+int parse_table(const char *path, std::vector<TuneRow> &rows) {
+    FILE *f = std::fopen(path, "r");
+    if (!f) return set_error(AMCFD_E_IO, std::string("cannot open the tuning table ") + path);
+    char line[512];
+    int lineno = 0, st = AMCFD_OK;
+    while (std::fgets(line, sizeof line, f)) {
+        ++lineno;
+        char *hash = std::strchr(line, '#');
+        if (hash) *hash = '\0';
+        long long cells;
+        int gb, bb;
+        char extra;
+        bool blank = true;
+        for (const char *c = line; *c; ++c) blank = blank && std::isspace((unsigned char)*c);
+        if (blank) continue;   // blank or comment
+        const int n = std::sscanf(line, "%lld %d %d %c", &cells, &gb, &bb, &extra);
+        if (n != 3 || cells < 1 || gb < 1 || gb > 64 || bb < 1 || bb > 4096) {
+            st = set_error(AMCFD_E_PARSE, std::string(path) + ":" + std::to_string(lineno) +
+                                              ": expected 'max_cells graph_block(1..64) box_blocks(1..4096)'");
+            break;
+        }
+        rows.push_back(TuneRow{cells, gb, bb});
+    }
+    std::fclose(f);
+    return st;
+}
+
+bool choose(const std::vector<TuneRow> &rows, long long cells, int *gb, int *bb) {
+    for (const TuneRow &r : rows)
+        if (cells <= r.max_cells) {
+            *gb = r.graph_block;
+            *bb = r.box_blocks;
+            return true;
+        }
+    return false;
+}
+
+std::string default_path() {
+    const char *env = std::getenv("AMCFD_TUNING");
+    if (env && *env) return env;
+    Dl_info info;
+    if (dladdr((void *)&default_path, &info) && info.dli_fname) {
+        std::string so = info.dli_fname;
+        const size_t slash = so.rfind('/');
+        return (slash == std::string::npos ? std::string(".") : so.substr(0, slash)) + "/tuning.txt";
+    }
+    return "tuning.txt";
+}
+
+std::once_flag table_once;
+std::vector<TuneRow> table;   // the library's table, loaded once
+
+void load_default() {
+    const std::string p = default_path();
+    FILE *f = std::fopen(p.c_str(), "r");
+    if (!f) return;   // no table: the built-in shapes
+    std::fclose(f);
+    std::vector<TuneRow> rows;
+    if (parse_table(p.c_str(), rows) == AMCFD_OK) table = rows;
+}
+
+}  // namespace
+
+void tuning_lookup(long long cells, int *graph_block, int *box_blocks) {
+    std::call_once(table_once, load_default);
+    choose(table, cells, graph_block, box_blocks);
+}
+
+}  // namespace amcfd
+
+extern "C" {
+
+int amcfd_tuning_choice(const char *path, long long cells, int *graph_block, int *box_blocks) {
+    using namespace amcfd;
+    if (!graph_block || !box_blocks) return set_error(AMCFD_E_ARG, "null argument");
+    if (cells < 1) return set_error(AMCFD_E_ARG, "cells must be positive");
+    *graph_block = 8;
+    *box_blocks = 256;
+    if (!path) {
+        tuning_lookup(cells, graph_block, box_blocks);
+        return AMCFD_OK;
+    }
+    std::vector<TuneRow> rows;
+    const int st = parse_table(path, rows);
+    if (st) return st;
+    choose(rows, cells, graph_block, box_blocks);
+    return AMCFD_OK;
+}
+
+}  // extern "C"

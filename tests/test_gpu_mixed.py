@@ -188,3 +188,33 @@ def test_mixed_schedules_with_y_and_z_passes(case_dir, amcfd, schedule):
         assert np.isfinite(out[-1]).all()
         s.close()
     assert np.array_equal(out[0], out[1])
+
+
+def test_solver_takes_its_launch_shapes_from_the_tuning_table(case_dir, amcfd, tmp_path, monkeypatch):
+    """A mixed-mode solver starts from the library's tuning-table row for its
+    interior cell count (am-cfd_amd/tuning.txt, written by autotune.py), and
+    amcfd_set_graph still overrides it."""
+    deck = amcfd.Deck(case_dir("tiny"))
+    p = deck.params
+    cells = (p.ni - 2) * (p.nj - 2) * (p.nk - 2)
+    s = amcfd.Solver(deck, mode=amcfd.MODE_MIXED)
+    assert s.graph_settings() == (True,) + amcfd.tuning_choice(cells)
+    s.set_graph(True, 16, 128)
+    assert s.graph_settings() == (True, 16, 128)
+
+
+def test_wide_grids_run_without_yz_passes(amcfd, tmp_path):
+    """y- and z-line lengths bound only the schedules that use y/z passes: a
+    grid with 400 cells in y runs the default x-only schedule, and a y/z
+    schedule on it is refused with AMCFD_E_CAPACITY."""
+    import decks
+    decks.CASES["tiny_wide_y"] = dict(
+        mesh=(([0.3e-3], [12], [1.0]), ([4.0e-3], [400], [1.0]), ([0.15e-3], [6], [1.0])),
+        deck=dict(finaltime=0.01e-3 - 1e-9, maxit=20, dt=5e-6, lasertime=0.01e-3, xstart=0.15e-3,
+                  ystart=2.0e-3, speed=1.0, power=50.0, radius=50e-6, flow=False))
+    s = amcfd.Solver(decks.write_case("tiny_wide_y", str(tmp_path)), mode=amcfd.MODE_MIXED)
+    with pytest.raises(amcfd.AmcfdError) as e:
+        s.set_mixed_schedule("xyzx")
+    assert e.value.code == -7
+    s.run()
+    assert np.isfinite(s.field("temp")).all()
