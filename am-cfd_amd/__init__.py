@@ -39,7 +39,8 @@ EXPORTS = ("amcfd_deck_load", "amcfd_deck_params", "amcfd_deck_free", "amcfd_cre
            "amcfd_get_scalars", "amcfd_dims", "amcfd_last_timing", "amcfd_set_profiling",
            "amcfd_kernel_stats", "amcfd_write_vts", "amcfd_write_pvd", "amcfd_append_residuals",
            "amcfd_append_final_geometry", "amcfd_deck_string", "amcfd_free_surface",
-           "amcfd_output_section", "amcfd_run_batch", "amcfd_set_mixed_schedule", "amcfd_set_graph", "amcfd_get_graph", "amcfd_tuning_choice", "amcfd_copy_state", "amcfd_test_expire_waits",
+           "amcfd_output_section", "amcfd_run_batch", "amcfd_set_mixed_schedule", "amcfd_set_graph", "amcfd_get_graph", "amcfd_set_box_fused",
+           "amcfd_tuning_choice", "amcfd_copy_state", "amcfd_test_expire_waits",
            "amcfd_tdma_lines", "amcfd_tdma_lines_noalloc",
            "amcfd_ws_create", "amcfd_ws_destroy", "amcfd_ws_attach", "amcfd_ws_counts", "amcfd_ws_train", "amcfd_ws_samples",
            "amcfd_ws_set_factors", "amcfd_ws_save", "amcfd_ws_load", "amcfd_ws_param_count", "amcfd_ws_params",
@@ -153,10 +154,9 @@ def load_library(path=LIB_PATH):
         "amcfd_set_mixed_schedule": (ip, [vp, cp, ip]),
         "amcfd_test_expire_waits": (ip, [vp]),
         "amcfd_set_graph": (ip, [vp, ip, ip, ip]),
-        "amcfd_get_graph": (ip, [vp, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int),
-                                ctypes.POINTER(ctypes.c_int)]),
-        "amcfd_tuning_choice": (ip, [cp, ctypes.c_longlong, ctypes.POINTER(ctypes.c_int),
-                                     ctypes.POINTER(ctypes.c_int)]),
+        "amcfd_get_graph": (ip, [vp] + [ctypes.POINTER(ctypes.c_int)] * 4),
+        "amcfd_set_box_fused": (ip, [vp, ip]),
+        "amcfd_tuning_choice": (ip, [cp, ctypes.c_longlong] + [ctypes.POINTER(ctypes.c_int)] * 3),
         "amcfd_copy_state": (ip, [vp, vp]),
         "amcfd_tdma_lines": (ip, [vp, vp, vp, vp, vp, ip, ip, ip, vp]),
         "amcfd_tdma_lines_noalloc": (ip, [vp, vp, vp, vp, vp, ip, ip, ip, vp]),
@@ -333,10 +333,17 @@ class Solver:
         return a.value, g.value
 
     def graph_settings(self):
-        """Mixed mode: (captured loop on, iterations per graph, box workgroups)."""
-        on, b, bb = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
-        _check(load_library().amcfd_get_graph(self._h, ctypes.byref(on), ctypes.byref(b), ctypes.byref(bb)))
-        return bool(on.value), b.value, bb.value
+        """Mixed mode: (captured loop on, iterations per graph, box workgroups,
+        box passes in one persistent launch)."""
+        on, b, bb, bf = ctypes.c_int(), ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        _check(load_library().amcfd_get_graph(self._h, ctypes.byref(on), ctypes.byref(b), ctypes.byref(bb),
+                                              ctypes.byref(bf)))
+        return bool(on.value), b.value, bb.value, bool(bf.value)
+
+    def set_box_fused(self, on=True):
+        """Mixed mode: a box variable's passes in one persistent launch (True)
+        or one launch per half pass (False); the same results."""
+        _check(load_library().amcfd_set_box_fused(self._h, int(bool(on))))
 
     def copy_state_from(self, other):
         """Continue from another solver's exact state (same deck)."""
@@ -369,12 +376,12 @@ def run_batch(solvers, max_steps=-1, workers=0):
 
 
 def tuning_choice(cells, path=None):
-    """(graph_block, box_blocks) the mixed mode takes for a grid of `cells`
-    interior cells, from the table at `path` or the library's own."""
-    gb, bb = ctypes.c_int(), ctypes.c_int()
+    """(graph_block, box_blocks, box_fused) the mixed mode takes for a grid
+    of `cells` interior cells, from the table at `path` or the library's own."""
+    gb, bb, bf = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
     _check(load_library().amcfd_tuning_choice(os.fsencode(path) if path else None, int(cells), ctypes.byref(gb),
-                                              ctypes.byref(bb)))
-    return gb.value, bb.value
+                                              ctypes.byref(bb), ctypes.byref(bf)))
+    return gb.value, bb.value, bool(bf.value)
 
 
 def tdma_lines(aw, ap, ae, d, phi, n, nlines, guard=0, stream=None):

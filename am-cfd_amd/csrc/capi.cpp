@@ -299,11 +299,18 @@ int amcfd_set_graph(amcfd_solver *s, int on, int block, int box_blocks) {
     GUARD(s->s.set_graph(on, block, box_blocks));
 }
 
-int amcfd_get_graph(const amcfd_solver *s, int *on, int *block, int *box_blocks) {
-    if (!s || !on || !block || !box_blocks) return amcfd::set_error(AMCFD_E_ARG, "null argument");
+int amcfd_get_graph(const amcfd_solver *s, int *on, int *block, int *box_blocks, int *box_fused) {
+    if (!s || !on || !block || !box_blocks || !box_fused) return amcfd::set_error(AMCFD_E_ARG, "null argument");
     if (s->s.mode() != AMCFD_MODE_MIXED) return amcfd::set_error(AMCFD_E_ARG, "the solver is not in the mixed mode");
     s->s.graph_settings(on, block, box_blocks);
+    *box_fused = s->s.box_fused();
     return AMCFD_OK;
+}
+
+int amcfd_set_box_fused(amcfd_solver *s, int on) {
+    if (!s) return amcfd::set_error(AMCFD_E_ARG, "null solver");
+    if (s->s.mode() != AMCFD_MODE_MIXED) return amcfd::set_error(AMCFD_E_ARG, "the solver is not in the mixed mode");
+    GUARD(s->s.set_box_fused(on));
 }
 
 int amcfd_copy_state(amcfd_solver *dst, const amcfd_solver *src) {

@@ -37,7 +37,7 @@ int ws_apply(WarmStart *w, Solver &s);
 void ws_attach(WarmStart *w, Solver *s);
 // the tuning table's choice for a grid of `cells` interior cells (tuning.cpp);
 // leaves the values alone when no table or no row applies
-void tuning_lookup(long long cells, int *graph_block, int *box_blocks);
+void tuning_lookup(long long cells, int *graph_block, int *box_blocks, int *box_fused);
 void ws_detach(WarmStart *w, Solver *s);
 
 // accumulated device time of one kernel family (HIP events on the solver stream)
@@ -74,6 +74,8 @@ class Solver {
         *block = graph_block_;
         *box_blocks = box_blocks_;
     }
+    int box_fused() const { return box_fused_; }
+    int set_box_fused(int on);
     // mixed mode: order of the enthalpy line passes ('x' red-black x-lines,
     // 'y' y-lines, 'z' z-lines; an even number of y/z) and red-black x
     // passes per melt-pool box variable
@@ -147,12 +149,19 @@ class Solver {
         int nk = -1, blocks = 0, box_blocks = 0;
         double delt = 0.0, scanvel = 0.0;
         std::string sched;
-        int box_passes = 0;
+        int box_passes = 0, box_fused = -1;
     } gkey_;
     StepDev *stepd_ = nullptr;       // device
     StepDev *steph_ = nullptr;       // pinned staging
     std::string mx_sched_ = "xxxx";   // mixed mode: order of the enthalpy line passes
     int mx_box_passes_ = 4;          // mixed mode: red-black x passes per box variable
+    // mixed mode: the box passes of one variable in one persistent launch (1)
+    // or one launch per half pass (0); the same results either way
+    int box_fused_ = 1;
+    double *box_bufs_ = nullptr;     // box_buf_passes_ pass buffers of ni*nj*nkf
+    int box_buf_passes_ = 0;
+    unsigned int *box_claim_ = nullptr;
+    int ensure_box_buffers(int passes);
     int profile_ = 0;
     bool shared_device_ = false;
     std::vector<hipEvent_t> ev_pool_;

@@ -219,6 +219,11 @@ void launch_mixed_xline(hipStream_t st, const Dims &d, const MxFields &m, double
 void launch_mixed_yzline(hipStream_t st, const Dims &d, const MxFields &m, const double *hin, double *hout, int dir,
                          int ibeg, int iend, int jbeg, int jend, int kbeg, int kend, const int *skip);
 // melt-pool box x-lines of one colour (FP64 coefficients, reference guard)
+// all red-black passes of one melt-pool box variable in one persistent
+// launch (plus the launch that prepares its pass buffers); nonzero if a
+// launch failed
+int launch_box_passes(hipStream_t st, const Dims &d, DevFields f, double *phi, double *bufs, size_t bstride,
+                      int passes, unsigned int *claim, int *error, const int *skip, const int *dbox, int nblocks);
 void launch_box_xline(hipStream_t st, const Dims &d, DevFields f, double *phi, int ibeg, int iend, int jbeg, int jend,
                       int kbeg, int kend, int colour, const int *skip, const int *dbox = nullptr, int nblocks = 0);
 

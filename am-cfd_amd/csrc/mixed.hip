@@ -26,6 +26,8 @@
 // assembly kernel writes; they and the 1-D correction along x are folded in
 // FP64 in a fixed order, so a run is reproducible bit for bit.
 
+#include <type_traits>
+
 #include "fv.h"
 
 // multiply-adds may fuse here: this mode does not reproduce the reference's
@@ -159,6 +161,31 @@ __device__ void wave_tridiag(int n, const T (&w)[M], const T (&dg)[M], const T (
     }
 }
 
+// The line in LDS (sw, sd, se: sub-, main and super-diagonal, sr: right-hand
+// side, row p at [p]) solved by the wave with M rows per lane; the solution
+// replaces sr.  Ends with the wave's writes to sr issued after a barrier.
+template <int M>
+__device__ __forceinline__ void lds_line_solve(int n, int lane, const float *sw, const float *sd, const float *se,
+                                               float *sr, bool guard) {
+    float w[M], dg[M], e[M], r[M];
+#pragma unroll
+    for (int q = 0; q < M; ++q) {
+        const int p = lane * M + q;
+        const bool in = p < n;
+        w[q] = in ? sw[p] : 0.f;
+        dg[q] = in ? sd[p] : 1.f;
+        e[q] = in ? se[p] : 0.f;
+        r[q] = in ? sr[p] : 0.f;
+    }
+    wave_tridiag<float, M>(n, w, dg, e, r, guard);
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < M; ++q) {
+        const int p = lane * M + q;
+        if (p < n) sr[p] = r[q];
+    }
+}
+
 template <typename TC>
 struct MxCoefs {
     const TC *ap, *an, *as, *ae, *aw, *at, *ab;
@@ -223,23 +250,13 @@ __global__ __launch_bounds__(64) void k_xline_mx(Dims d, MxCoefs<TC> cf, double 
             }
         }
         __syncthreads();
-        float w[M], dg[M], e[M], r[M];
-#pragma unroll
-        for (int q = 0; q < M; ++q) {
-            const int p = lane * M + q;
-            const bool in = p < n;
-            w[q] = in ? sw[p] : 0.f;
-            dg[q] = in ? sd[p] : 1.f;
-            e[q] = in ? se[p] : 0.f;
-            r[q] = in ? sr[p] : 0.f;
-        }
-        wave_tridiag<float, M>(n, w, dg, e, r, guard != 0);
-        __syncthreads();
-#pragma unroll
-        for (int q = 0; q < M; ++q) {
-            const int p = lane * M + q;
-            if (p < n) sr[p] = r[q];
-        }
+        // box lines (FP64 coefficients) of at most 128 cells take two rows per
+        // lane whatever the capacity M, which shortens the scans; the
+        // enthalpy's capacity fits its lines already
+        if (std::is_same<TC, double>::value && M > 2 && n <= 128)
+            lds_line_solve<2>(n, lane, sw, sd, se, sr, guard != 0);
+        else
+            lds_line_solve<M>(n, lane, sw, sd, se, sr, guard != 0);
         __syncthreads();
 #pragma unroll
         for (int q = 0; q < M; ++q) {
@@ -248,6 +265,170 @@ __global__ __launch_bounds__(64) void k_xline_mx(Dims d, MxCoefs<TC> cf, double 
         }
         __syncthreads();
     }
+}
+
+// ------------------------------------------------------------------------
+// Melt-pool box x-lines, all red-black passes of one variable in one
+// persistent launch.  Pass p writes its lines into pass buffer p (every box
+// cell of every buffer holds BX_PENDING beforehand, k_box_prep); the last
+// pass also writes the field.  A red line of pass p reads its own and its
+// black neighbours' values from buffer p-1 (the field for p = 0), a black
+// line its own from buffer p-1 and its red neighbours from buffer p; cells
+// outside the lines (the box's faces, the lines' ends) come from the field,
+// which no task writes before every reader of those cells is done.  Each
+// value is polled until it is no longer BX_PENDING, with device-coherent
+// relaxed atomic loads, so a value is its own ready flag.  Tasks (pass,
+// colour, line) are claimed from a counter in that order, and a task reads
+// only values of tasks claimed before it by running workgroups, so the
+// schedule cannot deadlock however many workgroups are resident.  Every
+// line sees the same values as in the launch-per-half-pass form
+// (k_xline_mx), so the results are the same bits.
+// ------------------------------------------------------------------------
+constexpr unsigned long long BX_PENDING = 0x7ff4dead0000beefull;   // a NaN no arithmetic produces
+constexpr long long BX_SPIN_LIMIT = 1LL << 20;                      // bounded wait (~ a second)
+
+__device__ __forceinline__ double bx_ld(const double *p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void bx_st(double *p, double v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ bool bx_pending(double v) {
+    return __builtin_bit_cast(unsigned long long, v) == BX_PENDING;
+}
+// v, already read from p, re-read while pending.  False once the bounded
+// wait expired or another wait of the solver has given up.
+__device__ __forceinline__ bool bx_await(const double *p, double &v, const int *error) {
+    long long spins = 0;
+    while (bx_pending(v)) {
+        v = bx_ld(p);
+        if (++spins > BX_SPIN_LIMIT) return false;
+        if ((spins & 255) == 0 && __hip_atomic_load(error, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0)
+            return false;
+    }
+    return true;
+}
+
+struct BoxRange {
+    int ibeg, iend, jbeg, jend, kbeg, kend;
+};
+__device__ __forceinline__ bool box_range(const int *dbox, int kend, int maxn, BoxRange &b) {
+    b.ibeg = dbox[0] + 1;
+    b.iend = dbox[1] - 1;
+    b.jbeg = dbox[2];
+    b.jend = dbox[3];
+    b.kbeg = dbox[4];
+    b.kend = kend;
+    return !(b.iend - b.ibeg + 1 > maxn || b.iend < b.ibeg || b.jend < b.jbeg || b.kend < b.kbeg);
+}
+
+// the pass buffers' box cells to BX_PENDING and the task counter to zero
+__global__ __launch_bounds__(256) void k_box_prep(Dims d, double *bufs, size_t bstride, int passes, int kend,
+                                                  unsigned int *claim, const int *__restrict__ skip,
+                                                  const int *__restrict__ dbox) {
+    if (skip && *skip) return;
+    BoxRange b;
+    if (!box_range(dbox, kend, 1 << 30, b)) return;
+    if (blockIdx.x == 0 && threadIdx.x == 0) *claim = 0u;
+    const int n = b.iend - b.ibeg + 1, nj = b.jend - b.jbeg + 1;
+    const size_t cells = (size_t)n * nj * (b.kend - b.kbeg + 1);
+    const double pend = __builtin_bit_cast(double, BX_PENDING);
+    for (size_t e = (size_t)blockIdx.x * 256 + threadIdx.x; e < cells * passes; e += (size_t)gridDim.x * 256) {
+        const size_t q = e / cells, r = e % cells;
+        const int i = b.ibeg + (int)(r % n);
+        const size_t line = r / n;
+        const int j = b.jbeg + (int)(line % nj), k = b.kbeg + (int)(line / nj);
+        bufs[q * bstride + cix(d, i, j, k)] = pend;
+    }
+}
+
+template <int M>
+__global__ __launch_bounds__(64) void k_box_lines(Dims d, MxCoefs<double> cf, double *phi, double *bufs,
+                                                   size_t bstride, int passes, int kend, int guard,
+                                                   unsigned int *claim, int *error, const int *__restrict__ skip,
+                                                   const int *__restrict__ dbox) {
+    __shared__ float sw[64 * M], sd[64 * M], se[64 * M], sr[64 * M];
+    __shared__ int s_task;
+    if (skip && *skip) return;
+    BoxRange b;
+    if (!box_range(dbox, kend, 64 * M, b)) return;
+    const int lane = threadIdx.x;
+    const int n = b.iend - b.ibeg + 1, nj = b.jend - b.jbeg + 1, nk = b.kend - b.kbeg + 1;
+    const int cm = (nj + 1) / 2;
+    const int per_half = nk * cm, total = 2 * passes * per_half;
+    bool ok = true;
+    for (;;) {
+        if (lane == 0) {
+            const unsigned int t = atomicAdd(claim, 1u);
+            s_task = t < (unsigned int)total ? (int)t : -1;
+        }
+        __syncthreads();
+        const int t = s_task;
+        __syncthreads();
+        if (t < 0) break;
+        const int h = t / per_half, u = t % per_half;
+        const int pass = h >> 1, colour = h & 1;
+        const int k = b.kbeg + u / cm;
+        const int j = b.jbeg + ((colour - k - b.jbeg) & 1) + 2 * (u % cm);
+        if (j > b.jend) continue;   // uniform over the wave
+        const double *own = pass == 0 ? phi : bufs + (size_t)(pass - 1) * bstride;
+        const double *nbr = colour == 0 ? own : bufs + (size_t)pass * bstride;
+        double *dst = bufs + (size_t)pass * bstride;
+        const bool last = pass == passes - 1;
+        // the line's neighbours: solved lines of the other colour inside the box, fixed values outside
+        const double *src_n = j < b.jend ? nbr : phi, *src_s = j > b.jbeg ? nbr : phi;
+        const double *src_t = k < b.kend ? nbr : phi, *src_b = k > b.kbeg ? nbr : phi;
+        const size_t row = (size_t)k * d.sk + (size_t)j * d.sj + b.ibeg;
+        double hp[M];
+#pragma unroll
+        for (int q = 0; q < M; ++q) {
+            const int ii = lane + 64 * q;
+            hp[q] = 0.0;
+            if (ii < n) {
+                const size_t c = row + ii;
+                const double *pw = (ii > 0 ? own : phi) + c - 1, *pe = (ii < n - 1 ? own : phi) + c + 1;
+                // every load first, then the values still pending are polled
+                double h0 = bx_ld(own + c), hw = bx_ld(pw), he = bx_ld(pe);
+                double hn = bx_ld(src_n + c + d.sj), hs = bx_ld(src_s + c - d.sj);
+                double ht = bx_ld(src_t + c + d.sk), hb = bx_ld(src_b + c - d.sk);
+                ok &= bx_await(own + c, h0, error);
+                ok &= bx_await(pw, hw, error);
+                ok &= bx_await(pe, he, error);
+                ok &= bx_await(src_n + c + d.sj, hn, error);
+                ok &= bx_await(src_s + c - d.sj, hs, error);
+                ok &= bx_await(src_t + c + d.sk, ht, error);
+                ok &= bx_await(src_b + c - d.sk, hb, error);
+                const double ap = cf.ap[c], aw = cf.aw[c], ae = cf.ae[c];
+                const double r = cf.su[c] + aw * hw + ae * he + cf.an[c] * hn + cf.as[c] * hs + cf.at[c] * ht +
+                                 cf.ab[c] * hb - ap * h0;
+                hp[q] = h0;
+                sw[ii] = (float)aw;
+                sd[ii] = (float)ap;
+                se[ii] = (float)ae;
+                sr[ii] = (float)r;
+            }
+        }
+        // one lane's expired wait ends the whole wave (the task loop stays uniform)
+        ok = !__any(!ok);
+        if (!ok) break;
+        __syncthreads();
+        // lines of at most 128 cells (the melt-pool box's) take two rows per
+        // lane whatever the capacity M, which shortens the scans
+        if (M > 2 && n <= 128) lds_line_solve<2>(n, lane, sw, sd, se, sr, guard != 0);
+        else lds_line_solve<M>(n, lane, sw, sd, se, sr, guard != 0);
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < M; ++q) {
+            const int ii = lane + 64 * q;
+            if (ii < n) {
+                const double v = hp[q] + (double)sr[ii];
+                bx_st(dst + row + ii, v);
+                if (last) phi[row + ii] = v;
+            }
+        }
+        __syncthreads();
+    }
+    if (!ok && lane == 0) atomicOr(error, 1);
 }
 
 // ------------------------------------------------------------------------
@@ -469,6 +650,25 @@ void launch_mixed_yzline(hipStream_t st, const Dims &d, const MxFields &m, const
                          int ibeg, int iend, int jbeg, int jend, int kbeg, int kend, const int *skip) {
     MxCoefs<float> cf{m.ap, m.an, m.as, m.ae, m.aw, m.at, m.ab, m.su};
     launch_yzline_t<float>(st, d, cf, hin, hout, dir, ibeg, iend, jbeg, jend, kbeg, kend, skip);
+}
+
+// all red-black passes of one box variable in one persistent launch
+int launch_box_passes(hipStream_t st, const Dims &d, DevFields f, double *phi, double *bufs, size_t bstride,
+                      int passes, unsigned int *claim, int *error, const int *skip, const int *dbox, int nblocks) {
+    MxCoefs<double> cf{f.ap, f.an, f.as, f.ae, f.aw, f.at, f.ab, f.su};
+    hipLaunchKernelGGL(k_box_prep, dim3(nblocks), dim3(256), 0, st, d, bufs, bstride, passes, d.nkm2, claim, skip,
+                       dbox);
+    const int n = d.nim2;   // an upper bound of the box's line length
+#define BL(MM) hipLaunchKernelGGL((k_box_lines<MM>), dim3(nblocks), dim3(64), 0, st, d, cf, phi, bufs, bstride, \
+                                  passes, d.nkm2, 1, claim, error, skip, dbox)
+    if (n <= 128) BL(2);
+    else if (n <= 256) BL(4);
+    else if (n <= 512) BL(8);
+    else if (n <= 1024) BL(16);
+    else if (n <= 2048) BL(32);
+    else BL(64);
+#undef BL
+    return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
 // melt-pool box lines (FP64 coefficients of the momentum / pressure

@@ -73,6 +73,24 @@ int Solver::set_graph(int on, int block, int box_blocks) {
     return AMCFD_OK;
 }
 
+int Solver::ensure_box_buffers(int passes) {
+    if (box_buf_passes_ >= passes) return AMCFD_OK;
+    // outside any capture: the buffers are allocated before a graph is taken
+    const size_t nn = (size_t)d_.ni * d_.nj * d_.nkf;
+    double *p = nullptr;
+    int st = alloc(&p, nn * passes);
+    if (st) return st;
+    box_bufs_ = p;   // an earlier, smaller set stays allocated until the solver goes
+    box_buf_passes_ = passes;
+    return AMCFD_OK;
+}
+
+int Solver::set_box_fused(int on) {
+    box_fused_ = on != 0;
+    release_graph();
+    return box_fused_ ? ensure_box_buffers(mx_box_passes_) : AMCFD_OK;
+}
+
 int Solver::set_mixed_schedule(const char *sched, int box_passes) {
     if (!sched || !*sched) return set_error(AMCFD_E_ARG, "empty pass schedule");
     int yz = 0;
@@ -97,7 +115,8 @@ int Solver::set_mixed_schedule(const char *sched, int box_passes) {
     }
     mx_sched_ = sched;
     mx_box_passes_ = box_passes;
-    return AMCFD_OK;
+    release_graph();
+    return box_fused_ ? ensure_box_buffers(mx_box_passes_) : AMCFD_OK;
 }
 
 int Solver::upload(double *dst, const std::vector<double> &src) {
@@ -138,7 +157,7 @@ int Solver::create(const amcfd_params &p, int mode) {
     int dev = 0;
     HIPCALL(hipGetDevice(&dev));
     if (mixed) {   // the captured loop's launch shapes for this size, from the tuning table if there is one
-        tuning_lookup((long long)(p.ni - 2) * (p.nj - 2) * (p.nk - 2), &graph_block_, &box_blocks_);
+        tuning_lookup((long long)(p.ni - 2) * (p.nj - 2) * (p.nk - 2), &graph_block_, &box_blocks_, &box_fused_);
         box_blocks_ = std::min(box_blocks_, max_box_blocks);
     }
     HIPCALL(hipStreamCreateWithFlags(&st_, hipStreamNonBlocking));
@@ -263,6 +282,14 @@ int Solver::create(const amcfd_params &p, int mode) {
         if ((st = alloc(&mx_.blkfold, (size_t)4 * mixed_blk_split() * d_.ni))) return st;
         if ((st = alloc(&mx_.delphi, (size_t)d_.ni + 2))) return st;
         if ((st = alloc(&mx_.halt, nn))) return st;
+        {
+            void *v = nullptr;
+            HIPCALL(hipMalloc(&v, sizeof(unsigned int)));
+            allocs_.push_back(v);
+            box_claim_ = (unsigned int *)v;
+            HIPCALL(hipMemsetAsync(box_claim_, 0, sizeof(unsigned int), st_));
+        }
+        if (box_fused_ && (st = ensure_box_buffers(mx_box_passes_))) return st;
         void *v = nullptr;
         HIPCALL(hipMalloc(&v, sizeof(StepDev)));
         allocs_.push_back(v);
@@ -863,10 +890,17 @@ int Solver::enqueue_mixed_iteration(bool ctl) {
             launch_bc_flow(st_, d_, g_, f_, box_, p_, ivar, bl);
             if (ivar < 4) launch_momentum(st_, d_, g_, f_, pr_, box_, p_, scanvel_, ivar, bl);
             else launch_pressure(st_, d_, g_, f_, pr_, box_, p_, bl);
-            for (int pass = 0; pass < mx_box_passes_; ++pass)
-                for (int colour = 0; colour < 2; ++colour)
-                    launch_box_xline(st_, d_, f_, phis[ivar], 1, d_.nim2, 1, d_.njm2, 1, d_.nkm2, colour, bl.skip,
-                                     bl.dbox, box_blocks_);
+            if (box_fused_) {
+                if ((st = ensure_box_buffers(mx_box_passes_))) return st;
+                if (launch_box_passes(st_, d_, f_, phis[ivar], box_bufs_, (size_t)d_.ni * d_.nj * d_.nkf,
+                                      mx_box_passes_, box_claim_, &sc_->error, bl.skip, bl.dbox, box_blocks_))
+                    return set_error(AMCFD_E_HIP, "box passes: launch failed");
+            } else {
+                for (int pass = 0; pass < mx_box_passes_; ++pass)
+                    for (int colour = 0; colour < 2; ++colour)
+                        launch_box_xline(st_, d_, f_, phis[ivar], 1, d_.nim2, 1, d_.njm2, 1, d_.nkm2, colour,
+                                         bl.skip, bl.dbox, box_blocks_);
+            }
             if (ivar == 4) launch_correct(st_, d_, f_, box_, p_, bl);
         }
     }
@@ -943,6 +977,7 @@ int Solver::capture_mixed_graph() {
     gkey_.scanvel = scanvel_;
     gkey_.sched = mx_sched_;
     gkey_.box_passes = mx_box_passes_;
+    gkey_.box_fused = box_fused_;
     return 0;
 }
 
@@ -952,7 +987,7 @@ int Solver::solve_step_mixed_graph() {
     // every launch argument the graph holds by value: rebuild when one moved
     if (!gexec_ || gkey_.nk != d_.nk || gkey_.blocks != graph_block_ || gkey_.box_blocks != box_blocks_ ||
         gkey_.delt != p_.delt || gkey_.scanvel != scanvel_ || gkey_.sched != mx_sched_ ||
-        gkey_.box_passes != mx_box_passes_) {
+        gkey_.box_passes != mx_box_passes_ || gkey_.box_fused != box_fused_) {
         int st = capture_mixed_graph();
         if (st) return st;
     }

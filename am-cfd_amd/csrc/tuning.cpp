@@ -1,15 +1,16 @@
 // The mixed mode's launch-shape table, written by am-cfd_amd/autotune.py and
 // read when a mixed-mode solver is created.
 //
-// The captured SIMPLE loop has two shapes that change its speed and not its
+// The captured SIMPLE loop has shapes that change its speed and not its
 // results: the iterations per graph launch (the scalars are read once per
-// launch) and the workgroups that stride over the melt-pool box.  Their best
-// values depend on the grid size, so the tuner measures each candidate on
-// representative grids and writes one row per size class:
-//     max_cells  graph_block  box_blocks
-// ('#' starts a comment).  A solver takes the first row whose max_cells is at
-// least its interior cell count; with no row (or no table) it keeps the
-// built-in 8 and 256.  The table is the file named by $AMCFD_TUNING, else
+// launch), the workgroups that stride over the melt-pool box, and whether a
+// box variable's passes run in one persistent launch or one launch per half
+// pass.  Their best values depend on the grid size, so the tuner measures
+// each candidate on representative grids and writes one row per size class:
+//     max_cells  graph_block  box_blocks  [box_fused]
+// ('#' starts a comment; box_fused 0 or 1, default 1).  A solver takes the
+// first row whose max_cells is at least its interior cell count; with no row
+// (or no table) it keeps the built-in 8, 256 and 1.  The table is the file named by $AMCFD_TUNING, else
 // tuning.txt beside libamcfd.so.
 
 #include <dlfcn.h>
@@ -30,7 +31,7 @@ namespace {
 
 struct TuneRow {
     long long max_cells;
-    int graph_block, box_blocks;
+    int graph_block, box_blocks, box_fused;
 };
 
 // rows in file order; a malformed line is an error (the whole table is refused)
@@ -44,28 +45,30 @@ int parse_table(const char *path, std::vector<TuneRow> &rows) {
         char *hash = std::strchr(line, '#');
         if (hash) *hash = '\0';
         long long cells;
-        int gb, bb;
+        int gb, bb, bf = 1;
         char extra;
         bool blank = true;
         for (const char *c = line; *c; ++c) blank = blank && std::isspace((unsigned char)*c);
         if (blank) continue;   // blank or comment
-        const int n = std::sscanf(line, "%lld %d %d %c", &cells, &gb, &bb, &extra);
-        if (n != 3 || cells < 1 || gb < 1 || gb > 64 || bb < 1 || bb > 4096) {
+        const int n = std::sscanf(line, "%lld %d %d %d %c", &cells, &gb, &bb, &bf, &extra);
+        if ((n != 3 && n != 4) || cells < 1 || gb < 1 || gb > 64 || bb < 1 || bb > 4096 || (bf != 0 && bf != 1)) {
             st = set_error(AMCFD_E_PARSE, std::string(path) + ":" + std::to_string(lineno) +
-                                              ": expected 'max_cells graph_block(1..64) box_blocks(1..4096)'");
+                                              ": expected 'max_cells graph_block(1..64) box_blocks(1..4096) "
+                                              "[box_fused(0|1)]'");
             break;
         }
-        rows.push_back(TuneRow{cells, gb, bb});
+        rows.push_back(TuneRow{cells, gb, bb, bf});
     }
     std::fclose(f);
     return st;
 }
 
-bool choose(const std::vector<TuneRow> &rows, long long cells, int *gb, int *bb) {
+bool choose(const std::vector<TuneRow> &rows, long long cells, int *gb, int *bb, int *bf) {
     for (const TuneRow &r : rows)
         if (cells <= r.max_cells) {
             *gb = r.graph_block;
             *bb = r.box_blocks;
+            *bf = r.box_fused;
             return true;
         }
     return false;
@@ -97,29 +100,30 @@ void load_default() {
 
 }  // namespace
 
-void tuning_lookup(long long cells, int *graph_block, int *box_blocks) {
+void tuning_lookup(long long cells, int *graph_block, int *box_blocks, int *box_fused) {
     std::call_once(table_once, load_default);
-    choose(table, cells, graph_block, box_blocks);
+    choose(table, cells, graph_block, box_blocks, box_fused);
 }
 
 }  // namespace amcfd
 
 extern "C" {
 
-int amcfd_tuning_choice(const char *path, long long cells, int *graph_block, int *box_blocks) {
+int amcfd_tuning_choice(const char *path, long long cells, int *graph_block, int *box_blocks, int *box_fused) {
     using namespace amcfd;
-    if (!graph_block || !box_blocks) return set_error(AMCFD_E_ARG, "null argument");
+    if (!graph_block || !box_blocks || !box_fused) return set_error(AMCFD_E_ARG, "null argument");
     if (cells < 1) return set_error(AMCFD_E_ARG, "cells must be positive");
     *graph_block = 8;
     *box_blocks = 256;
+    *box_fused = 1;
     if (!path) {
-        tuning_lookup(cells, graph_block, box_blocks);
+        tuning_lookup(cells, graph_block, box_blocks, box_fused);
         return AMCFD_OK;
     }
     std::vector<TuneRow> rows;
     const int st = parse_table(path, rows);
     if (st) return st;
-    choose(rows, cells, graph_block, box_blocks);
+    choose(rows, cells, graph_block, box_blocks, box_fused);
     return AMCFD_OK;
 }
 

@@ -300,19 +300,28 @@ int  amcfd_set_mixed_schedule(amcfd_solver *s, const char *schedule, int box_pas
  * for bit.  A step with a heat-source table row change always runs from the
  * host. */
 int  amcfd_set_graph(amcfd_solver *s, int on, int block, int box_blocks);
+/* Mixed mode: the box passes of each melt-pool box variable (u, v, w, p')
+ * in one persistent launch (on = 1, the default) or one launch per red or
+ * black half pass (on = 0).  Alone the two take the same time; beside other
+ * solvers (amcfd_run_batch) the persistent form's fewer launches measured
+ * 30% more throughput.  The persistent form claims its line tasks in
+ * pass order and hands line values over through per-pass buffers; its waits
+ * are bounded and an expired one ends the step with AMCFD_E_HIP.  Both forms
+ * give the same bits.  No reference counterpart. */
+int  amcfd_set_box_fused(amcfd_solver *s, int on);
 /* Mixed mode: the captured loop's current settings.  A mixed-mode solver
  * starts from the tuning table's row for its interior cell count (below). */
-int  amcfd_get_graph(const amcfd_solver *s, int *on, int *block, int *box_blocks);
+int  amcfd_get_graph(const amcfd_solver *s, int *on, int *block, int *box_blocks, int *box_fused);
 
 /* The launch-shape table of the mixed mode (no reference counterpart),
  * written by am-cfd_amd/autotune.py: lines "max_cells graph_block
- * box_blocks", '#' comments; the first row with max_cells >= the grid's
- * interior cells applies, else the built-in 8 and 256.  Both values change
- * the captured loop's speed, never its results.  The library's table is the
+ * box_blocks [box_fused]", '#' comments; the first row with max_cells >= the
+ * grid's interior cells applies, else the built-in 8, 256 and 1.  These
+ * values change the captured loop's speed, never its results.  The library's table is the
  * file named by $AMCFD_TUNING, else tuning.txt beside libamcfd.so, read
  * once.  amcfd_tuning_choice reports the choice for `cells` from the file at
  * `path`, or from the library's table when path is NULL; it needs no GPU. */
-int  amcfd_tuning_choice(const char *path, long long cells, int *graph_block, int *box_blocks);
+int  amcfd_tuning_choice(const char *path, long long cells, int *graph_block, int *box_blocks, int *box_fused);
 
 /* Copy the whole state of src into dst (same deck, any modes): every
  * solution field, the grid, the laser and time-loop state and the scalars,

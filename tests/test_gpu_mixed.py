@@ -200,7 +200,8 @@ def test_solver_takes_its_launch_shapes_from_the_tuning_table(case_dir, amcfd, t
     s = amcfd.Solver(deck, mode=amcfd.MODE_MIXED)
     assert s.graph_settings() == (True,) + amcfd.tuning_choice(cells)
     s.set_graph(True, 16, 128)
-    assert s.graph_settings() == (True, 16, 128)
+    s.set_box_fused(False)
+    assert s.graph_settings() == (True, 16, 128, False)
 
 
 def test_wide_grids_run_without_yz_passes(amcfd, tmp_path):
@@ -218,3 +219,43 @@ def test_wide_grids_run_without_yz_passes(amcfd, tmp_path):
     assert e.value.code == -7
     s.run()
     assert np.isfinite(s.field("temp")).all()
+
+
+FLOW_FIELDS = ("enthalpy", "uvel", "vvel", "wvel", "pp", "pressure")
+
+
+@pytest.mark.parametrize("case,steps", [("tiny", 12), ("tiny_vap", 10), ("tiny_tool", 10), ("tiny_powder", 30)])
+def test_fused_box_passes_equal_the_half_pass_launches(case_dir, amcfd, case, steps):
+    """The melt-pool box passes in one persistent launch per variable give
+    the bits of one launch per red or black half pass, step after step, from
+    the captured loop and from the host loop alike."""
+    deck = amcfd.Deck(case_dir(case))
+    fused = amcfd.Solver(deck, mode=amcfd.MODE_MIXED)
+    fused.set_box_fused(True)
+    split = amcfd.Solver(deck, mode=amcfd.MODE_MIXED)
+    split.set_box_fused(False)
+    host = amcfd.Solver(deck, mode=amcfd.MODE_MIXED)
+    host.set_box_fused(True)
+    host.set_graph(False)
+    for step in range(steps):
+        print("fused box passes, %s step %d" % (case, step + 1), flush=True)
+        for s in (fused, split, host):
+            s.step()
+        assert fused.scalars()["niter"] == split.scalars()["niter"] == host.scalars()["niter"]
+        for f in FLOW_FIELDS:
+            assert np.array_equal(fused.field(f), split.field(f)), (step, f)
+            assert np.array_equal(fused.field(f), host.field(f)), (step, f)
+    assert fused.scalars()["tpeak"] > fused.params.tsolid   # the box passes ran
+
+
+def test_expired_waits_end_a_mixed_step(case_dir, amcfd):
+    """The persistent box passes wait with a bound: with the device error
+    flag raised, every wait gives up at once, every kernel ends and the step
+    returns AMCFD_E_HIP."""
+    s = amcfd.Solver(case_dir("tiny"), mode=amcfd.MODE_MIXED)
+    s.run(max_steps=4)
+    assert s.scalars()["tpeak"] > s.params.tsolid
+    s.test_expire_waits()
+    with pytest.raises(amcfd.AmcfdError) as e:
+        s.step()
+    assert e.value.code == -5
