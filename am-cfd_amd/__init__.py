@@ -40,7 +40,7 @@ EXPORTS = ("amcfd_deck_load", "amcfd_deck_params", "amcfd_deck_free", "amcfd_cre
            "amcfd_kernel_stats", "amcfd_write_vts", "amcfd_write_pvd", "amcfd_append_residuals",
            "amcfd_append_final_geometry", "amcfd_deck_string", "amcfd_free_surface",
            "amcfd_output_section", "amcfd_run_batch", "amcfd_set_mixed_schedule", "amcfd_set_graph", "amcfd_get_graph", "amcfd_set_box_fused",
-           "amcfd_tuning_choice", "amcfd_copy_state", "amcfd_test_expire_waits",
+           "amcfd_tuning_choice", "amcfd_copy_state", "amcfd_test_expire_waits", "amcfd_test_libm",
            "amcfd_tdma_lines", "amcfd_tdma_lines_noalloc",
            "amcfd_ws_create", "amcfd_ws_destroy", "amcfd_ws_attach", "amcfd_ws_counts", "amcfd_ws_train", "amcfd_ws_samples",
            "amcfd_ws_set_factors", "amcfd_ws_save", "amcfd_ws_load", "amcfd_ws_param_count", "amcfd_ws_params",
@@ -153,6 +153,7 @@ def load_library(path=LIB_PATH):
         "amcfd_run_batch": (ip, [ctypes.POINTER(vp), ip, ip, ip, ctypes.POINTER(ip)]),
         "amcfd_set_mixed_schedule": (ip, [vp, cp, ip]),
         "amcfd_test_expire_waits": (ip, [vp]),
+        "amcfd_test_libm": (ip, [ip, dp, dp, ctypes.c_size_t]),
         "amcfd_set_graph": (ip, [vp, ip, ip, ip]),
         "amcfd_get_graph": (ip, [vp] + [ctypes.POINTER(ctypes.c_int)] * 4),
         "amcfd_set_box_fused": (ip, [vp, ip]),
@@ -382,6 +383,18 @@ def tuning_choice(cells, path=None):
     _check(load_library().amcfd_tuning_choice(os.fsencode(path) if path else None, int(cells), ctypes.byref(gb),
                                               ctypes.byref(bb), ctypes.byref(bf)))
     return gb.value, bb.value, bool(bf.value)
+
+
+def device_libm(fn, x):
+    """The device's exp (fn "exp") or log (fn "log") of the float64 array x,
+    as the kernels evaluate them."""
+    import numpy as np
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    y = np.empty_like(x)
+    code = {"exp": 0, "log": 1}[fn]
+    _check(load_library().amcfd_test_libm(code, x.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                                          y.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), x.size))
+    return y
 
 
 def tdma_lines(aw, ap, ae, d, phi, n, nlines, guard=0, stream=None):

@@ -324,6 +324,12 @@ int amcfd_test_expire_waits(amcfd_solver *s) {
     GUARD(s->s.raise_sweep_error());
 }
 
+int amcfd_test_libm(int fn, const double *x, double *y, size_t n) {
+    if ((fn != 0 && fn != 1) || (n && (!x || !y))) return amcfd::set_error(AMCFD_E_ARG, "fn must be 0 or 1, buffers non-null");
+    if (amcfd::eval_libm_device(fn, x, y, n)) return amcfd::set_error(AMCFD_E_HIP, "device evaluation failed");
+    return AMCFD_OK;
+}
+
 int amcfd_ws_attach(amcfd_solver *s, amcfd_ws *ws, int mode) {
     if (!s || (mode && !ws)) return amcfd::set_error(AMCFD_E_ARG, "null argument");
     if (mode < 0 || mode > 3) return amcfd::set_error(AMCFD_E_ARG, "mode must be 0, 1, 2 or 3");

@@ -1254,6 +1254,30 @@ void launch_end_step(hipStream_t st, const Dims &d, DevFields f, const amcfd_par
                        f.vnot, f.wnot, f.hnot, f.fraclnot, f.csfrac, f.solfrac, f.mpphase, f.mpliqphase);
 }
 
+// the device exp (fn 0) or log (fn 1) of n values, for tests against the C
+// library: the kernels evaluate exactly these functions
+__global__ __launch_bounds__(256) void k_libm_eval(int fn, size_t n, const double *__restrict__ x,
+                                                   double *__restrict__ y) {
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) y[t] = fn == 0 ? libm_exp(x[t]) : libm_log(x[t]);
+}
+
+int eval_libm_device(int fn, const double *x, double *y, size_t n) {
+    if (n == 0) return 0;
+    double *dx = nullptr, *dy = nullptr;
+    hipError_t e = hipMalloc((void **)&dx, n * sizeof(double));
+    if (e == hipSuccess) e = hipMalloc((void **)&dy, n * sizeof(double));
+    if (e == hipSuccess) e = hipMemcpy(dx, x, n * sizeof(double), hipMemcpyHostToDevice);
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(k_libm_eval, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, 0, fn, n, dx, dy);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemcpy(y, dy, n * sizeof(double), hipMemcpyDeviceToHost);
+    if (dx) (void)hipFree(dx);
+    if (dy) (void)hipFree(dy);
+    return e == hipSuccess ? 0 : -1;
+}
+
 int energy_partial_sizes(const Dims &d, int *nblk_e, int *nrow_e, int *nblk_t) {
     // sized for the larger of the enthalpy (assemble_grid) and species
     // (energy_grid) assembly partials

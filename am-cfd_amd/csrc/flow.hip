@@ -67,7 +67,7 @@ __device__ double surface_tension_gradient(double t, double a0, double dgdtp) {
     if (a0 < 0) return dgdtp;
     const double A = 4.3e-4, R = 8314, Ts = 2.03e-8, k1 = 1.38e-2, H0 = -1.46e8;
     double K = k1 * libm_exp(-H0 / R / t);
-    double g = (-A - R * Ts * log(1 + K * a0) - (K * a0 / (1 + K * a0)) * (Ts * H0 / t)) * 1;
+    double g = (-A - R * Ts * libm_log(1 + K * a0) - (K * a0 / (1 + K * a0)) * (Ts * H0 / t)) * 1;
     if (t >= 2500) g = 0;
     return g;
 }

@@ -5,6 +5,7 @@
 
 #include "internal.h"
 #include "libm_exp.h"
+#include "libm_log.h"
 
 namespace amcfd {
 

@@ -200,6 +200,9 @@ void launch_line_sweeps(hipStream_t st, const Dims &d, DevFields f, double *phi,
                         const SweepPlan &plan, int ibeg, int iend, int guard, int scan, unsigned long long *counter,
                         unsigned long long counter_base, DevScalars *sc, int nblocks, int shared);
 
+// the device exp (fn 0) or log (fn 1) of n host values (tests); nonzero on a HIP failure
+int eval_libm_device(int fn, const double *x, double *y, size_t n);
+
 // mixed-precision mode (mixed.hip, energy.hip)
 void launch_energy_assemble_mixed(hipStream_t st, const Dims &d, const DevGeom &g, DevFields f, Partials pr,
                                   const MxFields &m, const amcfd_params &p, const SourceParams &sp, const int *skip,

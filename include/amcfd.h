@@ -336,6 +336,14 @@ int  amcfd_copy_state(amcfd_solver *dst, const amcfd_solver *src);
  * returns AMCFD_E_HIP.  The flag stays raised for the solver's life. */
 int  amcfd_test_expire_waits(amcfd_solver *s);
 
+/* Test hook, no reference counterpart: evaluate the device's exp (fn 0) or
+ * log (fn 1), the functions the kernels use where the reference calls the C
+ * library's (std::exp at BoundCondManager.cpp:150, :604, :729 and
+ * CFDSolverManager.cpp:1735, :3100; std::log at BoundCondManager.cpp:730),
+ * on n host values x into y, so that tests can hold them to the C library
+ * bit for bit. */
+int  amcfd_test_libm(int fn, const double *x, double *y, size_t n);
+
 /* Learned warm start (no reference counterpart): a 3-D convolutional
  * encoder-decoder (4 levels, 16-128 channels, 3x3x3 kernels, BF16 on the
  * matrix cores, FP32 accumulation) that predicts a step's temperature
