@@ -39,7 +39,7 @@ EXPORTS = ("amcfd_deck_load", "amcfd_deck_params", "amcfd_deck_free", "amcfd_cre
            "amcfd_get_scalars", "amcfd_dims", "amcfd_last_timing", "amcfd_set_profiling",
            "amcfd_kernel_stats", "amcfd_write_vts", "amcfd_write_pvd", "amcfd_append_residuals",
            "amcfd_append_final_geometry", "amcfd_deck_string", "amcfd_free_surface",
-           "amcfd_output_section", "amcfd_run_batch", "amcfd_set_mixed_schedule", "amcfd_set_graph", "amcfd_get_graph", "amcfd_set_box_fused",
+           "amcfd_output_section", "amcfd_run_batch", "amcfd_run_grouped", "amcfd_set_mixed_schedule", "amcfd_set_graph", "amcfd_get_graph", "amcfd_set_box_fused",
            "amcfd_tuning_choice", "amcfd_copy_state", "amcfd_test_expire_waits", "amcfd_test_libm",
            "amcfd_tdma_lines", "amcfd_tdma_lines_noalloc",
            "amcfd_ws_create", "amcfd_ws_destroy", "amcfd_ws_attach", "amcfd_ws_counts", "amcfd_ws_train", "amcfd_ws_samples",
@@ -151,6 +151,7 @@ def load_library(path=LIB_PATH):
         "amcfd_free_surface": (ip, [vp, cp]),
         "amcfd_output_section": (ip, [vp, cp, ip, ctypes.POINTER(ip)]),
         "amcfd_run_batch": (ip, [ctypes.POINTER(vp), ip, ip, ip, ctypes.POINTER(ip)]),
+        "amcfd_run_grouped": (ip, [ctypes.POINTER(vp), ip, ip, ctypes.POINTER(ip)]),
         "amcfd_set_mixed_schedule": (ip, [vp, cp, ip]),
         "amcfd_test_expire_waits": (ip, [vp]),
         "amcfd_test_libm": (ip, [ip, dp, dp, ctypes.c_size_t]),
@@ -373,6 +374,16 @@ def run_batch(solvers, max_steps=-1, workers=0):
     handles = (ctypes.c_void_p * max(n, 1))(*[s._h.value for s in solvers])
     done = (ctypes.c_int * max(n, 1))()
     _check(load_library().amcfd_run_batch(handles, n, int(max_steps), int(workers), done))
+    return [done[i] for i in range(n)]
+
+
+def run_grouped(solvers, max_steps=-1):
+    """Advance mixed-mode solvers in lock step, each kernel of an iteration
+    one launch over all of them (captured); returns the steps each took."""
+    n = len(solvers)
+    handles = (ctypes.c_void_p * max(n, 1))(*[s._h.value for s in solvers])
+    done = (ctypes.c_int * max(n, 1))()
+    _check(load_library().amcfd_run_grouped(handles, n, int(max_steps), done))
     return [done[i] for i in range(n)]
 
 

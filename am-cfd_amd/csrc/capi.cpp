@@ -191,6 +191,20 @@ int amcfd_run_batch(amcfd_solver *const *s, int n, int max_steps, int workers, i
     return AMCFD_OK;
 }
 
+int amcfd_run_grouped(amcfd_solver *const *s, int n, int max_steps, int *steps_done) {
+    if (n < 0 || (n > 0 && (!s || !steps_done))) return amcfd::set_error(AMCFD_E_ARG, "null argument");
+    std::vector<amcfd::Solver *> v(n);
+    for (int i = 0; i < n; ++i) {
+        if (!s[i]) return amcfd::set_error(AMCFD_E_ARG, "null solver in group");
+        v[i] = &s[i]->s;
+    }
+    std::vector<amcfd::Solver *> seen(v);
+    std::sort(seen.begin(), seen.end());
+    if (std::adjacent_find(seen.begin(), seen.end()) != seen.end())
+        return amcfd::set_error(AMCFD_E_ARG, "the same solver appears twice in the group");
+    GUARD(amcfd::run_grouped(v, max_steps, steps_done));
+}
+
 int amcfd_get_field(amcfd_solver *s, const char *name, double *host, size_t n) {
     if (!s || !name || !host) return amcfd::set_error(AMCFD_E_ARG, "null argument");
     GUARD(s->s.get_field(name, host, n));

@@ -29,7 +29,9 @@ struct BcConst {
     int isevap;
 };
 
-__global__ __launch_bounds__(256) void k_bc_enthalpy(Dims d, BcConst k, const double *__restrict__ dxpwinv,
+struct BcEnthalpyK {
+    static constexpr int family = 1, rank = 0;
+    static __device__ __forceinline__ void run(const VB &vb, Dims d, BcConst k, const double *__restrict__ dxpwinv,
                                                      const double *__restrict__ dypsinv,
                                                      const double *__restrict__ dzpbinv,
                                                      const double *__restrict__ areaij,
@@ -40,7 +42,7 @@ __global__ __launch_bounds__(256) void k_bc_enthalpy(Dims d, BcConst k, const do
     __shared__ double scratch[8];
     if (skip && *skip) return;
     const int ntop = d.njm2 * d.nim2, new_ = d.nkm2 * d.njm2, nns = d.nkm2 * d.nim2;
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int t = vb.x * blockDim.x + threadIdx.x;
     double loss = 0.0;
     if (t < ntop) {
         int j = 1 + t / d.nim2, i = 1 + t % d.nim2;
@@ -86,8 +88,19 @@ __global__ __launch_bounds__(256) void k_bc_enthalpy(Dims d, BcConst k, const do
     if (lane == 0) {
         double s = 0.0;
         for (int w = 0; w < (int)(blockDim.x >> 6); ++w) s += scratch[w];
-        partial[blockIdx.x] = s;
+        partial[vb.x] = s;
     }
+}
+};
+__global__ __launch_bounds__(256) void k_bc_enthalpy(Dims d, BcConst k, const double *__restrict__ dxpwinv,
+                                                     const double *__restrict__ dypsinv,
+                                                     const double *__restrict__ dzpbinv,
+                                                     const double *__restrict__ areaij,
+                                                     const double *__restrict__ heatin,
+                                                     const double *__restrict__ temp,
+                                                     const double *__restrict__ diff, double *h,
+                                                     double *partial, const int *skip) {
+    BcEnthalpyK::run(vb_hw(), d, k, dxpwinv, dypsinv, dzpbinv, areaij, heatin, temp, diff, h, partial, skip);
 }
 
 // ------------------------------------------------------------------------
@@ -350,8 +363,9 @@ __device__ __forceinline__ float hmean_f32(float p, float q, float wq, float wp)
 }
 
 template <int SCH>
-__global__ __launch_bounds__(MTX *MTY) __attribute__((amdgpu_waves_per_eu(MX_ASM_WAVES, 8))) void
-k_energy_assemble_mx(Dims d, AsmConst q, DevGeom g, const double *__restrict__ uvel, const double *__restrict__ vvel,
+struct AssembleMxK {
+    static constexpr int family = 20 + SCH, rank = 0;
+    static __device__ __forceinline__ void run(const VB &vb, Dims d, AsmConst q, DevGeom g, const double *__restrict__ uvel, const double *__restrict__ vvel,
                      const double *__restrict__ wvel, const double *__restrict__ rho,
                      const double *__restrict__ diff, const double *__restrict__ hnot,
                      const double *__restrict__ fracl, const double *__restrict__ fraclnot,
@@ -367,9 +381,11 @@ k_energy_assemble_mx(Dims d, AsmConst q, DevGeom g, const double *__restrict__ u
     __shared__ double s_blk[4 * MTY * MTX];
     double cbl = 0.0, cblp = 0.0, cblm = 0.0, cblc = 0.0;   // this column's block-correction partials
     const int tx = threadIdx.x, ty = threadIdx.y;
-    const int gx = gridDim.x, gy = gridDim.y;
-    const int nb = gx * gy * gridDim.z;
-    const int t = xcd_tile(blockIdx.x + gx * (blockIdx.y + gy * blockIdx.z), nb);
+    const int gx = vb.gx, gy = vb.gy;
+    const int nb = gx * gy * vb.gz;
+    // a grouped launch has renumbered its blocks for the XCDs already
+    const int lin = vb.x + gx * (vb.y + gy * vb.z);
+    const int t = vb.hw ? xcd_tile(lin, nb) : lin;
     const int bx = t % gx, by = (t / gx) % gy, bz = t / (gx * gy);
     const int i = 1 + bx * MTX + tx;
     const int j = 1 + by * MTY + ty;
@@ -484,7 +500,7 @@ k_energy_assemble_mx(Dims d, AsmConst q, DevGeom g, const double *__restrict__ u
     s_blk[(3 * MTY + ty) * MTX + tx] = cblc;
     __syncthreads();
     if (ty < 4 && i <= d.nim2) {
-        const int ntile = gy * gridDim.z, tile = by + gy * bz;
+        const int ntile = gy * vb.gz, tile = by + gy * bz;
         const int qq = ty;   // one quantity per j-row of the first four
         double a = 0.0;
         for (int r = 0; r < MTY; ++r) a += s_blk[(qq * MTY + r) * MTX + tx];
@@ -497,6 +513,16 @@ k_energy_assemble_mx(Dims d, AsmConst q, DevGeom g, const double *__restrict__ u
         o.part_res[2 * t + 1] = v[1];
     }
 }
+};
+template <int SCH>
+__global__ __launch_bounds__(MTX *MTY) __attribute__((amdgpu_waves_per_eu(MX_ASM_WAVES, 8))) void k_energy_assemble_mx(Dims d, AsmConst q, DevGeom g, const double *__restrict__ uvel, const double *__restrict__ vvel,
+                     const double *__restrict__ wvel, const double *__restrict__ rho,
+                     const double *__restrict__ diff, const double *__restrict__ hnot,
+                     const double *__restrict__ fracl, const double *__restrict__ fraclnot,
+                     const double *__restrict__ h, AsmOut o, const int *__restrict__ skip,
+                     const StepDev *__restrict__ step) {
+    AssembleMxK<SCH>::run(vb_hw(), d, q, g, uvel, vvel, wvel, rho, diff, hnot, fracl, fraclnot, h, o, skip, step);
+}
 
 // ------------------------------------------------------------------------
 // phase-wise diffusivity / density / viscosity, CFDSolverManager.cpp:2137-2265
@@ -507,12 +533,14 @@ struct PropConst {
     int powder;
 };
 
-__global__ __launch_bounds__(256) void k_props(size_t n, PropConst c, const double *__restrict__ temp,
+struct PropsK {
+    static constexpr int family = 3, rank = 0;
+    static __device__ __forceinline__ void run(const VB &vb, size_t n, PropConst c, const double *__restrict__ temp,
                                                const double *__restrict__ fracl,
                                                const double *__restrict__ csfrac, double *diff, double *rho,
                                                double *vis, const int *skip) {
     if (skip && *skip) return;
-    size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    size_t t = (size_t)vb.x * blockDim.x + threadIdx.x;
     if (t >= n) return;
     double T = temp[t];
     double diffl = (c.kla * T + c.klb) / c.acpl;
@@ -543,6 +571,13 @@ __global__ __launch_bounds__(256) void k_props(size_t n, PropConst c, const doub
     diff[t] = dv;
     rho[t] = rv;
     vis[t] = vv;
+}
+};
+__global__ __launch_bounds__(256) void k_props(size_t n, PropConst c, const double *__restrict__ temp,
+                                               const double *__restrict__ fracl,
+                                               const double *__restrict__ csfrac, double *diff, double *rho,
+                                               double *vis, const int *skip) {
+    PropsK::run(vb_hw(), n, c, temp, fracl, csfrac, diff, rho, vis, skip);
 }
 
 // ------------------------------------------------------------------------
@@ -676,7 +711,9 @@ struct H2tConst {
     int steady;
 };
 
-__global__ __launch_bounds__(TX *TY) void k_h2t(Dims d, H2tConst q, DevGeom g, const double *__restrict__ h,
+struct H2tK {
+    static constexpr int family = 4, rank = 0;
+    static __device__ __forceinline__ void run(const VB &vb, Dims d, H2tConst q, DevGeom g, const double *__restrict__ h,
                                                  const double *__restrict__ hnot,
                                                  const double *__restrict__ fraclnot,
                                                  const double *__restrict__ rho,
@@ -686,14 +723,14 @@ __global__ __launch_bounds__(TX *TY) void k_h2t(Dims d, H2tConst q, DevGeom g, c
     __shared__ double scratch[10 * 4];
     if (skip && *skip) return;
     const int tx = threadIdx.x, ty = threadIdx.y;
-    const int i = blockIdx.x * TX + tx;
-    const int j = blockIdx.y * TY + ty;
+    const int i = vb.x * TX + tx;
+    const int j = vb.y * TY + ty;
     double tmax = -INFINITY;
     double acc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};   // accul heatvol W E B T S N
     const double dT = q.tliquid - q.tsolid;
     if (i < d.ni && j < d.nj) {
         for (int kk = 0; kk < KB; ++kk) {
-            const int k = blockIdx.z * KB + kk;
+            const int k = vb.z * KB + kk;
             if (k >= d.nk) break;
             const size_t c = cix(d, i, j, k);
             const double hv = h[c];
@@ -739,7 +776,7 @@ __global__ __launch_bounds__(TX *TY) void k_h2t(Dims d, H2tConst q, DevGeom g, c
     }
     __syncthreads();
     if (lane == 0) {
-        const int b = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
+        const int b = vb.x + vb.gx * (vb.y + vb.gy * vb.z);
         double m = scratch[0];
         for (int w = 1; w < nw; ++w) m = dmax(m, scratch[w]);
         partial[(size_t)b * 10] = m;
@@ -749,6 +786,16 @@ __global__ __launch_bounds__(TX *TY) void k_h2t(Dims d, H2tConst q, DevGeom g, c
             partial[(size_t)b * 10 + 1 + qq] = s;
         }
     }
+}
+};
+__global__ __launch_bounds__(TX *TY) void k_h2t(Dims d, H2tConst q, DevGeom g, const double *__restrict__ h,
+                                                 const double *__restrict__ hnot,
+                                                 const double *__restrict__ fraclnot,
+                                                 const double *__restrict__ rho,
+                                                 const double *__restrict__ diff,
+                                                 const double *__restrict__ src, double *temp, double *fracl,
+                                                 double *partial, const int *skip) {
+    H2tK::run(vb_hw(), d, q, g, h, hnot, fraclnot, rho, diff, src, temp, fracl, partial, skip);
 }
 
 // ------------------------------------------------------------------------
@@ -773,7 +820,9 @@ __device__ int block_max_int(int v, int *s) {
     return s[0];
 }
 
-__global__ __launch_bounds__(1024) void k_melt_pool(Dims d, int nblk, const double *__restrict__ partial,
+struct MeltPoolK {
+    static constexpr int family = 5, rank = 0;
+    static __device__ __forceinline__ void run(const VB &vb, Dims d, int nblk, const double *__restrict__ partial,
                                                     const double *__restrict__ temp,
                                                     const double *__restrict__ x, const double *__restrict__ y,
                                                     const double *__restrict__ z, double ts, int istart,
@@ -920,12 +969,23 @@ __global__ __launch_bounds__(1024) void k_melt_pool(Dims d, int nblk, const doub
         sc->kstat = max(kmin - 2, 2);
     }
 }
+};
+__global__ __launch_bounds__(1024) void k_melt_pool(Dims d, int nblk, const double *__restrict__ partial,
+                                                    const double *__restrict__ temp,
+                                                    const double *__restrict__ x, const double *__restrict__ y,
+                                                    const double *__restrict__ z, double ts, int istart,
+                                                    int jstart, DevScalars *sc, int ctl, int navierstokes,
+                                                    const StepDev *step) {
+    MeltPoolK::run(vb_hw(), d, nblk, partial, temp, x, y, z, ts, istart, jstart, sc, ctl, navierstokes, step);
+}
 
 // ------------------------------------------------------------------------
 // end-of-iteration scalars: residual folds, heat balance, energy ratio
 // (CFDSolverManager.cpp:1989-2071, BoundCondManager.cpp:846-857)
 // ------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_finalize(int nblk_t, int nblk_b, int nblk_f, int nblk_e, int nblk_c,
+struct FinalizeK {
+    static constexpr int family = 6, rank = 0;
+    static __device__ __forceinline__ void run(const VB &vb, int nblk_t, int nblk_b, int nblk_f, int nblk_e, int nblk_c,
                                                   const double *__restrict__ part_t,
                                                   const double *__restrict__ part_b,
                                                   const double *__restrict__ part_f,
@@ -1012,6 +1072,16 @@ __global__ __launch_bounds__(256) void k_finalize(int nblk_t, int nblk_b, int nb
     }
     if (species_done) sc->resorc = r[15] / (r[16] + AMCFD_SMALL);
 }
+};
+__global__ __launch_bounds__(256) void k_finalize(int nblk_t, int nblk_b, int nblk_f, int nblk_e, int nblk_c,
+                                                  const double *__restrict__ part_t,
+                                                  const double *__restrict__ part_b,
+                                                  const double *__restrict__ part_f,
+                                                  const double *__restrict__ part_e,
+                                                  const double *__restrict__ part_c, int flow_done, int species_done,
+                                                  int steady, double dens, double width, DevScalars *sc, int ctl) {
+    FinalizeK::run(vb_hw(), nblk_t, nblk_b, nblk_f, nblk_e, nblk_c, part_t, part_b, part_f, part_e, part_c, flow_done, species_done, steady, dens, width, sc, ctl);
+}
 
 // ------------------------------------------------------------------------
 // end-of-step push-back, CFDSolverManager.cpp:2625-2687
@@ -1055,7 +1125,7 @@ void launch_bc_enthalpy(hipStream_t st, const Dims &d, const DevGeom &g, DevFiel
               p.evaptemp, p.evapfact, p.molarmass, p.isevap};
     int n = d.njm2 * d.nim2 + d.nkm2 * d.njm2 + d.nkm2 * d.nim2;
     int nb = (n + 255) / 256;
-    hipLaunchKernelGGL(k_bc_enthalpy, dim3(nb), dim3(256), 0, st, d, k, g.dxpwinv, g.dypsinv, g.dzpbinv,
+    dispatch<BcEnthalpyK, 256>(k_bc_enthalpy, dim3(nb), dim3(256), 0, st, d, k, g.dxpwinv, g.dypsinv, g.dzpbinv,
                        g.areaij, f.heatin, f.temp, f.diff, f.enthalpy, pr.bc, skip);
 }
 
@@ -1078,22 +1148,47 @@ __global__ __launch_bounds__(256) void k_laser_map(int ni, int n, const double *
 }
 
 // heatinlaser = sum of areaij * heatin in the reference's (j, i) order
-// (BoundCondManager.cpp:158): one dependent add per cell, loads batched
-__global__ __launch_bounds__(64) void k_laser_total(int n, const double *__restrict__ contrib,
-                                                    DevScalars *sc) {
-    if (threadIdx.x != 0) return;
-    constexpr int U = 32;
+// (BoundCondManager.cpp:158): one dependent add per cell.  The chain is kept
+// fed from LDS: wave 1 streams chunks of the values into a double buffer
+// (coalesced loads, many in flight) while lane 0 of wave 0 adds the chunk
+// before it, so the chain waits on LDS reads issued ahead of it instead of
+// on memory round trips.
+constexpr int LT_CHUNK = 1024;
+__global__ __launch_bounds__(128) void k_laser_total(int n, const double *__restrict__ contrib,
+                                                     DevScalars *sc) {
+    __shared__ double buf[2][LT_CHUNK];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int nch = (n + LT_CHUNK - 1) / LT_CHUNK;
+    auto stage = [&](int ch) {
+        double *b = buf[ch & 1];
+#pragma unroll
+        for (int q = 0; q < LT_CHUNK / 64; ++q) {
+            const int r = q * 64 + lane, c = ch * LT_CHUNK + r;
+            b[r] = c < n ? contrib[c] : 0.0;
+        }
+    };
+    if (wave == 1 && nch > 0) stage(0);
+    __syncthreads();
     double total = 0.0;
-    int c = 0;
-    for (; c + U <= n; c += U) {
-        double v[U];
+    for (int ch = 0; ch < nch; ++ch) {
+        if (wave == 1) {
+            if (ch + 1 < nch) stage(ch + 1);
+        } else if (lane == 0) {
+            const double *b = buf[ch & 1];
+            const int m = min(LT_CHUNK, n - ch * LT_CHUNK);
+            int r = 0;
+            for (; r + 8 <= m; r += 8) {
+                double v[8];
 #pragma unroll
-        for (int r = 0; r < U; ++r) v[r] = contrib[c + r];
+                for (int u = 0; u < 8; ++u) v[u] = b[r + u];
 #pragma unroll
-        for (int r = 0; r < U; ++r) total += v[r];
+                for (int u = 0; u < 8; ++u) total += v[u];
+            }
+            for (; r < m; ++r) total += b[r];
+        }
+        __syncthreads();
     }
-    for (; c < n; ++c) total += contrib[c];
-    sc->heatinlaser = total;
+    if (threadIdx.x == 0) sc->heatinlaser = total;
 }
 
 void launch_laser_map(hipStream_t st, const Dims &d, const DevGeom &g, DevFields f, Partials pr,
@@ -1101,7 +1196,7 @@ void launch_laser_map(hipStream_t st, const Dims &d, const DevGeom &g, DevFields
     const int n = d.ni * d.nj;
     hipLaunchKernelGGL(k_laser_map, dim3((n + 255) / 256), dim3(256), 0, st, d.ni, n, g.x, g.y, g.areaij, q,
                        f.heatin, pr.laser);
-    hipLaunchKernelGGL(k_laser_total, dim3(1), dim3(64), 0, st, n, pr.laser, sc);
+    hipLaunchKernelGGL(k_laser_total, dim3(1), dim3(128), 0, st, n, pr.laser, sc);
 }
 
 static dim3 energy_grid(const Dims &d) {
@@ -1136,11 +1231,14 @@ void launch_energy_assemble_mixed(hipStream_t st, const Dims &d, const DevGeom &
                s.beamposx, s.beamposy, s.powerindicator, p.discretization, mixed_has_volumetric_source(p)};
     AsmOut o{nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, m.su, f.sourceinput, pr.energy,
              nullptr, nullptr, m.an, m.as, m.ae, m.aw, m.at, m.ab, m.ap, m.blkpart};
-    auto kern = p.discretization == AMCFD_DISC_POWERLAW     ? k_energy_assemble_mx<AMCFD_DISC_POWERLAW>
-                : p.discretization == AMCFD_DISC_EXPONENTIAL ? k_energy_assemble_mx<AMCFD_DISC_EXPONENTIAL>
-                                                             : k_energy_assemble_mx<AMCFD_DISC_UPWIND>;
-    hipLaunchKernelGGL(kern, assemble_grid_mx(d), dim3(MTX, MTY), 0, st, d, q, g, f.uvel, f.vvel, f.wvel, f.rho,
-                       f.diff, f.hnot, f.fracl, f.fraclnot, f.enthalpy, o, skip, step);
+#define ASM(SCH)                                                                                                   \
+    dispatch<AssembleMxK<SCH>, MTX * MTY>(k_energy_assemble_mx<SCH>, assemble_grid_mx(d), dim3(MTX, MTY), 0, st, d, q, \
+                                           g, f.uvel, f.vvel, f.wvel, f.rho, f.diff, f.hnot, f.fracl, f.fraclnot,      \
+                                           f.enthalpy, o, skip, step)
+    if (p.discretization == AMCFD_DISC_POWERLAW) ASM(AMCFD_DISC_POWERLAW);
+    else if (p.discretization == AMCFD_DISC_EXPONENTIAL) ASM(AMCFD_DISC_EXPONENTIAL);
+    else ASM(AMCFD_DISC_UPWIND);
+#undef ASM
 }
 
 // A fixed volumetric source of 0 W (finite factors, positive radius and
@@ -1164,7 +1262,7 @@ void launch_props(hipStream_t st, const Dims &d, DevFields f, const amcfd_params
                 p.thconsb, p.thconsc, p.thconla, p.thconlb, p.powderden, p.powdercpa, p.powdercpb,
                 p.powderthcona, p.powderthconb, p.powderbed};
     size_t n = (size_t)d.ni * d.nj * d.nk;
-    hipLaunchKernelGGL(k_props, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, n, c, f.temp, f.fracl,
+    dispatch<PropsK, 256>(k_props, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, n, c, f.temp, f.fracl,
                        f.csfrac, f.diff, f.rho, f.vis, skip);
 }
 
@@ -1185,7 +1283,7 @@ static dim3 node_grid(const Dims &d) {
 void launch_h2t(hipStream_t st, const Dims &d, const DevGeom &g, DevFields f, Partials pr, const amcfd_params &p,
                 const int *skip, bool src_zero) {
     H2tConst q{p.hlcal, p.hsmelt, p.acpa, p.acpb, p.acpl, p.tliquid, p.tsolid, p.hlatnt, p.delt, p.steady};
-    hipLaunchKernelGGL(k_h2t, node_grid(d), dim3(TX, TY), 0, st, d, q, g, f.enthalpy, f.hnot, f.fraclnot, f.rho,
+    dispatch<H2tK, TX * TY>(k_h2t, node_grid(d), dim3(TX, TY), 0, st, d, q, g, f.enthalpy, f.hnot, f.fraclnot, f.rho,
                        f.diff, src_zero ? nullptr : f.sourceinput, f.temp, f.fracl, pr.h2t, skip);
 }
 
@@ -1194,7 +1292,7 @@ void launch_melt_pool(hipStream_t st, const Dims &d, const DevGeom &g, DevFields
                       const StepDev *step) {
     dim3 gr = node_grid(d);
     int nblk = gr.x * gr.y * gr.z;
-    hipLaunchKernelGGL(k_melt_pool, dim3(1), dim3(1024), 0, st, d, nblk, pr.h2t, f.temp, g.x, g.y, g.z, p.tsolid,
+    dispatch<MeltPoolK, 1024>(k_melt_pool, dim3(1), dim3(1024), 0, st, d, nblk, pr.h2t, f.temp, g.x, g.y, g.z, p.tsolid,
                        istart, jstart, sc, (int)ctl, p.navierstokes, step);
 }
 
@@ -1204,7 +1302,7 @@ void launch_finalize(hipStream_t st, const Dims &d, Partials pr, DevScalars *sc,
     int nblk_t = gr.x * gr.y * gr.z;
     const int nblk_b = (d.njm2 * d.nim2 + d.nkm2 * d.njm2 + d.nkm2 * d.nim2 + 255) / 256;
     const dim3 e = assemble_grid(d), c = energy_grid(d);   // enthalpy and species assembly partials
-    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, st, nblk_t, nblk_b, pr.nblk_f, (int)(e.x * e.y * e.z),
+    dispatch<FinalizeK, 256>(k_finalize, dim3(1), dim3(256), 0, st, nblk_t, nblk_b, pr.nblk_f, (int)(e.x * e.y * e.z),
                        (int)(c.x * c.y * c.z), pr.h2t, pr.bc, pr.flow, pr.energy, pr.species, flow_done, species_done, steady, p.dens,
                        width, sc, (int)ctl);
 }
@@ -1212,7 +1310,9 @@ void launch_finalize(hipStream_t st, const Dims &d, Partials pr, DevScalars *sc,
 // the convergence test of CFDSolverManager.cpp:690-716 on the device: the
 // first largest of resorm, resoru, resorv, resorw; the heat-balance band with
 // the laser on (or in steady mode), resorh < 1e-6 with it off
-__global__ void k_converge(DevScalars *sc, const StepDev *step, int steady, double tol) {
+struct ConvergeK {
+    static constexpr int family = 7, rank = 0;
+    static __device__ __forceinline__ void run(const VB &vb, DevScalars *sc, const StepDev *step, int steady, double tol) {
     if (sc->done) return;
     sc->niter += 1;
     double res[4] = {sc->resorm, sc->resoru, sc->resorv, sc->resorw};
@@ -1230,6 +1330,10 @@ __global__ void k_converge(DevScalars *sc, const StepDev *step, int steady, doub
         sc->noflow = 1;
     }
 }
+};
+__global__ void k_converge(DevScalars *sc, const StepDev *step, int steady, double tol) {
+    ConvergeK::run(vb_hw(), sc, step, steady, tol);
+}
 
 __global__ void k_loop_reset(DevScalars *sc) {
     sc->niter = 0;
@@ -1238,8 +1342,20 @@ __global__ void k_loop_reset(DevScalars *sc) {
     sc->noflow = 1;
 }
 
+// grouped runs: whether every instance's loop is done, into host memory
+__global__ __launch_bounds__(64) void k_group_done(int *const *__restrict__ done, int n, int *out) {
+    int all = 1;
+    for (int i = threadIdx.x; i < n; i += 64) all &= *done[i] != 0;
+    all = __all(all);
+    if (threadIdx.x == 0) *out = all;
+}
+
+void launch_group_done(hipStream_t st, int *const *done, int n, int *out) {
+    hipLaunchKernelGGL(k_group_done, dim3(1), dim3(64), 0, st, done, n, out);
+}
+
 void launch_converge(hipStream_t st, DevScalars *sc, const StepDev *step, const amcfd_params &p) {
-    hipLaunchKernelGGL(k_converge, dim3(1), dim3(1), 0, st, sc, step, p.steady, p.nonlintol);
+    dispatch<ConvergeK, 64>(k_converge, dim3(1), dim3(1), 0, st, sc, step, p.steady, p.nonlintol);
 }
 
 void launch_loop_reset(hipStream_t st, DevScalars *sc) {

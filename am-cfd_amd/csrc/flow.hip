@@ -39,17 +39,17 @@ __device__ __forceinline__ BoxTiles box_tiles(const Dims &d, BoxDev b, const int
     t.n = (ni_b < 1 || nj_b < 1 || nk_b < 1) ? 0 : t.gx * t.gy * nk_b;
     return t;
 }
-__device__ __forceinline__ int block_lin() { return blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z); }
-__device__ __forceinline__ int grid_size() { return gridDim.x * gridDim.y * gridDim.z; }
 
 // zero face velocities next to solid cells, CFDSolverManager.cpp:2577-2620
-__global__ __launch_bounds__(TX *TY) void k_clean(Dims d, BoxDev b0, double ts, const double *__restrict__ temp,
+struct CleanK {
+    static constexpr int family = 15, rank = 0;
+    static __device__ __forceinline__ void run(const VB &vb, Dims d, BoxDev b0, double ts, const double *__restrict__ temp,
                                                    double *u, double *v, double *w, const int *dbox,
                                                    const int *skip) {
     if (skip && *skip) return;
     const BoxTiles bt = box_tiles(d, b0, dbox);
     const BoxDev b = bt.b;
-    for (int t = block_lin(); t < bt.n; t += grid_size()) {
+    for (int t = vb.lin(); t < bt.n; t += vb.count()) {
         const int i = b.istat + 1 + (t % bt.gx) * TX + threadIdx.x;
         const int j = b.jstat + ((t / bt.gx) % bt.gy) * TY + threadIdx.y;
         const int k = b.kstat + t / (bt.gx * bt.gy);
@@ -60,6 +60,12 @@ __global__ __launch_bounds__(TX *TY) void k_clean(Dims d, BoxDev b0, double ts, 
         if (dmin(tP, temp[c + d.sj]) <= ts) v[c + d.sj] = 0.0;
         if (dmin(tP, temp[c + d.sk]) <= ts) w[c + d.sk] = 0.0;
     }
+}
+};
+__global__ __launch_bounds__(TX *TY) void k_clean(Dims d, BoxDev b0, double ts, const double *__restrict__ temp,
+                                                   double *u, double *v, double *w, const int *dbox,
+                                                   const int *skip) {
+    CleanK::run(vb_hw(), d, b0, ts, temp, u, v, w, dbox, skip);
 }
 
 // Fe-O surfactant surface-tension gradient, BoundCondManager.cpp:706-741
@@ -74,7 +80,9 @@ __device__ double surface_tension_gradient(double t, double a0, double dgdtp) {
 
 // Marangoni top ghost (ivar 1: u, 2: v) and zeroed box sides (ivar 1..4),
 // BoundCondManager.cpp:472-564
-__global__ __launch_bounds__(256) void k_bc_flow(Dims d, BoxDev b0, int ivar, double a_oxygen, double dgdtp,
+struct BcFlowK {
+    static constexpr int family = 16, rank = 0;
+    static __device__ __forceinline__ void run(const VB &vb, Dims d, BoxDev b0, int ivar, double a_oxygen, double dgdtp,
                                                  const double *__restrict__ dxpwinv,
                                                  const double *__restrict__ dypsinv,
                                                  const double *__restrict__ dzpbinv,
@@ -90,7 +98,7 @@ __global__ __launch_bounds__(256) void k_bc_flow(Dims d, BoxDev b0, int ivar, do
     if (ni_b < 1 || nj_b < 1) return;
     const int ntop = (ivar <= 2) ? ni_b * nj_b : 0;
     const int nall = ntop + (nk_b > 0 ? nk_b * nj_b : 0);
-    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < nall; t += gridDim.x * blockDim.x) {
+    for (int t = vb.x * blockDim.x + threadIdx.x; t < nall; t += vb.gx * blockDim.x) {
     if (t < ntop) {
         const int j = b.jstat + t / ni_b, i = b.istat + 1 + t % ni_b;
         const size_t top = cix(d, i, j, d.nkm1), below = top - d.sk;
@@ -118,6 +126,19 @@ __global__ __launch_bounds__(256) void k_bc_flow(Dims d, BoxDev b0, int ivar, do
     }
     }
 }
+};
+__global__ __launch_bounds__(256) void k_bc_flow(Dims d, BoxDev b0, int ivar, double a_oxygen, double dgdtp,
+                                                 const double *__restrict__ dxpwinv,
+                                                 const double *__restrict__ dypsinv,
+                                                 const double *__restrict__ dzpbinv,
+                                                 const double *__restrict__ fracx,
+                                                 const double *__restrict__ fracy,
+                                                 const double *__restrict__ temp,
+                                                 const double *__restrict__ fracl,
+                                                 const double *__restrict__ vis, double *phi, const int *dbox,
+                                                 const int *skip) {
+    BcFlowK::run(vb_hw(), d, b0, ivar, a_oxygen, dgdtp, dxpwinv, dypsinv, dzpbinv, fracx, fracy, temp, fracl, vis, phi, dbox, skip);
+}
 
 struct MomConst {
     double delt, tsolid, urf, darcy, grav, beta, scanvel;
@@ -131,7 +152,9 @@ struct MomConst {
 //   calculateResiduals(1/2/3)  CFDSolverManager.cpp:1960-2046
 // IVAR selects the staggering: 1 = u (x faces), 2 = v (y faces), 3 = w (z faces).
 template <int IVAR>
-__global__ __launch_bounds__(TX *TY) void k_momentum(Dims d, BoxDev b0, MomConst q, DevGeom g,
+struct MomentumK {
+    static constexpr int family = 16 + IVAR, rank = 0;
+    static __device__ __forceinline__ void run(const VB &vb, Dims d, BoxDev b0, MomConst q, DevGeom g,
                                                      const double *__restrict__ U, const double *__restrict__ Vv,
                                                      const double *__restrict__ W, const double *__restrict__ V,
                                                      const double *__restrict__ rho,
@@ -148,7 +171,7 @@ __global__ __launch_bounds__(TX *TY) void k_momentum(Dims d, BoxDev b0, MomConst
     const BoxDev b = bt.b;
     const int sch = q.scheme;
     double resid = 0.0, umax = 0.0;
-    for (int tt = block_lin(); tt < bt.n; tt += grid_size()) {
+    for (int tt = vb.lin(); tt < bt.n; tt += vb.count()) {
     const int i = b.istat + 1 + (tt % bt.gx) * TX + threadIdx.x;
     const int j = b.jstat + ((tt / bt.gx) % bt.gy) * TY + threadIdx.y;
     const int k = b.kstat + tt / (bt.gx * bt.gy);
@@ -231,7 +254,7 @@ __global__ __launch_bounds__(TX *TY) void k_momentum(Dims d, BoxDev b0, MomConst
                          ab * phi[c - sk] + su - ap * phi[c];
             resid += fabs(res);
             // first tile: plain assignment, as with one tile per workgroup
-            if (k == d.nkm2) umax = (tt == block_lin()) ? fabs(U[c + sk]) : dmax(umax, fabs(U[c + sk]));
+            if (k == d.nkm2) umax = (tt == vb.lin()) ? fabs(U[c + sk]) : dmax(umax, fabs(U[c + sk]));
         } else if (IVAR == 2) {
             const double fy = g.fracy[j - 1];
             double vn = (Vv[c] + Vv[c + sj]) * 0.5, vs = (Vv[c] + Vv[c - sj]) * 0.5;
@@ -385,7 +408,7 @@ __global__ __launch_bounds__(TX *TY) void k_momentum(Dims d, BoxDev b0, MomConst
     }
     __syncthreads();
     if (lane == 0) {
-        const int bid = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
+        const int bid = vb.x + vb.gx * (vb.y + vb.gy * vb.z);
         double s = 0.0, m = 0.0;
         for (int w = 0; w < 4; ++w) { s += scratch[w]; m = dmax(m, scratch[4 + w]); }
         partial[(size_t)bid * 3] = s;
@@ -393,10 +416,27 @@ __global__ __launch_bounds__(TX *TY) void k_momentum(Dims d, BoxDev b0, MomConst
         partial[(size_t)bid * 3 + 2] = 0.0;
     }
 }
+};
+template <int IVAR>
+__global__ __launch_bounds__(TX *TY) void k_momentum(Dims d, BoxDev b0, MomConst q, DevGeom g,
+                                                     const double *__restrict__ U, const double *__restrict__ Vv,
+                                                     const double *__restrict__ W, const double *__restrict__ V,
+                                                     const double *__restrict__ rho,
+                                                     const double *__restrict__ pressure,
+                                                     const double *__restrict__ phinot,
+                                                     const double *__restrict__ fracl,
+                                                     const double *__restrict__ temp, double *an_o, double *as_o,
+                                                     double *ae_o, double *aw_o, double *at_o, double *ab_o,
+                                                     double *ap_o, double *su_o, double *dco, double *partial,
+                                                     const int *dbox, const int *skip) {
+    MomentumK<IVAR>::run(vb_hw(), d, b0, q, g, U, Vv, W, V, rho, pressure, phinot, fracl, temp, an_o, as_o, ae_o, aw_o, at_o, ab_o, ap_o, su_o, dco, partial, dbox, skip);
+}
 
 // pressure-correction coefficients, mass imbalance, solid pinning and the
 // continuity normaliser: CFDSolverManager.cpp:1112-1156, 1645-1676, 2047-2071
-__global__ __launch_bounds__(TX *TY) void k_pressure(Dims d, BoxDev b0, double tsolid, DevGeom g,
+struct PressureK {
+    static constexpr int family = 20, rank = 0;
+    static __device__ __forceinline__ void run(const VB &vb, Dims d, BoxDev b0, double tsolid, DevGeom g,
                                                      const double *__restrict__ U, const double *__restrict__ Vv,
                                                      const double *__restrict__ W, const double *__restrict__ rho,
                                                      const double *__restrict__ dux, const double *__restrict__ dvy,
@@ -410,7 +450,7 @@ __global__ __launch_bounds__(TX *TY) void k_pressure(Dims d, BoxDev b0, double t
     const BoxTiles bt = box_tiles(d, b0, dbox);
     const BoxDev b = bt.b;
     double dsum = 0.0, msum = 0.0;
-    for (int tt = block_lin(); tt < bt.n; tt += grid_size()) {
+    for (int tt = vb.lin(); tt < bt.n; tt += vb.count()) {
     const int i = b.istat + 1 + (tt % bt.gx) * TX + threadIdx.x;
     const int j = b.jstat + ((tt / bt.gx) % bt.gy) * TY + threadIdx.y;
     const int k = b.kstat + tt / (bt.gx * bt.gy);
@@ -448,7 +488,7 @@ __global__ __launch_bounds__(TX *TY) void k_pressure(Dims d, BoxDev b0, double t
     }
     __syncthreads();
     if (lane == 0) {
-        const int bid = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
+        const int bid = vb.x + vb.gx * (vb.y + vb.gy * vb.z);
         double s = 0.0, m = 0.0;
         for (int w = 0; w < 4; ++w) { s += scratch[w]; m += scratch[4 + w]; }
         partial[(size_t)bid * 3] = 0.0;
@@ -456,9 +496,23 @@ __global__ __launch_bounds__(TX *TY) void k_pressure(Dims d, BoxDev b0, double t
         partial[(size_t)bid * 3 + 2] = m;
     }
 }
+};
+__global__ __launch_bounds__(TX *TY) void k_pressure(Dims d, BoxDev b0, double tsolid, DevGeom g,
+                                                     const double *__restrict__ U, const double *__restrict__ Vv,
+                                                     const double *__restrict__ W, const double *__restrict__ rho,
+                                                     const double *__restrict__ dux, const double *__restrict__ dvy,
+                                                     const double *__restrict__ dwz,
+                                                     const double *__restrict__ temp, double *an_o, double *as_o,
+                                                     double *ae_o, double *aw_o, double *at_o, double *ab_o,
+                                                     double *ap_o, double *su_o, double *partial, const int *dbox,
+                                                     const int *skip) {
+    PressureK::run(vb_hw(), d, b0, tsolid, g, U, Vv, W, rho, dux, dvy, dwz, temp, an_o, as_o, ae_o, aw_o, at_o, ab_o, ap_o, su_o, partial, dbox, skip);
+}
 
 // velocity correction from p', CFDSolverManager.cpp:2719-2742
-__global__ __launch_bounds__(TX *TY) void k_correct_vel(Dims d, BoxDev b0, double ts, const double *__restrict__ temp,
+struct CorrectVelK {
+    static constexpr int family = 21, rank = 0;
+    static __device__ __forceinline__ void run(const VB &vb, Dims d, BoxDev b0, double ts, const double *__restrict__ temp,
                                                         const double *__restrict__ pp,
                                                         const double *__restrict__ dux,
                                                         const double *__restrict__ dvy,
@@ -467,7 +521,7 @@ __global__ __launch_bounds__(TX *TY) void k_correct_vel(Dims d, BoxDev b0, doubl
     if (skip && *skip) return;
     const BoxTiles bt = box_tiles(d, b0, dbox);
     const BoxDev b = bt.b;
-    for (int tt = block_lin(); tt < bt.n; tt += grid_size()) {
+    for (int tt = vb.lin(); tt < bt.n; tt += vb.count()) {
         const int i = b.istat + 1 + (tt % bt.gx) * TX + threadIdx.x;
         const int j = b.jstat + ((tt / bt.gx) % bt.gy) * TY + threadIdx.y;
         const int k = b.kstat + tt / (bt.gx * bt.gy);
@@ -479,15 +533,26 @@ __global__ __launch_bounds__(TX *TY) void k_correct_vel(Dims d, BoxDev b0, doubl
         if (dmin(tP, temp[c - d.sk]) > ts) w[c] += dwz[c] * (pp[c - d.sk] - pp[c]);
     }
 }
+};
+__global__ __launch_bounds__(TX *TY) void k_correct_vel(Dims d, BoxDev b0, double ts, const double *__restrict__ temp,
+                                                        const double *__restrict__ pp,
+                                                        const double *__restrict__ dux,
+                                                        const double *__restrict__ dvy,
+                                                        const double *__restrict__ dwz, double *u, double *v,
+                                                        double *w, const int *dbox, const int *skip) {
+    CorrectVelK::run(vb_hw(), d, b0, ts, temp, pp, dux, dvy, dwz, u, v, w, dbox, skip);
+}
 
 // pressure update, CFDSolverManager.cpp:2744-2757
-__global__ __launch_bounds__(TX *TY) void k_correct_p(Dims d, BoxDev b0, double ts, double urfp,
+struct CorrectPK {
+    static constexpr int family = 22, rank = 0;
+    static __device__ __forceinline__ void run(const VB &vb, Dims d, BoxDev b0, double ts, double urfp,
                                                       const double *__restrict__ temp, double *pp,
                                                       double *pressure, const int *dbox, const int *skip) {
     if (skip && *skip) return;
     const BoxTiles bt = box_tiles(d, b0, dbox);
     const BoxDev b = bt.b;
-    for (int tt = block_lin(); tt < bt.n; tt += grid_size()) {
+    for (int tt = vb.lin(); tt < bt.n; tt += vb.count()) {
         const int i = b.istat + 1 + (tt % bt.gx) * TX + threadIdx.x;
         const int j = b.jstat + ((tt / bt.gx) % bt.gy) * TY + threadIdx.y;
         const int k = b.kstat + tt / (bt.gx * bt.gy);
@@ -498,6 +563,12 @@ __global__ __launch_bounds__(TX *TY) void k_correct_p(Dims d, BoxDev b0, double 
             pp[c] = 0.0;
         }
     }
+}
+};
+__global__ __launch_bounds__(TX *TY) void k_correct_p(Dims d, BoxDev b0, double ts, double urfp,
+                                                      const double *__restrict__ temp, double *pp,
+                                                      double *pressure, const int *dbox, const int *skip) {
+    CorrectPK::run(vb_hw(), d, b0, ts, urfp, temp, pp, pressure, dbox, skip);
 }
 
 dim3 box_grid(const Box &b, const Dims &d) {
@@ -526,7 +597,7 @@ void launch_clean_velocities(hipStream_t st, const Dims &d, DevFields f, const B
                              const BoxLaunch &bl) {
     dim3 g = bl_grid(b, d, bl);
     if (!g.x) return;
-    hipLaunchKernelGGL(k_clean, g, dim3(TX, TY), 0, st, d, to_dev(b), tsolid, f.temp, f.uvel, f.vvel, f.wvel, bl.dbox,
+    dispatch<CleanK, TX * TY>(k_clean, g, dim3(TX, TY), 0, st, d, to_dev(b), tsolid, f.temp, f.uvel, f.vvel, f.wvel, bl.dbox,
                        bl.skip);
 }
 
@@ -542,7 +613,7 @@ void launch_bc_flow(hipStream_t st, const Dims &d, const DevGeom &g, DevFields f
         if (n <= 0) return;
         nb = (n + 255) / 256;
     }
-    hipLaunchKernelGGL(k_bc_flow, dim3(nb), dim3(256), 0, st, d, to_dev(b), ivar, p.a_oxygen, p.dgdtp, g.dxpwinv,
+    dispatch<BcFlowK, 256>(k_bc_flow, dim3(nb), dim3(256), 0, st, d, to_dev(b), ivar, p.a_oxygen, p.dgdtp, g.dxpwinv,
                        g.dypsinv, g.dzpbinv, g.fracx, g.fracy, f.temp, f.fracl, f.vis, phi, bl.dbox, bl.skip);
 }
 
@@ -554,15 +625,15 @@ void launch_momentum(hipStream_t st, const Dims &d, const DevGeom &g, DevFields 
     MomConst q{p.delt, p.tsolid, urf, p.darcyKo, p.grav, p.beta, scanvel, p.discretization, p.steady};
     double *part = pr.flow + (size_t)(ivar - 1) * pr.nblk_f * 3;
     if (ivar == 1)
-        hipLaunchKernelGGL(k_momentum<1>, gr, dim3(TX, TY), 0, st, d, to_dev(b), q, g, f.uvel, f.vvel, f.wvel, f.vis,
+        dispatch<MomentumK<1>, TX * TY>(k_momentum<1>, gr, dim3(TX, TY), 0, st, d, to_dev(b), q, g, f.uvel, f.vvel, f.wvel, f.vis,
                            f.rho, f.pressure, f.unot, f.fracl, f.temp, f.an, f.as, f.ae, f.aw, f.at, f.ab, f.ap, f.su,
                            f.dux, part, bl.dbox, bl.skip);
     else if (ivar == 2)
-        hipLaunchKernelGGL(k_momentum<2>, gr, dim3(TX, TY), 0, st, d, to_dev(b), q, g, f.uvel, f.vvel, f.wvel, f.vis,
+        dispatch<MomentumK<2>, TX * TY>(k_momentum<2>, gr, dim3(TX, TY), 0, st, d, to_dev(b), q, g, f.uvel, f.vvel, f.wvel, f.vis,
                            f.rho, f.pressure, f.vnot, f.fracl, f.temp, f.an, f.as, f.ae, f.aw, f.at, f.ab, f.ap, f.su,
                            f.dvy, part, bl.dbox, bl.skip);
     else
-        hipLaunchKernelGGL(k_momentum<3>, gr, dim3(TX, TY), 0, st, d, to_dev(b), q, g, f.uvel, f.vvel, f.wvel, f.vis,
+        dispatch<MomentumK<3>, TX * TY>(k_momentum<3>, gr, dim3(TX, TY), 0, st, d, to_dev(b), q, g, f.uvel, f.vvel, f.wvel, f.vis,
                            f.rho, f.pressure, f.wnot, f.fracl, f.temp, f.an, f.as, f.ae, f.aw, f.at, f.ab, f.ap, f.su,
                            f.dwz, part, bl.dbox, bl.skip);
 }
@@ -572,7 +643,7 @@ void launch_pressure(hipStream_t st, const Dims &d, const DevGeom &g, DevFields 
     dim3 gr = bl_grid(b, d, bl);
     if (!gr.x) return;
     double *part = pr.flow + (size_t)3 * pr.nblk_f * 3;
-    hipLaunchKernelGGL(k_pressure, gr, dim3(TX, TY), 0, st, d, to_dev(b), p.tsolid, g, f.uvel, f.vvel, f.wvel, f.rho,
+    dispatch<PressureK, TX * TY>(k_pressure, gr, dim3(TX, TY), 0, st, d, to_dev(b), p.tsolid, g, f.uvel, f.vvel, f.wvel, f.rho,
                        f.dux, f.dvy, f.dwz, f.temp, f.an, f.as, f.ae, f.aw, f.at, f.ab, f.ap, f.su, part, bl.dbox,
                        bl.skip);
 }
@@ -581,9 +652,9 @@ void launch_correct(hipStream_t st, const Dims &d, DevFields f, const Box &b, co
                     const BoxLaunch &bl) {
     dim3 gr = bl_grid(b, d, bl);
     if (!gr.x) return;
-    hipLaunchKernelGGL(k_correct_vel, gr, dim3(TX, TY), 0, st, d, to_dev(b), p.tsolid, f.temp, f.pp, f.dux, f.dvy,
+    dispatch<CorrectVelK, TX * TY>(k_correct_vel, gr, dim3(TX, TY), 0, st, d, to_dev(b), p.tsolid, f.temp, f.pp, f.dux, f.dvy,
                        f.dwz, f.uvel, f.vvel, f.wvel, bl.dbox, bl.skip);
-    hipLaunchKernelGGL(k_correct_p, gr, dim3(TX, TY), 0, st, d, to_dev(b), p.tsolid, p.urfp, f.temp, f.pp,
+    dispatch<CorrectPK, TX * TY>(k_correct_p, gr, dim3(TX, TY), 0, st, d, to_dev(b), p.tsolid, p.urfp, f.temp, f.pp,
                        f.pressure, bl.dbox, bl.skip);
 }
 

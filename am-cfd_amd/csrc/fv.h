@@ -3,6 +3,7 @@
 // tridiagonal solve.
 #pragma once
 
+#include "group.h"
 #include "internal.h"
 #include "libm_exp.h"
 #include "libm_log.h"

@@ -2,6 +2,7 @@
 // (laser update, SIMPLE iterations, push-back) through the HIP kernels.
 #pragma once
 
+#include "group.h"
 #include "internal.h"
 
 #include <map>
@@ -118,7 +119,25 @@ class Solver {
     int beam_j() const { return jstart_; }
     std::map<std::string, KernelTimer> timers;
 
+    // grouped runs (group.cpp): many mixed-mode solvers advance in lock step,
+    // each kernel of an iteration one launch over all of them.
+    // nonzero (with the error set) if this solver cannot take part
+    int group_check() const;
+    // every launch argument a recorded iteration holds by value
+    std::string group_key() const;
+    // the solve's start on the solver stream: warm start, the step's device
+    // inputs, the loop reset; `ready` recorded after them
+    int group_begin_solve(hipEvent_t ready);
+    // one SIMPLE iteration's launches, recorded instead of issued
+    int group_record_iteration(LaunchRecorder &rec);
+    // the device's done flag of the step's loop
+    int *group_done_flag() const { return &sc_->done; }
+    // the solve's end once `done` (recorded after the grouped loop) has
+    // passed: scalars, the step's status, the warm start's sample
+    int group_end_solve(hipEvent_t done, float ms);
+
   private:
+    double group_alenold_ = 0.0;
     amcfd_params p_{};
     std::vector<double> tool_, dtt_, heatt_;
     HostGrid hg_;
@@ -226,6 +245,9 @@ class Solver {
     void collect_timers();
     double *field_ptr(const char *name);
 };
+
+// grouped lock-step runs of mixed-mode solvers (group.cpp)
+int run_grouped(const std::vector<Solver *> &s, int max_steps, int *steps_done);
 
 // output writers (output.cpp)
 int write_vts(Solver &s, const char *path, const char *const *names, int nnames);

@@ -185,6 +185,8 @@ void launch_finalize(hipStream_t st, const Dims &d, Partials pr, DevScalars *sc,
 // device-side convergence test of CFDSolverManager.cpp:690-716 after a
 // finalize: counts the iteration, sets done / converged
 void launch_converge(hipStream_t st, DevScalars *sc, const StepDev *step, const amcfd_params &p);
+// grouped runs: out = whether all n flags done[i] are set
+void launch_group_done(hipStream_t st, int *const *done, int n, int *out);
 // resets the device loop control before a step's captured iterations
 void launch_loop_reset(hipStream_t st, DevScalars *sc);
 void launch_end_step(hipStream_t st, const Dims &d, DevFields f, const amcfd_params &p, double track,

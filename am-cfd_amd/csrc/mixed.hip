@@ -186,6 +186,22 @@ __device__ __forceinline__ void lds_line_solve(int n, int lane, const float *sw,
     }
 }
 
+// lds_line_solve with the smallest row count per lane (2, 4, ... up to the
+// capacity M) that holds the line: the scans' grouping of rows, and so the
+// FP32 roundings, depend on the line alone, not on the capacity a launch
+// was built for (grouped launches serve lines of several sizes)
+template <int M>
+__device__ __forceinline__ void lds_line_solve_fit(int n, int lane, const float *sw, const float *sd,
+                                                   const float *se, float *sr, bool guard) {
+    if constexpr (M > 2) {
+        if (n <= 64 * (M / 2)) {
+            lds_line_solve_fit<M / 2>(n, lane, sw, sd, se, sr, guard);
+            return;
+        }
+    }
+    lds_line_solve<M>(n, lane, sw, sd, se, sr, guard);
+}
+
 template <typename TC>
 struct MxCoefs {
     const TC *ap, *an, *as, *ae, *aw, *at, *ab;
@@ -200,7 +216,9 @@ struct MxCoefs {
 // read only lines of the other, so the pass is race-free and in place.
 // ------------------------------------------------------------------------
 template <typename TC, int M>
-__global__ __launch_bounds__(64) void k_xline_mx(Dims d, MxCoefs<TC> cf, double *phi, int ibeg, int iend, int jbeg,
+struct XlineK {
+    static constexpr int family = (std::is_same<TC, float>::value ? 11 : 12), rank = M;
+    static __device__ __forceinline__ void run(const VB &vb, Dims d, MxCoefs<TC> cf, double *phi, int ibeg, int iend, int jbeg,
                                                   int jend, int kbeg, int kend, int colour, int guard,
                                                   const int *__restrict__ skip, const int *__restrict__ dbox) {
     __shared__ float sw[64 * M], sd[64 * M], se[64 * M], sr[64 * M];
@@ -220,8 +238,8 @@ __global__ __launch_bounds__(64) void k_xline_mx(Dims d, MxCoefs<TC> cf, double 
     // workgroups go to the 8 XCDs round-robin by id; give each XCD one
     // contiguous run of lines, so that the lines sharing a neighbour line
     // (j +- 2 and k +- 2 of the same colour) read it through one L2
-    const int G = gridDim.x, q8 = G / 8, r8 = G % 8, x8 = blockIdx.x % 8, i8 = blockIdx.x / 8;
-    const int t = (x8 < r8) ? x8 * (q8 + 1) + i8 : r8 * (q8 + 1) + (x8 - r8) * q8 + i8;
+    const int G = vb.gx, q8 = G / 8, r8 = G % 8, x8 = vb.x % 8, i8 = vb.x / 8;
+    const int t = !vb.hw ? vb.x : (x8 < r8) ? x8 * (q8 + 1) + i8 : r8 * (q8 + 1) + (x8 - r8) * q8 + i8;
     const long long chunk = (nlines + G - 1) / G;
     const long long u0 = (long long)t * chunk, u1 = min(u0 + chunk, nlines);
     for (long long u = u0; u < u1; ++u) {
@@ -250,13 +268,9 @@ __global__ __launch_bounds__(64) void k_xline_mx(Dims d, MxCoefs<TC> cf, double 
             }
         }
         __syncthreads();
-        // box lines (FP64 coefficients) of at most 128 cells take two rows per
-        // lane whatever the capacity M, which shortens the scans; the
-        // enthalpy's capacity fits its lines already
-        if (std::is_same<TC, double>::value && M > 2 && n <= 128)
-            lds_line_solve<2>(n, lane, sw, sd, se, sr, guard != 0);
-        else
-            lds_line_solve<M>(n, lane, sw, sd, se, sr, guard != 0);
+        // the fewest rows per lane that hold the line (box lines are much
+        // shorter than the capacity the domain sets)
+        lds_line_solve_fit<M>(n, lane, sw, sd, se, sr, guard != 0);
         __syncthreads();
 #pragma unroll
         for (int q = 0; q < M; ++q) {
@@ -265,6 +279,13 @@ __global__ __launch_bounds__(64) void k_xline_mx(Dims d, MxCoefs<TC> cf, double 
         }
         __syncthreads();
     }
+}
+};
+template <typename TC, int M>
+__global__ __launch_bounds__(64) void k_xline_mx(Dims d, MxCoefs<TC> cf, double *phi, int ibeg, int iend, int jbeg,
+                                                  int jend, int kbeg, int kend, int colour, int guard,
+                                                  const int *__restrict__ skip, const int *__restrict__ dbox) {
+    XlineK<TC, M>::run(vb_hw(), d, cf, phi, ibeg, iend, jbeg, jend, kbeg, kend, colour, guard, skip, dbox);
 }
 
 // ------------------------------------------------------------------------
@@ -323,17 +344,19 @@ __device__ __forceinline__ bool box_range(const int *dbox, int kend, int maxn, B
 }
 
 // the pass buffers' box cells to BX_PENDING and the task counter to zero
-__global__ __launch_bounds__(256) void k_box_prep(Dims d, double *bufs, size_t bstride, int passes, int kend,
+struct BoxPrepK {
+    static constexpr int family = 13, rank = 0;
+    static __device__ __forceinline__ void run(const VB &vb, Dims d, double *bufs, size_t bstride, int passes, int kend,
                                                   unsigned int *claim, const int *__restrict__ skip,
                                                   const int *__restrict__ dbox) {
     if (skip && *skip) return;
     BoxRange b;
     if (!box_range(dbox, kend, 1 << 30, b)) return;
-    if (blockIdx.x == 0 && threadIdx.x == 0) *claim = 0u;
+    if (vb.x == 0 && threadIdx.x == 0) *claim = 0u;
     const int n = b.iend - b.ibeg + 1, nj = b.jend - b.jbeg + 1;
     const size_t cells = (size_t)n * nj * (b.kend - b.kbeg + 1);
     const double pend = __builtin_bit_cast(double, BX_PENDING);
-    for (size_t e = (size_t)blockIdx.x * 256 + threadIdx.x; e < cells * passes; e += (size_t)gridDim.x * 256) {
+    for (size_t e = (size_t)vb.x * 256 + threadIdx.x; e < cells * passes; e += (size_t)vb.gx * 256) {
         const size_t q = e / cells, r = e % cells;
         const int i = b.ibeg + (int)(r % n);
         const size_t line = r / n;
@@ -341,9 +364,17 @@ __global__ __launch_bounds__(256) void k_box_prep(Dims d, double *bufs, size_t b
         bufs[q * bstride + cix(d, i, j, k)] = pend;
     }
 }
+};
+__global__ __launch_bounds__(256) void k_box_prep(Dims d, double *bufs, size_t bstride, int passes, int kend,
+                                                  unsigned int *claim, const int *__restrict__ skip,
+                                                  const int *__restrict__ dbox) {
+    BoxPrepK::run(vb_hw(), d, bufs, bstride, passes, kend, claim, skip, dbox);
+}
 
 template <int M>
-__global__ __launch_bounds__(64) void k_box_lines(Dims d, MxCoefs<double> cf, double *phi, double *bufs,
+struct BoxLinesK {
+    static constexpr int family = 14, rank = M;
+    static __device__ __forceinline__ void run(const VB &vb, Dims d, MxCoefs<double> cf, double *phi, double *bufs,
                                                    size_t bstride, int passes, int kend, int guard,
                                                    unsigned int *claim, int *error, const int *__restrict__ skip,
                                                    const int *__restrict__ dbox) {
@@ -412,10 +443,9 @@ __global__ __launch_bounds__(64) void k_box_lines(Dims d, MxCoefs<double> cf, do
         ok = !__any(!ok);
         if (!ok) break;
         __syncthreads();
-        // lines of at most 128 cells (the melt-pool box's) take two rows per
-        // lane whatever the capacity M, which shortens the scans
-        if (M > 2 && n <= 128) lds_line_solve<2>(n, lane, sw, sd, se, sr, guard != 0);
-        else lds_line_solve<M>(n, lane, sw, sd, se, sr, guard != 0);
+        // the fewest rows per lane that hold the line (the melt-pool box's
+        // lines are much shorter than the domain's)
+        lds_line_solve_fit<M>(n, lane, sw, sd, se, sr, guard != 0);
         __syncthreads();
 #pragma unroll
         for (int q = 0; q < M; ++q) {
@@ -429,6 +459,14 @@ __global__ __launch_bounds__(64) void k_box_lines(Dims d, MxCoefs<double> cf, do
         __syncthreads();
     }
     if (!ok && lane == 0) atomicOr(error, 1);
+}
+};
+template <int M>
+__global__ __launch_bounds__(64) void k_box_lines(Dims d, MxCoefs<double> cf, double *phi, double *bufs,
+                                                   size_t bstride, int passes, int kend, int guard,
+                                                   unsigned int *claim, int *error, const int *__restrict__ skip,
+                                                   const int *__restrict__ dbox) {
+    BoxLinesK<M>::run(vb_hw(), d, cf, phi, bufs, bstride, passes, kend, guard, claim, error, skip, dbox);
 }
 
 // ------------------------------------------------------------------------
@@ -513,37 +551,31 @@ __global__ __launch_bounds__(64) void k_yzline_mx(Dims d, MxCoefs<TC> cf, const 
 //   bl_i d_i - blm_i d_{i-1} - blp_i d_{i+1} = blc_i
 // (CFDSolverManager.cpp:2334-2349) solved by one wave, then phi += d_i.
 // ------------------------------------------------------------------------
-__global__ __launch_bounds__(64) void k_blk_fold(int ni, int ntile, const double *__restrict__ part, double *out,
+struct BlkFoldK {
+    static constexpr int family = 8, rank = 0;
+    static __device__ __forceinline__ void run(const VB &vb, int ni, int ntile, const double *__restrict__ part, double *out,
                                                   const int *__restrict__ skip) {
     if (skip && *skip) return;
-    const int i = blockIdx.x * 64 + threadIdx.x, q = blockIdx.y, s = blockIdx.z;
+    const int i = vb.x * 64 + threadIdx.x, q = vb.y, s = vb.z;
     if (i >= ni) return;
     const double *p = part + (size_t)q * ntile * ni + i;
     double a = 0.0;
     for (int t = s; t < ntile; t += BF_SPLIT) a += p[(size_t)t * ni];
     out[((size_t)q * BF_SPLIT + s) * ni + i] = a;
 }
+};
+__global__ __launch_bounds__(64) void k_blk_fold(int ni, int ntile, const double *__restrict__ part, double *out,
+                                                  const int *__restrict__ skip) {
+    BlkFoldK::run(vb_hw(), ni, ntile, part, out, skip);
+}
 
 constexpr int BS_NT = 1024;
 constexpr int BS_MAXNI = 4100;   // ni of the longest x-line, rounded up
 
+// the 1-D system along x with the fewest rows per lane that hold it (the
+// roundings depend on the system alone, whatever capacity M a launch has)
 template <int M>
-__global__ __launch_bounds__(BS_NT) void k_blk_solve_mx(int nim2, int ni, const double *__restrict__ f1,
-                                                        double *delphi, const int *__restrict__ skip) {
-    __shared__ double sq[4][BS_MAXNI];
-    if (skip && *skip) return;
-    // second fold level, all threads: sq[q][i] = sum over the splits
-    for (int e = threadIdx.x; e < 4 * ni; e += BS_NT) {
-        const int q = e / ni, i = e % ni;
-        double a = 0.0;
-#pragma unroll
-        for (int sp = 0; sp < BF_SPLIT; ++sp) a += f1[((size_t)q * BF_SPLIT + sp) * ni + i];
-        sq[q][i] = a;
-    }
-    __syncthreads();
-    if (threadIdx.x >= 64) return;
-    const int lane = threadIdx.x;
-    const int n = nim2;
+__device__ __forceinline__ void blk_solve_rows(int n, int lane, const double (*sq)[BS_MAXNI], double *delphi) {
     double w[M], dg[M], e[M], r[M];
 #pragma unroll
     for (int q = 0; q < M; ++q) {
@@ -561,15 +593,58 @@ __global__ __launch_bounds__(BS_NT) void k_blk_solve_mx(int nim2, int ni, const 
         if (p < n) delphi[p + 1] = r[q];
     }
 }
+template <int M>
+__device__ __forceinline__ void blk_solve_fit(int n, int lane, const double (*sq)[BS_MAXNI], double *delphi) {
+    if constexpr (M > 2) {
+        if (n <= 64 * (M / 2)) {
+            blk_solve_fit<M / 2>(n, lane, sq, delphi);
+            return;
+        }
+    }
+    blk_solve_rows<M>(n, lane, sq, delphi);
+}
 
-__global__ __launch_bounds__(256) void k_add_delphi_mx(Dims d, const double *__restrict__ delphi, double *phi,
+template <int M>
+struct BlkSolveK {
+    static constexpr int family = 9, rank = M;
+    static __device__ __forceinline__ void run(const VB &vb, int nim2, int ni, const double *__restrict__ f1,
+                                                        double *delphi, const int *__restrict__ skip) {
+    __shared__ double sq[4][BS_MAXNI];
+    if (skip && *skip) return;
+    // second fold level, all threads: sq[q][i] = sum over the splits
+    for (int e = threadIdx.x; e < 4 * ni; e += BS_NT) {
+        const int q = e / ni, i = e % ni;
+        double a = 0.0;
+#pragma unroll
+        for (int sp = 0; sp < BF_SPLIT; ++sp) a += f1[((size_t)q * BF_SPLIT + sp) * ni + i];
+        sq[q][i] = a;
+    }
+    __syncthreads();
+    if (threadIdx.x >= 64) return;
+    blk_solve_fit<M>(nim2, threadIdx.x, sq, delphi);
+}
+};
+template <int M>
+__global__ __launch_bounds__(BS_NT) void k_blk_solve_mx(int nim2, int ni, const double *__restrict__ f1,
+                                                        double *delphi, const int *__restrict__ skip) {
+    BlkSolveK<M>::run(vb_hw(), nim2, ni, f1, delphi, skip);
+}
+
+struct AddDelphiMxK {
+    static constexpr int family = 10, rank = 0;
+    static __device__ __forceinline__ void run(const VB &vb, Dims d, const double *__restrict__ delphi, double *phi,
                                                       const int *__restrict__ skip) {
     if (skip && *skip) return;
-    const int i = 1 + blockIdx.x * 64 + threadIdx.x;
-    const int j = 1 + blockIdx.y * 4 + threadIdx.y;
+    const int i = 1 + vb.x * 64 + threadIdx.x;
+    const int j = 1 + vb.y * 4 + threadIdx.y;
     if (i > d.nim2 || j > d.njm2) return;
     const double dp = delphi[i];
-    for (int k = 1 + blockIdx.z * 4; k < 1 + (blockIdx.z + 1) * 4 && k <= d.nkm2; ++k) phi[cix(d, i, j, k)] += dp;
+    for (int k = 1 + vb.z * 4; k < 1 + (vb.z + 1) * 4 && k <= d.nkm2; ++k) phi[cix(d, i, j, k)] += dp;
+}
+};
+__global__ __launch_bounds__(256) void k_add_delphi_mx(Dims d, const double *__restrict__ delphi, double *phi,
+                                                      const int *__restrict__ skip) {
+    AddDelphiMxK::run(vb_hw(), d, delphi, phi, skip);
 }
 
 template <typename TC>
@@ -581,7 +656,7 @@ void launch_xline_t(hipStream_t st, const Dims &d, const MxCoefs<TC> &cf, double
     const int n = iend - ibeg + 1;
     const long long nlines = (long long)(kend - kbeg + 1) * ((jend - jbeg + 2) / 2);
     const int nb = dbox ? nblocks : (int)std::min<long long>(nlines, 16384);
-#define XL(MM) hipLaunchKernelGGL((k_xline_mx<TC, MM>), dim3(nb), dim3(64), 0, st, d, cf, phi, ibeg, iend, jbeg, jend, \
+#define XL(MM) dispatch<XlineK<TC, MM>, 64>(k_xline_mx<TC, MM>, dim3(nb), dim3(64), 0, st, d, cf, phi, ibeg, iend, jbeg, jend, \
                                   kbeg, kend, colour, guard, skip, dbox)
     if (n <= 128) XL(2);
     else if (n <= 256) XL(4);
@@ -609,6 +684,10 @@ void launch_yzline_t(hipStream_t st, const Dims &d, const MxCoefs<TC> &cf, const
                                   (int)((size_t)YZ_MAXL * 64 * 2 * sizeof(float)));
         attr_set = true;
     }
+    if (LaunchRecorder *r = tls_recorder()) {   // y/z passes have no grouped form
+        r->unsupported = true;
+        return;
+    }
     if (dir == 1)
         hipLaunchKernelGGL((k_yzline_mx<TC, 1>), grid, dim3(64), lds, st, d, cf, hin, hout, ibeg, iend, jbeg, jend,
                            kbeg, kend, skip);
@@ -625,10 +704,10 @@ size_t mixed_yz_lds(int len) { return (size_t)len * 64 * 2 * sizeof(float); }
 int mixed_blk_split() { return BF_SPLIT; }
 
 void launch_mixed_block_correction(hipStream_t st, const Dims &d, const MxFields &m, double *phi, const int *skip) {
-    hipLaunchKernelGGL(k_blk_fold, dim3((d.ni + 63) / 64, 4, BF_SPLIT), dim3(64), 0, st, d.ni, mixed_tile_count(d),
+    dispatch<BlkFoldK, 64>(k_blk_fold, dim3((d.ni + 63) / 64, 4, BF_SPLIT), dim3(64), 0, st, d.ni, mixed_tile_count(d),
                        m.blkpart, m.blkfold, skip);
     const int n = d.nim2;
-#define BS(MM) hipLaunchKernelGGL((k_blk_solve_mx<MM>), dim3(1), dim3(BS_NT), 0, st, d.nim2, d.ni, m.blkfold, m.delphi, skip)
+#define BS(MM) dispatch<BlkSolveK<MM>, BS_NT>(k_blk_solve_mx<MM>, dim3(1), dim3(BS_NT), 0, st, d.nim2, d.ni, m.blkfold, m.delphi, skip)
     if (n <= 128) BS(2);
     else if (n <= 256) BS(4);
     else if (n <= 512) BS(8);
@@ -636,7 +715,7 @@ void launch_mixed_block_correction(hipStream_t st, const Dims &d, const MxFields
     else if (n <= 2048) BS(32);
     else BS(64);
 #undef BS
-    hipLaunchKernelGGL(k_add_delphi_mx, dim3((d.nim2 + 63) / 64, (d.njm2 + 3) / 4, (d.nkm2 + 3) / 4), dim3(64, 4), 0,
+    dispatch<AddDelphiMxK, 256>(k_add_delphi_mx, dim3((d.nim2 + 63) / 64, (d.njm2 + 3) / 4, (d.nkm2 + 3) / 4), dim3(64, 4), 0,
                        st, d, m.delphi, phi, skip);
 }
 
@@ -656,10 +735,10 @@ void launch_mixed_yzline(hipStream_t st, const Dims &d, const MxFields &m, const
 int launch_box_passes(hipStream_t st, const Dims &d, DevFields f, double *phi, double *bufs, size_t bstride,
                       int passes, unsigned int *claim, int *error, const int *skip, const int *dbox, int nblocks) {
     MxCoefs<double> cf{f.ap, f.an, f.as, f.ae, f.aw, f.at, f.ab, f.su};
-    hipLaunchKernelGGL(k_box_prep, dim3(nblocks), dim3(256), 0, st, d, bufs, bstride, passes, d.nkm2, claim, skip,
+    dispatch<BoxPrepK, 256>(k_box_prep, dim3(nblocks), dim3(256), 0, st, d, bufs, bstride, passes, d.nkm2, claim, skip,
                        dbox);
     const int n = d.nim2;   // an upper bound of the box's line length
-#define BL(MM) hipLaunchKernelGGL((k_box_lines<MM>), dim3(nblocks), dim3(64), 0, st, d, cf, phi, bufs, bstride, \
+#define BL(MM) dispatch<BoxLinesK<MM>, 64>(k_box_lines<MM>, dim3(nblocks), dim3(64), 0, st, d, cf, phi, bufs, bstride, \
                                   passes, d.nkm2, 1, claim, error, skip, dbox)
     if (n <= 128) BL(2);
     else if (n <= 256) BL(4);

@@ -12,6 +12,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
 #include <cstring>
 
 namespace amcfd {
@@ -1017,6 +1018,71 @@ int Solver::solve_step_mixed_graph() {
     if (!std::isfinite(sc_host_->resorh))
         return set_error(AMCFD_E_NAN, "the enthalpy residual is not finite after step " + std::to_string(step_));
     return rc;
+}
+
+// ---------------------------------------------------------------------------
+// grouped runs: the captured loop's pieces for a runner that launches each
+// kernel of the iteration once for many solvers (group.cpp)
+// ---------------------------------------------------------------------------
+int Solver::group_check() const {
+    if (mode_ != AMCFD_MODE_MIXED) return set_error(AMCFD_E_UNSUPPORTED, "grouped runs take mixed-mode solvers");
+    if (!use_graph_) return set_error(AMCFD_E_UNSUPPORTED, "grouped runs take solvers with the captured loop on");
+    if (!heatt_.empty())
+        return set_error(AMCFD_E_UNSUPPORTED, "a heat-source table advances between iterations (host loop)");
+    if (mx_sched_.find_first_of("yz") != std::string::npos)
+        return set_error(AMCFD_E_UNSUPPORTED, "y/z line passes have no grouped form");
+    return 0;
+}
+
+std::string Solver::group_key() const {
+    char buf[512];
+    std::snprintf(buf, sizeof buf, "%p:%d:%d:%d:%a:%a:%s:%d:%d:%p;", (const void *)this, d_.nk, graph_block_,
+                  box_blocks_, p_.delt, scanvel_, mx_sched_.c_str(), mx_box_passes_, box_fused_,
+                  (const void *)box_bufs_);
+    return buf;
+}
+
+int Solver::group_begin_solve(hipEvent_t ready) {
+    int st = warm_start_begin();
+    if (st) return st;
+    group_alenold_ = alen_;
+    SourceParams sp = source_params();
+    *steph_ = StepDev{sp, powerindicator_, (int)p_.maxit, istart_, jstart_};
+    HIPCALL(hipMemcpyAsync(stepd_, steph_, sizeof(StepDev), hipMemcpyHostToDevice, st_));
+    launch_loop_reset(st_, sc_);
+    HIPCALL(hipGetLastError());
+    HIPCALL(hipEventRecord(ready, st_));
+    return 0;
+}
+
+int Solver::group_record_iteration(LaunchRecorder &rec) {
+    if (int st = ensure_box_buffers(mx_box_passes_)) return st;   // allocated before any recording
+    const int prof = profile_;
+    profile_ = 0;
+    tls_recorder() = &rec;
+    const int st = enqueue_mixed_iteration(true);
+    tls_recorder() = nullptr;
+    profile_ = prof;
+    if (!st && rec.unsupported) return set_error(AMCFD_E_UNSUPPORTED, "an iteration launch has no grouped form");
+    return st;
+}
+
+int Solver::group_end_solve(hipEvent_t done, float ms) {
+    HIPCALL(hipStreamWaitEvent(st_, done, 0));
+    int st = read_scalars();
+    if (st) return st;
+    niter_ = sc_host_->niter;
+    itertot_ += niter_;
+    alen_ = sc_host_->alen;
+    const int rc = sc_host_->converged ? AMCFD_OK : AMCFD_I_MAXIT;
+    if (rc == AMCFD_I_MAXIT && std::fabs(alen_ - group_alenold_) < 1.0e-6) steadylength_ = true;
+    last_total_ms = ms;
+    last_tdma_ms = -1.0;
+    if (!std::isfinite(sc_host_->resorh))
+        return set_error(AMCFD_E_NAN, "the enthalpy residual is not finite after step " + std::to_string(step_));
+    last_solve_rc_ = rc;
+    st = warm_start_end();
+    return st ? st : rc;
 }
 
 int Solver::end_step() {

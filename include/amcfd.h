@@ -255,6 +255,19 @@ int  amcfd_output_section(amcfd_solver *s, const char *base, int finish, int *wr
  * process. */
 int  amcfd_run_batch(amcfd_solver *const *s, int n, int max_steps, int workers, int *steps_done);
 
+/* n mixed-mode solvers (captured loop on, x-line schedule, no heat-source
+ * table) advanced in lock step, up to max_steps steps each (negative: until
+ * each finishes): every kernel of a SIMPLE iteration is one launch over all
+ * of them, the instances' tiles and line batches side by side in one grid,
+ * captured with graph_block iterations into one graph that is relaunched
+ * until every instance's convergence test (CFDSolverManager.cpp:690-716,
+ * on the device) has ended its step.  Instances may differ in grid, deck
+ * and parameters; each computes exactly what it computes alone.
+ * steps_done[i] receives the steps instance i took.  Returns
+ * AMCFD_E_UNSUPPORTED (before any step) for a solver that cannot take part.
+ * No reference counterpart: the reference runs one deck per process. */
+int  amcfd_run_grouped(amcfd_solver *const *s, int n, int max_steps, int *steps_done);
+
 /* The reference's x-line TDMA over many independent lines held in device
  * memory: the recurrence of CFDSolverManager::solveEntireDomain
  * (CFDSolverManager.cpp:2500-2522; guard = 0) or, with the |den| <= 1e-12
