@@ -39,7 +39,7 @@ EXPORTS = ("amcfd_deck_load", "amcfd_deck_params", "amcfd_deck_free", "amcfd_cre
            "amcfd_get_scalars", "amcfd_dims", "amcfd_last_timing", "amcfd_set_profiling",
            "amcfd_kernel_stats", "amcfd_write_vts", "amcfd_write_pvd", "amcfd_append_residuals",
            "amcfd_append_final_geometry", "amcfd_deck_string", "amcfd_free_surface",
-           "amcfd_output_section", "amcfd_run_batch", "amcfd_run_grouped", "amcfd_set_mixed_schedule", "amcfd_set_graph", "amcfd_get_graph", "amcfd_set_box_fused",
+           "amcfd_output_section", "amcfd_run_batch", "amcfd_run_grouped", "amcfd_set_mixed_schedule", "amcfd_set_graph", "amcfd_get_graph", "amcfd_set_box_fused", "amcfd_set_line_batch",
            "amcfd_tuning_choice", "amcfd_copy_state", "amcfd_test_expire_waits", "amcfd_test_libm",
            "amcfd_tdma_lines", "amcfd_tdma_lines_noalloc",
            "amcfd_ws_create", "amcfd_ws_destroy", "amcfd_ws_attach", "amcfd_ws_counts", "amcfd_ws_train", "amcfd_ws_samples",
@@ -156,9 +156,10 @@ def load_library(path=LIB_PATH):
         "amcfd_test_expire_waits": (ip, [vp]),
         "amcfd_test_libm": (ip, [ip, dp, dp, ctypes.c_size_t]),
         "amcfd_set_graph": (ip, [vp, ip, ip, ip]),
-        "amcfd_get_graph": (ip, [vp] + [ctypes.POINTER(ctypes.c_int)] * 4),
+        "amcfd_get_graph": (ip, [vp] + [ctypes.POINTER(ctypes.c_int)] * 5),
+        "amcfd_set_line_batch": (ip, [vp, ip]),
         "amcfd_set_box_fused": (ip, [vp, ip]),
-        "amcfd_tuning_choice": (ip, [cp, ctypes.c_longlong] + [ctypes.POINTER(ctypes.c_int)] * 3),
+        "amcfd_tuning_choice": (ip, [cp, ctypes.c_longlong] + [ctypes.POINTER(ctypes.c_int)] * 4),
         "amcfd_copy_state": (ip, [vp, vp]),
         "amcfd_tdma_lines": (ip, [vp, vp, vp, vp, vp, ip, ip, ip, vp]),
         "amcfd_tdma_lines_noalloc": (ip, [vp, vp, vp, vp, vp, ip, ip, ip, vp]),
@@ -336,11 +337,16 @@ class Solver:
 
     def graph_settings(self):
         """Mixed mode: (captured loop on, iterations per graph, box workgroups,
-        box passes in one persistent launch)."""
-        on, b, bb, bf = ctypes.c_int(), ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        box passes in one persistent launch, x-lines per workgroup)."""
+        on, b, bb, bf, lb = ctypes.c_int(), ctypes.c_int(), ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
         _check(load_library().amcfd_get_graph(self._h, ctypes.byref(on), ctypes.byref(b), ctypes.byref(bb),
-                                              ctypes.byref(bf)))
-        return bool(on.value), b.value, bb.value, bool(bf.value)
+                                              ctypes.byref(bf), ctypes.byref(lb)))
+        return bool(on.value), b.value, bb.value, bool(bf.value), lb.value
+
+    def set_line_batch(self, lines=1):
+        """Mixed mode: consecutive x-lines per workgroup in the enthalpy's
+        passes; the same results."""
+        _check(load_library().amcfd_set_line_batch(self._h, int(lines)))
 
     def set_box_fused(self, on=True):
         """Mixed mode: a box variable's passes in one persistent launch (True)
@@ -388,12 +394,13 @@ def run_grouped(solvers, max_steps=-1):
 
 
 def tuning_choice(cells, path=None):
-    """(graph_block, box_blocks, box_fused) the mixed mode takes for a grid
-    of `cells` interior cells, from the table at `path` or the library's own."""
-    gb, bb, bf = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+    """(graph_block, box_blocks, box_fused, line_batch) the mixed mode takes
+    for a grid of `cells` interior cells, from the table at `path` or the
+    library's own."""
+    gb, bb, bf, lb = ctypes.c_int(), ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
     _check(load_library().amcfd_tuning_choice(os.fsencode(path) if path else None, int(cells), ctypes.byref(gb),
-                                              ctypes.byref(bb), ctypes.byref(bf)))
-    return gb.value, bb.value, bool(bf.value)
+                                              ctypes.byref(bb), ctypes.byref(bf), ctypes.byref(lb)))
+    return gb.value, bb.value, bool(bf.value), lb.value
 
 
 def device_libm(fn, x):

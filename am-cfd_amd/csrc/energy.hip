@@ -820,6 +820,7 @@ __device__ int block_max_int(int v, int *s) {
     return s[0];
 }
 
+constexpr int MP_MAXNI = 4100;   // x-nodes of the longest grid a line solver takes
 struct MeltPoolK {
     static constexpr int family = 5, rank = 0;
     static __device__ __forceinline__ void run(const VB &vb, Dims d, int nblk, const double *__restrict__ partial,
@@ -836,6 +837,8 @@ struct MeltPoolK {
     }
     __shared__ double sd[32];
     __shared__ double tp_s;
+    __shared__ int s_kf[MP_MAXNI], s_ju[MP_MAXNI], s_jd[MP_MAXNI], s_rd[MP_MAXNI], s_rw[MP_MAXNI];
+    __shared__ int s_nd, s_nw;
     const int tid = threadIdx.x, nt = blockDim.x;
     // peak temperature (first-argument-wins max over block partials)
     double m = -INFINITY;
@@ -875,30 +878,67 @@ struct MeltPoolK {
     for (int i = 1 + tid; i <= istart; i += nt)
         if (T(i, js, top) < ts) cand = max(cand, i);
     const int imin = block_max_int(cand, si);
-    // depth: lowest k of the run that starts at nkm2 and stays >= ts
-    cand = d.nkm2;
-    for (int i = 1 + tid; i <= d.nim2; i += nt) {
-        for (int k = d.nkm2; k > 0; --k) {
-            if (T(i, js, k) < ts) break;
-            cand = min(cand, k);
-        }
+    // depth: lowest k of the run that starts at nkm2 and stays >= ts; width:
+    // highest / lowest j of the runs from row js along the top plane.  A row
+    // whose starting cell is below ts has no run (it leaves the fold
+    // unchanged); for the others, the run ends at the first cell below ts:
+    //   kf[i] = highest k < nkm2 with T(i, js, k) < ts (0 if none),
+    //   ju[i] = lowest j in (js, njm2] with T(i, j, top) < ts (njm1 if none),
+    //   jd[i] = highest j in [1, js) with T(i, j, top) < ts (0 if none),
+    // and the run covers k in [kf + 1, nkm2], j in [js, ju - 1], j in
+    // [jd + 1, js]; the folds below are the loops' min / max.  Only the rows
+    // with a run are scanned, all their cells at once, eight loads in flight
+    // per thread.
+    if (tid == 0) {
+        s_nd = 0;
+        s_nw = 0;
     }
-    const int kmin = block_min_int(cand, si) - 1;
-    // width: highest / lowest j of the runs along the top plane
-    cand = js;
-    for (int i = 1 + tid; i <= d.nim2; i += nt)
-        for (int j = js; j < d.njm1; ++j) {
-            if (T(i, j, top) < ts) break;
-            cand = max(cand, j);
+    __syncthreads();
+    for (int i = 1 + tid; i <= d.nim2; i += nt) {
+        const bool dr = !(T(i, js, d.nkm2) < ts), wr = !(T(i, js, top) < ts);
+        s_kf[i] = dr ? 0 : d.nkm2;
+        s_ju[i] = wr ? d.njm1 : js;
+        s_jd[i] = wr ? 0 : js;
+        if (dr) s_rd[atomicAdd(&s_nd, 1)] = i;
+        if (wr) s_rw[atomicAdd(&s_nw, 1)] = i;
+    }
+    __syncthreads();
+    const int nrd = s_nd, nrw = s_nw;
+    auto scan = [&](int total, auto cell, auto hit) {
+        constexpr int U = 8;
+        for (int e0 = tid; e0 < total; e0 += U * nt) {
+            double v[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int e = e0 + u * nt;
+                v[u] = e < total ? cell(e) : ts;
+            }
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int e = e0 + u * nt;
+                if (e < total && v[u] < ts) hit(e);
+            }
         }
-    const int jmax = block_max_int(cand, si) + 1;
-    cand = js;
-    for (int i = 1 + tid; i <= d.nim2; i += nt)
-        for (int j = js; j > 0; --j) {
-            if (T(i, j, top) < ts) break;
-            cand = min(cand, j);
-        }
-    const int jmin = block_min_int(cand, si) - 1;
+    };
+    const int nkd = d.nkm2 - 1, nju = d.njm2 - js, njd = js - 1;
+    scan(nrd * nkd, [&](int e) { return T(s_rd[e % nrd], js, 1 + e / nrd); },
+         [&](int e) { atomicMax(&s_kf[s_rd[e % nrd]], 1 + e / nrd); });
+    if (nju > 0)
+        scan(nrw * nju, [&](int e) { return T(s_rw[e % nrw], js + 1 + e / nrw, top); },
+             [&](int e) { atomicMin(&s_ju[s_rw[e % nrw]], js + 1 + e / nrw); });
+    if (njd > 0)
+        scan(nrw * njd, [&](int e) { return T(s_rw[e % nrw], 1 + e / nrw, top); },
+             [&](int e) { atomicMax(&s_jd[s_rw[e % nrw]], 1 + e / nrw); });
+    __syncthreads();
+    int ck = d.nkm2, cju = js, cjd = js;
+    for (int i = 1 + tid; i <= d.nim2; i += nt) {
+        ck = min(ck, s_kf[i] + 1);
+        cju = max(cju, s_ju[i] - 1);
+        cjd = min(cjd, s_jd[i] + 1);
+    }
+    const int kmin = block_min_int(ck, si) - 1;
+    const int jmax = block_max_int(cju, si) + 1;
+    const int jmin = block_min_int(cjd, si) - 1;
     // interpolated extents (max / min over i of per-row values)
     double dep = -INFINITY, ymx = -INFINITY, ymn = INFINITY;
     if (kmin != 0) {
@@ -1177,12 +1217,12 @@ __global__ __launch_bounds__(128) void k_laser_total(int n, const double *__rest
             const double *b = buf[ch & 1];
             const int m = min(LT_CHUNK, n - ch * LT_CHUNK);
             int r = 0;
-            for (; r + 8 <= m; r += 8) {
-                double v[8];
+            for (; r + 32 <= m; r += 32) {
+                double v[32];
 #pragma unroll
-                for (int u = 0; u < 8; ++u) v[u] = b[r + u];
+                for (int u = 0; u < 32; ++u) v[u] = b[r + u];
 #pragma unroll
-                for (int u = 0; u < 8; ++u) total += v[u];
+                for (int u = 0; u < 32; ++u) total += v[u];
             }
             for (; r < m; ++r) total += b[r];
         }

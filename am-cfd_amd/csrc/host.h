@@ -38,7 +38,7 @@ int ws_apply(WarmStart *w, Solver &s);
 void ws_attach(WarmStart *w, Solver *s);
 // the tuning table's choice for a grid of `cells` interior cells (tuning.cpp);
 // leaves the values alone when no table or no row applies
-void tuning_lookup(long long cells, int *graph_block, int *box_blocks, int *box_fused);
+void tuning_lookup(long long cells, int *graph_block, int *box_blocks, int *box_fused, int *line_batch);
 void ws_detach(WarmStart *w, Solver *s);
 
 // accumulated device time of one kernel family (HIP events on the solver stream)
@@ -77,6 +77,9 @@ class Solver {
     }
     int box_fused() const { return box_fused_; }
     int set_box_fused(int on);
+    // mixed mode: consecutive x-lines per workgroup in the enthalpy's passes
+    int line_batch() const { return line_batch_; }
+    int set_line_batch(int lines);
     // mixed mode: order of the enthalpy line passes ('x' red-black x-lines,
     // 'y' y-lines, 'z' z-lines; an even number of y/z) and red-black x
     // passes per melt-pool box variable
@@ -168,7 +171,7 @@ class Solver {
         int nk = -1, blocks = 0, box_blocks = 0;
         double delt = 0.0, scanvel = 0.0;
         std::string sched;
-        int box_passes = 0, box_fused = -1;
+        int box_passes = 0, box_fused = -1, line_batch = 0;
     } gkey_;
     StepDev *stepd_ = nullptr;       // device
     StepDev *steph_ = nullptr;       // pinned staging
@@ -177,6 +180,7 @@ class Solver {
     // mixed mode: the box passes of one variable in one persistent launch (1)
     // or one launch per half pass (0); the same results either way
     int box_fused_ = 1;
+    int line_batch_ = 1;             // mixed mode: x-lines per workgroup of the enthalpy passes
     double *box_bufs_ = nullptr;     // box_buf_passes_ pass buffers of ni*nj*nkf
     int box_buf_passes_ = 0;
     unsigned int *box_claim_ = nullptr;

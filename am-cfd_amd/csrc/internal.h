@@ -217,9 +217,10 @@ int mixed_max_yz();
 size_t mixed_yz_lds(int len);   // LDS bytes of one y/z line workgroup
 int mixed_blk_split();
 void launch_mixed_block_correction(hipStream_t st, const Dims &d, const MxFields &m, double *phi, const int *skip);
-// x-lines of one colour ((j + k) % 2 == colour), in place
+// x-lines of one colour ((j + k) % 2 == colour), in place; line_batch
+// consecutive lines per workgroup
 void launch_mixed_xline(hipStream_t st, const Dims &d, const MxFields &m, double *phi, int ibeg, int iend, int jbeg,
-                        int jend, int kbeg, int kend, int colour, const int *skip);
+                        int jend, int kbeg, int kend, int colour, const int *skip, int line_batch = 1);
 // y-lines (dir 1) or z-lines (dir 2), hin -> hout
 void launch_mixed_yzline(hipStream_t st, const Dims &d, const MxFields &m, const double *hin, double *hout, int dir,
                          int ibeg, int iend, int jbeg, int jend, int kbeg, int kend, const int *skip);

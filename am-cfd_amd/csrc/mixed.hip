@@ -330,6 +330,81 @@ __device__ __forceinline__ bool bx_await(const double *p, double &v, const int *
     return true;
 }
 
+// the cells of one box line task and where its values come from
+struct BoxRow {
+    const double *own, *phi, *src_n, *src_s, *src_t, *src_b;
+    size_t row;
+    int n, lane;
+};
+// the registers of v are live here: their loads are issued before this point
+// and not moved after it (an empty statement that reads them, no code)
+template <typename T>
+__device__ __forceinline__ void live_one(T v) {
+    asm volatile("" ::"v"(v));
+}
+template <typename... T>
+__device__ __forceinline__ void live_here(T... v) {
+    (live_one(v), ...);
+}
+// K rows per lane of a box line: every row's coefficient and value loads
+// issued together, the tridiagonal coefficients to LDS, then the values
+// still pending polled and the residual formed.  The coefficients hold no
+// pass's values, so their loads overlap the wait for the neighbours'; the
+// results are those of loading row by row.
+template <int K>
+__device__ __forceinline__ bool box_rows(const Dims &d, const MxCoefs<double> &cf, const BoxRow &br, double *hp,
+                                         float *sw, float *sd, float *se, float *sr, int *error) {
+    double ap[K], aw[K], ae[K], an[K], as[K], at[K], ab[K], su[K];
+    double h0[K], hw[K], he[K], hn[K], hs[K], ht[K], hb[K];
+#pragma unroll
+    for (int q = 0; q < K; ++q) {
+        const int ii = br.lane + 64 * q;
+        if (ii < br.n) {
+            const size_t c = br.row + ii;
+            ap[q] = cf.ap[c]; aw[q] = cf.aw[c]; ae[q] = cf.ae[c]; an[q] = cf.an[c];
+            as[q] = cf.as[c]; at[q] = cf.at[c]; ab[q] = cf.ab[c]; su[q] = cf.su[c];
+            h0[q] = bx_ld(br.own + c);
+            hw[q] = bx_ld((ii > 0 ? br.own : br.phi) + c - 1);
+            he[q] = bx_ld((ii < br.n - 1 ? br.own : br.phi) + c + 1);
+            hn[q] = bx_ld(br.src_n + c + d.sj);
+            hs[q] = bx_ld(br.src_s + c - d.sj);
+            ht[q] = bx_ld(br.src_t + c + d.sk);
+            hb[q] = bx_ld(br.src_b + c - d.sk);
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < K; ++q) {
+        const int ii = br.lane + 64 * q;
+        if (ii < br.n) {
+            sw[ii] = (float)aw[q];
+            sd[ii] = (float)ap[q];
+            se[ii] = (float)ae[q];
+            live_here(an[q], as[q], at[q], ab[q], su[q]);
+        }
+    }
+    bool ok = true;
+#pragma unroll
+    for (int q = 0; q < K; ++q) {
+        const int ii = br.lane + 64 * q;
+        if (ii < br.n) {
+            const size_t c = br.row + ii;
+            const double *pw = (ii > 0 ? br.own : br.phi) + c - 1, *pe = (ii < br.n - 1 ? br.own : br.phi) + c + 1;
+            ok &= bx_await(br.own + c, h0[q], error);
+            ok &= bx_await(pw, hw[q], error);
+            ok &= bx_await(pe, he[q], error);
+            ok &= bx_await(br.src_n + c + d.sj, hn[q], error);
+            ok &= bx_await(br.src_s + c - d.sj, hs[q], error);
+            ok &= bx_await(br.src_t + c + d.sk, ht[q], error);
+            ok &= bx_await(br.src_b + c - d.sk, hb[q], error);
+            const double r = su[q] + aw[q] * hw[q] + ae[q] * he[q] + an[q] * hn[q] + as[q] * hs[q] +
+                             at[q] * ht[q] + ab[q] * hb[q] - ap[q] * h0[q];
+            hp[q] = h0[q];
+            sr[ii] = (float)r;
+        }
+    }
+    return ok;
+}
+
 struct BoxRange {
     int ibeg, iend, jbeg, jend, kbeg, kend;
 };
@@ -412,9 +487,25 @@ struct BoxLinesK {
         const size_t row = (size_t)k * d.sk + (size_t)j * d.sj + b.ibeg;
         double hp[M];
 #pragma unroll
+        for (int q = 0; q < M; ++q) hp[q] = 0.0;
+        const BoxRow br{own, phi, src_n, src_s, src_t, src_b, row, n, lane};
+        // lines of up to 256 cells (the box's, in practice): every load of
+        // every row at once; longer ones row by row
+        if constexpr (M >= 2) {
+            if (n <= 128) {
+                ok &= box_rows<2>(d, cf, br, hp, sw, sd, se, sr, error);
+                goto loaded;
+            }
+        }
+        if constexpr (M >= 4) {
+            if (n <= 256) {
+                ok &= box_rows<4>(d, cf, br, hp, sw, sd, se, sr, error);
+                goto loaded;
+            }
+        }
+#pragma unroll
         for (int q = 0; q < M; ++q) {
             const int ii = lane + 64 * q;
-            hp[q] = 0.0;
             if (ii < n) {
                 const size_t c = row + ii;
                 const double *pw = (ii > 0 ? own : phi) + c - 1, *pe = (ii < n - 1 ? own : phi) + c + 1;
@@ -439,6 +530,7 @@ struct BoxLinesK {
                 sr[ii] = (float)r;
             }
         }
+    loaded:
         // one lane's expired wait ends the whole wave (the task loop stays uniform)
         ok = !__any(!ok);
         if (!ok) break;
@@ -650,12 +742,12 @@ __global__ __launch_bounds__(256) void k_add_delphi_mx(Dims d, const double *__r
 template <typename TC>
 void launch_xline_t(hipStream_t st, const Dims &d, const MxCoefs<TC> &cf, double *phi, int ibeg, int iend, int jbeg,
                     int jend, int kbeg, int kend, int colour, int guard, const int *skip, const int *dbox = nullptr,
-                    int nblocks = 0) {
+                    int nblocks = 0, int line_batch = 1) {
     // with dbox the line range is read on the device: the host range is an
     // upper bound (the whole interior) and fixes the kernel's row capacity
     const int n = iend - ibeg + 1;
     const long long nlines = (long long)(kend - kbeg + 1) * ((jend - jbeg + 2) / 2);
-    const int nb = dbox ? nblocks : (int)std::min<long long>(nlines, 16384);
+    const int nb = dbox ? nblocks : (int)std::min<long long>((nlines + line_batch - 1) / line_batch, 16384);
 #define XL(MM) dispatch<XlineK<TC, MM>, 64>(k_xline_mx<TC, MM>, dim3(nb), dim3(64), 0, st, d, cf, phi, ibeg, iend, jbeg, jend, \
                                   kbeg, kend, colour, guard, skip, dbox)
     if (n <= 128) XL(2);
@@ -720,9 +812,10 @@ void launch_mixed_block_correction(hipStream_t st, const Dims &d, const MxFields
 }
 
 void launch_mixed_xline(hipStream_t st, const Dims &d, const MxFields &m, double *phi, int ibeg, int iend, int jbeg,
-                        int jend, int kbeg, int kend, int colour, const int *skip) {
+                        int jend, int kbeg, int kend, int colour, const int *skip, int line_batch) {
     MxCoefs<float> cf{m.ap, m.an, m.as, m.ae, m.aw, m.at, m.ab, m.su};
-    launch_xline_t<float>(st, d, cf, phi, ibeg, iend, jbeg, jend, kbeg, kend, colour, 0, skip);
+    launch_xline_t<float>(st, d, cf, phi, ibeg, iend, jbeg, jend, kbeg, kend, colour, 0, skip, nullptr, 0,
+                          std::max(1, line_batch));
 }
 
 void launch_mixed_yzline(hipStream_t st, const Dims &d, const MxFields &m, const double *hin, double *hout, int dir,

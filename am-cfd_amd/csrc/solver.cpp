@@ -92,6 +92,12 @@ int Solver::set_box_fused(int on) {
     return box_fused_ ? ensure_box_buffers(mx_box_passes_) : AMCFD_OK;
 }
 
+int Solver::set_line_batch(int lines) {
+    if (lines < 1 || lines > 64) return set_error(AMCFD_E_ARG, "line batch must be 1..64");
+    line_batch_ = lines;
+    return AMCFD_OK;
+}
+
 int Solver::set_mixed_schedule(const char *sched, int box_passes) {
     if (!sched || !*sched) return set_error(AMCFD_E_ARG, "empty pass schedule");
     int yz = 0;
@@ -158,7 +164,8 @@ int Solver::create(const amcfd_params &p, int mode) {
     int dev = 0;
     HIPCALL(hipGetDevice(&dev));
     if (mixed) {   // the captured loop's launch shapes for this size, from the tuning table if there is one
-        tuning_lookup((long long)(p.ni - 2) * (p.nj - 2) * (p.nk - 2), &graph_block_, &box_blocks_, &box_fused_);
+        tuning_lookup((long long)(p.ni - 2) * (p.nj - 2) * (p.nk - 2), &graph_block_, &box_blocks_, &box_fused_,
+                      &line_batch_);
         box_blocks_ = std::min(box_blocks_, max_box_blocks);
     }
     HIPCALL(hipStreamCreateWithFlags(&st_, hipStreamNonBlocking));
@@ -832,8 +839,8 @@ int Solver::energy_sweeps_mixed(const int *skip) {
     double *cur = f_.enthalpy, *other = mx_.halt;
     for (char c : mx_sched_) {
         if (c == 'x') {
-            launch_mixed_xline(st_, d_, mx_, cur, 1, d_.nim2, 1, d_.njm2, 1, d_.nkm2, 0, skip);
-            launch_mixed_xline(st_, d_, mx_, cur, 1, d_.nim2, 1, d_.njm2, 1, d_.nkm2, 1, skip);
+            launch_mixed_xline(st_, d_, mx_, cur, 1, d_.nim2, 1, d_.njm2, 1, d_.nkm2, 0, skip, line_batch_);
+            launch_mixed_xline(st_, d_, mx_, cur, 1, d_.nim2, 1, d_.njm2, 1, d_.nkm2, 1, skip, line_batch_);
         } else {
             launch_mixed_yzline(st_, d_, mx_, cur, other, c == 'y' ? 1 : 2, 1, d_.nim2, 1, d_.njm2, 1, d_.nkm2, skip);
             std::swap(cur, other);
@@ -979,6 +986,7 @@ int Solver::capture_mixed_graph() {
     gkey_.sched = mx_sched_;
     gkey_.box_passes = mx_box_passes_;
     gkey_.box_fused = box_fused_;
+    gkey_.line_batch = line_batch_;
     return 0;
 }
 
@@ -988,7 +996,7 @@ int Solver::solve_step_mixed_graph() {
     // every launch argument the graph holds by value: rebuild when one moved
     if (!gexec_ || gkey_.nk != d_.nk || gkey_.blocks != graph_block_ || gkey_.box_blocks != box_blocks_ ||
         gkey_.delt != p_.delt || gkey_.scanvel != scanvel_ || gkey_.sched != mx_sched_ ||
-        gkey_.box_passes != mx_box_passes_ || gkey_.box_fused != box_fused_) {
+        gkey_.box_passes != mx_box_passes_ || gkey_.box_fused != box_fused_ || gkey_.line_batch != line_batch_) {
         int st = capture_mixed_graph();
         if (st) return st;
     }
@@ -1036,8 +1044,8 @@ int Solver::group_check() const {
 
 std::string Solver::group_key() const {
     char buf[512];
-    std::snprintf(buf, sizeof buf, "%p:%d:%d:%d:%a:%a:%s:%d:%d:%p;", (const void *)this, d_.nk, graph_block_,
-                  box_blocks_, p_.delt, scanvel_, mx_sched_.c_str(), mx_box_passes_, box_fused_,
+    std::snprintf(buf, sizeof buf, "%p:%d:%d:%d:%a:%a:%s:%d:%d:%d:%p;", (const void *)this, d_.nk, graph_block_,
+                  box_blocks_, p_.delt, scanvel_, mx_sched_.c_str(), mx_box_passes_, box_fused_, line_batch_,
                   (const void *)box_bufs_);
     return buf;
 }

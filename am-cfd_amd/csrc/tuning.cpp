@@ -5,12 +5,15 @@
 // results: the iterations per graph launch (the scalars are read once per
 // launch), the workgroups that stride over the melt-pool box, and whether a
 // box variable's passes run in one persistent launch or one launch per half
-// pass.  Their best values depend on the grid size, so the tuner measures
-// each candidate on representative grids and writes one row per size class:
-//     max_cells  graph_block  box_blocks  [box_fused]
-// ('#' starts a comment; box_fused 0 or 1, default 1).  A solver takes the
-// first row whose max_cells is at least its interior cell count; with no row
-// (or no table) it keeps the built-in 8, 256 and 1.  The table is the file named by $AMCFD_TUNING, else
+// pass, and the line batch of the enthalpy's x-line passes (consecutive
+// lines per workgroup).  Their best values depend on the grid size, so the
+// tuner measures each candidate on representative grids and writes one row
+// per size class:
+//     max_cells  graph_block  box_blocks  [box_fused  [line_batch]]
+// ('#' starts a comment; box_fused 0 or 1, default 1; line_batch 1..64,
+// default 1).  A solver takes the first row whose max_cells is at least its
+// interior cell count; with no row (or no table) it keeps the built-in 8,
+// 256, 1 and 1.  The table is the file named by $AMCFD_TUNING, else
 // tuning.txt beside libamcfd.so.
 
 #include <dlfcn.h>
@@ -31,7 +34,7 @@ namespace {
 
 struct TuneRow {
     long long max_cells;
-    int graph_block, box_blocks, box_fused;
+    int graph_block, box_blocks, box_fused, line_batch;
 };
 
 // rows in file order; a malformed line is an error (the whole table is refused)
@@ -45,30 +48,32 @@ int parse_table(const char *path, std::vector<TuneRow> &rows) {
         char *hash = std::strchr(line, '#');
         if (hash) *hash = '\0';
         long long cells;
-        int gb, bb, bf = 1;
+        int gb, bb, bf = 1, lb = 1;
         char extra;
         bool blank = true;
         for (const char *c = line; *c; ++c) blank = blank && std::isspace((unsigned char)*c);
         if (blank) continue;   // blank or comment
-        const int n = std::sscanf(line, "%lld %d %d %d %c", &cells, &gb, &bb, &bf, &extra);
-        if ((n != 3 && n != 4) || cells < 1 || gb < 1 || gb > 64 || bb < 1 || bb > 4096 || (bf != 0 && bf != 1)) {
+        const int n = std::sscanf(line, "%lld %d %d %d %d %c", &cells, &gb, &bb, &bf, &lb, &extra);
+        if (n < 3 || n > 5 || cells < 1 || gb < 1 || gb > 64 || bb < 1 || bb > 4096 || (bf != 0 && bf != 1) ||
+            lb < 1 || lb > 64) {
             st = set_error(AMCFD_E_PARSE, std::string(path) + ":" + std::to_string(lineno) +
                                               ": expected 'max_cells graph_block(1..64) box_blocks(1..4096) "
-                                              "[box_fused(0|1)]'");
+                                              "[box_fused(0|1) [line_batch(1..64)]]'");
             break;
         }
-        rows.push_back(TuneRow{cells, gb, bb, bf});
+        rows.push_back(TuneRow{cells, gb, bb, bf, lb});
     }
     std::fclose(f);
     return st;
 }
 
-bool choose(const std::vector<TuneRow> &rows, long long cells, int *gb, int *bb, int *bf) {
+bool choose(const std::vector<TuneRow> &rows, long long cells, int *gb, int *bb, int *bf, int *lb) {
     for (const TuneRow &r : rows)
         if (cells <= r.max_cells) {
             *gb = r.graph_block;
             *bb = r.box_blocks;
             *bf = r.box_fused;
+            *lb = r.line_batch;
             return true;
         }
     return false;
@@ -100,30 +105,32 @@ void load_default() {
 
 }  // namespace
 
-void tuning_lookup(long long cells, int *graph_block, int *box_blocks, int *box_fused) {
+void tuning_lookup(long long cells, int *graph_block, int *box_blocks, int *box_fused, int *line_batch) {
     std::call_once(table_once, load_default);
-    choose(table, cells, graph_block, box_blocks, box_fused);
+    choose(table, cells, graph_block, box_blocks, box_fused, line_batch);
 }
 
 }  // namespace amcfd
 
 extern "C" {
 
-int amcfd_tuning_choice(const char *path, long long cells, int *graph_block, int *box_blocks, int *box_fused) {
+int amcfd_tuning_choice(const char *path, long long cells, int *graph_block, int *box_blocks, int *box_fused,
+                        int *line_batch) {
     using namespace amcfd;
-    if (!graph_block || !box_blocks || !box_fused) return set_error(AMCFD_E_ARG, "null argument");
+    if (!graph_block || !box_blocks || !box_fused || !line_batch) return set_error(AMCFD_E_ARG, "null argument");
     if (cells < 1) return set_error(AMCFD_E_ARG, "cells must be positive");
     *graph_block = 8;
     *box_blocks = 256;
     *box_fused = 1;
+    *line_batch = 1;
     if (!path) {
-        tuning_lookup(cells, graph_block, box_blocks, box_fused);
+        tuning_lookup(cells, graph_block, box_blocks, box_fused, line_batch);
         return AMCFD_OK;
     }
     std::vector<TuneRow> rows;
     const int st = parse_table(path, rows);
     if (st) return st;
-    choose(rows, cells, graph_block, box_blocks, box_fused);
+    choose(rows, cells, graph_block, box_blocks, box_fused, line_batch);
     return AMCFD_OK;
 }
 

@@ -181,6 +181,24 @@ def batch_solo(amcfd, decks, tmp, n, warmup, steps):
     return {"seconds": ts, "runs": runs}
 
 
+def batch_grouped(amcfd, decks, tmp, n, warmup, steps):
+    """Instances 0..n-1 in the mixed mode, grouped (amcfd_run_grouped: every
+    kernel of an iteration one launch over all instances, captured): seconds
+    over the timed steps, iterations, and per instance its iteration count
+    and temperature digest."""
+    ss = [amcfd.Solver(amcfd.Deck(decks.write_sweep_instance(s, tmp)), mode=amcfd.MODE_MIXED) for s in range(n)]
+    amcfd.run_grouped(ss, warmup)
+    i0 = sum(s.scalars()["itertot"] for s in ss)
+    t0 = time.perf_counter()
+    amcfd.run_grouped(ss, steps)
+    tg = time.perf_counter() - t0
+    iters = sum(s.scalars()["itertot"] for s in ss) - i0
+    runs = [[s.scalars()["itertot"], temp_digest(s)] for s in ss]
+    for s in ss:
+        s.close()
+    return tg, iters, runs
+
+
 def batch_side_by_side(amcfd, decks, tmp, n, warmup, steps, workers, record_steps, train_iters, ws_out):
     """The seeded parameter sweep side by side (amcfd_run_batch, one stream
     per instance) in the FP64 mode and in the mixed mode (captured loops);
@@ -212,9 +230,9 @@ def batch_side_by_side(amcfd, decks, tmp, n, warmup, steps, workers, record_step
 
     tb, iters, runs = side_by_side(amcfd.MODE_FP64, 0)
     log("batch: FP64 side by side %.3f s" % tb)
-    tm, itm, _ = side_by_side(amcfd.MODE_MIXED, record_steps)
+    tm, itm, mruns = side_by_side(amcfd.MODE_MIXED, record_steps)
     log("batch: mixed side by side %.3f s" % tm)
-    out = {"cells": cells, "fp64": [tb, iters, runs], "mixed": [tm, itm], "train": None}
+    out = {"cells": cells, "fp64": [tb, iters, runs], "mixed": [tm, itm, mruns], "train": None}
     if net is not None and net.samples() > 0:
         t0 = time.perf_counter()
         n1 = (3 * train_iters) // 5
@@ -241,6 +259,8 @@ def batch_leg(amcfd, decks, tmp, args, ws_out):
     solo = batch_solo(amcfd, decks, tmp, n, warmup, steps)
     ts = solo["seconds"]
     log("batch leg: FP64 one at a time %.3f s" % ts)
+    tg, itg, gruns = batch_grouped(amcfd, decks, tmp, n, warmup, steps)
+    log("batch leg: mixed grouped %.3f s" % tg)
     cmd = [sys.executable, os.path.abspath(__file__), "--batch-child", "--batch", str(n), "--batch-warmup",
            str(warmup), "--batch-steps", str(steps), "--batch-workers", str(args.batch_workers),
            "--record-steps", str(args.record_steps), "--train-iters", str(args.train_iters), "--ws-out", ws_out]
@@ -248,7 +268,7 @@ def batch_leg(amcfd, decks, tmp, args, ws_out):
     c = json.loads(child.strip().splitlines()[-1])
     work = sum(c["cells"]) * steps
     tb, iters, runs = c["fp64"]
-    tm, itm = c["mixed"]
+    tm, itm, mruns = c["mixed"]
     res = {"instances": n, "interior_cells": sum(c["cells"]), "warmup_steps": warmup, "timed_steps": steps,
            "workers": args.batch_workers,
            "fp64_one_at_a_time": {"value": work / ts, "seconds": ts, "hw_queues": 4},
@@ -258,7 +278,13 @@ def batch_leg(amcfd, decks, tmp, args, ws_out):
            "mixed_side_by_side": {"value": work / tm, "seconds": tm, "iters_per_instance_step": itm / (n * steps),
                                   "speedup_vs_fp64_one_at_a_time": ts / tm, "speedup_vs_fp64_side_by_side": tb / tm,
                                   "hw_queues": 32},
-           "note": "instances 0..n-1 of the seeded sweep (numpy PCG64 seed 20261203+s); side by side = "
+           "mixed_grouped": {"value": work / tg, "seconds": tg, "iters_per_instance_step": itg / (n * steps),
+                             "identical_to_side_by_side": gruns == mruns,
+                             "speedup_vs_fp64_one_at_a_time": ts / tg, "speedup_vs_fp64_side_by_side": tb / tg,
+                             "speedup_vs_mixed_side_by_side": tm / tg, "hw_queues": 4},
+           "note": "instances 0..n-1 of the seeded sweep (numpy PCG64 seed 20261203+s); grouped = "
+                   "amcfd_run_grouped, the instances in lock step with every kernel of an iteration one launch "
+                   "over all of them (one stream, captured graph, in this process); side by side = "
                    "amcfd_run_batch, each instance on its own stream, the mixed instances running their SIMPLE "
                    "loops from captured graphs (a child process with 32 hardware queues); FP64 side by side "
                    "compared with one at a time by iteration count and temperature digest"}

@@ -322,19 +322,27 @@ int  amcfd_set_graph(amcfd_solver *s, int on, int block, int box_blocks);
  * are bounded and an expired one ends the step with AMCFD_E_HIP.  Both forms
  * give the same bits.  No reference counterpart. */
 int  amcfd_set_box_fused(amcfd_solver *s, int on);
-/* Mixed mode: the captured loop's current settings.  A mixed-mode solver
- * starts from the tuning table's row for its interior cell count (below). */
-int  amcfd_get_graph(const amcfd_solver *s, int *on, int *block, int *box_blocks, int *box_fused);
+/* Mixed mode: consecutive x-lines each workgroup solves in the enthalpy's
+ * red-black passes (1..64, default 1).  The lines of a half pass are
+ * independent, so this changes the passes' speed and never their results.
+ * No reference counterpart. */
+int  amcfd_set_line_batch(amcfd_solver *s, int lines);
+/* Mixed mode: the captured loop's current settings and the line batch.  A
+ * mixed-mode solver starts from the tuning table's row for its interior cell
+ * count (below). */
+int  amcfd_get_graph(const amcfd_solver *s, int *on, int *block, int *box_blocks, int *box_fused, int *line_batch);
 
 /* The launch-shape table of the mixed mode (no reference counterpart),
  * written by am-cfd_amd/autotune.py: lines "max_cells graph_block
- * box_blocks [box_fused]", '#' comments; the first row with max_cells >= the
- * grid's interior cells applies, else the built-in 8, 256 and 1.  These
+ * box_blocks [box_fused [line_batch]]", '#' comments; the first row with
+ * max_cells >= the grid's interior cells applies, else the built-in 8, 256,
+ * 1 and 1.  These
  * values change the captured loop's speed, never its results.  The library's table is the
  * file named by $AMCFD_TUNING, else tuning.txt beside libamcfd.so, read
  * once.  amcfd_tuning_choice reports the choice for `cells` from the file at
  * `path`, or from the library's table when path is NULL; it needs no GPU. */
-int  amcfd_tuning_choice(const char *path, long long cells, int *graph_block, int *box_blocks, int *box_fused);
+int  amcfd_tuning_choice(const char *path, long long cells, int *graph_block, int *box_blocks, int *box_fused,
+                         int *line_batch);
 
 /* Copy the whole state of src into dst (same deck, any modes): every
  * solution field, the grid, the laser and time-loop state and the scalars,
