@@ -379,6 +379,13 @@ struct AssembleMxK {
     }
     __shared__ double scratch[2 * 4];
     __shared__ double s_blk[4 * MTY * MTX];
+    // plane k of the tile and its one-cell ring (row 0 / MTY + 1: south /
+    // north, column 0 / MTX + 1: west / east), double-buffered in k: the
+    // fields read at horizontal neighbours, FP32 where the coefficients use
+    // them in FP32, the enthalpy in FP64
+    __shared__ float s_df[2][MTY + 2][MTX + 2], s_rh[2][MTY + 2][MTX + 2], s_u[2][MTY + 2][MTX + 2];
+    __shared__ float s_v[2][MTY + 2][MTX + 2], s_fl[2][MTY + 2][MTX + 2];
+    __shared__ double s_h[2][MTY + 2][MTX + 2];
     double cbl = 0.0, cblp = 0.0, cblm = 0.0, cblc = 0.0;   // this column's block-correction partials
     const int tx = threadIdx.x, ty = threadIdx.y;
     const int gx = vb.gx, gy = vb.gy;
@@ -390,45 +397,111 @@ struct AssembleMxK {
     const int i = 1 + bx * MTX + tx;
     const int j = 1 + by * MTY + ty;
     const int k0 = 1 + bz * KA;
+    const int kend = min(k0 + KA - 1, d.nkm2);
     const bool col_ok = (i <= d.nim2) && (j <= d.njm2) && (k0 <= d.nkm2);
+    // the staged cells: every thread its own (interior or the east / north
+    // ghost), and 136 threads one cell of the ring each
+    const bool own_ok = i <= d.nim1 && j <= d.njm1;
+    int hi = -1, hj = -1, hii = 0, hjj = 0;   // the ring cell of this thread
+    if (ty == 0) { hi = i; hj = j - 1; hii = tx + 1; hjj = 0; }
+    else if (ty == 1) { hi = i; hj = 1 + by * MTY + MTY; hii = tx + 1; hjj = MTY + 1; }
+    else if (ty == 2 && tx < MTY) { hi = 1 + bx * MTX - 1; hj = 1 + by * MTY + tx; hii = 0; hjj = tx + 1; }
+    else if (ty == 3 && tx < MTY) { hi = 1 + bx * MTX + MTX; hj = 1 + by * MTY + tx; hii = MTX + 1; hjj = tx + 1; }
+    const bool ring_ok = hi >= 0 && hi <= d.nim1 && hj >= 0 && hj <= d.njm1;
+    const size_t own_c = own_ok ? cix(d, i, j, 0) : 0, ring_c = ring_ok ? cix(d, hi, hj, 0) : 0;
+    // one staged cell's values (FP64 as loaded)
+    struct Staged {
+        double df, rh, u, v, fl, h;
+    };
+    auto load_cell = [&](size_t c, Staged &v) {
+        v.df = diff[c]; v.rh = rho[c]; v.u = uvel[c]; v.v = vvel[c]; v.fl = fracl[c]; v.h = h[c];
+    };
+    auto store_cell = [&](int buf, int jj, int ii, const Staged &v) {
+        s_df[buf][jj][ii] = (float)v.df; s_rh[buf][jj][ii] = (float)v.rh; s_u[buf][jj][ii] = (float)v.u;
+        s_v[buf][jj][ii] = (float)v.v; s_fl[buf][jj][ii] = (float)v.fl; s_h[buf][jj][ii] = v.h;
+    };
     double sumd = 0.0, sumh = 0.0;
+    // column constants
+    float fxi = 0.f, fxw = 0.f, fyj = 0.f, fys = 0.f, ryn = 0.f, rys = 0.f, rxe = 0.f, rxw = 0.f, faij = 0.f;
+    double dxy = 0.0, src_col = 0.0, ztop = 0.0;
     if (col_ok) {
-        const double ztop = g.z[d.nkm1];
-        const float fxi = (float)g.fracx[i], fxw = (float)g.fracx[i - 1];
-        const float fyj = (float)g.fracy[j], fys = (float)g.fracy[j - 1];
-        const float ryn = (float)g.dypsinv[j < d.njm2 ? j + 1 : d.njm1], rys = (float)g.dypsinv[j > 1 ? j : 1];
-        const float rxe = (float)g.dxpwinv[i < d.nim2 ? i + 1 : d.nim1], rxw = (float)g.dxpwinv[i > 1 ? i : 1];
-        const double dxy = (g.xu[i + 1] - g.xu[i]) * (g.yv[j + 1] - g.yv[j]);
-        const double aij = g.areaij[j * d.ni + i];
-        const float faij = (float)aij;
+        ztop = g.z[d.nkm1];
+        fxi = (float)g.fracx[i]; fxw = (float)g.fracx[i - 1];
+        fyj = (float)g.fracy[j]; fys = (float)g.fracy[j - 1];
+        ryn = (float)g.dypsinv[j < d.njm2 ? j + 1 : d.njm1]; rys = (float)g.dypsinv[j > 1 ? j : 1];
+        rxe = (float)g.dxpwinv[i < d.nim2 ? i + 1 : d.nim1]; rxw = (float)g.dxpwinv[i > 1 ? i : 1];
+        dxy = (g.xu[i + 1] - g.xu[i]) * (g.yv[j + 1] - g.yv[j]);
+        faij = (float)g.areaij[j * d.ni + i];
         // volumetric source: its horizontal Gaussian is the column's own
-        double src_col = 0.0;
         if (q.heatthick > 0.0) {
             const double xb = q.beamx - g.x[i], yb = q.beamy - g.y[j];
             src_col = q.avolpow * q.avolfact / M_PI / (q.heatrb * q.heatrb) / q.heatthick * q.apowseta *
                       libm_exp(-q.avolfact / (q.heatrb * q.heatrb) * ((xb * xb) + (yb * yb))) * (double)q.pi;
         }
-        const double rdelt = 1.0 / q.delt;
-        const float rurfh = 1.0f / (float)q.urfh;
-        const int kend = min(k0 + KA - 1, d.nkm2);
-        size_t c = cix(d, i, j, k0);
-        float dB = (float)diff[c - d.sk], dP = (float)diff[c];
-        double rP = rho[c];
-        float wb = (float)wvel[c];
-        float flB = (float)fracl[c - d.sk];
-        double flP = fracl[c];
-        double hB = h[c - d.sk], hP = h[c];
-        for (int k = k0; k <= kend; ++k, c += d.sk) {
-            const float dT = (float)diff[c + d.sk], wt = (float)wvel[c + d.sk];
-            const double rT = rho[c + d.sk], flT = fracl[c + d.sk], hT = h[c + d.sk];
-            const float dN = (float)diff[c + d.sj], dS = (float)diff[c - d.sj];
-            const float dE = (float)diff[c + 1], dW = (float)diff[c - 1];
-            const float rN = (float)rho[c + d.sj], rE = (float)rho[c + 1], rPf = (float)rP, rTf = (float)rT;
-            const float vs = (float)vvel[c], vn = (float)vvel[c + d.sj], uw = (float)uvel[c], ue = (float)uvel[c + 1];
-            const float flE = (float)fracl[c + 1], flW = (float)fracl[c - 1];
-            const float flN = (float)fracl[c + d.sj], flS = (float)fracl[c - d.sj];
-            const double hN = h[c + d.sj], hS = h[c - d.sj], hE = h[c + 1], hW = h[c - 1];
-            const double hnP = hnot[c], flnP = fraclnot[c];
+    }
+    const double rdelt = 1.0 / q.delt;
+    const float rurfh = 1.0f / (float)q.urfh;
+    // the march: plane k from the staged buffer k & 1, its bottom and centre
+    // values carried in registers, its top values (plane k + 1) loaded one
+    // plane ahead and staged for the next plane
+    Staged own{}, ring{};
+    double wnext = 0.0, hn_next = 0.0, fln_next = 0.0;
+    float dB = 0.f, dP = 0.f, wb = 0.f, flB = 0.f;
+    double rP = 0.0, flP = 0.0, hB = 0.0, hP = 0.0;
+    if (k0 <= d.nkm2) {
+        if (own_ok) {
+            const size_t c = own_c + (size_t)k0 * d.sk;
+            Staged p0;
+            load_cell(c, p0);
+            store_cell(k0 & 1, ty + 1, tx + 1, p0);
+            dB = (float)diff[c - d.sk]; flB = (float)fracl[c - d.sk]; hB = h[c - d.sk];
+            dP = (float)p0.df; rP = p0.rh; wb = (float)wvel[c]; flP = p0.fl; hP = p0.h;
+            load_cell(c + d.sk, own);
+            wnext = wvel[c + d.sk];
+            hn_next = hnot[c];
+            fln_next = fraclnot[c];
+        }
+        if (ring_ok) {
+            const size_t c = ring_c + (size_t)k0 * d.sk;
+            Staged r0;
+            load_cell(c, r0);
+            store_cell(k0 & 1, hjj, hii, r0);
+            load_cell(c + d.sk, ring);
+        }
+    }
+    __syncthreads();
+    for (int k = k0; k <= kend; ++k) {
+        // plane k + 1 to its buffer (values loaded last plane); its top
+        // values are this plane's top neighbours
+        const int cur = k & 1, nxt = cur ^ 1;
+        if (own_ok) store_cell(nxt, ty + 1, tx + 1, own);
+        if (ring_ok) store_cell(nxt, hjj, hii, ring);
+        const float dT = (float)own.df, wt = (float)wnext;
+        const double rT = own.rh, flT = own.fl, hT = own.h;
+        const double hnP = hn_next, flnP = fln_next;
+        // plane k + 2 and this plane's successor's old values, in flight
+        // while plane k is computed
+        if (k + 1 <= kend) {
+            if (own_ok) {
+                const size_t c = own_c + (size_t)(k + 1) * d.sk;
+                load_cell(c + d.sk, own);
+                wnext = wvel[c + d.sk];
+                hn_next = hnot[c];
+                fln_next = fraclnot[c];
+            }
+            if (ring_ok) load_cell(ring_c + (size_t)(k + 2) * d.sk, ring);
+        }
+        if (col_ok) {
+            const size_t c = own_c + (size_t)k * d.sk;
+            const int jj = ty + 1, ii = tx + 1;
+            const float dN = s_df[cur][jj + 1][ii], dS = s_df[cur][jj - 1][ii];
+            const float dE = s_df[cur][jj][ii + 1], dW = s_df[cur][jj][ii - 1];
+            const float rN = s_rh[cur][jj + 1][ii], rE = s_rh[cur][jj][ii + 1], rPf = (float)rP, rTf = (float)rT;
+            const float vs = s_v[cur][jj][ii], vn = s_v[cur][jj + 1][ii], uw = s_u[cur][jj][ii], ue = s_u[cur][jj][ii + 1];
+            const float flE = s_fl[cur][jj][ii + 1], flW = s_fl[cur][jj][ii - 1];
+            const float flN = s_fl[cur][jj + 1][ii], flS = s_fl[cur][jj - 1][ii];
+            const double hN = s_h[cur][jj + 1][ii], hS = s_h[cur][jj - 1][ii], hE = s_h[cur][jj][ii + 1],
+                         hW = s_h[cur][jj][ii - 1];
             const float aik = (float)g.areaik[k * d.ni + i], ajk = (float)g.areajk[k * d.nj + j];
             const double vol = dxy * (g.zw[k + 1] - g.zw[k]);
             float an, as, ae, aw, at, ab;
@@ -486,12 +559,13 @@ struct AssembleMxK {
             cblp += (double)ae;
             cblm += (double)aw;
             cblc += num - rap * hP;
-            // the next plane's bottom and centre values
-            dB = dP; dP = dT;
-            rP = rT; wb = wt;
-            flB = (float)flP; flP = flT;
-            hB = hP; hP = hT;
         }
+        // the next plane's bottom and centre values
+        dB = dP; dP = dT;
+        rP = rT; wb = wt;
+        flB = (float)flP; flP = flT;
+        hB = hP; hP = hT;
+        __syncthreads();
     }
     // column partials of this tile: the j-rows of each i, in order
     s_blk[(0 * MTY + ty) * MTX + tx] = cbl;
