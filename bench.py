@@ -448,6 +448,12 @@ def main():
     ulp.set_field("enthalpy", np.nextafter(ulp.field("enthalpy"), np.inf))
     ws_d = amcfd.Solver(deck, mode=amcfd.MODE_MIXED)
     ws_d.copy_state_from(fp)
+    # the FP64 mode's own iteration error on these steps: the same start with
+    # twice the iterations per step
+    it2_dir = os.path.join(tmp, "maxit2")
+    os.makedirs(it2_dir, exist_ok=True)
+    it2 = amcfd.Solver(amcfd.Deck(decks.write_case(args.case, it2_dir, maxit=2 * int(p.maxit))), mode=amcfd.MODE_FP64)
+    it2.copy_state_from(fp)
     wsa_d = None
     if train:
         net.attach(ws_d, amcfd.WS_APPLY)
@@ -462,7 +468,8 @@ def main():
     n1, ms1 = fp.kernel_stats("sweep_h")
     v64 = dom * args.steps / el64
     sweep64_ms = (ms1 - ms0) / max(n1 - n0, 1)
-    runs = [(mx_d, "mixed (drift)"), (ulp, "fp64 + 1 ulp"), (ws_d, "mixed + warm start (drift)")]
+    runs = [(mx_d, "mixed (drift)"), (ulp, "fp64 + 1 ulp"), (ws_d, "mixed + warm start (drift)"),
+            (it2, "fp64 at twice maxit")]
     if wsa_d is not None:
         runs.append((wsa_d, "mixed + unguarded warm start (drift)"))
     for s, label in runs:
@@ -471,10 +478,14 @@ def main():
         log("%s: %d steps" % (label, args.steps))
     drift_block = {"from": "one FP64 state after %d warm-up steps, %d steps on" % (args.warmup, args.steps),
                    "mixed": drift(np, mx_d, fp), "mixed_warm_start": drift(np, ws_d, fp),
-                   "fp64_one_ulp": drift(np, ulp, fp),
+                   "fp64_one_ulp": drift(np, ulp, fp), "fp64_twice_maxit": drift(np, it2, fp),
+                   "mixed_vs_fp64_twice_maxit": drift(np, mx_d, it2),
                    "note": "each run starts from the same FP64 state (amcfd_copy_state) and runs the same steps; "
                            "fp64_one_ulp moves that state's enthalpy up by one ulp; a step here runs to maxit "
-                           "without reaching the tolerance, so a last-bit difference grows into this floor"}
+                           "without reaching the tolerance, so a last-bit difference grows into this floor; "
+                           "fp64_twice_maxit runs the FP64 mode with twice the iterations per step: its distance "
+                           "from the plain FP64 run is the iteration error the maxit-bound steps carry; "
+                           "mixed_vs_fp64_twice_maxit is the mixed run against that one"}
     drift_block["mixed_warm_start"]["steps_from_guess"], drift_block["mixed_warm_start"]["steps_guarded"] = \
         ws_d.warm_start_counts()
     if wsa_d is not None:
@@ -483,7 +494,7 @@ def main():
             "every step started from the guess (mode 3); these steps end at maxit, so the start moves the "
             "answer: the reason mode 2 keeps such runs on the plain start")
         wsa_d.close()
-    for s in (mx_d, ulp, ws_d):
+    for s in (mx_d, ulp, ws_d, it2):
         s.close()
     fp.close()
     # mixed, and mixed + warm start, from the start of the run (captured loops)

@@ -150,6 +150,47 @@ def test_gpu_matches_oracle_surfactant(case_dir, amcfd, oracle_mod):
     assert steps >= 20
 
 
+def test_gpu_plate120_whole_run_matches_the_reference(case_dir, amcfd):
+    """The reference's CPU configuration (the 120x48x32 plate, 400 steps,
+    39908 SIMPLE iterations) run whole in the plain FP64 mode and held to the
+    compiled reference's own fixtures, with no oracle in the loop: every
+    step's iteration counts, beam cell, melt-pool box and melt-pool scalars
+    equal to the reference's, its residual and heat-balance scalars within
+    1e-6 (they are folded by trees), and the SHA-256 of all 20 fields equal
+    to the reference's at every 50th step."""
+    import csv
+    import hashlib
+    import json
+    from conftest import golden_path
+    deck = amcfd.Deck(case_dir("plate120"))
+    gpu = amcfd.Solver(deck)
+    rows = list(csv.DictReader(open(golden_path("plate120_scalars.csv"))))
+    dig = json.load(open(golden_path("plate120_digests.json")))
+    assert len(rows) == 400
+    checked, fin = 0, False
+    for step, row in enumerate(rows, start=1):
+        fin = gpu.step()
+        s = gpu.scalars()
+        if step % 50 == 0:
+            print("plate120 step %d: itertot %d" % (step, s["itertot"]), flush=True)
+        for key in ("niter", "itertot", "powerindicator", "istart", "jstart", "nk"):
+            assert s[key] == int(row[key]), (step, key, s[key], row[key])
+        if s["tpeak"] > deck.params.tsolid:
+            for key in ("istat", "iend", "jstat", "jend", "kstat"):
+                assert s[key] == int(row[key]), (step, key, s[key], row[key])
+        for key in ("timet", "tpeak", "alen", "depth", "width", "heatinlaser", "beamposx", "beamposy"):
+            a, b = s[key], float(row[key])
+            assert a == b or (np.isnan(a) and np.isnan(b)), (step, key, a, b)
+        for key in ("resorh", "ratio"):
+            a, b = s[key], float(row[key])
+            assert abs(a - b) <= 1e-6 * max(abs(b), 1e-30), (step, key, a, b)
+        for fld, rec in dig["steps"].get(str(step), {}).items():
+            h = hashlib.sha256(np.ascontiguousarray(gpu.field(fld), dtype="<f8").tobytes()).hexdigest()
+            assert h == rec["sha256"], (step, fld)
+            checked += 1
+    assert checked == 8 * 20 and fin
+
+
 def test_gpu_matches_oracle_plate120_first_steps(case_dir, amcfd, oracle_mod):
     steps, worst = compare_run(amcfd, oracle_mod, case_dir("plate120"), max_steps=4)
     print("plate120", steps, {k: "%.1e" % v for k, v in worst.items()})
