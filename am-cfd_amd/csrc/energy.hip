@@ -364,7 +364,7 @@ __device__ __forceinline__ float hmean_f32(float p, float q, float wq, float wp)
 
 template <int SCH>
 struct AssembleMxK {
-    static constexpr int family = 20 + SCH, rank = 0;
+    static constexpr int family = 30 + SCH, rank = 0;
     static __device__ __forceinline__ void run(const VB &vb, Dims d, AsmConst q, DevGeom g, const double *__restrict__ uvel, const double *__restrict__ vvel,
                      const double *__restrict__ wvel, const double *__restrict__ rho,
                      const double *__restrict__ diff, const double *__restrict__ hnot,

@@ -3,6 +3,7 @@
 // i in [istat+1, iend-1] (BoundCondManager.cpp:438-446); blocks are 64 (i) x
 // 4 (j) threads, one box plane per grid z-slice.
 
+#include "boxprep.h"
 #include "fv.h"
 
 namespace amcfd {
@@ -138,6 +139,42 @@ __global__ __launch_bounds__(256) void k_bc_flow(Dims d, BoxDev b0, int ivar, do
                                                  const double *__restrict__ vis, double *phi, const int *dbox,
                                                  const int *skip) {
     BcFlowK::run(vb_hw(), d, b0, ivar, a_oxygen, dgdtp, dxpwinv, dypsinv, dzpbinv, fracx, fracy, temp, fracl, vis, phi, dbox, skip);
+}
+
+// k_bc_flow and the preparation of the box passes' buffers (BoxPrepK) in
+// one launch: they touch different memory, and both end before the box
+// passes start
+struct BcFlowPrepK {
+    static constexpr int family = 23, rank = 0;
+    static __device__ __forceinline__ void run(const VB &vb, Dims d, BoxDev b0, int ivar, double a_oxygen, double dgdtp,
+                                               const double *__restrict__ dxpwinv,
+                                               const double *__restrict__ dypsinv,
+                                               const double *__restrict__ dzpbinv,
+                                               const double *__restrict__ fracx,
+                                               const double *__restrict__ fracy,
+                                               const double *__restrict__ temp,
+                                               const double *__restrict__ fracl,
+                                               const double *__restrict__ vis, double *phi, const int *dbox,
+                                               const int *skip, double *bufs, size_t bstride, int passes,
+                                               unsigned int *claim) {
+        BoxPrepK::run(vb, d, bufs, bstride, passes, d.nkm2, claim, skip, dbox);
+        BcFlowK::run(vb, d, b0, ivar, a_oxygen, dgdtp, dxpwinv, dypsinv, dzpbinv, fracx, fracy, temp, fracl, vis, phi,
+                     dbox, skip);
+    }
+};
+__global__ __launch_bounds__(256) void k_bc_flow_prep(Dims d, BoxDev b0, int ivar, double a_oxygen, double dgdtp,
+                                                      const double *__restrict__ dxpwinv,
+                                                      const double *__restrict__ dypsinv,
+                                                      const double *__restrict__ dzpbinv,
+                                                      const double *__restrict__ fracx,
+                                                      const double *__restrict__ fracy,
+                                                      const double *__restrict__ temp,
+                                                      const double *__restrict__ fracl,
+                                                      const double *__restrict__ vis, double *phi, const int *dbox,
+                                                      const int *skip, double *bufs, size_t bstride, int passes,
+                                                      unsigned int *claim) {
+    BcFlowPrepK::run(vb_hw(), d, b0, ivar, a_oxygen, dgdtp, dxpwinv, dypsinv, dzpbinv, fracx, fracy, temp, fracl, vis,
+                     phi, dbox, skip, bufs, bstride, passes, claim);
 }
 
 struct MomConst {
@@ -602,7 +639,7 @@ void launch_clean_velocities(hipStream_t st, const Dims &d, DevFields f, const B
 }
 
 void launch_bc_flow(hipStream_t st, const Dims &d, const DevGeom &g, DevFields f, const Box &b,
-                    const amcfd_params &p, int ivar, const BoxLaunch &bl) {
+                    const amcfd_params &p, int ivar, const BoxLaunch &bl, const BoxPrep *prep) {
     double *phi = ivar == 1 ? f.uvel : ivar == 2 ? f.vvel : ivar == 3 ? f.wvel : f.pp;
     int nb;
     if (bl.dbox) {
@@ -613,8 +650,14 @@ void launch_bc_flow(hipStream_t st, const Dims &d, const DevGeom &g, DevFields f
         if (n <= 0) return;
         nb = (n + 255) / 256;
     }
-    dispatch<BcFlowK, 256>(k_bc_flow, dim3(nb), dim3(256), 0, st, d, to_dev(b), ivar, p.a_oxygen, p.dgdtp, g.dxpwinv,
-                       g.dypsinv, g.dzpbinv, g.fracx, g.fracy, f.temp, f.fracl, f.vis, phi, bl.dbox, bl.skip);
+    if (prep && bl.dbox)
+        dispatch<BcFlowPrepK, 256>(k_bc_flow_prep, dim3(nb), dim3(256), 0, st, d, to_dev(b), ivar, p.a_oxygen, p.dgdtp,
+                                   g.dxpwinv, g.dypsinv, g.dzpbinv, g.fracx, g.fracy, f.temp, f.fracl, f.vis, phi, bl.dbox,
+                                   bl.skip, prep->bufs, prep->bstride, prep->passes, prep->claim);
+    else
+        dispatch<BcFlowK, 256>(k_bc_flow, dim3(nb), dim3(256), 0, st, d, to_dev(b), ivar, p.a_oxygen, p.dgdtp,
+                               g.dxpwinv, g.dypsinv, g.dzpbinv, g.fracx, g.fracy, f.temp, f.fracl, f.vis, phi, bl.dbox,
+                               bl.skip);
 }
 
 void launch_momentum(hipStream_t st, const Dims &d, const DevGeom &g, DevFields f, Partials pr, const Box &b,

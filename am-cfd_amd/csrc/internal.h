@@ -172,8 +172,20 @@ struct BoxLaunch {
 };
 void launch_clean_velocities(hipStream_t st, const Dims &d, DevFields f, const Box &b, double tsolid,
                              const BoxLaunch &bl = BoxLaunch());
+// the pass buffers of the persistent box passes (mixed mode) that a
+// launch prepares: every box cell of passes buffers bstride apart to the
+// pending pattern, the task counter to zero
+struct BoxPrep {
+    double *bufs = nullptr;
+    size_t bstride = 0;
+    int passes = 0;
+    unsigned int *claim = nullptr;
+};
+// with prep (and a device box) the same launch also prepares the box
+// passes' buffers
 void launch_bc_flow(hipStream_t st, const Dims &d, const DevGeom &g, DevFields f, const Box &b,
-                    const amcfd_params &p, int ivar, const BoxLaunch &bl = BoxLaunch());
+                    const amcfd_params &p, int ivar, const BoxLaunch &bl = BoxLaunch(),
+                    const BoxPrep *prep = nullptr);
 void launch_momentum(hipStream_t st, const Dims &d, const DevGeom &g, DevFields f, Partials pr,
                      const Box &b, const amcfd_params &p, double scanvel, int ivar, const BoxLaunch &bl = BoxLaunch());
 void launch_pressure(hipStream_t st, const Dims &d, const DevGeom &g, DevFields f, Partials pr,
@@ -226,10 +238,11 @@ void launch_mixed_yzline(hipStream_t st, const Dims &d, const MxFields &m, const
                          int ibeg, int iend, int jbeg, int jend, int kbeg, int kend, const int *skip);
 // melt-pool box x-lines of one colour (FP64 coefficients, reference guard)
 // all red-black passes of one melt-pool box variable in one persistent
-// launch (plus the launch that prepares its pass buffers); nonzero if a
-// launch failed
+// launch (plus, with prep, the launch that prepares its pass buffers;
+// without, an earlier launch has prepared them); nonzero if a launch failed
 int launch_box_passes(hipStream_t st, const Dims &d, DevFields f, double *phi, double *bufs, size_t bstride,
-                      int passes, unsigned int *claim, int *error, const int *skip, const int *dbox, int nblocks);
+                      int passes, unsigned int *claim, int *error, const int *skip, const int *dbox, int nblocks,
+                      bool prep = true);
 void launch_box_xline(hipStream_t st, const Dims &d, DevFields f, double *phi, int ibeg, int iend, int jbeg, int jend,
                       int kbeg, int kend, int colour, const int *skip, const int *dbox = nullptr, int nblocks = 0);
 

@@ -28,6 +28,7 @@
 
 #include <type_traits>
 
+#include "boxprep.h"
 #include "fv.h"
 
 // multiply-adds may fuse here: this mode does not reproduce the reference's
@@ -291,7 +292,7 @@ __global__ __launch_bounds__(64) void k_xline_mx(Dims d, MxCoefs<TC> cf, double 
 // ------------------------------------------------------------------------
 // Melt-pool box x-lines, all red-black passes of one variable in one
 // persistent launch.  Pass p writes its lines into pass buffer p (every box
-// cell of every buffer holds BX_PENDING beforehand, k_box_prep); the last
+// cell of every buffer holds BX_PENDING beforehand, BoxPrepK); the last
 // pass also writes the field.  A red line of pass p reads its own and its
 // black neighbours' values from buffer p-1 (the field for p = 0), a black
 // line its own from buffer p-1 and its red neighbours from buffer p; cells
@@ -305,7 +306,6 @@ __global__ __launch_bounds__(64) void k_xline_mx(Dims d, MxCoefs<TC> cf, double 
 // line sees the same values as in the launch-per-half-pass form
 // (k_xline_mx), so the results are the same bits.
 // ------------------------------------------------------------------------
-constexpr unsigned long long BX_PENDING = 0x7ff4dead0000beefull;   // a NaN no arithmetic produces
 constexpr long long BX_SPIN_LIMIT = 1LL << 20;                      // bounded wait (~ a second)
 
 __device__ __forceinline__ double bx_ld(const double *p) {
@@ -405,41 +405,6 @@ __device__ __forceinline__ bool box_rows(const Dims &d, const MxCoefs<double> &c
     return ok;
 }
 
-struct BoxRange {
-    int ibeg, iend, jbeg, jend, kbeg, kend;
-};
-__device__ __forceinline__ bool box_range(const int *dbox, int kend, int maxn, BoxRange &b) {
-    b.ibeg = dbox[0] + 1;
-    b.iend = dbox[1] - 1;
-    b.jbeg = dbox[2];
-    b.jend = dbox[3];
-    b.kbeg = dbox[4];
-    b.kend = kend;
-    return !(b.iend - b.ibeg + 1 > maxn || b.iend < b.ibeg || b.jend < b.jbeg || b.kend < b.kbeg);
-}
-
-// the pass buffers' box cells to BX_PENDING and the task counter to zero
-struct BoxPrepK {
-    static constexpr int family = 13, rank = 0;
-    static __device__ __forceinline__ void run(const VB &vb, Dims d, double *bufs, size_t bstride, int passes, int kend,
-                                                  unsigned int *claim, const int *__restrict__ skip,
-                                                  const int *__restrict__ dbox) {
-    if (skip && *skip) return;
-    BoxRange b;
-    if (!box_range(dbox, kend, 1 << 30, b)) return;
-    if (vb.x == 0 && threadIdx.x == 0) *claim = 0u;
-    const int n = b.iend - b.ibeg + 1, nj = b.jend - b.jbeg + 1;
-    const size_t cells = (size_t)n * nj * (b.kend - b.kbeg + 1);
-    const double pend = __builtin_bit_cast(double, BX_PENDING);
-    for (size_t e = (size_t)vb.x * 256 + threadIdx.x; e < cells * passes; e += (size_t)vb.gx * 256) {
-        const size_t q = e / cells, r = e % cells;
-        const int i = b.ibeg + (int)(r % n);
-        const size_t line = r / n;
-        const int j = b.jbeg + (int)(line % nj), k = b.kbeg + (int)(line / nj);
-        bufs[q * bstride + cix(d, i, j, k)] = pend;
-    }
-}
-};
 __global__ __launch_bounds__(256) void k_box_prep(Dims d, double *bufs, size_t bstride, int passes, int kend,
                                                   unsigned int *claim, const int *__restrict__ skip,
                                                   const int *__restrict__ dbox) {
@@ -826,10 +791,12 @@ void launch_mixed_yzline(hipStream_t st, const Dims &d, const MxFields &m, const
 
 // all red-black passes of one box variable in one persistent launch
 int launch_box_passes(hipStream_t st, const Dims &d, DevFields f, double *phi, double *bufs, size_t bstride,
-                      int passes, unsigned int *claim, int *error, const int *skip, const int *dbox, int nblocks) {
+                      int passes, unsigned int *claim, int *error, const int *skip, const int *dbox, int nblocks,
+                      bool prep) {
     MxCoefs<double> cf{f.ap, f.an, f.as, f.ae, f.aw, f.at, f.ab, f.su};
-    dispatch<BoxPrepK, 256>(k_box_prep, dim3(nblocks), dim3(256), 0, st, d, bufs, bstride, passes, d.nkm2, claim, skip,
-                       dbox);
+    if (prep)
+        dispatch<BoxPrepK, 256>(k_box_prep, dim3(nblocks), dim3(256), 0, st, d, bufs, bstride, passes, d.nkm2, claim,
+                                skip, dbox);
     const int n = d.nim2;   // an upper bound of the box's line length
 #define BL(MM) dispatch<BoxLinesK<MM>, 64>(k_box_lines<MM>, dim3(nblocks), dim3(64), 0, st, d, cf, phi, bufs, bstride, \
                                   passes, d.nkm2, 1, claim, error, skip, dbox)

@@ -894,14 +894,18 @@ int Solver::enqueue_mixed_iteration(bool ctl) {
         bl.nblocks = box_blocks_;
         launch_clean_velocities(st_, d_, f_, box_, p_.tsolid, bl);
         double *phis[5] = {nullptr, f_.uvel, f_.vvel, f_.wvel, f_.pp};
+        BoxPrep prep;
+        if (box_fused_) {   // the boundary launch of each variable prepares its box passes
+            if ((st = ensure_box_buffers(mx_box_passes_))) return st;
+            prep = BoxPrep{box_bufs_, (size_t)d_.ni * d_.nj * d_.nkf, mx_box_passes_, box_claim_};
+        }
         for (int ivar = 1; ivar <= 4; ++ivar) {
-            launch_bc_flow(st_, d_, g_, f_, box_, p_, ivar, bl);
+            launch_bc_flow(st_, d_, g_, f_, box_, p_, ivar, bl, box_fused_ ? &prep : nullptr);
             if (ivar < 4) launch_momentum(st_, d_, g_, f_, pr_, box_, p_, scanvel_, ivar, bl);
             else launch_pressure(st_, d_, g_, f_, pr_, box_, p_, bl);
             if (box_fused_) {
-                if ((st = ensure_box_buffers(mx_box_passes_))) return st;
-                if (launch_box_passes(st_, d_, f_, phis[ivar], box_bufs_, (size_t)d_.ni * d_.nj * d_.nkf,
-                                      mx_box_passes_, box_claim_, &sc_->error, bl.skip, bl.dbox, box_blocks_))
+                if (launch_box_passes(st_, d_, f_, phis[ivar], prep.bufs, prep.bstride, prep.passes, prep.claim,
+                                      &sc_->error, bl.skip, bl.dbox, box_blocks_, false))
                     return set_error(AMCFD_E_HIP, "box passes: launch failed");
             } else {
                 for (int pass = 0; pass < mx_box_passes_; ++pass)
