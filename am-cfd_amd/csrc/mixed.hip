@@ -107,10 +107,14 @@ __device__ void wave_tridiag(int n, const T (&w)[M], const T (&dg)[M], const T (
 #pragma unroll
     for (int q = 0; q < M; ++q)
         if (lane * M + q < n) S.row(w[q], dg[q], e[q], r[q]);
-    // inclusive prefix over lanes: P_l = S_l * ... * S_0
+    // inclusive prefix over lanes: P_l = S_l * ... * S_0.  A step of
+    // offset off changes only lanes >= off, so once off reaches the lanes
+    // that hold rows the rest change nothing that is read (short lines)
+    const int used = (n + M - 1) / M;
     Mob<T> P = S;
 #pragma unroll
     for (int off = 1; off < 64; off <<= 1) {
+        if (off >= used) break;
         Mob<T> L;
         L.a = shfl_up_t(P.a, off); L.b = shfl_up_t(P.b, off); L.c = shfl_up_t(P.c, off);
         L.d = shfl_up_t(P.d, off); L.f = shfl_up_t(P.f, off); L.g = shfl_up_t(P.g, off);
@@ -382,20 +386,38 @@ __device__ __forceinline__ bool box_rows(const Dims &d, const MxCoefs<double> &c
             live_here(an[q], as[q], at[q], ab[q], su[q]);
         }
     }
+    // every value still pending is reloaded at once, round after round, so
+    // the residual can be formed one round trip after the last neighbour's
+    // store lands, in whatever order the neighbours land (bounded as
+    // bx_await is)
     bool ok = true;
+    for (long long spins = 0;;) {
+        bool any = false;
+#pragma unroll
+        for (int q = 0; q < K; ++q) {
+            const int ii = br.lane + 64 * q;
+            if (ii < br.n) {
+                const size_t c = br.row + ii;
+                if (bx_pending(h0[q])) { any = true; h0[q] = bx_ld(br.own + c); }
+                if (bx_pending(hw[q])) { any = true; hw[q] = bx_ld((ii > 0 ? br.own : br.phi) + c - 1); }
+                if (bx_pending(he[q])) { any = true; he[q] = bx_ld((ii < br.n - 1 ? br.own : br.phi) + c + 1); }
+                if (bx_pending(hn[q])) { any = true; hn[q] = bx_ld(br.src_n + c + d.sj); }
+                if (bx_pending(hs[q])) { any = true; hs[q] = bx_ld(br.src_s + c - d.sj); }
+                if (bx_pending(ht[q])) { any = true; ht[q] = bx_ld(br.src_t + c + d.sk); }
+                if (bx_pending(hb[q])) { any = true; hb[q] = bx_ld(br.src_b + c - d.sk); }
+            }
+        }
+        if (!any) break;
+        if (++spins > BX_SPIN_LIMIT ||
+            ((spins & 255) == 0 && __hip_atomic_load(error, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0)) {
+            ok = false;
+            break;
+        }
+    }
 #pragma unroll
     for (int q = 0; q < K; ++q) {
         const int ii = br.lane + 64 * q;
         if (ii < br.n) {
-            const size_t c = br.row + ii;
-            const double *pw = (ii > 0 ? br.own : br.phi) + c - 1, *pe = (ii < br.n - 1 ? br.own : br.phi) + c + 1;
-            ok &= bx_await(br.own + c, h0[q], error);
-            ok &= bx_await(pw, hw[q], error);
-            ok &= bx_await(pe, he[q], error);
-            ok &= bx_await(br.src_n + c + d.sj, hn[q], error);
-            ok &= bx_await(br.src_s + c - d.sj, hs[q], error);
-            ok &= bx_await(br.src_t + c + d.sk, ht[q], error);
-            ok &= bx_await(br.src_b + c - d.sk, hb[q], error);
             const double r = su[q] + aw[q] * hw[q] + ae[q] * he[q] + an[q] * hn[q] + as[q] * hs[q] +
                              at[q] * ht[q] + ab[q] * hb[q] - ap[q] * h0[q];
             hp[q] = h0[q];
