@@ -916,7 +916,18 @@ struct MeltPoolK {
     const int tid = threadIdx.x, nt = blockDim.x;
     // peak temperature (first-argument-wins max over block partials)
     double m = -INFINITY;
-    for (int b = tid; b < nblk; b += nt) m = dmax(m, partial[(size_t)b * 10]);
+    {   // each thread's blocks in order, eight blocks' loads in flight at a time
+        constexpr int U = 8;
+        int b = tid;
+        for (; b + (U - 1) * nt < nblk; b += U * nt) {
+            double v[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) v[u] = partial[(size_t)(b + u * nt) * 10];
+#pragma unroll
+            for (int u = 0; u < U; ++u) m = dmax(m, v[u]);
+        }
+        for (; b < nblk; b += nt) m = dmax(m, partial[(size_t)b * 10]);
+    }
     m = wave_max(m);
     if ((tid & 63) == 0) sd[tid >> 6] = m;
     __syncthreads();

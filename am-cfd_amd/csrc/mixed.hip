@@ -639,7 +639,17 @@ struct BlkFoldK {
     if (i >= ni) return;
     const double *p = part + (size_t)q * ntile * ni + i;
     double a = 0.0;
-    for (int t = s; t < ntile; t += BF_SPLIT) a += p[(size_t)t * ni];
+    // the tiles in order, eight tiles' loads in flight at a time
+    constexpr int U = 8;
+    int t = s;
+    for (; t + (U - 1) * BF_SPLIT < ntile; t += U * BF_SPLIT) {
+        double v[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) v[u] = p[(size_t)(t + u * BF_SPLIT) * ni];
+#pragma unroll
+        for (int u = 0; u < U; ++u) a += v[u];
+    }
+    for (; t < ntile; t += BF_SPLIT) a += p[(size_t)t * ni];
     out[((size_t)q * BF_SPLIT + s) * ni + i] = a;
 }
 };
