@@ -869,12 +869,16 @@ int Solver::enqueue_mixed_iteration(bool ctl) {
     tic("assemble_h", &slot);
     launch_energy_assemble_mixed(st_, d_, g_, f_, pr_, mx_, p_, SourceParams{}, done, stepd_);
     toc("assemble_h", slot);
-    launch_props(st_, d_, f_, p_, done);
     launch_mixed_block_correction(st_, d_, mx_, f_.enthalpy, done);
     tic("sweep_h", &slot);
     int st = energy_sweeps_mixed(done);
     toc("sweep_h", slot);
     if (st) return st;
+    // the properties from T (CFDSolverManager.cpp:624): nothing between the
+    // assembly and here reads or writes what they read or write, so they run
+    // after the line passes, which then find more of the coefficients the
+    // assembly has just written still in the caches
+    launch_props(st_, d_, f_, p_, done);
     tic("h2t", &slot);
     launch_h2t(st_, d_, g_, f_, pr_, p_, done, !mixed_has_volumetric_source(p_));
     toc("h2t", slot);
