@@ -39,7 +39,7 @@ EXPORTS = ("amcfd_deck_load", "amcfd_deck_params", "amcfd_deck_free", "amcfd_cre
            "amcfd_get_scalars", "amcfd_dims", "amcfd_last_timing", "amcfd_set_profiling",
            "amcfd_kernel_stats", "amcfd_write_vts", "amcfd_write_pvd", "amcfd_append_residuals",
            "amcfd_append_final_geometry", "amcfd_deck_string", "amcfd_free_surface",
-           "amcfd_output_section", "amcfd_run_batch", "amcfd_run_grouped", "amcfd_set_mixed_schedule", "amcfd_set_graph", "amcfd_get_graph", "amcfd_set_box_fused", "amcfd_set_line_batch",
+           "amcfd_output_section", "amcfd_run_batch", "amcfd_run_grouped", "amcfd_set_mixed_schedule", "amcfd_set_graph", "amcfd_get_graph", "amcfd_set_box_fused", "amcfd_set_line_batch", "amcfd_set_assembly_overlap",
            "amcfd_tuning_choice", "amcfd_copy_state", "amcfd_test_expire_waits", "amcfd_test_libm",
            "amcfd_tdma_lines", "amcfd_tdma_lines_noalloc",
            "amcfd_ws_create", "amcfd_ws_destroy", "amcfd_ws_attach", "amcfd_ws_counts", "amcfd_ws_train", "amcfd_ws_samples",
@@ -158,6 +158,7 @@ def load_library(path=LIB_PATH):
         "amcfd_set_graph": (ip, [vp, ip, ip, ip]),
         "amcfd_get_graph": (ip, [vp] + [ctypes.POINTER(ctypes.c_int)] * 5),
         "amcfd_set_line_batch": (ip, [vp, ip]),
+        "amcfd_set_assembly_overlap": (ip, [vp, ip]),
         "amcfd_set_box_fused": (ip, [vp, ip]),
         "amcfd_tuning_choice": (ip, [cp, ctypes.c_longlong] + [ctypes.POINTER(ctypes.c_int)] * 4),
         "amcfd_copy_state": (ip, [vp, vp]),
@@ -347,6 +348,12 @@ class Solver:
         """Mixed mode: consecutive x-lines per workgroup in the enthalpy's
         passes; the same results."""
         _check(load_library().amcfd_set_line_batch(self._h, int(lines)))
+
+    def set_assembly_overlap(self, on=True):
+        """Mixed mode, captured loop: the next iteration's assembly away from
+        the melt-pool box beside the flow part (True, or a workgroup count
+        for it) or after it (False); the same results."""
+        _check(load_library().amcfd_set_assembly_overlap(self._h, int(on)))
 
     def set_box_fused(self, on=True):
         """Mixed mode: a box variable's passes in one persistent launch (True)

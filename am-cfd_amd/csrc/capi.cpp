@@ -329,6 +329,12 @@ int amcfd_set_line_batch(amcfd_solver *s, int lines) {
     GUARD(s->s.set_line_batch(lines));
 }
 
+int amcfd_set_assembly_overlap(amcfd_solver *s, int on) {
+    if (!s) return amcfd::set_error(AMCFD_E_ARG, "null solver");
+    if (s->s.mode() != AMCFD_MODE_MIXED) return amcfd::set_error(AMCFD_E_ARG, "the solver is not in the mixed mode");
+    GUARD(s->s.set_assembly_overlap(on));
+}
+
 int amcfd_set_box_fused(amcfd_solver *s, int on) {
     if (!s) return amcfd::set_error(AMCFD_E_ARG, "null solver");
     if (s->s.mode() != AMCFD_MODE_MIXED) return amcfd::set_error(AMCFD_E_ARG, "the solver is not in the mixed mode");

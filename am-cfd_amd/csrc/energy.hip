@@ -29,6 +29,36 @@ struct BcConst {
     int isevap;
 };
 
+// The ghost cells the enthalpy boundary kernel sets, numbered g = 0 .. n-1:
+// the top face (nkm1) over the interior columns, then the west / east pair
+// (i = 0, nim1) of each interior (j, k), then the south / north pair (j = 0,
+// njm1) of each interior (i, k).
+__device__ __forceinline__ int bc_ghost_count(const Dims &d) {
+    return d.njm2 * d.nim2 + 2 * d.nkm2 * d.njm2 + 2 * d.nkm2 * d.nim2;
+}
+__device__ __forceinline__ size_t bc_ghost_cell(const Dims &d, int g) {
+    const int ntop = d.njm2 * d.nim2, nwe = 2 * d.nkm2 * d.njm2;
+    if (g < ntop) return cix(d, 1 + g % d.nim2, 1 + g / d.nim2, d.nkm1);
+    g -= ntop;
+    if (g < nwe) {
+        const int r = g >> 1;
+        return cix(d, (g & 1) ? d.nim1 : 0, 1 + r % d.njm2, 1 + r / d.njm2);
+    }
+    g -= nwe;
+    const int r = g >> 1;
+    return cix(d, 1 + r % d.nim2, (g & 1) ? d.njm1 : 0, 1 + r / d.nim2);
+}
+
+// A boundary launch that runs ahead of the convergence test of the
+// iteration before it (captured loops): it keeps the ghost values it
+// overwrites in save (numbered as above) and sets *spec when it runs (clears
+// it when skipped), so that the iteration's later launch can put them back
+// if that test ends the loop.
+struct BcSave {
+    double *save;
+    int *spec;
+};
+
 struct BcEnthalpyK {
     static constexpr int family = 1, rank = 0;
     static __device__ __forceinline__ void run(const VB &vb, Dims d, BcConst k, const double *__restrict__ dxpwinv,
@@ -38,9 +68,13 @@ struct BcEnthalpyK {
                                                      const double *__restrict__ heatin,
                                                      const double *__restrict__ temp,
                                                      const double *__restrict__ diff, double *h,
-                                                     double *partial, const int *skip) {
+                                                     double *partial, const int *skip, BcSave sv) {
     __shared__ double scratch[8];
-    if (skip && *skip) return;
+    if (skip && *skip) {
+        if (sv.spec && vb.x == 0 && threadIdx.x == 0) *sv.spec = 0;
+        return;
+    }
+    if (sv.spec && vb.x == 0 && threadIdx.x == 0) *sv.spec = 1;
     const int ntop = d.njm2 * d.nim2, new_ = d.nkm2 * d.njm2, nns = d.nkm2 * d.nim2;
     const int t = vb.x * blockDim.x + threadIdx.x;
     double loss = 0.0;
@@ -57,12 +91,17 @@ struct BcEnthalpyK {
         double ta = k.tempamb;
         double hrad = k.emiss * k.sigm * (T * T * T * T - ta * ta * ta * ta);
         double c1 = diff[below] * dzpbinv[d.nkm1];
+        if (sv.save) sv.save[t] = h[top];
         h[top] = h[below] + (heatin[j * d.ni + i] - hrad - hrad - hevap) / c1;
         loss = (hrad + hrad + hevap) * areaij[j * d.ni + i];
     } else if (t < ntop + new_) {
         int r = t - ntop;
         int kk = 1 + r / d.njm2, j = 1 + r % d.njm2;
         size_t w = cix(d, 0, j, kk), e = cix(d, d.nim1, j, kk);
+        if (sv.save) {
+            sv.save[ntop + 2 * r] = h[w];
+            sv.save[ntop + 2 * r + 1] = h[e];
+        }
         double q = k.htci * (temp[w] - k.tempamb);
         double c1 = diff[w + 1] * dxpwinv[1];
         h[w] = h[w + 1] - q / c1;
@@ -73,6 +112,10 @@ struct BcEnthalpyK {
         int r = t - ntop - new_;
         int kk = 1 + r / d.nim2, i = 1 + r % d.nim2;
         size_t s = cix(d, i, 0, kk), n = cix(d, i, d.njm1, kk);
+        if (sv.save) {
+            sv.save[ntop + 2 * new_ + 2 * r] = h[s];
+            sv.save[ntop + 2 * new_ + 2 * r + 1] = h[n];
+        }
         double q = k.htcj * (temp[s] - k.tempamb);
         double c1 = diff[s + d.sj] * dypsinv[1];
         h[s] = h[s + d.sj] - q / c1;
@@ -99,8 +142,8 @@ __global__ __launch_bounds__(256) void k_bc_enthalpy(Dims d, BcConst k, const do
                                                      const double *__restrict__ heatin,
                                                      const double *__restrict__ temp,
                                                      const double *__restrict__ diff, double *h,
-                                                     double *partial, const int *skip) {
-    BcEnthalpyK::run(vb_hw(), d, k, dxpwinv, dypsinv, dzpbinv, areaij, heatin, temp, diff, h, partial, skip);
+                                                     double *partial, const int *skip, BcSave sv) {
+    BcEnthalpyK::run(vb_hw(), d, k, dxpwinv, dypsinv, dzpbinv, areaij, heatin, temp, diff, h, partial, skip, sv);
 }
 
 // ------------------------------------------------------------------------
@@ -146,6 +189,18 @@ struct AsmOut {
     double *an, *as, *ae, *aw, *at, *ab, *ap, *su, *src, *part_res, *term_bl, *term_blc;
     float *fan, *fas, *fae, *faw, *fat, *fab, *fap;
     double *blkpart;
+    double *h_out;   // the enthalpy, where the late split launch restores ghost values
+};
+// the mixed assembly split around the flow part of the iteration before
+// (captured loops): mode 0 every tile, 1 the tiles away from the device box,
+// 2 the tiles near it, which also restores the ghost values kept in gsave
+// when the loop has ended and the iteration's boundary launch ran (gspec)
+struct AsmSplit {
+    const int *dbox;
+    int mode;
+    const double *gsave;
+    const int *gspec;
+    int tgx, tgy, tgz;   // mode 1 with a smaller grid: the tile grid its workgroups stride over
 };
 
 template <int SCH>
@@ -370,8 +425,18 @@ struct AssembleMxK {
                      const double *__restrict__ diff, const double *__restrict__ hnot,
                      const double *__restrict__ fracl, const double *__restrict__ fraclnot,
                      const double *__restrict__ h, AsmOut o, const int *__restrict__ skip,
-                     const StepDev *__restrict__ step) {
-    if (skip && *skip) return;
+                     const StepDev *__restrict__ step, AsmSplit sp2) {
+    if (skip && *skip) {
+        // the iteration does not run: the ghost values its boundary launch
+        // set ahead of the convergence test go back (the late launch only)
+        if (sp2.mode == 2 && sp2.gspec && *sp2.gspec) {
+            const int ng = bc_ghost_count(d);
+            const int nt = MTX * MTY, tid = threadIdx.x + MTX * threadIdx.y;
+            const int nbk = vb.gx * vb.gy * vb.gz, lin0 = vb.x + vb.gx * (vb.y + vb.gy * vb.z);
+            for (int gi = lin0 * nt + tid; gi < ng; gi += nbk * nt) o.h_out[bc_ghost_cell(d, gi)] = sp2.gsave[gi];
+        }
+        return;
+    }
     if (step) {   // a captured loop: the step's source parameters from device memory
         const SourceParams sp = step->sp;
         q.avolpow = sp.avolpow; q.apowseta = sp.apowseta; q.heatthick = sp.heatthick; q.heatrb = sp.heatrb;
@@ -398,6 +463,17 @@ struct AssembleMxK {
     const int j = 1 + by * MTY + ty;
     const int k0 = 1 + bz * KA;
     const int kend = min(k0 + KA - 1, d.nkm2);
+    if (sp2.mode) {
+        // The flow part of the iteration before changes velocities only in
+        // [istat, iend] x [jstat, jend + 1] x [kstat, nkm1] of the device box;
+        // a tile reads velocities one cell beyond its own cells.  Tiles
+        // within two cells of that region (near) wait for the flow part (the
+        // late launch, mode 2); the others run alongside it (mode 1).
+        const int *b = sp2.dbox;
+        const int i0 = 1 + bx * MTX, i1 = i0 + MTX - 1, j0 = 1 + by * MTY, j1 = j0 + MTY - 1;
+        const bool near = i1 + 2 >= b[0] && i0 - 2 <= b[1] && j1 + 2 >= b[2] && j0 - 2 <= b[3] + 1 && kend + 2 >= b[4];
+        if (near != (sp2.mode == 2)) return;
+    }
     const bool col_ok = (i <= d.nim2) && (j <= d.njm2) && (k0 <= d.nkm2);
     // the staged cells: every thread its own (interior or the east / north
     // ghost), and 136 threads one cell of the ring each
@@ -594,8 +670,18 @@ __global__ __launch_bounds__(MTX *MTY) __attribute__((amdgpu_waves_per_eu(MX_ASM
                      const double *__restrict__ diff, const double *__restrict__ hnot,
                      const double *__restrict__ fracl, const double *__restrict__ fraclnot,
                      const double *__restrict__ h, AsmOut o, const int *__restrict__ skip,
-                     const StepDev *__restrict__ step) {
-    AssembleMxK<SCH>::run(vb_hw(), d, q, g, uvel, vvel, wvel, rho, diff, hnot, fracl, fraclnot, h, o, skip, step);
+                     const StepDev *__restrict__ step, AsmSplit sp2) {
+    if (sp2.tgx) {   // fewer workgroups than tiles (room for the launches beside it): stride over the tiles
+        const int nt = sp2.tgx * sp2.tgy * sp2.tgz;
+        for (int v = blockIdx.x; v < nt; v += gridDim.x) {
+            const VB vb{v % sp2.tgx, (v / sp2.tgx) % sp2.tgy, v / (sp2.tgx * sp2.tgy), sp2.tgx, sp2.tgy, sp2.tgz, 1};
+            AssembleMxK<SCH>::run(vb, d, q, g, uvel, vvel, wvel, rho, diff, hnot, fracl, fraclnot, h, o, skip, step,
+                                  sp2);
+            __syncthreads();   // the tile's LDS is reused by the next
+        }
+        return;
+    }
+    AssembleMxK<SCH>::run(vb_hw(), d, q, g, uvel, vvel, wvel, rho, diff, hnot, fracl, fraclnot, h, o, skip, step, sp2);
 }
 
 // ------------------------------------------------------------------------
@@ -1245,13 +1331,13 @@ __global__ __launch_bounds__(256) void k_end_step(size_t n, double tsolid, doubl
 // launchers
 // ---------------------------------------------------------------------------
 void launch_bc_enthalpy(hipStream_t st, const Dims &d, const DevGeom &g, DevFields f, Partials pr,
-                        const amcfd_params &p, const int *skip) {
+                        const amcfd_params &p, const int *skip, double *save, int *spec) {
     BcConst k{p.sigm, p.emiss, p.htci, p.htcj, p.tempamb, p.pressamb, p.gasconst, p.evaplatent,
               p.evaptemp, p.evapfact, p.molarmass, p.isevap};
     int n = d.njm2 * d.nim2 + d.nkm2 * d.njm2 + d.nkm2 * d.nim2;
     int nb = (n + 255) / 256;
     dispatch<BcEnthalpyK, 256>(k_bc_enthalpy, dim3(nb), dim3(256), 0, st, d, k, g.dxpwinv, g.dypsinv, g.dzpbinv,
-                       g.areaij, f.heatin, f.temp, f.diff, f.enthalpy, pr.bc, skip);
+                       g.areaij, f.heatin, f.temp, f.diff, f.enthalpy, pr.bc, skip, BcSave{save, spec});
 }
 
 // Gaussian surface flux map, BoundCondManager.cpp:128-159: one thread per
@@ -1341,7 +1427,7 @@ void launch_energy_assemble(hipStream_t st, const Dims &d, const DevGeom &g, Dev
     AsmConst q{p.delt, p.hlatnt, p.urfh, s.avolpow, s.apowseta, s.heatthick, s.heatrb, s.avolfact,
                s.beamposx, s.beamposy, s.powerindicator, p.discretization};
     AsmOut o{f.an, f.as, f.ae, f.aw, f.at, f.ab, f.ap, f.su, f.sourceinput, pr.energy, f.sp, f.apnot,
-             nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+             nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
     auto kern = p.discretization == AMCFD_DISC_POWERLAW     ? k_energy_assemble<AMCFD_DISC_POWERLAW>
                 : p.discretization == AMCFD_DISC_EXPONENTIAL ? k_energy_assemble<AMCFD_DISC_EXPONENTIAL>
                                                              : k_energy_assemble<AMCFD_DISC_UPWIND>;
@@ -1351,15 +1437,25 @@ void launch_energy_assemble(hipStream_t st, const Dims &d, const DevGeom &g, Dev
 
 void launch_energy_assemble_mixed(hipStream_t st, const Dims &d, const DevGeom &g, DevFields f, Partials pr,
                                   const MxFields &m, const amcfd_params &p, const SourceParams &s, const int *skip,
-                                  const StepDev *step) {
+                                  const StepDev *step, const int *split_box, int split_mode, const double *gsave,
+                                  const int *gspec, int split_blocks) {
     AsmConst q{p.delt, p.hlatnt, p.urfh, s.avolpow, s.apowseta, s.heatthick, s.heatrb, s.avolfact,
                s.beamposx, s.beamposy, s.powerindicator, p.discretization, mixed_has_volumetric_source(p)};
     AsmOut o{nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, m.su, f.sourceinput, pr.energy,
-             nullptr, nullptr, m.an, m.as, m.ae, m.aw, m.at, m.ab, m.ap, m.blkpart};
+             nullptr, nullptr, m.an, m.as, m.ae, m.aw, m.at, m.ab, m.ap, m.blkpart, f.enthalpy};
+    const dim3 tg = assemble_grid_mx(d);
+    AsmSplit sp2{split_box, split_box ? split_mode : 0, gsave, gspec, 0, 0, 0};
+    dim3 grid = tg;
+    if (sp2.mode == 1 && split_blocks > 0 && (unsigned)split_blocks < tg.x * tg.y * tg.z) {
+        sp2.tgx = (int)tg.x;
+        sp2.tgy = (int)tg.y;
+        sp2.tgz = (int)tg.z;
+        grid = dim3((unsigned)split_blocks);
+    }
 #define ASM(SCH)                                                                                                   \
-    dispatch<AssembleMxK<SCH>, MTX * MTY>(k_energy_assemble_mx<SCH>, assemble_grid_mx(d), dim3(MTX, MTY), 0, st, d, q, \
+    dispatch<AssembleMxK<SCH>, MTX * MTY>(k_energy_assemble_mx<SCH>, grid, dim3(MTX, MTY), 0, st, d, q,            \
                                            g, f.uvel, f.vvel, f.wvel, f.rho, f.diff, f.hnot, f.fracl, f.fraclnot,      \
-                                           f.enthalpy, o, skip, step)
+                                           f.enthalpy, o, skip, step, sp2)
     if (p.discretization == AMCFD_DISC_POWERLAW) ASM(AMCFD_DISC_POWERLAW);
     else if (p.discretization == AMCFD_DISC_EXPONENTIAL) ASM(AMCFD_DISC_EXPONENTIAL);
     else ASM(AMCFD_DISC_UPWIND);

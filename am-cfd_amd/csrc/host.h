@@ -80,6 +80,7 @@ class Solver {
     // mixed mode: consecutive x-lines per workgroup in the enthalpy's passes
     int line_batch() const { return line_batch_; }
     int set_line_batch(int lines);
+    int set_assembly_overlap(int on);
     // mixed mode: order of the enthalpy line passes ('x' red-black x-lines,
     // 'y' y-lines, 'z' z-lines; an even number of y/z) and red-black x
     // passes per melt-pool box variable
@@ -151,6 +152,14 @@ class Solver {
     DevScalars *sc_ = nullptr;       // device
     DevScalars *sc_host_ = nullptr;  // pinned mirror
     hipStream_t st_ = nullptr;
+    // captured mixed loops: the second stream of the forked launches, its
+    // fork / join events, the ghost values a launch run ahead keeps, whether
+    // it ran, and the second set of the partials that alternate by iteration
+    hipStream_t side_st_ = nullptr;
+    hipEvent_t ev_fork_ = nullptr, ev_join_ = nullptr;
+    double *ghost_save_ = nullptr;
+    int *ghost_spec_ = nullptr;
+    double *energy_alt_ = nullptr, *bc_alt_ = nullptr;
     int mode_ = AMCFD_MODE_FP64;
     hipEvent_t step_ev_[2] = {nullptr, nullptr};   // brackets one solve_step
     std::vector<void *> allocs_;
@@ -171,7 +180,7 @@ class Solver {
         int nk = -1, blocks = 0, box_blocks = 0;
         double delt = 0.0, scanvel = 0.0;
         std::string sched;
-        int box_passes = 0, box_fused = -1, line_batch = 0;
+        int box_passes = 0, box_fused = -1, line_batch = 0, fork_asm = -1;
     } gkey_;
     StepDev *stepd_ = nullptr;       // device
     StepDev *steph_ = nullptr;       // pinned staging
@@ -181,6 +190,12 @@ class Solver {
     // or one launch per half pass (0); the same results either way
     int box_fused_ = 1;
     int line_batch_ = 1;             // mixed mode: x-lines per workgroup of the enthalpy passes
+    // captured mixed loops: the next iteration's boundary launch and far
+    // assembly tiles run alongside the flow part (1) or after it (0); the
+    // same results either way
+    int fork_asm_ = 1;
+    int fork_blocks_ = 0;           // workgroups of the far tiles' launch (0: one per tile)
+    int fork_blocks_default_ = 0;   // its default, from the compute-unit count
     double *box_bufs_ = nullptr;     // box_buf_passes_ pass buffers of ni*nj*nkf
     int box_buf_passes_ = 0;
     unsigned int *box_claim_ = nullptr;
@@ -238,8 +253,13 @@ class Solver {
     int solve_step_mixed_graph();
     int solve_step_fp64();
     // one mixed-mode SIMPLE iteration; ctl: every decision on the device
-    // (flags in DevScalars, the step's inputs in stepd_), for capture
-    int enqueue_mixed_iteration(bool ctl);
+    // (flags in DevScalars, the step's inputs in stepd_), for capture.
+    // it: the iteration's place in a captured block (its partial buffers
+    // alternate); ahead: its boundary launch and the tiles of its assembly
+    // away from the melt-pool box were enqueued by the iteration before, on
+    // side_st_; fork_next: enqueue those of the next iteration on side_st_,
+    // alongside this iteration's flow part
+    int enqueue_mixed_iteration(bool ctl, int it = 0, bool ahead = false, bool fork_next = false);
     int capture_mixed_graph();
     void release_graph();
     int energy_sweeps_mixed(const int *skip);

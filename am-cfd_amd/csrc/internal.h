@@ -143,8 +143,16 @@ struct SweepPlan {
 // ---------------------------------------------------------------------------
 // kernel launchers (defined in the .hip files)
 // ---------------------------------------------------------------------------
+// with save, the ghost values the launch overwrites are kept there and *spec
+// tells whether it ran (captured loops that run it ahead of the previous
+// iteration's convergence test)
 void launch_bc_enthalpy(hipStream_t st, const Dims &d, const DevGeom &g, DevFields f, Partials pr,
-                        const amcfd_params &p, const int *skip = nullptr);
+                        const amcfd_params &p, const int *skip = nullptr, double *save = nullptr,
+                        int *spec = nullptr);
+// the number of ghost values the enthalpy boundary launch sets
+inline size_t bc_ghost_values(const Dims &d) {
+    return (size_t)d.njm2 * d.nim2 + 2 * (size_t)d.nkm2 * d.njm2 + 2 * (size_t)d.nkm2 * d.nim2;
+}
 void launch_energy_assemble(hipStream_t st, const Dims &d, const DevGeom &g, DevFields f, Partials pr,
                             const amcfd_params &p, const SourceParams &sp);
 void launch_laser_map(hipStream_t st, const Dims &d, const DevGeom &g, DevFields f, Partials pr,
@@ -218,9 +226,14 @@ void launch_line_sweeps(hipStream_t st, const Dims &d, DevFields f, double *phi,
 int eval_libm_device(int fn, const double *x, double *y, size_t n);
 
 // mixed-precision mode (mixed.hip, energy.hip)
+// split_mode with the device box split_box: 1 assembles the tiles away from
+// the box (alongside the previous iteration's flow part), 2 the tiles near
+// it, and when the loop has ended puts back the ghost values gsave holds if
+// *gspec says the boundary launch ran ahead
 void launch_energy_assemble_mixed(hipStream_t st, const Dims &d, const DevGeom &g, DevFields f, Partials pr,
                                   const MxFields &m, const amcfd_params &p, const SourceParams &sp, const int *skip,
-                                  const StepDev *step = nullptr);
+                                  const StepDev *step = nullptr, const int *split_box = nullptr, int split_mode = 0,
+                                  const double *gsave = nullptr, const int *gspec = nullptr, int split_blocks = 0);
 int mixed_tile_count(const Dims &d);
 // the mixed assembly writes the volumetric-source field only for decks that have one
 int mixed_has_volumetric_source(const amcfd_params &p);

@@ -42,6 +42,12 @@ Solver::~Solver() {
     for (auto &e : ev_pool_) (void)hipEventDestroy(e);
     for (auto &e : step_ev_)
         if (e) (void)hipEventDestroy(e);
+    if (ev_fork_) (void)hipEventDestroy(ev_fork_);
+    if (ev_join_) (void)hipEventDestroy(ev_join_);
+    if (side_st_) {
+        (void)hipStreamSynchronize(side_st_);
+        (void)hipStreamDestroy(side_st_);
+    }
     if (st_) (void)hipStreamDestroy(st_);
 }
 
@@ -90,6 +96,13 @@ int Solver::set_box_fused(int on) {
     box_fused_ = on != 0;
     release_graph();
     return box_fused_ ? ensure_box_buffers(mx_box_passes_) : AMCFD_OK;
+}
+
+int Solver::set_assembly_overlap(int on) {
+    if (on < 0) return set_error(AMCFD_E_ARG, "overlap: 0, 1 or a workgroup count");
+    fork_asm_ = on != 0;
+    fork_blocks_ = on > 1 ? on : fork_blocks_default_;
+    return AMCFD_OK;
 }
 
 int Solver::set_line_batch(int lines) {
@@ -167,6 +180,13 @@ int Solver::create(const amcfd_params &p, int mode) {
         tuning_lookup((long long)(p.ni - 2) * (p.nj - 2) * (p.nk - 2), &graph_block_, &box_blocks_, &box_fused_,
                       &line_batch_);
         box_blocks_ = std::min(box_blocks_, max_box_blocks);
+        // the far assembly tiles beside the flow part: about one and a half
+        // workgroups per compute unit, which leaves room on every unit for
+        // the flow part's launches (measured best on the 400x160x80 track)
+        int ncu = 0;
+        HIPCALL(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+        fork_blocks_ = std::max(8, 3 * ncu / 2);
+        fork_blocks_default_ = fork_blocks_;
     }
     HIPCALL(hipStreamCreateWithFlags(&st_, hipStreamNonBlocking));
     HIPCALL(hipEventCreate(&step_ev_[0]));
@@ -281,7 +301,26 @@ int Solver::create(const amcfd_params &p, int mode) {
         float **c32[] = {&mx_.ap, &mx_.an, &mx_.as, &mx_.ae, &mx_.aw, &mx_.at, &mx_.ab};
         for (auto *q : c32)
             if ((st = allocf(q, nn))) return st;
-        mx_.su = f_.su;
+        // the energy equation's source of its own (the momentum equations'
+        // use f.su while the next iteration's assembly runs alongside them)
+        if ((st = alloc(&mx_.su, nn))) return st;
+        if ((st = alloc(&energy_alt_, (size_t)2 * pr_.nblk_e))) return st;
+        if ((st = alloc(&bc_alt_, (size_t)pr_.nblk_b))) return st;
+        {
+            Dims full = d_;   // the ghost layer of the full z extent
+            full.nk = d_.nkf;
+            full.nkm1 = d_.nkf - 1;
+            full.nkm2 = d_.nkf - 2;
+            if ((st = alloc(&ghost_save_, bc_ghost_values(full)))) return st;
+            void *v = nullptr;
+            HIPCALL(hipMalloc(&v, sizeof(int)));
+            allocs_.push_back(v);
+            ghost_spec_ = (int *)v;
+            HIPCALL(hipMemsetAsync(ghost_spec_, 0, sizeof(int), st_));
+        }
+        HIPCALL(hipStreamCreateWithFlags(&side_st_, hipStreamNonBlocking));
+        HIPCALL(hipEventCreateWithFlags(&ev_fork_, hipEventDisableTiming));
+        HIPCALL(hipEventCreateWithFlags(&ev_join_, hipEventDisableTiming));
         Dims full = d_;   // partials sized for the full z extent
         full.nk = d_.nkf;
         full.nkm1 = d_.nkf - 1;
@@ -857,7 +896,7 @@ int Solver::energy_sweeps_mixed(const int *skip) {
 // melt pool (flow or not) and after the finalize (convergence), as the FP64
 // mode does; with ctl the kernels skip themselves on the device flags and
 // k_converge takes the decision.
-int Solver::enqueue_mixed_iteration(bool ctl) {
+int Solver::enqueue_mixed_iteration(bool ctl, int it, bool ahead, bool fork_next) {
     const int *done = ctl ? &sc_->done : nullptr;
     int slot;
     if (!ctl) {
@@ -865,9 +904,23 @@ int Solver::enqueue_mixed_iteration(bool ctl) {
         *steph_ = StepDev{sp, powerindicator_, (int)p_.maxit, istart_, jstart_};
         HIPCALL(hipMemcpyAsync(stepd_, steph_, sizeof(StepDev), hipMemcpyHostToDevice, st_));
     }
-    launch_bc_enthalpy(st_, d_, g_, f_, pr_, p_, done);
+    // the partials the finalize of this iteration folds (boundary and energy
+    // residual) alternate with the next iteration's, which may be written
+    // before this finalize runs
+    Partials pq = pr_;
+    if (it & 1) {
+        pq.energy = energy_alt_;
+        pq.bc = bc_alt_;
+    }
     tic("assemble_h", &slot);
-    launch_energy_assemble_mixed(st_, d_, g_, f_, pr_, mx_, p_, SourceParams{}, done, stepd_);
+    if (!ahead) {
+        launch_bc_enthalpy(st_, d_, g_, f_, pq, p_, done);
+        launch_energy_assemble_mixed(st_, d_, g_, f_, pq, mx_, p_, SourceParams{}, done, stepd_);
+    } else {   // the tiles near the box, once the flow part and the far tiles are done
+        HIPCALL(hipStreamWaitEvent(st_, ev_join_, 0));
+        launch_energy_assemble_mixed(st_, d_, g_, f_, pq, mx_, p_, SourceParams{}, done, stepd_, &sc_->istat, 2,
+                                     ghost_save_, ghost_spec_);
+    }
     toc("assemble_h", slot);
     launch_mixed_block_correction(st_, d_, mx_, f_.enthalpy, done);
     tic("sweep_h", &slot);
@@ -891,6 +944,27 @@ int Solver::enqueue_mixed_iteration(bool ctl) {
         flow = sc_host_->tpeak > p_.tsolid && p_.navierstokes;
     }
     pr_.nblk_f = box_blocks_;
+    pq.nblk_f = box_blocks_;
+    if (fork_next) {
+        // The next iteration's boundary launch and the tiles of its assembly
+        // away from the box, alongside this iteration's flow part: they read
+        // nothing the flow part writes, and write nothing it or this
+        // iteration's finalize reads (its partials are the other set, its
+        // source term is the energy equation's own).  The boundary launch
+        // keeps the ghost values it overwrites, which the next iteration's
+        // late assembly launch puts back if this iteration ends the loop.
+        Partials pn = pr_;
+        if (!(it & 1)) {
+            pn.energy = energy_alt_;
+            pn.bc = bc_alt_;
+        }
+        HIPCALL(hipEventRecord(ev_fork_, st_));
+        HIPCALL(hipStreamWaitEvent(side_st_, ev_fork_, 0));
+        launch_bc_enthalpy(side_st_, d_, g_, f_, pn, p_, done, ghost_save_, ghost_spec_);
+        launch_energy_assemble_mixed(side_st_, d_, g_, f_, pn, mx_, p_, SourceParams{}, done, stepd_, &sc_->istat, 1,
+                                     nullptr, nullptr, fork_blocks_);
+        HIPCALL(hipEventRecord(ev_join_, side_st_));
+    }
     if (flow) {
         BoxLaunch bl;
         bl.dbox = &sc_->istat;
@@ -920,7 +994,7 @@ int Solver::enqueue_mixed_iteration(bool ctl) {
             if (ivar == 4) launch_correct(st_, d_, f_, box_, p_, bl);
         }
     }
-    launch_finalize(st_, d_, pr_, sc_, p_, flow, 0, p_.steady, ctl ? 0.0 : sc_host_->width, ctl);
+    launch_finalize(st_, d_, pq, sc_, p_, flow, 0, p_.steady, ctl ? 0.0 : sc_host_->width, ctl);
     if (ctl) launch_converge(st_, sc_, stepd_, p_);
     HIPCALL(hipGetLastError());
     return 0;
@@ -975,7 +1049,10 @@ int Solver::capture_mixed_graph() {
     profile_ = 0;   // no events inside a capture
     HIPCALL(hipStreamBeginCapture(st_, hipStreamCaptureModeThreadLocal));
     int st = 0;
-    for (int it = 0; it < graph_block_ && !st; ++it) st = enqueue_mixed_iteration(true);
+    // each iteration but the block's last runs the next one's boundary
+    // launch and far assembly tiles alongside its own flow part
+    for (int it = 0; it < graph_block_ && !st; ++it)
+        st = enqueue_mixed_iteration(true, it, it > 0 && fork_asm_, it + 1 < graph_block_ && fork_asm_);
     hipGraph_t g = nullptr;
     const hipError_t e = hipStreamEndCapture(st_, &g);
     profile_ = prof;
@@ -995,6 +1072,7 @@ int Solver::capture_mixed_graph() {
     gkey_.box_passes = mx_box_passes_;
     gkey_.box_fused = box_fused_;
     gkey_.line_batch = line_batch_;
+    gkey_.fork_asm = fork_asm_ + 2 * fork_blocks_;
     return 0;
 }
 
@@ -1004,7 +1082,8 @@ int Solver::solve_step_mixed_graph() {
     // every launch argument the graph holds by value: rebuild when one moved
     if (!gexec_ || gkey_.nk != d_.nk || gkey_.blocks != graph_block_ || gkey_.box_blocks != box_blocks_ ||
         gkey_.delt != p_.delt || gkey_.scanvel != scanvel_ || gkey_.sched != mx_sched_ ||
-        gkey_.box_passes != mx_box_passes_ || gkey_.box_fused != box_fused_ || gkey_.line_batch != line_batch_) {
+        gkey_.box_passes != mx_box_passes_ || gkey_.box_fused != box_fused_ || gkey_.line_batch != line_batch_ ||
+        gkey_.fork_asm != fork_asm_ + 2 * fork_blocks_) {
         int st = capture_mixed_graph();
         if (st) return st;
     }

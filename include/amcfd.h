@@ -327,6 +327,16 @@ int  amcfd_set_box_fused(amcfd_solver *s, int on);
  * independent, so this changes the passes' speed and never their results.
  * No reference counterpart. */
 int  amcfd_set_line_batch(amcfd_solver *s, int lines);
+/* Mixed mode, captured loop: while one iteration's flow part (the melt-pool
+ * box solves, latency-bound and on few compute units) runs, the next
+ * iteration's enthalpy boundary launch and the tiles of its assembly away
+ * from the box run beside it on a second stream (on = 1, default, with about
+ * 1.5 workgroups per compute unit striding over those tiles; on > 1 sets
+ * that workgroup count); the tiles near the box follow the flow part.  The
+ * boundary launch keeps the ghost values it overwrites and they are put back
+ * if the convergence test ends the loop first.  on = 0 runs everything in
+ * order.  Every setting gives the same bits.  No reference counterpart. */
+int  amcfd_set_assembly_overlap(amcfd_solver *s, int on);
 /* Mixed mode: the captured loop's current settings and the line batch.  A
  * mixed-mode solver starts from the tuning table's row for its interior cell
  * count (below). */

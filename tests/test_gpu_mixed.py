@@ -260,3 +260,37 @@ def test_expired_waits_end_a_mixed_step(case_dir, amcfd):
     with pytest.raises(amcfd.AmcfdError) as e:
         s.step()
     assert e.value.code == -5
+
+
+@pytest.mark.parametrize("inst", [0, 3])
+def test_assembly_overlap_gives_the_same_bits(amcfd, tmp_path, inst):
+    """The captured loop with the next iteration's boundary launch and far
+    assembly tiles beside the flow part gives the bits of the loop without
+    that overlap and of the host loop, on sweep instances whose steps end by
+    convergence inside a block of iterations (the ghost values set ahead are
+    put back), every field including the ghost cells."""
+    import decks
+    deck = amcfd.Deck(decks.write_sweep_instance(inst, str(tmp_path)))
+    on = amcfd.Solver(deck, mode=amcfd.MODE_MIXED)
+    on.set_assembly_overlap(True)
+    few = amcfd.Solver(deck, mode=amcfd.MODE_MIXED)
+    few.set_assembly_overlap(16)   # many tiles per workgroup
+    off = amcfd.Solver(deck, mode=amcfd.MODE_MIXED)
+    off.set_assembly_overlap(False)
+    host = amcfd.Solver(deck, mode=amcfd.MODE_MIXED)
+    host.set_graph(False)
+    converged = 0
+    for step in range(4):
+        print("assembly overlap, instance %d step %d" % (inst, step + 1), flush=True)
+        for s in (on, few, off, host):
+            s.step()
+        sa = on.scalars()
+        converged += sa["niter"] < on.params.maxit
+        for other in (few, off, host):
+            sb = other.scalars()
+            assert sa["niter"] == sb["niter"], (step, sa["niter"], sb["niter"])
+            for key in ("tpeak", "ratio", "resorh", "resorm", "alen", "depth", "width"):
+                assert sa[key] == sb[key] or (np.isnan(sa[key]) and np.isnan(sb[key])), (step, key, sa[key], sb[key])
+            for f in ("enthalpy", "temp", "fracl", "uvel", "vvel", "wvel", "pressure", "pp"):
+                assert np.array_equal(on.field(f), other.field(f)), (step, f)
+    assert converged >= 1   # a step ended inside a block of iterations
