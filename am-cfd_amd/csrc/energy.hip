@@ -671,17 +671,23 @@ __global__ __launch_bounds__(MTX *MTY) __attribute__((amdgpu_waves_per_eu(MX_ASM
                      const double *__restrict__ fracl, const double *__restrict__ fraclnot,
                      const double *__restrict__ h, AsmOut o, const int *__restrict__ skip,
                      const StepDev *__restrict__ step, AsmSplit sp2) {
-    if (sp2.tgx) {   // fewer workgroups than tiles (room for the launches beside it): stride over the tiles
-        const int nt = sp2.tgx * sp2.tgy * sp2.tgz;
-        for (int v = blockIdx.x; v < nt; v += gridDim.x) {
-            const VB vb{v % sp2.tgx, (v / sp2.tgx) % sp2.tgy, v / (sp2.tgx * sp2.tgy), sp2.tgx, sp2.tgy, sp2.tgz, 1};
-            AssembleMxK<SCH>::run(vb, d, q, g, uvel, vvel, wvel, rho, diff, hnot, fracl, fraclnot, h, o, skip, step,
-                                  sp2);
-            __syncthreads();   // the tile's LDS is reused by the next
-        }
-        return;
-    }
     AssembleMxK<SCH>::run(vb_hw(), d, q, g, uvel, vvel, wvel, rho, diff, hnot, fracl, fraclnot, h, o, skip, step, sp2);
+}
+// fewer workgroups than tiles (room for the launches beside it): each
+// strides over the tile grid sp2.tgx x tgy x tgz
+template <int SCH>
+__global__ __launch_bounds__(MTX *MTY) __attribute__((amdgpu_waves_per_eu(MX_ASM_WAVES, 8))) void k_energy_assemble_mx_strided(Dims d, AsmConst q, DevGeom g, const double *__restrict__ uvel, const double *__restrict__ vvel,
+                     const double *__restrict__ wvel, const double *__restrict__ rho,
+                     const double *__restrict__ diff, const double *__restrict__ hnot,
+                     const double *__restrict__ fracl, const double *__restrict__ fraclnot,
+                     const double *__restrict__ h, AsmOut o, const int *__restrict__ skip,
+                     const StepDev *__restrict__ step, AsmSplit sp2) {
+    const int nt = sp2.tgx * sp2.tgy * sp2.tgz;
+    for (int v = blockIdx.x; v < nt; v += gridDim.x) {
+        const VB vb{v % sp2.tgx, (v / sp2.tgx) % sp2.tgy, v / (sp2.tgx * sp2.tgy), sp2.tgx, sp2.tgy, sp2.tgz, 1};
+        AssembleMxK<SCH>::run(vb, d, q, g, uvel, vvel, wvel, rho, diff, hnot, fracl, fraclnot, h, o, skip, step, sp2);
+        __syncthreads();   // the tile's LDS is reused by the next
+    }
 }
 
 // ------------------------------------------------------------------------
@@ -1453,9 +1459,18 @@ void launch_energy_assemble_mixed(hipStream_t st, const Dims &d, const DevGeom &
         grid = dim3((unsigned)split_blocks);
     }
 #define ASM(SCH)                                                                                                   \
-    dispatch<AssembleMxK<SCH>, MTX * MTY>(k_energy_assemble_mx<SCH>, grid, dim3(MTX, MTY), 0, st, d, q,            \
-                                           g, f.uvel, f.vvel, f.wvel, f.rho, f.diff, f.hnot, f.fracl, f.fraclnot,      \
-                                           f.enthalpy, o, skip, step, sp2)
+    do {                                                                                                           \
+        if (sp2.tgx) {                                                                                             \
+            if (LaunchRecorder *r = tls_recorder()) r->unsupported = true;                                         \
+            else                                                                                                   \
+                hipLaunchKernelGGL(k_energy_assemble_mx_strided<SCH>, grid, dim3(MTX, MTY), 0, st, d, q, g, f.uvel,  \
+                                   f.vvel, f.wvel, f.rho, f.diff, f.hnot, f.fracl, f.fraclnot, f.enthalpy, o, skip, \
+                                   step, sp2);                                                                     \
+        } else                                                                                                     \
+            dispatch<AssembleMxK<SCH>, MTX * MTY>(k_energy_assemble_mx<SCH>, grid, dim3(MTX, MTY), 0, st, d, q, g, \
+                                                   f.uvel, f.vvel, f.wvel, f.rho, f.diff, f.hnot, f.fracl,         \
+                                                   f.fraclnot, f.enthalpy, o, skip, step, sp2);                    \
+    } while (0)
     if (p.discretization == AMCFD_DISC_POWERLAW) ASM(AMCFD_DISC_POWERLAW);
     else if (p.discretization == AMCFD_DISC_EXPONENTIAL) ASM(AMCFD_DISC_EXPONENTIAL);
     else ASM(AMCFD_DISC_UPWIND);

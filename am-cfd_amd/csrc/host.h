@@ -196,6 +196,10 @@ class Solver {
     int fork_asm_ = 1;
     int fork_blocks_ = 0;           // workgroups of the far tiles' launch (0: one per tile)
     int fork_blocks_default_ = 0;   // its default, from the compute-unit count
+    // the overlap a capture uses: none beside other solvers' streams
+    // (amcfd_run_batch), where a stream waiting on another blocks the
+    // hardware queue it shares with other solvers (measured 5x slower)
+    int fork_key() const { return (fork_asm_ && !shared_device_) ? 1 + 2 * fork_blocks_ : 0; }
     double *box_bufs_ = nullptr;     // box_buf_passes_ pass buffers of ni*nj*nkf
     int box_buf_passes_ = 0;
     unsigned int *box_claim_ = nullptr;

@@ -1051,8 +1051,9 @@ int Solver::capture_mixed_graph() {
     int st = 0;
     // each iteration but the block's last runs the next one's boundary
     // launch and far assembly tiles alongside its own flow part
+    const bool fork = fork_key() != 0;
     for (int it = 0; it < graph_block_ && !st; ++it)
-        st = enqueue_mixed_iteration(true, it, it > 0 && fork_asm_, it + 1 < graph_block_ && fork_asm_);
+        st = enqueue_mixed_iteration(true, it, it > 0 && fork, it + 1 < graph_block_ && fork);
     hipGraph_t g = nullptr;
     const hipError_t e = hipStreamEndCapture(st_, &g);
     profile_ = prof;
@@ -1072,7 +1073,7 @@ int Solver::capture_mixed_graph() {
     gkey_.box_passes = mx_box_passes_;
     gkey_.box_fused = box_fused_;
     gkey_.line_batch = line_batch_;
-    gkey_.fork_asm = fork_asm_ + 2 * fork_blocks_;
+    gkey_.fork_asm = fork_key();
     return 0;
 }
 
@@ -1083,7 +1084,7 @@ int Solver::solve_step_mixed_graph() {
     if (!gexec_ || gkey_.nk != d_.nk || gkey_.blocks != graph_block_ || gkey_.box_blocks != box_blocks_ ||
         gkey_.delt != p_.delt || gkey_.scanvel != scanvel_ || gkey_.sched != mx_sched_ ||
         gkey_.box_passes != mx_box_passes_ || gkey_.box_fused != box_fused_ || gkey_.line_batch != line_batch_ ||
-        gkey_.fork_asm != fork_asm_ + 2 * fork_blocks_) {
+        gkey_.fork_asm != fork_key()) {
         int st = capture_mixed_graph();
         if (st) return st;
     }

@@ -335,7 +335,9 @@ int  amcfd_set_line_batch(amcfd_solver *s, int lines);
  * that workgroup count); the tiles near the box follow the flow part.  The
  * boundary launch keeps the ghost values it overwrites and they are put back
  * if the convergence test ends the loop first.  on = 0 runs everything in
- * order.  Every setting gives the same bits.  No reference counterpart. */
+ * order, as amcfd_run_batch does with several workers (there a stream that
+ * waits on another holds up the hardware queue it shares with other
+ * solvers).  Every setting gives the same bits.  No reference counterpart. */
 int  amcfd_set_assembly_overlap(amcfd_solver *s, int on);
 /* Mixed mode: the captured loop's current settings and the line batch.  A
  * mixed-mode solver starts from the tuning table's row for its interior cell
