@@ -969,23 +969,6 @@ __global__ __launch_bounds__(TX *TY) void k_h2t(Dims d, H2tConst q, DevGeom g, c
 // BoundCondManager.cpp:280-447 (one workgroup; every scan of the reference
 // is a first-crossing or extremum search, done here as parallel min/max).
 // ------------------------------------------------------------------------
-__device__ int block_min_int(int v, int *s) {
-    __syncthreads();
-    if (threadIdx.x == 0) s[0] = 0x7fffffff;
-    __syncthreads();
-    atomicMin(&s[0], v);
-    __syncthreads();
-    return s[0];
-}
-__device__ int block_max_int(int v, int *s) {
-    __syncthreads();
-    if (threadIdx.x == 0) s[0] = -0x7fffffff;
-    __syncthreads();
-    atomicMax(&s[0], v);
-    __syncthreads();
-    return s[0];
-}
-
 constexpr int MP_MAXNI = 4100;   // x-nodes of the longest grid a line solver takes
 struct MeltPoolK {
     static constexpr int family = 5, rank = 0;
@@ -995,7 +978,7 @@ struct MeltPoolK {
                                                     const double *__restrict__ z, double ts, int istart,
                                                     int jstart, DevScalars *sc, int ctl, int navierstokes,
                                                     const StepDev *step) {
-    __shared__ int si[1];
+    __shared__ int si[5];
     if (ctl && sc->done) return;
     if (step) {   // the beam's nearest node of this step (captured loops)
         istart = step->istart;
@@ -1046,19 +1029,11 @@ struct MeltPoolK {
     };
     auto Y = [&](int j) { return (j >= 0 && j < d.nj) ? y[j] : 0.0; };
     const int js = jstart, top = d.nkm1;
-    // length
-    int cand = d.nim2;
-    for (int i = istart + tid; i <= d.nim2; i += nt)
-        if (T(i, js, top) <= ts) cand = min(cand, i);
-    const int imax = block_min_int(cand, si);
-    cand = 1;
-    for (int i = 1 + tid; i <= istart; i += nt)
-        if (T(i, js, top) < ts) cand = max(cand, i);
-    const int imin = block_max_int(cand, si);
-    // depth: lowest k of the run that starts at nkm2 and stays >= ts; width:
-    // highest / lowest j of the runs from row js along the top plane.  A row
-    // whose starting cell is below ts has no run (it leaves the fold
-    // unchanged); for the others, the run ends at the first cell below ts:
+    // One pass over row js: the length (first cell at or below ts from the
+    // beam on, last one below ts before it) and the rows that start a depth
+    // run (at nkm2) or a width run (on the top plane).  A row whose starting
+    // cell is below ts has no run (it leaves the fold unchanged); for the
+    // others, the run ends at the first cell below ts:
     //   kf[i] = highest k < nkm2 with T(i, js, k) < ts (0 if none),
     //   ju[i] = lowest j in (js, njm2] with T(i, j, top) < ts (njm1 if none),
     //   jd[i] = highest j in [1, js) with T(i, j, top) < ts (0 if none),
@@ -1069,17 +1044,29 @@ struct MeltPoolK {
     if (tid == 0) {
         s_nd = 0;
         s_nw = 0;
+        si[0] = 0x7fffffff;
+        si[1] = -0x7fffffff;
     }
     __syncthreads();
-    for (int i = 1 + tid; i <= d.nim2; i += nt) {
-        const bool dr = !(T(i, js, d.nkm2) < ts), wr = !(T(i, js, top) < ts);
-        s_kf[i] = dr ? 0 : d.nkm2;
-        s_ju[i] = wr ? d.njm1 : js;
-        s_jd[i] = wr ? 0 : js;
-        if (dr) s_rd[atomicAdd(&s_nd, 1)] = i;
-        if (wr) s_rw[atomicAdd(&s_nw, 1)] = i;
+    {   // the length scans span istart..nim2 and 1..istart (istart is 0..nim1)
+        int cmax = d.nim2, cmin = 1;
+        for (int i = tid; i <= d.nim1; i += nt) {
+            const double tt = T(i, js, top);
+            if (i >= istart && i <= d.nim2 && tt <= ts) cmax = min(cmax, i);
+            if (i >= 1 && i <= istart && tt < ts) cmin = max(cmin, i);
+            if (i < 1 || i > d.nim2) continue;
+            const bool dr = !(T(i, js, d.nkm2) < ts), wr = !(tt < ts);
+            s_kf[i] = dr ? 0 : d.nkm2;
+            s_ju[i] = wr ? d.njm1 : js;
+            s_jd[i] = wr ? 0 : js;
+            if (dr) s_rd[atomicAdd(&s_nd, 1)] = i;
+            if (wr) s_rw[atomicAdd(&s_nw, 1)] = i;
+        }
+        atomicMin(&si[0], cmax);
+        atomicMax(&si[1], cmin);
     }
     __syncthreads();
+    const int imax = si[0], imin = si[1];
     const int nrd = s_nd, nrw = s_nw;
     auto scan = [&](int total, auto cell, auto hit) {
         constexpr int U = 8;
@@ -1113,30 +1100,36 @@ struct MeltPoolK {
         cju = max(cju, s_ju[i] - 1);
         cjd = min(cjd, s_jd[i] + 1);
     }
-    const int kmin = block_min_int(ck, si) - 1;
-    const int jmax = block_max_int(cju, si) + 1;
-    const int jmin = block_min_int(cjd, si) - 1;
+    if (tid == 0) {   // slots of their own: si[0..1] are still read above
+        si[2] = 0x7fffffff;
+        si[3] = -0x7fffffff;
+        si[4] = 0x7fffffff;
+    }
+    __syncthreads();
+    atomicMin(&si[2], ck);
+    atomicMax(&si[3], cju);
+    atomicMin(&si[4], cjd);
+    __syncthreads();
+    const int kmin = si[2] - 1, jmax = si[3] + 1, jmin = si[4] - 1;
     // interpolated extents (max / min over i of per-row values)
+    // the three interpolations of each row in one pass (each folds as its
+    // own loop would)
     double dep = -INFINITY, ymx = -INFINITY, ymn = INFINITY;
-    if (kmin != 0) {
-        for (int i = 1 + tid; i <= d.nim2; i += nt) {
-            if (T(i, js, kmin + 1) < ts) continue;
+    const bool kd = kmin != 0, ju = jmax != js, jd = jmin != js;
+    for (int i = 1 + tid; i <= d.nim2; i += nt) {
+        if (kd && !(T(i, js, kmin + 1) < ts)) {
             double dtdz = (z[kmin] - z[kmin - 1]) / (T(i, js, kmin) - T(i, js, kmin - 1));
             dep = dmax(z[top] - z[kmin] + (T(i, js, kmin) - ts) * dtdz, dep);
         }
-    }
-    if (jmax != js)
-        for (int i = 1 + tid; i <= d.nim2; i += nt) {
-            if (T(i, jmax - 1, top) < ts) continue;
+        if (ju && !(T(i, jmax - 1, top) < ts)) {
             double dtdy = (Y(jmax) - Y(jmax + 1)) / (T(i, jmax, top) - T(i, jmax + 1, top));
             ymx = dmax(Y(jmax) + (ts - T(i, jmax, top)) * dtdy, ymx);
         }
-    if (jmin != js)
-        for (int i = 1 + tid; i <= d.nim2; i += nt) {
-            if (T(i, jmin + 1, top) < ts) continue;
+        if (jd && !(T(i, jmin + 1, top) < ts)) {
             double dtdy = (Y(jmin) - Y(jmin - 1)) / (T(i, jmin, top) - T(i, jmin - 1, top));
             ymn = dmin(Y(jmin) + (ts - T(i, jmin, top)) * dtdy, ymn);
         }
+    }
     dep = wave_max(dep);
     ymx = wave_max(ymx);
     ymn = -wave_max(-ymn);
@@ -1200,6 +1193,28 @@ __global__ __launch_bounds__(1024) void k_melt_pool(Dims d, int nblk, const doub
 // end-of-iteration scalars: residual folds, heat balance, energy ratio
 // (CFDSolverManager.cpp:1989-2071, BoundCondManager.cpp:846-857)
 // ------------------------------------------------------------------------
+// the convergence test of CFDSolverManager.cpp:690-716 on the device, after
+// the folds (captured loops): counts the iteration; the first largest of
+// resorm, resoru, resorv, resorw; the heat-balance band with the laser on
+// (or in steady mode), resorh < 1e-6 with it off; done at maxit
+__device__ __forceinline__ void converge_test(DevScalars *sc, const StepDev *step, int steady, double tol) {
+    sc->niter += 1;
+    double res[4] = {sc->resorm, sc->resoru, sc->resorv, sc->resorw};
+    double amaxres = res[0];
+    for (int r = 1; r < 4; ++r)
+        if (amaxres < res[r]) amaxres = res[r];
+    const double ratio = sc->ratio;
+    bool ok;
+    if (steady) ok = ratio <= 1.001 && ratio >= 0.999 && amaxres < tol;
+    else if (step->powerindicator == 1) ok = ratio <= 1.01 && ratio >= 0.99 && amaxres < tol;
+    else ok = sc->resorh < 1e-6 && amaxres < tol;
+    if (ok || sc->niter >= step->maxit) {
+        sc->converged = ok ? 1 : 0;
+        sc->done = 1;
+        sc->noflow = 1;
+    }
+}
+
 struct FinalizeK {
     static constexpr int family = 6, rank = 0;
     static __device__ __forceinline__ void run(const VB &vb, int nblk_t, int nblk_b, int nblk_f, int nblk_e, int nblk_c,
@@ -1208,7 +1223,8 @@ struct FinalizeK {
                                                   const double *__restrict__ part_f,
                                                   const double *__restrict__ part_e,
                                                   const double *__restrict__ part_c, int flow_done, int species_done,
-                                                  int steady, double dens, double width, DevScalars *sc, int ctl) {
+                                                  int steady, double dens, double width, DevScalars *sc, int ctl,
+                                                  const StepDev *step, double tol) {
     if (ctl) {   // the loop's state lives on the device
         if (sc->done) return;
         flow_done = !sc->noflow;
@@ -1288,6 +1304,7 @@ struct FinalizeK {
         sc->resorm = msum / (r[14] * dens + AMCFD_SMALL);
     }
     if (species_done) sc->resorc = r[15] / (r[16] + AMCFD_SMALL);
+    if (ctl && step) converge_test(sc, step, steady, tol);   // the same thread: it reads what it wrote
 }
 };
 __global__ __launch_bounds__(256) void k_finalize(int nblk_t, int nblk_b, int nblk_f, int nblk_e, int nblk_c,
@@ -1296,8 +1313,10 @@ __global__ __launch_bounds__(256) void k_finalize(int nblk_t, int nblk_b, int nb
                                                   const double *__restrict__ part_f,
                                                   const double *__restrict__ part_e,
                                                   const double *__restrict__ part_c, int flow_done, int species_done,
-                                                  int steady, double dens, double width, DevScalars *sc, int ctl) {
-    FinalizeK::run(vb_hw(), nblk_t, nblk_b, nblk_f, nblk_e, nblk_c, part_t, part_b, part_f, part_e, part_c, flow_done, species_done, steady, dens, width, sc, ctl);
+                                                  int steady, double dens, double width, DevScalars *sc, int ctl,
+                                                  const StepDev *step, double tol) {
+    FinalizeK::run(vb_hw(), nblk_t, nblk_b, nblk_f, nblk_e, nblk_c, part_t, part_b, part_f, part_e, part_c, flow_done,
+                   species_done, steady, dens, width, sc, ctl, step, tol);
 }
 
 // ------------------------------------------------------------------------
@@ -1533,42 +1552,14 @@ void launch_melt_pool(hipStream_t st, const Dims &d, const DevGeom &g, DevFields
 }
 
 void launch_finalize(hipStream_t st, const Dims &d, Partials pr, DevScalars *sc, const amcfd_params &p,
-                     int flow_done, int species_done, int steady, double width, bool ctl) {
+                     int flow_done, int species_done, int steady, double width, bool ctl, const StepDev *step) {
     dim3 gr = node_grid(d);
     int nblk_t = gr.x * gr.y * gr.z;
     const int nblk_b = (d.njm2 * d.nim2 + d.nkm2 * d.njm2 + d.nkm2 * d.nim2 + 255) / 256;
     const dim3 e = assemble_grid(d), c = energy_grid(d);   // enthalpy and species assembly partials
     dispatch<FinalizeK, 256>(k_finalize, dim3(1), dim3(256), 0, st, nblk_t, nblk_b, pr.nblk_f, (int)(e.x * e.y * e.z),
                        (int)(c.x * c.y * c.z), pr.h2t, pr.bc, pr.flow, pr.energy, pr.species, flow_done, species_done, steady, p.dens,
-                       width, sc, (int)ctl);
-}
-
-// the convergence test of CFDSolverManager.cpp:690-716 on the device: the
-// first largest of resorm, resoru, resorv, resorw; the heat-balance band with
-// the laser on (or in steady mode), resorh < 1e-6 with it off
-struct ConvergeK {
-    static constexpr int family = 7, rank = 0;
-    static __device__ __forceinline__ void run(const VB &vb, DevScalars *sc, const StepDev *step, int steady, double tol) {
-    if (sc->done) return;
-    sc->niter += 1;
-    double res[4] = {sc->resorm, sc->resoru, sc->resorv, sc->resorw};
-    double amaxres = res[0];
-    for (int r = 1; r < 4; ++r)
-        if (amaxres < res[r]) amaxres = res[r];
-    const double ratio = sc->ratio;
-    bool ok;
-    if (steady) ok = ratio <= 1.001 && ratio >= 0.999 && amaxres < tol;
-    else if (step->powerindicator == 1) ok = ratio <= 1.01 && ratio >= 0.99 && amaxres < tol;
-    else ok = sc->resorh < 1e-6 && amaxres < tol;
-    if (ok || sc->niter >= step->maxit) {
-        sc->converged = ok ? 1 : 0;
-        sc->done = 1;
-        sc->noflow = 1;
-    }
-}
-};
-__global__ void k_converge(DevScalars *sc, const StepDev *step, int steady, double tol) {
-    ConvergeK::run(vb_hw(), sc, step, steady, tol);
+                       width, sc, (int)ctl, step, p.nonlintol);
 }
 
 __global__ void k_loop_reset(DevScalars *sc) {
@@ -1588,10 +1579,6 @@ __global__ __launch_bounds__(64) void k_group_done(int *const *__restrict__ done
 
 void launch_group_done(hipStream_t st, int *const *done, int n, int *out) {
     hipLaunchKernelGGL(k_group_done, dim3(1), dim3(64), 0, st, done, n, out);
-}
-
-void launch_converge(hipStream_t st, DevScalars *sc, const StepDev *step, const amcfd_params &p) {
-    dispatch<ConvergeK, 64>(k_converge, dim3(1), dim3(1), 0, st, sc, step, p.steady, p.nonlintol);
 }
 
 void launch_loop_reset(hipStream_t st, DevScalars *sc) {

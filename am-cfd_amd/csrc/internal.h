@@ -200,11 +200,12 @@ void launch_pressure(hipStream_t st, const Dims &d, const DevGeom &g, DevFields 
                      const Box &b, const amcfd_params &p, const BoxLaunch &bl = BoxLaunch());
 void launch_correct(hipStream_t st, const Dims &d, DevFields f, const Box &b, const amcfd_params &p,
                     const BoxLaunch &bl = BoxLaunch());
+// with ctl and step, the same launch then applies the convergence test of
+// CFDSolverManager.cpp:690-716 on the device: counts the iteration, sets
+// done / converged
 void launch_finalize(hipStream_t st, const Dims &d, Partials pr, DevScalars *sc, const amcfd_params &p,
-                     int flow_done, int species_done, int steady, double width, bool ctl = false);
-// device-side convergence test of CFDSolverManager.cpp:690-716 after a
-// finalize: counts the iteration, sets done / converged
-void launch_converge(hipStream_t st, DevScalars *sc, const StepDev *step, const amcfd_params &p);
+                     int flow_done, int species_done, int steady, double width, bool ctl = false,
+                     const StepDev *step = nullptr);
 // grouped runs: out = whether all n flags done[i] are set
 void launch_group_done(hipStream_t st, int *const *done, int n, int *out);
 // resets the device loop control before a step's captured iterations

@@ -895,7 +895,7 @@ int Solver::energy_sweeps_mixed(const int *skip) {
 // the very same launches.  Without ctl the host reads the scalars after the
 // melt pool (flow or not) and after the finalize (convergence), as the FP64
 // mode does; with ctl the kernels skip themselves on the device flags and
-// k_converge takes the decision.
+// the finalize launch takes the decision.
 int Solver::enqueue_mixed_iteration(bool ctl, int it, bool ahead, bool fork_next) {
     const int *done = ctl ? &sc_->done : nullptr;
     int slot;
@@ -994,8 +994,8 @@ int Solver::enqueue_mixed_iteration(bool ctl, int it, bool ahead, bool fork_next
             if (ivar == 4) launch_correct(st_, d_, f_, box_, p_, bl);
         }
     }
-    launch_finalize(st_, d_, pq, sc_, p_, flow, 0, p_.steady, ctl ? 0.0 : sc_host_->width, ctl);
-    if (ctl) launch_converge(st_, sc_, stepd_, p_);
+    launch_finalize(st_, d_, pq, sc_, p_, flow, 0, p_.steady, ctl ? 0.0 : sc_host_->width, ctl,
+                    ctl ? stepd_ : nullptr);
     HIPCALL(hipGetLastError());
     return 0;
 }
