@@ -6,7 +6,7 @@
 // It is trained on the GPU from steps the solver itself runs (record mode)
 // and then supplies each step's initial iterate (apply mode).  The SIMPLE
 // iteration (CFDSolverManager.cpp:595-722) still runs to the same
-// convergence test, so the method stays the judge: the guess can only
+// convergence test, so the method still decides the answer: the guess can only
 // change where the iteration starts (CFDSolverManager.cpp:608-612).
 //
 // Network (input D x H x W, D, H, W multiples of 8):

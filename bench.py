@@ -413,8 +413,9 @@ def main():
         profile_leg(args)
         return
     if args.gpus != 1:
-        print("bench.py measures one GPU: every GPU box of this build has one, and the solve's line sweeps "
-              "span the whole grid of one instance; --gpus must be 1.", file=sys.stderr)
+        print("bench.py measures one GPU: the solve of one instance runs on one device (its line sweeps span "
+              "the whole grid); --gpus must be 1. Several instances share a device with amcfd_run_grouped.",
+              file=sys.stderr)
         sys.exit(2)
 
     import numpy as np

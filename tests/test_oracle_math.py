@@ -95,7 +95,7 @@ def test_grid_closed_form(amcfd, oracle_mod, case_dir):
 
 
 def test_plate120_first_steps_bitwise(amcfd, oracle_mod, case_dir):
-    """configs[0]-sized deck: the first 6 steps reproduce the reference's
+    """The 120x48x32 plate: the first 6 steps reproduce the reference's
     per-step scalars exactly (iteration counts, residuals, pool extents)."""
     deck = amcfd.Deck(case_dir("plate120"))
     o = oracle_mod.Oracle(deck.params)
