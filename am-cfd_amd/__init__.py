@@ -156,11 +156,11 @@ def load_library(path=LIB_PATH):
         "amcfd_test_expire_waits": (ip, [vp]),
         "amcfd_test_libm": (ip, [ip, dp, dp, ctypes.c_size_t]),
         "amcfd_set_graph": (ip, [vp, ip, ip, ip]),
-        "amcfd_get_graph": (ip, [vp] + [ctypes.POINTER(ctypes.c_int)] * 5),
+        "amcfd_get_graph": (ip, [vp] + [ctypes.POINTER(ctypes.c_int)] * 6),
         "amcfd_set_line_batch": (ip, [vp, ip]),
         "amcfd_set_assembly_overlap": (ip, [vp, ip]),
         "amcfd_set_box_fused": (ip, [vp, ip]),
-        "amcfd_tuning_choice": (ip, [cp, ctypes.c_longlong] + [ctypes.POINTER(ctypes.c_int)] * 4),
+        "amcfd_tuning_choice": (ip, [cp, ctypes.c_longlong] + [ctypes.POINTER(ctypes.c_int)] * 5),
         "amcfd_copy_state": (ip, [vp, vp]),
         "amcfd_tdma_lines": (ip, [vp, vp, vp, vp, vp, ip, ip, ip, vp]),
         "amcfd_tdma_lines_noalloc": (ip, [vp, vp, vp, vp, vp, ip, ip, ip, vp]),
@@ -338,11 +338,12 @@ class Solver:
 
     def graph_settings(self):
         """Mixed mode: (captured loop on, iterations per graph, box workgroups,
-        box passes in one persistent launch, x-lines per workgroup)."""
-        on, b, bb, bf, lb = ctypes.c_int(), ctypes.c_int(), ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        box passes in one persistent launch, x-lines per workgroup, assembly
+        overlap on)."""
+        on, b, bb, bf, lb, ov = (ctypes.c_int() for _ in range(6))
         _check(load_library().amcfd_get_graph(self._h, ctypes.byref(on), ctypes.byref(b), ctypes.byref(bb),
-                                              ctypes.byref(bf), ctypes.byref(lb)))
-        return bool(on.value), b.value, bb.value, bool(bf.value), lb.value
+                                              ctypes.byref(bf), ctypes.byref(lb), ctypes.byref(ov)))
+        return bool(on.value), b.value, bb.value, bool(bf.value), lb.value, bool(ov.value)
 
     def set_line_batch(self, lines=1):
         """Mixed mode: consecutive x-lines per workgroup in the enthalpy's
@@ -401,13 +402,13 @@ def run_grouped(solvers, max_steps=-1):
 
 
 def tuning_choice(cells, path=None):
-    """(graph_block, box_blocks, box_fused, line_batch) the mixed mode takes
-    for a grid of `cells` interior cells, from the table at `path` or the
-    library's own."""
-    gb, bb, bf, lb = ctypes.c_int(), ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+    """(graph_block, box_blocks, box_fused, line_batch, overlap) the mixed
+    mode takes for a grid of `cells` interior cells, from the table at `path`
+    or the library's own."""
+    gb, bb, bf, lb, ov = ctypes.c_int(), ctypes.c_int(), ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
     _check(load_library().amcfd_tuning_choice(os.fsencode(path) if path else None, int(cells), ctypes.byref(gb),
-                                              ctypes.byref(bb), ctypes.byref(bf), ctypes.byref(lb)))
-    return gb.value, bb.value, bool(bf.value), lb.value
+                                              ctypes.byref(bb), ctypes.byref(bf), ctypes.byref(lb), ctypes.byref(ov)))
+    return gb.value, bb.value, bool(bf.value), lb.value, bool(ov.value)
 
 
 def device_libm(fn, x):

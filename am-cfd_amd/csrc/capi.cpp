@@ -313,13 +313,15 @@ int amcfd_set_graph(amcfd_solver *s, int on, int block, int box_blocks) {
     GUARD(s->s.set_graph(on, block, box_blocks));
 }
 
-int amcfd_get_graph(const amcfd_solver *s, int *on, int *block, int *box_blocks, int *box_fused, int *line_batch) {
-    if (!s || !on || !block || !box_blocks || !box_fused || !line_batch)
+int amcfd_get_graph(const amcfd_solver *s, int *on, int *block, int *box_blocks, int *box_fused, int *line_batch,
+                    int *overlap) {
+    if (!s || !on || !block || !box_blocks || !box_fused || !line_batch || !overlap)
         return amcfd::set_error(AMCFD_E_ARG, "null argument");
     if (s->s.mode() != AMCFD_MODE_MIXED) return amcfd::set_error(AMCFD_E_ARG, "the solver is not in the mixed mode");
     s->s.graph_settings(on, block, box_blocks);
     *box_fused = s->s.box_fused();
     *line_batch = s->s.line_batch();
+    *overlap = s->s.assembly_overlap();
     return AMCFD_OK;
 }
 

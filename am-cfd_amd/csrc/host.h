@@ -38,7 +38,8 @@ int ws_apply(WarmStart *w, Solver &s);
 void ws_attach(WarmStart *w, Solver *s);
 // the tuning table's choice for a grid of `cells` interior cells (tuning.cpp);
 // leaves the values alone when no table or no row applies
-void tuning_lookup(long long cells, int *graph_block, int *box_blocks, int *box_fused, int *line_batch);
+void tuning_lookup(long long cells, int *graph_block, int *box_blocks, int *box_fused, int *line_batch,
+                   int *overlap);
 void ws_detach(WarmStart *w, Solver *s);
 
 // accumulated device time of one kernel family (HIP events on the solver stream)
@@ -81,6 +82,7 @@ class Solver {
     int line_batch() const { return line_batch_; }
     int set_line_batch(int lines);
     int set_assembly_overlap(int on);
+    int assembly_overlap() const { return fork_asm_ ? 1 : 0; }
     // mixed mode: order of the enthalpy line passes ('x' red-black x-lines,
     // 'y' y-lines, 'z' z-lines; an even number of y/z) and red-black x
     // passes per melt-pool box variable

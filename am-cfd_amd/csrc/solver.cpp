@@ -178,7 +178,7 @@ int Solver::create(const amcfd_params &p, int mode) {
     HIPCALL(hipGetDevice(&dev));
     if (mixed) {   // the captured loop's launch shapes for this size, from the tuning table if there is one
         tuning_lookup((long long)(p.ni - 2) * (p.nj - 2) * (p.nk - 2), &graph_block_, &box_blocks_, &box_fused_,
-                      &line_batch_);
+                      &line_batch_, &fork_asm_);
         box_blocks_ = std::min(box_blocks_, max_box_blocks);
         // the far assembly tiles beside the flow part: about one and a half
         // workgroups per compute unit, which leaves room on every unit for

@@ -5,16 +5,20 @@
 // results: the iterations per graph launch (the scalars are read once per
 // launch), the workgroups that stride over the melt-pool box, and whether a
 // box variable's passes run in one persistent launch or one launch per half
-// pass, and the line batch of the enthalpy's x-line passes (consecutive
-// lines per workgroup).  Their best values depend on the grid size, so the
-// tuner measures each candidate on representative grids and writes one row
-// per size class:
-//     max_cells  graph_block  box_blocks  [box_fused  [line_batch]]
+// pass, the line batch of the enthalpy's x-line passes (consecutive lines
+// per workgroup), and whether the next iteration's assembly runs beside the
+// flow part.  Their best values depend on the grid size, so the tuner
+// measures each candidate on representative grids and writes one row per
+// size class:
+//     max_cells  graph_block  box_blocks  [box_fused  [line_batch  [overlap]]]
 // ('#' starts a comment; box_fused 0 or 1, default 1; line_batch 1..64,
-// default 1).  A solver takes the first row whose max_cells is at least its
-// interior cell count; with no row (or no table) it keeps the built-in 8,
-// 256, 1 and 1.  The table is the file named by $AMCFD_TUNING, else
-// tuning.txt beside libamcfd.so.
+// default 1; overlap 0 or 1, default 0).  A solver takes the first row whose
+// max_cells is at least its interior cell count; with no row (or no table)
+// it keeps the built-in 8, 256, 1, 1, 0.  The overlap is off by default
+// because it pays on some grids only: measured 5% faster on the 5.12 M-cell
+// track, 2-8% slower on sweep instances of 0.3-2 M cells and on the
+// 9.7 M-cell multilayer deck.  The table is the file named by $AMCFD_TUNING,
+// else tuning.txt beside libamcfd.so.
 
 #include <dlfcn.h>
 
@@ -34,7 +38,7 @@ namespace {
 
 struct TuneRow {
     long long max_cells;
-    int graph_block, box_blocks, box_fused, line_batch;
+    int graph_block, box_blocks, box_fused, line_batch, overlap;   // overlap -1: not given
 };
 
 // rows in file order; a malformed line is an error (the whole table is refused)
@@ -48,35 +52,45 @@ int parse_table(const char *path, std::vector<TuneRow> &rows) {
         char *hash = std::strchr(line, '#');
         if (hash) *hash = '\0';
         long long cells;
-        int gb, bb, bf = 1, lb = 1;
+        int gb, bb, bf = 1, lb = 1, ov = -1;
         char extra;
         bool blank = true;
         for (const char *c = line; *c; ++c) blank = blank && std::isspace((unsigned char)*c);
         if (blank) continue;   // blank or comment
-        const int n = std::sscanf(line, "%lld %d %d %d %d %c", &cells, &gb, &bb, &bf, &lb, &extra);
-        if (n < 3 || n > 5 || cells < 1 || gb < 1 || gb > 64 || bb < 1 || bb > 4096 || (bf != 0 && bf != 1) ||
-            lb < 1 || lb > 64) {
+        const int n = std::sscanf(line, "%lld %d %d %d %d %d %c", &cells, &gb, &bb, &bf, &lb, &ov, &extra);
+        if (n < 3 || n > 6 || cells < 1 || gb < 1 || gb > 64 || bb < 1 || bb > 4096 || (bf != 0 && bf != 1) ||
+            lb < 1 || lb > 64 || (n == 6 && ov != 0 && ov != 1)) {
             st = set_error(AMCFD_E_PARSE, std::string(path) + ":" + std::to_string(lineno) +
                                               ": expected 'max_cells graph_block(1..64) box_blocks(1..4096) "
-                                              "[box_fused(0|1) [line_batch(1..64)]]'");
+                                              "[box_fused(0|1) [line_batch(1..64) [overlap(0|1)]]]'");
             break;
         }
-        rows.push_back(TuneRow{cells, gb, bb, bf, lb});
+        rows.push_back(TuneRow{cells, gb, bb, bf, lb, n == 6 ? ov : -1});
     }
     std::fclose(f);
     return st;
 }
 
-bool choose(const std::vector<TuneRow> &rows, long long cells, int *gb, int *bb, int *bf, int *lb) {
+bool choose(const std::vector<TuneRow> &rows, long long cells, int *gb, int *bb, int *bf, int *lb, int *ov) {
     for (const TuneRow &r : rows)
         if (cells <= r.max_cells) {
             *gb = r.graph_block;
             *bb = r.box_blocks;
             *bf = r.box_fused;
             *lb = r.line_batch;
+            if (r.overlap >= 0) *ov = r.overlap;
             return true;
         }
     return false;
+}
+
+// the shapes a solver of `cells` interior cells starts from without a table row
+void built_in(int *gb, int *bb, int *bf, int *lb, int *ov) {
+    *gb = 8;
+    *bb = 256;
+    *bf = 1;
+    *lb = 1;
+    *ov = 0;
 }
 
 std::string default_path() {
@@ -105,9 +119,11 @@ void load_default() {
 
 }  // namespace
 
-void tuning_lookup(long long cells, int *graph_block, int *box_blocks, int *box_fused, int *line_batch) {
+void tuning_lookup(long long cells, int *graph_block, int *box_blocks, int *box_fused, int *line_batch,
+                   int *overlap) {
     std::call_once(table_once, load_default);
-    choose(table, cells, graph_block, box_blocks, box_fused, line_batch);
+    built_in(graph_block, box_blocks, box_fused, line_batch, overlap);
+    choose(table, cells, graph_block, box_blocks, box_fused, line_batch, overlap);
 }
 
 }  // namespace amcfd
@@ -115,22 +131,20 @@ void tuning_lookup(long long cells, int *graph_block, int *box_blocks, int *box_
 extern "C" {
 
 int amcfd_tuning_choice(const char *path, long long cells, int *graph_block, int *box_blocks, int *box_fused,
-                        int *line_batch) {
+                        int *line_batch, int *overlap) {
     using namespace amcfd;
-    if (!graph_block || !box_blocks || !box_fused || !line_batch) return set_error(AMCFD_E_ARG, "null argument");
+    if (!graph_block || !box_blocks || !box_fused || !line_batch || !overlap)
+        return set_error(AMCFD_E_ARG, "null argument");
     if (cells < 1) return set_error(AMCFD_E_ARG, "cells must be positive");
-    *graph_block = 8;
-    *box_blocks = 256;
-    *box_fused = 1;
-    *line_batch = 1;
     if (!path) {
-        tuning_lookup(cells, graph_block, box_blocks, box_fused, line_batch);
+        tuning_lookup(cells, graph_block, box_blocks, box_fused, line_batch, overlap);
         return AMCFD_OK;
     }
     std::vector<TuneRow> rows;
     const int st = parse_table(path, rows);
     if (st) return st;
-    choose(rows, cells, graph_block, box_blocks, box_fused, line_batch);
+    built_in(graph_block, box_blocks, box_fused, line_batch, overlap);
+    choose(rows, cells, graph_block, box_blocks, box_fused, line_batch, overlap);
     return AMCFD_OK;
 }
 

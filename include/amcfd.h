@@ -330,31 +330,35 @@ int  amcfd_set_line_batch(amcfd_solver *s, int lines);
 /* Mixed mode, captured loop: while one iteration's flow part (the melt-pool
  * box solves, latency-bound and on few compute units) runs, the next
  * iteration's enthalpy boundary launch and the tiles of its assembly away
- * from the box run beside it on a second stream (on = 1, default, with about
- * 1.5 workgroups per compute unit striding over those tiles; on > 1 sets
- * that workgroup count); the tiles near the box follow the flow part.  The
+ * from the box run beside it on a second stream (on = 1, with about 1.5
+ * workgroups per compute unit striding over those tiles; on > 1 sets that
+ * workgroup count); the tiles near the box follow the flow part.  A solver
+ * starts from the tuning table's choice (off unless a row turns it on).  The
  * boundary launch keeps the ghost values it overwrites and they are put back
  * if the convergence test ends the loop first.  on = 0 runs everything in
  * order, as amcfd_run_batch does with several workers (there a stream that
  * waits on another holds up the hardware queue it shares with other
  * solvers).  Every setting gives the same bits.  No reference counterpart. */
 int  amcfd_set_assembly_overlap(amcfd_solver *s, int on);
-/* Mixed mode: the captured loop's current settings and the line batch.  A
- * mixed-mode solver starts from the tuning table's row for its interior cell
- * count (below). */
-int  amcfd_get_graph(const amcfd_solver *s, int *on, int *block, int *box_blocks, int *box_fused, int *line_batch);
+/* Mixed mode: the captured loop's current settings, the line batch and
+ * whether the assembly overlap is on.  A mixed-mode solver starts from the
+ * tuning table's row for its interior cell count (below). */
+int  amcfd_get_graph(const amcfd_solver *s, int *on, int *block, int *box_blocks, int *box_fused, int *line_batch,
+                     int *overlap);
 
 /* The launch-shape table of the mixed mode (no reference counterpart),
  * written by am-cfd_amd/autotune.py: lines "max_cells graph_block
- * box_blocks [box_fused [line_batch]]", '#' comments; the first row with
- * max_cells >= the grid's interior cells applies, else the built-in 8, 256,
- * 1 and 1.  These
- * values change the captured loop's speed, never its results.  The library's table is the
- * file named by $AMCFD_TUNING, else tuning.txt beside libamcfd.so, read
- * once.  amcfd_tuning_choice reports the choice for `cells` from the file at
+ * box_blocks [box_fused [line_batch [overlap]]]", '#' comments; the first
+ * row with max_cells >= the grid's interior cells applies, else the built-in
+ * 8, 256, 1, 1 and the assembly overlap (amcfd_set_assembly_overlap) off;
+ * the shipped table turns the overlap on where it measured faster.  These
+ * values change the captured loop's speed, never its results.  The
+ * library's table is the file named by $AMCFD_TUNING, else tuning.txt beside
+ * libamcfd.so, read once.
+ * amcfd_tuning_choice reports the choice for `cells` from the file at
  * `path`, or from the library's table when path is NULL; it needs no GPU. */
 int  amcfd_tuning_choice(const char *path, long long cells, int *graph_block, int *box_blocks, int *box_fused,
-                         int *line_batch);
+                         int *line_batch, int *overlap);
 
 /* Copy the whole state of src into dst (same deck, any modes): every
  * solution field, the grid, the laser and time-loop state and the scalars,

@@ -202,7 +202,8 @@ def test_solver_takes_its_launch_shapes_from_the_tuning_table(case_dir, amcfd, t
     s.set_graph(True, 16, 128)
     s.set_box_fused(False)
     s.set_line_batch(3)
-    assert s.graph_settings() == (True, 16, 128, False, 3)
+    s.set_assembly_overlap(False)
+    assert s.graph_settings() == (True, 16, 128, False, 3, False)
 
 
 def test_wide_grids_run_without_yz_passes(amcfd, tmp_path):

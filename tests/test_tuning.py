@@ -15,24 +15,28 @@ def write(tmp_path, text):
 
 def test_first_row_that_holds_the_grid_applies(tmp_path):
     amcfd = amcfd_module()
-    path = write(tmp_path, "# max_cells graph_block box_blocks [box_fused [line_batch]]\n"
-                           "1000000 16 128 0 4   # small grids\n"
+    path = write(tmp_path, "# max_cells graph_block box_blocks [box_fused [line_batch [overlap]]]\n"
+                           "1000000 16 128 0 4 1   # small grids\n"
                            "\n"
                            "6000000 8 512\n")
-    assert amcfd.tuning_choice(1, path) == (16, 128, False, 4)
-    assert amcfd.tuning_choice(1000000, path) == (16, 128, False, 4)
-    assert amcfd.tuning_choice(1000001, path) == (8, 512, True, 1)   # box_fused and line_batch default to 1
-    assert amcfd.tuning_choice(6000000, path) == (8, 512, True, 1)
-    assert amcfd.tuning_choice(6000001, path) == (8, 256, True, 1)   # past the last row: the built-in shapes
+    assert amcfd.tuning_choice(1, path) == (16, 128, False, 4, True)
+    assert amcfd.tuning_choice(1000000, path) == (16, 128, False, 4, True)
+    # box_fused and line_batch default to 1, the overlap to off
+    assert amcfd.tuning_choice(1000001, path) == (8, 512, True, 1, False)
+    assert amcfd.tuning_choice(6000000, path) == (8, 512, True, 1, False)
+    assert amcfd.tuning_choice(6000001, path) == (8, 256, True, 1, False)   # past the last row: the built-in shapes
 
 
 def test_empty_table_gives_the_built_in_shapes(tmp_path):
     amcfd = amcfd_module()
-    assert amcfd.tuning_choice(5120000, write(tmp_path, "# nothing tuned\n")) == (8, 256, True, 1)
+    empty = write(tmp_path, "# nothing tuned\n")
+    for cells in (1, 300000, 5120000, 10 ** 9):
+        assert amcfd.tuning_choice(cells, empty) == (8, 256, True, 1, False)
 
 
 @pytest.mark.parametrize("bad", ["1000 8\n", "1000 0 256\n", "1000 8 5000\n", "0 8 256\n", "1000 8 256 7\n",
-                                 "1000 8 256 1 0\n", "1000 8 256 1 65\n", "1000 8 256 1 2 3\n", "x 8 256\n"])
+                                 "1000 8 256 1 0\n", "1000 8 256 1 65\n", "1000 8 256 1 2 3\n",
+                                 "1000 8 256 1 2 1 4\n", "x 8 256\n"])
 def test_malformed_rows_are_refused(tmp_path, bad):
     amcfd = amcfd_module()
     with pytest.raises(amcfd.AmcfdError) as e:
