@@ -699,18 +699,14 @@ struct PropConst {
     int powder;
 };
 
-struct PropsK {
-    static constexpr int family = 3, rank = 0;
-    static __device__ __forceinline__ void run(const VB &vb, size_t n, PropConst c, const double *__restrict__ temp,
-                                               const double *__restrict__ fracl,
-                                               const double *__restrict__ csfrac, double *diff, double *rho,
-                                               double *vis, const int *skip) {
-    if (skip && *skip) return;
-    size_t t = (size_t)vb.x * blockDim.x + threadIdx.x;
-    if (t >= n) return;
-    double T = temp[t];
+// the properties of node t from its temperature, liquid fraction and
+// powder fraction (CFDSolverManager.cpp:624 and the property routines it calls)
+__device__ __forceinline__ void props_at(const PropConst &c, size_t t, double T, const double *__restrict__ fracl,
+                                         const double *__restrict__ csfrac, double &dv, double &rv, double &vv) {
     double diffl = (c.kla * T + c.klb) / c.acpl;
-    double dv = diffl, rv = c.denl, vv = c.viscos;
+    dv = diffl;
+    rv = c.denl;
+    vv = c.viscos;
     if (!(T >= c.tliquid)) {
         double diffs, rhos;
         if (!c.powder) {
@@ -734,6 +730,19 @@ struct PropsK {
             vv = c.viscos;
         }
     }
+}
+
+struct PropsK {
+    static constexpr int family = 3, rank = 0;
+    static __device__ __forceinline__ void run(const VB &vb, size_t n, PropConst c, const double *__restrict__ temp,
+                                               const double *__restrict__ fracl,
+                                               const double *__restrict__ csfrac, double *diff, double *rho,
+                                               double *vis, const int *skip) {
+    if (skip && *skip) return;
+    size_t t = (size_t)vb.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    double dv, rv, vv;
+    props_at(c, t, temp[t], fracl, csfrac, dv, rv, vv);
     diff[t] = dv;
     rho[t] = rv;
     vis[t] = vv;
@@ -877,15 +886,18 @@ struct H2tConst {
     int steady;
 };
 
-struct H2tK {
-    static constexpr int family = 4, rank = 0;
-    static __device__ __forceinline__ void run(const VB &vb, Dims d, H2tConst q, DevGeom g, const double *__restrict__ h,
-                                                 const double *__restrict__ hnot,
-                                                 const double *__restrict__ fraclnot,
-                                                 const double *__restrict__ rho,
-                                                 const double *__restrict__ diff,
-                                                 const double *__restrict__ src, double *temp, double *fracl,
-                                                 double *partial, const int *skip) {
+// PROPS: the node's properties are computed first, from the temperature and
+// liquid fraction the node holds on entry, stored, and used by the heat-balance
+// terms in place of the stored ones (the same values, one read less); no node
+// reads another node's properties, so the fusion keeps every result.
+template <bool PROPS>
+__device__ __forceinline__ void h2t_body(const VB &vb, const Dims &d, const H2tConst &q, const PropConst &pc,
+                                         const DevGeom &g, const double *__restrict__ h,
+                                         const double *__restrict__ hnot, const double *__restrict__ fraclnot,
+                                         const double *__restrict__ csfrac, const double *__restrict__ src,
+                                         double *__restrict__ temp, double *__restrict__ fracl,
+                                         double *__restrict__ diff, double *__restrict__ rho,
+                                         double *__restrict__ vis, double *partial, const int *skip) {
     __shared__ double scratch[10 * 4];
     if (skip && *skip) return;
     const int tx = threadIdx.x, ty = threadIdx.y;
@@ -895,11 +907,38 @@ struct H2tK {
     double acc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};   // accul heatvol W E B T S N
     const double dT = q.tliquid - q.tsolid;
     if (i < d.ni && j < d.nj) {
+        // every plane's inputs are loaded before any is used, so a thread has
+        // its KB planes' loads in flight at once
+        double hpre[KB], tpre[KB], hnpre[KB], fnpre[KB], rpre[KB];
+#pragma unroll
         for (int kk = 0; kk < KB; ++kk) {
             const int k = vb.z * KB + kk;
             if (k >= d.nk) break;
             const size_t c = cix(d, i, j, k);
-            const double hv = h[c];
+            hpre[kk] = h[c];
+            if constexpr (PROPS) tpre[kk] = temp[c];
+            if (!q.steady) {
+                hnpre[kk] = hnot[c];
+                fnpre[kk] = fraclnot[c];
+                if constexpr (!PROPS) rpre[kk] = rho[c];
+            }
+        }
+#pragma unroll
+        for (int kk = 0; kk < KB; ++kk) {
+            const int k = vb.z * KB + kk;
+            if (k >= d.nk) break;
+            const size_t c = cix(d, i, j, k);
+            double dv = 0.0, rv = 0.0;
+            if constexpr (PROPS) {
+                double vv;
+                props_at(pc, c, tpre[kk], fracl, csfrac, dv, rv, vv);
+                diff[c] = dv;
+                rho[c] = rv;
+                vis[c] = vv;
+            } else {
+                rv = rpre[kk];
+            }
+            const double hv = hpre[kk];
             double T, fl;
             if (hv >= q.hlcal) {
                 fl = 1.0;
@@ -918,16 +957,16 @@ struct H2tK {
             if (!in) continue;
             if (!q.steady) {
                 double vol = (g.xu[i + 1] - g.xu[i]) * (g.yv[j + 1] - g.yv[j]) * (g.zw[k + 1] - g.zw[k]);
-                double dh = hv - hnot[c] + (fl - fraclnot[c]) * q.hlatnt;
-                acc[0] += vol * rho[c] * dh / q.delt;
+                double dh = hv - hnpre[kk] + (fl - fnpre[kk]) * q.hlatnt;
+                acc[0] += vol * rv * dh / q.delt;
                 if (src) acc[1] += src[c] * vol;   // null: no volumetric source, the field is zero
             }
-            if (i == 1) acc[2] += g.areajk[k * d.nj + j] * (diff[c] * (h[c - 1] - hv) * g.dxpwinv[1]);
-            if (i == d.nim2) acc[3] += g.areajk[k * d.nj + j] * (diff[c] * (h[c + 1] - hv) * g.dxpwinv[d.nim1]);
-            if (k == 1) acc[4] += g.areaij[j * d.ni + i] * (diff[c] * (h[c - d.sk] - hv) * g.dzpbinv[1]);
-            if (k == d.nkm2) acc[5] += g.areaij[j * d.ni + i] * (diff[c] * (h[c + d.sk] - hv) * g.dzpbinv[d.nkm1]);
-            if (j == 1) acc[6] += g.areaik[k * d.ni + i] * (diff[c] * (h[c - d.sj] - hv) * g.dypsinv[1]);
-            if (j == d.njm2) acc[7] += g.areaik[k * d.ni + i] * (diff[c] * (h[c + d.sj] - hv) * g.dypsinv[d.njm1]);
+            if (i == 1) acc[2] += g.areajk[k * d.nj + j] * ((PROPS ? dv : diff[c]) * (h[c - 1] - hv) * g.dxpwinv[1]);
+            if (i == d.nim2) acc[3] += g.areajk[k * d.nj + j] * ((PROPS ? dv : diff[c]) * (h[c + 1] - hv) * g.dxpwinv[d.nim1]);
+            if (k == 1) acc[4] += g.areaij[j * d.ni + i] * ((PROPS ? dv : diff[c]) * (h[c - d.sk] - hv) * g.dzpbinv[1]);
+            if (k == d.nkm2) acc[5] += g.areaij[j * d.ni + i] * ((PROPS ? dv : diff[c]) * (h[c + d.sk] - hv) * g.dzpbinv[d.nkm1]);
+            if (j == 1) acc[6] += g.areaik[k * d.ni + i] * ((PROPS ? dv : diff[c]) * (h[c - d.sj] - hv) * g.dypsinv[1]);
+            if (j == d.njm2) acc[7] += g.areaik[k * d.ni + i] * ((PROPS ? dv : diff[c]) * (h[c + d.sj] - hv) * g.dypsinv[d.njm1]);
         }
     }
     // block reduction: max of T, sums of the rest
@@ -953,6 +992,33 @@ struct H2tK {
         }
     }
 }
+
+struct H2tK {
+    static constexpr int family = 4, rank = 0;
+    static __device__ __forceinline__ void run(const VB &vb, Dims d, H2tConst q, DevGeom g, const double *__restrict__ h,
+                                                 const double *__restrict__ hnot,
+                                                 const double *__restrict__ fraclnot,
+                                                 const double *__restrict__ rho,
+                                                 const double *__restrict__ diff,
+                                                 const double *__restrict__ src, double *temp, double *fracl,
+                                                 double *partial, const int *skip) {
+        h2t_body<false>(vb, d, q, PropConst{}, g, h, hnot, fraclnot, nullptr, src, temp, fracl,
+                        const_cast<double *>(diff), const_cast<double *>(rho), nullptr, partial, skip);
+    }
+};
+
+// the properties (CFDSolverManager.cpp:624) and H -> T in one pass, for loops
+// where nothing between the two reads or writes what they read or write
+struct PropsH2tK {
+    static constexpr int family = 40, rank = 0;
+    static __device__ __forceinline__ void run(const VB &vb, Dims d, H2tConst q, PropConst pc, DevGeom g,
+                                               const double *__restrict__ h, const double *__restrict__ hnot,
+                                               const double *__restrict__ fraclnot,
+                                               const double *__restrict__ csfrac, const double *__restrict__ src,
+                                               double *temp, double *fracl, double *diff, double *rho, double *vis,
+                                               double *partial, const int *skip) {
+        h2t_body<true>(vb, d, q, pc, g, h, hnot, fraclnot, csfrac, src, temp, fracl, diff, rho, vis, partial, skip);
+    }
 };
 __global__ __launch_bounds__(TX *TY) void k_h2t(Dims d, H2tConst q, DevGeom g, const double *__restrict__ h,
                                                  const double *__restrict__ hnot,
@@ -962,6 +1028,16 @@ __global__ __launch_bounds__(TX *TY) void k_h2t(Dims d, H2tConst q, DevGeom g, c
                                                  const double *__restrict__ src, double *temp, double *fracl,
                                                  double *partial, const int *skip) {
     H2tK::run(vb_hw(), d, q, g, h, hnot, fraclnot, rho, diff, src, temp, fracl, partial, skip);
+}
+
+__global__ __launch_bounds__(TX *TY) void k_props_h2t(Dims d, H2tConst q, PropConst pc, DevGeom g,
+                                                       const double *__restrict__ h, const double *__restrict__ hnot,
+                                                       const double *__restrict__ fraclnot,
+                                                       const double *__restrict__ csfrac,
+                                                       const double *__restrict__ src, double *temp, double *fracl,
+                                                       double *diff, double *rho, double *vis, double *partial,
+                                                       const int *skip) {
+    PropsH2tK::run(vb_hw(), d, q, pc, g, h, hnot, fraclnot, csfrac, src, temp, fracl, diff, rho, vis, partial, skip);
 }
 
 // ------------------------------------------------------------------------
@@ -1512,10 +1588,14 @@ int mixed_tile_count(const Dims &d) {
     return (int)(a.y * a.z);
 }
 
+static PropConst prop_const(const amcfd_params &p) {
+    return PropConst{p.tliquid, p.tsolid, p.viscos, p.dens, p.denl, p.acpa, p.acpb, p.acpl, p.thconsa,
+                     p.thconsb, p.thconsc, p.thconla, p.thconlb, p.powderden, p.powdercpa, p.powdercpb,
+                     p.powderthcona, p.powderthconb, p.powderbed};
+}
+
 void launch_props(hipStream_t st, const Dims &d, DevFields f, const amcfd_params &p, const int *skip) {
-    PropConst c{p.tliquid, p.tsolid, p.viscos, p.dens, p.denl, p.acpa, p.acpb, p.acpl, p.thconsa,
-                p.thconsb, p.thconsc, p.thconla, p.thconlb, p.powderden, p.powdercpa, p.powdercpb,
-                p.powderthcona, p.powderthconb, p.powderbed};
+    PropConst c = prop_const(p);
     size_t n = (size_t)d.ni * d.nj * d.nk;
     dispatch<PropsK, 256>(k_props, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, n, c, f.temp, f.fracl,
                        f.csfrac, f.diff, f.rho, f.vis, skip);
@@ -1540,6 +1620,14 @@ void launch_h2t(hipStream_t st, const Dims &d, const DevGeom &g, DevFields f, Pa
     H2tConst q{p.hlcal, p.hsmelt, p.acpa, p.acpb, p.acpl, p.tliquid, p.tsolid, p.hlatnt, p.delt, p.steady};
     dispatch<H2tK, TX * TY>(k_h2t, node_grid(d), dim3(TX, TY), 0, st, d, q, g, f.enthalpy, f.hnot, f.fraclnot, f.rho,
                        f.diff, src_zero ? nullptr : f.sourceinput, f.temp, f.fracl, pr.h2t, skip);
+}
+
+void launch_props_h2t(hipStream_t st, const Dims &d, const DevGeom &g, DevFields f, Partials pr,
+                      const amcfd_params &p, const int *skip, bool src_zero) {
+    H2tConst q{p.hlcal, p.hsmelt, p.acpa, p.acpb, p.acpl, p.tliquid, p.tsolid, p.hlatnt, p.delt, p.steady};
+    dispatch<PropsH2tK, TX * TY>(k_props_h2t, node_grid(d), dim3(TX, TY), 0, st, d, q, prop_const(p), g, f.enthalpy,
+                                 f.hnot, f.fraclnot, f.csfrac, src_zero ? nullptr : f.sourceinput, f.temp, f.fracl,
+                                 f.diff, f.rho, f.vis, pr.h2t, skip);
 }
 
 void launch_melt_pool(hipStream_t st, const Dims &d, const DevGeom &g, DevFields f, Partials pr,

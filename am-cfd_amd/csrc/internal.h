@@ -165,6 +165,10 @@ void launch_block_correction(hipStream_t st, const Dims &d, DevFields f, Partial
 // and is not read (mixed mode)
 void launch_h2t(hipStream_t st, const Dims &d, const DevGeom &g, DevFields f, Partials pr,
                 const amcfd_params &p, const int *skip = nullptr, bool src_zero = false);
+// launch_props then launch_h2t in one kernel (the same results), where nothing
+// between them reads or writes what they read or write
+void launch_props_h2t(hipStream_t st, const Dims &d, const DevGeom &g, DevFields f, Partials pr,
+                      const amcfd_params &p, const int *skip = nullptr, bool src_zero = false);
 // with device control (ctl), also sets sc->noflow for the flow kernels
 void launch_melt_pool(hipStream_t st, const Dims &d, const DevGeom &g, DevFields f, Partials pr,
                       const amcfd_params &p, int istart, int jstart, DevScalars *sc, bool ctl = false,

@@ -812,12 +812,13 @@ int Solver::solve_step_fp64() {
         tic("assemble_h", &slot);
         launch_energy_assemble(st_, d_, g_, f_, pr_, p_, sp);
         toc("assemble_h", slot);
-        launch_props(st_, d_, f_, p_);
         launch_block_correction(st_, d_, f_, pr_, f_.enthalpy, shared_device_);
         int st = sweep(f_.enthalpy, plan_dom_, 1, d_.nim2, 1, d_.njm2, 1, d_.nkm2, 0, "sweep_h");
         if (st) return st;
+        // the properties (CFDSolverManager.cpp:624) in the H -> T launch: the
+        // block correction and the sweep read and write none of their fields
         tic("h2t", &slot);
-        launch_h2t(st_, d_, g_, f_, pr_, p_);
+        launch_props_h2t(st_, d_, g_, f_, pr_, p_);
         toc("h2t", slot);
         launch_melt_pool(st_, d_, g_, f_, pr_, p_, istart_, jstart_, sc_);
         HIPCALL(hipGetLastError());
@@ -929,11 +930,9 @@ int Solver::enqueue_mixed_iteration(bool ctl, int it, bool ahead, bool fork_next
     if (st) return st;
     // the properties from T (CFDSolverManager.cpp:624): nothing between the
     // assembly and here reads or writes what they read or write, so they run
-    // after the line passes, which then find more of the coefficients the
-    // assembly has just written still in the caches
-    launch_props(st_, d_, f_, p_, done);
+    // after the line passes, in the H -> T launch
     tic("h2t", &slot);
-    launch_h2t(st_, d_, g_, f_, pr_, p_, done, !mixed_has_volumetric_source(p_));
+    launch_props_h2t(st_, d_, g_, f_, pr_, p_, done, !mixed_has_volumetric_source(p_));
     toc("h2t", slot);
     launch_melt_pool(st_, d_, g_, f_, pr_, p_, istart_, jstart_, sc_, ctl, stepd_);
     HIPCALL(hipGetLastError());
