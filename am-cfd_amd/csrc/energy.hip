@@ -1291,6 +1291,30 @@ __device__ __forceinline__ void converge_test(DevScalars *sc, const StepDev *ste
     }
 }
 
+// a[q] += p[b * stride + off + q] for q < NQV over b = tid, tid + nt, ... < n,
+// in that order (the sums' bits are those of the plain loop), with the loads
+// of U blocks in flight at once: the partials were just written and sit in
+// the caches, so the plain loop is a chain of cache latencies
+template <int NQV, int U>
+__device__ __forceinline__ void strided_fold(const double *__restrict__ p, int n, int stride, int off, int tid,
+                                             int nt, double *a) {
+    int b = tid;
+    for (; b + (U - 1) * nt < n; b += U * nt) {
+        double v[U][NQV];
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+#pragma unroll
+            for (int q = 0; q < NQV; ++q) v[u][q] = p[(size_t)(b + u * nt) * stride + off + q];
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+#pragma unroll
+            for (int q = 0; q < NQV; ++q) a[q] += v[u][q];
+    }
+    for (; b < n; b += nt)
+#pragma unroll
+        for (int q = 0; q < NQV; ++q) a[q] += p[(size_t)b * stride + off + q];
+}
+
 struct FinalizeK {
     static constexpr int family = 6, rank = 0;
     static __device__ __forceinline__ void run(const VB &vb, int nblk_t, int nblk_b, int nblk_f, int nblk_e, int nblk_c,
@@ -1314,19 +1338,10 @@ struct FinalizeK {
     double a[NQ + 1];
 #pragma unroll
     for (int q = 0; q <= NQ; ++q) a[q] = 0.0;
-    for (int b = tid; b < nblk_t; b += nt)
-#pragma unroll
-        for (int q = 0; q < 8; ++q) a[q] += part_t[(size_t)b * 10 + 1 + q];
-    for (int b = tid; b < nblk_b; b += nt) a[8] += part_b[b];
-    for (int b = tid; b < nblk_e; b += nt) {
-        a[9] += part_e[2 * b];
-        a[10] += part_e[2 * b + 1];
-    }
-    if (species_done)
-        for (int b = tid; b < nblk_c; b += nt) {
-            a[15] += part_c[2 * b];
-            a[16] += part_c[2 * b + 1];
-        }
+    strided_fold<8, 8>(part_t, nblk_t, 10, 1, tid, nt, a);
+    strided_fold<1, 8>(part_b, nblk_b, 1, 0, tid, nt, a + 8);
+    strided_fold<2, 8>(part_e, nblk_e, 2, 0, tid, nt, a + 9);
+    if (species_done) strided_fold<2, 8>(part_c, nblk_c, 2, 0, tid, nt, a + 15);
     if (flow_done) {
         for (int b = tid; b < nblk_f; b += nt) {
             const double *pu = part_f + (size_t)b * 3, *pv = part_f + ((size_t)nblk_f + b) * 3;

@@ -728,7 +728,15 @@ struct AddDelphiMxK {
     const int j = 1 + vb.y * 4 + threadIdx.y;
     if (i > d.nim2 || j > d.njm2) return;
     const double dp = delphi[i];
-    for (int k = 1 + vb.z * 4; k < 1 + (vb.z + 1) * 4 && k <= d.nkm2; ++k) phi[cix(d, i, j, k)] += dp;
+    // the four planes' loads before any store, so they are in flight together
+    const int k0 = 1 + vb.z * 4, nk = min(4, d.nkm2 - k0 + 1);
+    double v[4];
+#pragma unroll
+    for (int kk = 0; kk < 4; ++kk)
+        if (kk < nk) v[kk] = phi[cix(d, i, j, k0 + kk)];
+#pragma unroll
+    for (int kk = 0; kk < 4; ++kk)
+        if (kk < nk) phi[cix(d, i, j, k0 + kk)] = v[kk] + dp;
 }
 };
 __global__ __launch_bounds__(256) void k_add_delphi_mx(Dims d, const double *__restrict__ delphi, double *phi,
